@@ -1,0 +1,196 @@
+/*
+ * rfhip.h - MI355X (gfx950) HIP backend for rapidfem-td's DGTD time-stepper.
+ *
+ * Drop-in C ABI for the reference's optional GPU backend
+ * `rapidfem_td::gpu::{GpuContext, GpuOperator}` (crates/rapidfem-td/src/gpu/
+ * mod.rs:25-26), which session.rs calls at :266-277, :589-612 and :866-970.
+ * Each entry point below names the reference method it replaces. Plain
+ * pointers and sizes only; no torch or HIP types cross this boundary.
+ *
+ * Conventions
+ *  - Every function returns an int status, RFHIP_OK (0) on success, and never
+ *    aborts. The message of the last failure on the calling thread is read
+ *    with rfhip_last_error(). The reference returns Result<_, String>; its
+ *    length-mismatch asserts (gpu/operator.rs:518,552) become RFHIP_EINVAL.
+ *  - The caller owns every host buffer: inputs are read during the call,
+ *    outputs are preallocated ((steps+1)*n_dof when recording, else n_dof).
+ *  - The library owns device memory for the lifetime of a handle.
+ *  - One handle = one host thread = one HIP stream (the reference's &mut self
+ *    plus an in-order queue, gpu/operator.rs:436-438).
+ *  - State layout at the boundary is the reference's node-major AoS,
+ *    y[(e*Np + node)*6 + c], c = Ex,Ey,Ez,Hx,Hy,Hz (rhs.rs:492-497).
+ */
+#ifndef RFHIP_H
+#define RFHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RFHIP_OK 0
+#define RFHIP_EINVAL 1      /* bad argument or length mismatch */
+#define RFHIP_ENODEV 2      /* no usable HIP device ("no OpenCL GPU device found", gpu/mod.rs:48-50) */
+#define RFHIP_ENOMEM 3      /* device allocation failed */
+#define RFHIP_EHIP 4        /* a HIP runtime call failed */
+#define RFHIP_ECOLLAPSE 5   /* adaptive step collapsed below min_step_factor*dt (gpu/operator.rs:881-888) */
+#define RFHIP_EUNSUPPORTED 6 /* operator data this form cannot run (e.g. non-conforming faces for a fused form) */
+
+/* Kernel forms of one operator.
+ *  PLAIN_F32  the reference GPU structure and precision: one work-item per
+ *             node apply kernel + a separate stage kernel, f32 state
+ *             (gpu/apply.cl, gpu/lserk.cl).
+ *  PLAIN_F64  the same structure in FP64: the baseline "plain FP64 GPU form".
+ *  FUSED_F64  one fused kernel per RK stage (volume curl + upwind flux + LIFT
+ *             + materials + stage update), FP64 state and arithmetic.
+ *  FUSED_F64C the fused kernel with FP64 arithmetic and compressed
+ *             registers: y stored as f32 hi + bf16 lo, p as f32. */
+#define RFHIP_FORM_PLAIN_F32 0
+#define RFHIP_FORM_PLAIN_F64 1
+#define RFHIP_FORM_FUSED_F64 2
+#define RFHIP_FORM_FUSED_F64C 3
+
+/* Drive of dy/dt = A*y + b*g(t) (session.rs:107-114 Drive). */
+#define RFHIP_DRIVE_FREE 0
+#define RFHIP_DRIVE_POINT 1
+#define RFHIP_DRIVE_VECTOR 2
+
+typedef struct rfhip_ctx rfhip_ctx;
+typedef struct rfhip_op rfhip_op;
+
+/* Copies the calling thread's last error message (NUL-terminated, truncated
+ * to len). Returns the full message length. */
+size_t rfhip_last_error(char *buf, size_t len);
+
+/* ---- GpuContext (gpu/mod.rs:33-135) ------------------------------------- */
+
+/* GpuContext::new (gpu/mod.rs:46-65): first HIP device. */
+int rfhip_ctx_create(rfhip_ctx **out);
+void rfhip_ctx_destroy(rfhip_ctx *ctx);
+/* GpuContext::device_name (gpu/mod.rs:35). */
+int rfhip_ctx_device_name(const rfhip_ctx *ctx, char *buf, size_t len);
+/* GpuContext::fp64 (gpu/mod.rs:38): always 1 on gfx950. */
+int rfhip_ctx_has_fp64(const rfhip_ctx *ctx);
+
+/* ---- GpuOperator (gpu/operator.rs:78-1486) ------------------------------ */
+
+/* Everything GpuOperator::new reads from MaxwellOperator's fields
+ * (gpu/operator.rs:216-292), as the f64 host data MaxwellOperator holds.
+ * face arrays are flattened over e*4 + local_face. */
+typedef struct {
+  int32_t np, nfp;                /* Np, Nfp */
+  int64_t n_elem;                 /* K */
+  const double *diff_r;           /* Np*Np row-major (re.diff_r) */
+  const double *diff_s;
+  const double *diff_t;
+  const double *lift;             /* Np*(4*Nfp) row-major (re.lift) */
+  const int32_t *face_nodes;      /* 4*Nfp (re.face_nodes) */
+  const double *jinv;             /* 9*K, jacobian_inv[i][k] row-major */
+  const double *inv_eps;          /* 3*K */
+  const double *inv_mu;           /* 3*K */
+  const double *sigma_eps;        /* 3*K */
+  const double *sigma_mu;         /* 3*K */
+  const double *face_normal;      /* 12*K */
+  const double *face_fscale;      /* 4*K */
+  const int32_t *face_neighbor;   /* 4*K, -1 on a boundary face */
+  const int32_t *face_nbr_local;  /* 4*K */
+  const int32_t *face_port;       /* 4*K, -1 for none */
+  const int32_t *face_perm;       /* 4*K*Nfp, 0-padded on boundary faces */
+  double flux_alpha;              /* 0 central .. 1 upwind */
+} rfhip_op_desc;
+
+/* GpuOperator::new (gpu/operator.rs:207-396). form: RFHIP_FORM_*. */
+int rfhip_op_create(rfhip_ctx *ctx, const rfhip_op_desc *desc, int form,
+                    rfhip_op **out);
+void rfhip_op_destroy(rfhip_op *op);
+/* GpuOperator::n_dof (gpu/operator.rs:430). */
+int64_t rfhip_op_n_dof(const rfhip_op *op);
+int rfhip_op_form(const rfhip_op *op);
+
+/* GpuOperator::apply (gpu/operator.rs:513-523): dy = A*y, f32 in/out. */
+int rfhip_op_apply(rfhip_op *op, const float *y, float *dy);
+/* FP64 variant (no reference counterpart: the reference GPU apply is f32
+ * only). */
+int rfhip_op_apply_f64(rfhip_op *op, const double *y, double *dy);
+
+/* GpuOperator::transient (gpu/operator.rs:906-953): LSERK4, `steps` frames
+ * of `substeps` steps of dt/substeps, source b*g held per substep
+ * (g has steps*substeps samples unless free). record != 0 writes
+ * (steps+1)*n_dof rows (row 0 = y0), else the final state (n_dof). */
+int rfhip_op_transient(rfhip_op *op, const float *y0, float dt, int64_t steps,
+                       int64_t substeps, int drive_kind, int64_t point_dof,
+                       const double *src_vec, const float *g, int64_t n_g,
+                       int record, float *out);
+/* FP64 boundary variant: the session's f64<->f32 round trip
+ * (session.rs:912-916) would otherwise cap accuracy at f32. */
+int rfhip_op_transient_f64(rfhip_op *op, const double *y0, double dt,
+                           int64_t steps, int64_t substeps, int drive_kind,
+                           int64_t point_dof, const double *src_vec,
+                           const double *g, int64_t n_g, int record,
+                           double *out);
+
+/* session.rs:130-140 Controller, all f64. */
+typedef struct {
+  double atol, rtol, safety, growth_limit, shrink_limit, pi_alpha, pi_beta,
+      min_step_factor;
+} rfhip_controller;
+
+/* KclStats (session.rs:218-223); h in operator time. */
+typedef struct {
+  int64_t accepted, rejected;
+  double h_min, h_max;
+} rfhip_kcl_stats;
+
+/* GpuOperator::transient_kcl (gpu/operator.rs:823-893): KCL RK4(3)5[2R+]C
+ * with the PI controller, g[k] held across frame k (steps samples unless
+ * free). out receives (steps+1)*n_dof rows. RFHIP_ECOLLAPSE when the step
+ * collapses. */
+int rfhip_op_transient_kcl(rfhip_op *op, const float *y0, float dt,
+                           int64_t steps, int drive_kind, int64_t point_dof,
+                           const double *src_vec, const float *g, int64_t n_g,
+                           const rfhip_controller *ctl, float *out,
+                           rfhip_kcl_stats *stats);
+int rfhip_op_transient_kcl_f64(rfhip_op *op, const double *y0, double dt,
+                               int64_t steps, int drive_kind,
+                               int64_t point_dof, const double *src_vec,
+                               const double *g, int64_t n_g,
+                               const rfhip_controller *ctl, double *out,
+                               rfhip_kcl_stats *stats);
+
+/* GpuOperator::expmv (gpu/operator.rs:1175-1197): out = exp(t*A)*v, FP64
+ * Arnoldi (CGS2) with the Hessenberg exponential, following
+ * propagator.rs:186-326: max_dim caps the dimension, tol > 0 enables the
+ * a-posteriori stop (0 = fixed dimension). The basis is held whole in HBM,
+ * so no KRYLOV_CHUNK sub-stepping. dim_used may be NULL. */
+int rfhip_op_expmv(rfhip_op *op, const double *v, double t, int32_t max_dim,
+                   double tol, double *out, int32_t *dim_used);
+/* GpuOperator::etd_step (gpu/operator.rs:1267-1298): out = exp(hA)y +
+ * h*phi1(hA)b via the augmented [[A,b],[0,0]] system
+ * (propagator.rs:338-373). */
+int rfhip_op_etd_step(rfhip_op *op, const double *y, const double *b,
+                      double h, int32_t max_dim, double tol, double *out,
+                      int32_t *dim_used);
+
+/* ---- device-resident runs (no reference counterpart) -------------------
+ * For callers that keep the state in HBM between calls (benchmarks, long
+ * runs without per-frame recording): load once, step, read back. */
+int rfhip_op_state_load(rfhip_op *op, const double *y0);
+int rfhip_op_state_store(rfhip_op *op, double *y);
+/* nsteps free LSERK4 steps of dt on the resident state. */
+int rfhip_op_state_lserk(rfhip_op *op, double dt, int64_t nsteps);
+/* As rfhip_op_state_lserk, timed with HIP events on the operator's stream:
+ * total_ms for the whole run and stage_ms_avg, the mean duration of one
+ * launch of the dominant kernel (the fused stage kernel, or the plain form's
+ * apply kernel). */
+int rfhip_op_state_lserk_timed(rfhip_op *op, double dt, int64_t nsteps,
+                               double *total_ms, double *stage_ms_avg);
+/* Energy 0.5*y^T M~ y of the resident state (rhs.rs:1509-1544), FP64. */
+int rfhip_op_state_energy(rfhip_op *op, const double *mass_ref,
+                          const double *det, double *energy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,18 @@
+"""rapidfem_amd: MI355X (gfx950) backend for rapidfem-td's nodal DGTD
+Maxwell time-stepper.
+
+The compute path is librfhip.so (hand-written HIP kernels behind a C ABI,
+include/rfhip.h); this package is the Python host mirror of the reference's
+GpuContext / GpuOperator / TdSession GPU routes and the operator builder.
+"""
+from ._lib import RfhipError  # noqa: F401  (fails loudly if librfhip.so is missing)
+from .gpu import Controller, Drive, GpuContext, GpuOperator, KclStats  # noqa: F401
+from .maxwell import (ElemMaterial, MaxwellOperator, Mesh, ReferenceElement, flux_alpha,  # noqa: F401
+                      structured_box, structured_box_jittered)
+from .session import GaussianPulse, TdSession  # noqa: F401
+
+__all__ = [
+    "RfhipError", "Controller", "Drive", "GpuContext", "GpuOperator", "KclStats", "ElemMaterial",
+    "MaxwellOperator", "Mesh", "ReferenceElement", "flux_alpha", "structured_box", "structured_box_jittered",
+    "GaussianPulse", "TdSession",
+]

@@ -1,0 +1,1053 @@
+// C ABI of the MI355X DGTD backend (include/rfhip.h): the drop-in for
+// rapidfem_td::gpu::{GpuContext, GpuOperator} (crates/rapidfem-td/src/gpu/).
+// Host orchestration only; kernels live in rfhip_plain.hip, rfhip_fused.hip
+// and rfhip_vec.hip.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <functional>
+#include <map>
+#include <memory>
+#include <unordered_map>
+
+#include "rfhip_impl.hpp"
+
+namespace rfhip {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+// LSERK4 (constants.rs:106-122), KCL RK4(3)5[2R+]C (constants.rs:151-178).
+constexpr double kLsA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                            -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+constexpr double kLsB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                            1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                            2277821191437.0 / 14882151754819.0};
+constexpr double kKclA[4] = {970286171893.0 / 4311952581923.0, 6584761158862.0 / 12103376702013.0,
+                             2251764453980.0 / 15575788980749.0, 26877169314380.0 / 34165994151039.0};
+constexpr double kKclB[5] = {1153189308089.0 / 22510343858157.0, 1772645290293.0 / 4653164025191.0,
+                             -1672844663538.0 / 4480602732383.0, 2114624349019.0 / 3568978502595.0,
+                             5198255086312.0 / 14908931495163.0};
+constexpr double kKclBhat[5] = {1016888040809.0 / 7410784769900.0, 11231460423587.0 / 58533540763752.0,
+                                -1563879915014.0 / 6823010717585.0, 606302364029.0 / 971179775848.0,
+                                1097981568119.0 / 3980877426909.0};
+constexpr double kKclErrFloor = 1e-12;   // constants.rs KCL_ERR_FLOOR
+constexpr double kKclNonfinite = 10.0;   // constants.rs KCL_NONFINITE_ERR
+constexpr double kArnoldiBreakdown = 1e-12;  // constants.rs:62
+constexpr int kMaxKrylov = 64;
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return RFHIP_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return RFHIP_ENOMEM;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return RFHIP_EINVAL;
+  }
+}
+
+bool is_fused(int form) { return form == RFHIP_FORM_FUSED_F64 || form == RFHIP_FORM_FUSED_F64C; }
+bool f64_regs(int form) { return form != RFHIP_FORM_PLAIN_F32; }
+
+// ---- reference-element conventions used to check and encode faces -------
+
+struct Ijk {
+  int i, j, k;
+};
+std::vector<Ijk> node_ijk(int p) {  // dg_basis.rs:270-280
+  std::vector<Ijk> v;
+  for (int i = 0; i <= p; ++i)
+    for (int j = 0; j <= p - i; ++j)
+      for (int k = 0; k <= p - i - j; ++k) v.push_back({i, j, k});
+  return v;
+}
+int order_of_np(int np) {
+  for (int p = 1; p <= 8; ++p)
+    if ((p + 1) * (p + 2) * (p + 3) / 6 == np) return p;
+  return -1;
+}
+constexpr int kFaceVerts[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 2, 3}};  // core mesh.rs:45-50
+int bary(const Ijk &q, int a, int p) {
+  switch (a) {
+    case 0: return p - q.i - q.j - q.k;
+    case 1: return q.i;
+    case 2: return q.j;
+    default: return q.k;
+  }
+}
+// The six slot correspondences s -> sigma[s] between the two copies of a
+// shared triangle.
+constexpr int kSigma[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+template <class T>
+std::vector<T> cvt(const double *src, size_t n) {
+  std::vector<T> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = (T)src[i];
+  return v;
+}
+
+void sync(rfhip_op *op) { RF_HIP(hipStreamSynchronize(op->stream)); }
+
+void check_drive(const rfhip_op *op, int kind, int64_t dof, const double *src, int64_t n_g, int64_t need, const void *g) {
+  RF_REQUIRE(kind >= RFHIP_DRIVE_FREE && kind <= RFHIP_DRIVE_VECTOR, "unknown drive kind");
+  if (kind == RFHIP_DRIVE_POINT) RF_REQUIRE(dof >= 0 && dof < op->n_dof, "source_dof out of range");
+  if (kind == RFHIP_DRIVE_VECTOR) RF_REQUIRE(src != nullptr, "a vector drive needs a source pattern");
+  if (kind != RFHIP_DRIVE_FREE) {
+    RF_REQUIRE(g != nullptr, "a driven run needs waveform samples");
+    RF_REQUIRE(n_g == need, "wrong number of waveform samples");
+  }
+}
+
+// ---- operator construction --------------------------------------------
+
+void build_plain(rfhip_op *op, const rfhip_op_desc &d) {
+  const bool f64 = op->form == RFHIP_FORM_PLAIN_F64;
+  const size_t K = (size_t)d.n_elem, np = d.np, nfp = d.nfp;
+  hipStream_t s = op->stream;
+  auto up = [&](DevBuf &b, const double *src, size_t n) {
+    if (f64) {
+      upload(b, src, n, s);
+    } else {
+      auto v = cvt<float>(src, n);
+      upload(b, v.data(), n, s);
+      sync(op);  // keep v alive until the copy is done
+    }
+  };
+  up(op->dr, d.diff_r, np * np);
+  up(op->ds, d.diff_s, np * np);
+  up(op->dt, d.diff_t, np * np);
+  up(op->lift, d.lift, np * 4 * nfp);
+  up(op->jinv, d.jinv, 9 * K);
+  up(op->inv_eps, d.inv_eps, 3 * K);
+  up(op->inv_mu, d.inv_mu, 3 * K);
+  up(op->sig_eps, d.sigma_eps, 3 * K);
+  up(op->sig_mu, d.sigma_mu, 3 * K);
+  up(op->normal, d.face_normal, 12 * K);
+  up(op->fscale, d.face_fscale, 4 * K);
+  upload(op->face_nodes, d.face_nodes, 4 * nfp, s);
+  upload(op->nbr, d.face_neighbor, 4 * K, s);
+  upload(op->nbr_lf, d.face_nbr_local, 4 * K, s);
+  upload(op->port, d.face_port, 4 * K, s);
+  upload(op->perm, d.face_perm, 4 * K * nfp, s);
+  const size_t w = f64 ? 8 : 4;
+  op->y[0].alloc(w * op->n_dof);
+  op->k.alloc(w * op->n_dof);
+  op->p.alloc(w * op->n_dof);
+  DevOp &dv = op->dev;
+  dv.dr = op->dr.p;
+  dv.ds = op->ds.p;
+  dv.dt = op->dt.p;
+  dv.lift = op->lift.p;
+  dv.face_nodes = op->face_nodes.as<int>();
+  dv.jinv = op->jinv.p;
+  dv.inv_eps = op->inv_eps.p;
+  dv.inv_mu = op->inv_mu.p;
+  dv.sig_eps = op->sig_eps.p;
+  dv.sig_mu = op->sig_mu.p;
+  dv.normal = op->normal.p;
+  dv.fscale = op->fscale.p;
+  dv.nbr = op->nbr.as<int>();
+  dv.nbr_lf = op->nbr_lf.as<int>();
+  dv.port = op->port.as<int>();
+  dv.perm = op->perm.as<int>();
+}
+
+void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
+  const int64_t K = d.n_elem;
+  const int np = d.np, nfp = d.nfp, nf4 = 4 * nfp;
+  const int p = order_of_np(np);
+  hipStream_t s = op->stream;
+  // Node convention check: the fused kernels encode neighbour nodes through
+  // the equispaced node order of ReferenceElement::new.
+  const auto ijk = node_ijk(p);
+  {
+    int c[4] = {0, 0, 0, 0};
+    std::vector<int32_t> fn(4 * nfp, -1);
+    for (int a = 0; a < np; ++a) {
+      const Ijk &q = ijk[a];
+      const bool on[4] = {q.k == 0, q.i == 0, q.j == 0, q.i + q.j + q.k == p};
+      for (int f = 0; f < 4; ++f)
+        if (on[f]) fn[f * nfp + c[f]++] = a;
+    }
+    for (int q = 0; q < 4 * nfp; ++q)
+      if (fn[q] != d.face_nodes[q])
+        throw Error{RFHIP_EUNSUPPORTED, "fused forms need the equispaced node order of ReferenceElement::new"};
+  }
+  // Face node lookup by barycentric weights on the local face slots.
+  std::vector<int> lookup(4 * (p + 1) * (p + 1), -1);
+  for (int f = 0; f < 4; ++f)
+    for (int m = 0; m < nfp; ++m) {
+      const Ijk &q = ijk[d.face_nodes[f * nfp + m]];
+      lookup[(f * (p + 1) + bary(q, kFaceVerts[f][1], p)) * (p + 1) + bary(q, kFaceVerts[f][2], p)] = m;
+    }
+  // Candidate permutations perm[(f, nlf, sigma)][m] and the neighbour-node table.
+  std::vector<int32_t> cand(4 * 4 * 6 * nfp), nbr_node(4 * 4 * 6 * nfp);
+  for (int f = 0; f < 4; ++f)
+    for (int g = 0; g < 4; ++g)
+      for (int o = 0; o < 6; ++o)
+        for (int m = 0; m < nfp; ++m) {
+          const Ijk &q = ijk[d.face_nodes[f * nfp + m]];
+          int w[3];
+          for (int sl = 0; sl < 3; ++sl) w[kSigma[o][sl]] = bary(q, kFaceVerts[f][sl], p);
+          const int m2 = lookup[(g * (p + 1) + w[1]) * (p + 1) + w[2]];
+          const size_t at = ((size_t)(f * 4 + g) * 6 + o) * nfp + m;
+          cand[at] = m2;
+          nbr_node[at] = m2 < 0 ? 0 : d.face_nodes[g * nfp + m2];
+        }
+  // Per-face codes, and a check that normal*fscale is the J^-1 gradient.
+  std::vector<int32_t> fcode(4 * K);
+  double worst = 0.0;
+  for (int64_t e = 0; e < K; ++e) {
+    const double *ji = d.jinv + 9 * e;
+    for (int f = 0; f < 4; ++f) {
+      const int64_t q = 4 * e + f;
+      double g[3];
+      for (int x = 0; x < 3; ++x) {
+        const double r0 = ji[0 * 3 + x], r1 = ji[1 * 3 + x], r2 = ji[2 * 3 + x];
+        g[x] = f == 0 ? -r2 : f == 1 ? -r0 : f == 2 ? -r1 : (r0 + r1 + r2);
+      }
+      const double gn = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+      for (int x = 0; x < 3; ++x)
+        worst = std::max(worst, std::fabs(g[x] - d.face_fscale[q] * d.face_normal[3 * q + x]) / gn);
+      const int nb = d.face_neighbor[q];
+      if (d.face_port[q] >= 0 && nb < 0) {
+        fcode[q] = -2 - d.face_port[q];
+        continue;
+      }
+      if (nb < 0) {
+        fcode[q] = kCodePec;
+        continue;
+      }
+      RF_REQUIRE(nb < K && d.face_nbr_local[q] >= 0 && d.face_nbr_local[q] < 4, "bad neighbour data");
+      if (nb >= (1 << 26)) throw Error{RFHIP_EUNSUPPORTED, "fused forms encode neighbours in 26 bits"};
+      const int g2 = d.face_nbr_local[q];
+      int found = -1;
+      for (int o = 0; o < 6 && found < 0; ++o) {
+        bool ok = true;
+        for (int m = 0; m < nfp && ok; ++m)
+          ok = cand[((size_t)(f * 4 + g2) * 6 + o) * nfp + m] == d.face_perm[q * nfp + m];
+        if (ok) found = o;
+      }
+      if (found < 0) throw Error{RFHIP_EUNSUPPORTED, "face permutation is not a triangle symmetry (non-conforming face)"};
+      fcode[q] = (int32_t)((nb << 5) | (g2 << 3) | found);
+    }
+  }
+  if (worst > 1e-9)
+    throw Error{RFHIP_EUNSUPPORTED, "face normal*fscale is not the J^-1 face gradient (rel " + std::to_string(worst) + ")"};
+  // Material table.
+  std::vector<int32_t> mat(K);
+  std::vector<double> tab;
+  std::map<std::array<double, 12>, int> seen;
+  for (int64_t e = 0; e < K; ++e) {
+    std::array<double, 12> row;
+    for (int c = 0; c < 3; ++c) {
+      row[c] = d.inv_eps[3 * e + c];
+      row[3 + c] = d.inv_mu[3 * e + c];
+      row[6 + c] = d.sigma_eps[3 * e + c];
+      row[9 + c] = d.sigma_mu[3 * e + c];
+    }
+    auto it = seen.find(row);
+    if (it == seen.end()) {
+      it = seen.emplace(row, (int)(tab.size() / 12)).first;
+      tab.insert(tab.end(), row.begin(), row.end());
+    }
+    mat[e] = it->second;
+  }
+  // Transposed reference matrices.
+  std::vector<double> dT(3 * np * np), liftT(nf4 * np);
+  const double *D[3] = {d.diff_r, d.diff_s, d.diff_t};
+  for (int k = 0; k < 3; ++k)
+    for (int j = 0; j < np; ++j)
+      for (int i = 0; i < np; ++i) dT[(k * np + j) * np + i] = D[k][i * np + j];
+  for (int fm = 0; fm < nf4; ++fm)
+    for (int i = 0; i < np; ++i) liftT[fm * np + i] = d.lift[i * nf4 + fm];
+  upload(op->dT, dT.data(), dT.size(), s);
+  upload(op->liftT, liftT.data(), liftT.size(), s);
+  upload(op->jinv64, d.jinv, 9 * (size_t)K, s);
+  upload(op->fcode, fcode.data(), fcode.size(), s);
+  upload(op->mat, mat.data(), mat.size(), s);
+  upload(op->mat_tab, tab.data(), tab.size(), s);
+  upload(op->nbr_node, nbr_node.data(), nbr_node.size(), s);
+  upload(op->face_nodes, d.face_nodes, 4 * (size_t)nfp, s);
+  sync(op);
+  const size_t n = op->n_dof;
+  if (op->form == RFHIP_FORM_FUSED_F64) {
+    op->y[0].alloc(8 * n);
+    op->y[1].alloc(8 * n);
+    op->p.alloc(8 * n);
+  } else {
+    op->y[0].alloc(4 * n);
+    op->y[1].alloc(4 * n);
+    op->ylo[0].alloc(2 * n);
+    op->ylo[1].alloc(2 * n);
+    op->p.alloc(4 * n);
+  }
+  op->k.alloc(8 * n);
+  DevOp &dv = op->dev;
+  dv.dT = op->dT.as<double>();
+  dv.liftT = op->liftT.as<double>();
+  dv.jinv64 = op->jinv64.as<double>();
+  dv.fcode = op->fcode.as<int>();
+  dv.mat = op->mat.as<int>();
+  dv.mat_tab = op->mat_tab.as<double>();
+  dv.nbr_node = op->nbr_node.as<int>();
+  dv.face_nodes = op->face_nodes.as<int>();
+}
+
+// ---- stepping -----------------------------------------------------------
+
+// k = A*x (+ b*g) for the KCL / Krylov paths; x and k in the register
+// precision of the form (f32 for PLAIN_F32, f64 otherwise).
+void rhs_into(rfhip_op *op, const void *x, void *k, int kind, int64_t dof, double g) {
+  hipStream_t s = op->stream;
+  if (is_fused(op->form))
+    launch_fused_apply(op, static_cast<const double *>(x), static_cast<double *>(k), s);
+  else
+    launch_plain_apply(op, x, k, s);
+  if (kind == RFHIP_DRIVE_POINT)
+    launch_add_point(op, k, dof, g, s);
+  else if (kind == RFHIP_DRIVE_VECTOR)
+    launch_add_vec(op, k, op->src.p, g, s);
+}
+
+// One LSERK4 step on the resident state; stage_ev (optional) gets an event
+// pair around each dominant-kernel launch.
+void lserk_step(rfhip_op *op, double h, int kind, int64_t dof, double g, hipEvent_t *stage_ev) {
+  hipStream_t s = op->stream;
+  for (int st = 0; st < 5; ++st) {
+    if (stage_ev) RF_HIP(hipEventRecord(stage_ev[2 * st], s));
+    if (is_fused(op->form)) {
+      FusedIO io{};
+      io.yin = op->y[op->cur].p;
+      io.yin_lo = op->ylo[op->cur].p;
+      io.yout = op->y[1 - op->cur].p;
+      io.yout_lo = op->ylo[1 - op->cur].p;
+      io.p = op->p.p;
+      io.src_kind = kind;
+      io.src_dof = dof;
+      io.src_vec = op->src.as<double>();
+      io.g = g;
+      launch_fused_lserk(op, io, kLsA[st], kLsB[st], h, s);
+      if (stage_ev) RF_HIP(hipEventRecord(stage_ev[2 * st + 1], s));
+      op->cur = 1 - op->cur;
+    } else {
+      launch_plain_apply(op, op->y[0].p, op->k.p, s);
+      if (stage_ev) RF_HIP(hipEventRecord(stage_ev[2 * st + 1], s));
+      if (kind == RFHIP_DRIVE_POINT)
+        launch_add_point(op, op->k.p, dof, g, s);
+      else if (kind == RFHIP_DRIVE_VECTOR)
+        launch_add_vec(op, op->k.p, op->src.p, g, s);
+      launch_plain_stage(op, op->p.p, op->k.p, op->y[0].p, kLsA[st], kLsB[st], h, s);
+    }
+  }
+}
+
+void zero_p(rfhip_op *op) {
+  const size_t w = (op->form == RFHIP_FORM_PLAIN_F32 || op->form == RFHIP_FORM_FUSED_F64C) ? 4 : 8;
+  launch_fill(op->p.p, (int64_t)(w * op->n_dof), op->stream);
+}
+
+// Load an f64 host state into the form's state registers.
+void state_from_host_f64(rfhip_op *op, const double *y0) {
+  const int64_t n = op->n_dof;
+  hipStream_t s = op->stream;
+  op->cur = 0;
+  if (op->form == RFHIP_FORM_PLAIN_F32) {
+    op->kst.alloc(8 * n);
+    RF_HIP(hipMemcpyAsync(op->kst.p, y0, 8 * n, hipMemcpyHostToDevice, s));
+    launch_cast_d2f(op->kst.as<double>(), op->y[0].as<float>(), n, s);
+  } else if (op->form == RFHIP_FORM_FUSED_F64C) {
+    RF_HIP(hipMemcpyAsync(op->k.p, y0, 8 * n, hipMemcpyHostToDevice, s));
+    launch_f64_to_c(op->k.as<double>(), op->y[0].as<float>(), op->ylo[0].as<uint16_t>(), n, s);
+  } else {
+    RF_HIP(hipMemcpyAsync(op->y[0].p, y0, 8 * n, hipMemcpyHostToDevice, s));
+  }
+}
+void state_from_host_f32(rfhip_op *op, const float *y0) {
+  const int64_t n = op->n_dof;
+  hipStream_t s = op->stream;
+  op->cur = 0;
+  if (op->form == RFHIP_FORM_PLAIN_F32) {
+    RF_HIP(hipMemcpyAsync(op->y[0].p, y0, 4 * n, hipMemcpyHostToDevice, s));
+    return;
+  }
+  op->kst.alloc(8 * n);
+  float *tmp = op->kst.as<float>();
+  RF_HIP(hipMemcpyAsync(tmp, y0, 4 * n, hipMemcpyHostToDevice, s));
+  double *dst = op->form == RFHIP_FORM_FUSED_F64C ? op->k.as<double>() : op->y[0].as<double>();
+  launch_cast_f2d(tmp, dst, n, s);
+  if (op->form == RFHIP_FORM_FUSED_F64C)
+    launch_f64_to_c(dst, op->y[0].as<float>(), op->ylo[0].as<uint16_t>(), n, s);
+}
+// Current state as f64 on the device (may stage through op->k).
+const double *state_f64_device(rfhip_op *op) {
+  const int64_t n = op->n_dof;
+  hipStream_t s = op->stream;
+  if (op->form == RFHIP_FORM_PLAIN_F32) {
+    op->kst.alloc(8 * n);
+    launch_cast_f2d(op->y[0].as<float>(), op->kst.as<double>(), n, s);
+    return op->kst.as<double>();
+  }
+  if (op->form == RFHIP_FORM_FUSED_F64C) {
+    launch_c_to_f64(op->y[op->cur].as<float>(), op->ylo[op->cur].as<uint16_t>(), op->k.as<double>(), n, s);
+    return op->k.as<double>();
+  }
+  return op->y[is_fused(op->form) ? op->cur : 0].as<double>();
+}
+void state_to_host_f64(rfhip_op *op, double *out) {
+  const double *src = state_f64_device(op);
+  RF_HIP(hipMemcpyAsync(out, src, 8 * op->n_dof, hipMemcpyDeviceToHost, op->stream));
+  sync(op);
+}
+void state_to_host_f32(rfhip_op *op, float *out) {
+  const int64_t n = op->n_dof;
+  if (op->form == RFHIP_FORM_PLAIN_F32) {
+    RF_HIP(hipMemcpyAsync(out, op->y[0].p, 4 * n, hipMemcpyDeviceToHost, op->stream));
+    sync(op);
+    return;
+  }
+  const double *src = state_f64_device(op);
+  op->kb_x.alloc(4 * n);
+  launch_cast_d2f(src, op->kb_x.as<float>(), n, op->stream);
+  RF_HIP(hipMemcpyAsync(out, op->kb_x.p, 4 * n, hipMemcpyDeviceToHost, op->stream));
+  sync(op);
+}
+
+void load_source(rfhip_op *op, int kind, const double *src) {
+  if (kind != RFHIP_DRIVE_VECTOR) return;
+  upload(op->src, src, op->n_dof, op->stream);
+}
+
+template <class G>
+void transient_impl(rfhip_op *op, int64_t steps, int64_t substeps, double dt, int kind, int64_t dof,
+                    const G *g, bool record, const std::function<void(int64_t row)> &emit) {
+  substeps = std::max<int64_t>(1, substeps);
+  const double h = dt / (double)substeps;
+  zero_p(op);
+  if (record) emit(0);
+  for (int64_t k = 0; k < steps; ++k) {
+    for (int64_t j = 0; j < substeps; ++j) {
+      const double gv = kind == RFHIP_DRIVE_FREE ? 0.0 : (double)g[k * substeps + j];
+      lserk_step(op, h, kind, dof, gv, nullptr);
+    }
+    if (record) emit(k + 1);
+  }
+  if (!record) emit(-1);
+}
+
+// ---- KCL ------------------------------------------------------------------
+
+double factor(const rfhip_controller &c, double err_norm, double prev, bool reject) {  // session.rs:174-186
+  const double err = std::max(err_norm, kKclErrFloor);
+  const double f = (reject || prev <= 0.0) ? c.safety * std::pow(err, -c.pi_alpha)
+                                           : c.safety * std::pow(err, -c.pi_alpha) * std::pow(prev, c.pi_beta);
+  if (reject) return std::max(f, c.shrink_limit);
+  return std::min(std::max(f, c.shrink_limit), c.growth_limit);
+}
+
+// KCL transient on registers y=op->y[0]-like buffer `ys` (f32 or f64).
+template <class G>
+void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_t dof, const G *g,
+              const rfhip_controller &ctl, rfhip_kcl_stats &st, const std::function<void(int64_t)> &emit) {
+  const int64_t n = op->n_dof;
+  const bool f64 = f64_regs(op->form);
+  const size_t w = f64 ? 8 : 4;
+  hipStream_t s = op->stream;
+  op->kst.alloc(w * n);
+  op->kdtf.alloc(w * n);
+  op->kerr.alloc(w * n);
+  op->kbak.alloc(w * n);
+  if (!is_fused(op->form)) op->k.alloc(w * n);
+  const int nblk = weighted_err_blocks(n);
+  op->red.alloc(8 * (nblk + 1));
+  double *part = op->red.as<double>();
+  double *res = part + nblk;
+  double h = dt, prev = 0.0, hmin = INFINITY, hmax = 0.0;
+  int64_t acc = 0, rej = 0;
+  const double h_floor = ctl.min_step_factor * dt;
+  emit(0);
+  for (int64_t fr = 0; fr < steps; ++fr) {
+    double t_rel = 0.0;
+    const double gv = kind == RFHIP_DRIVE_FREE ? 0.0 : (double)g[fr];
+    while (t_rel < dt) {
+      const double ht = std::min(h, dt - t_rel);
+      launch_copy(op->kbak.p, ys, (int64_t)(w * n), s);
+      // explicit_adaptive.rs:107-149
+      rhs_into(op, ys, op->k.p, kind, dof, gv);
+      launch_kcl_stage0(f64, ys, op->kdtf.p, op->kerr.p, op->k.p, ht, kKclB[0], kKclBhat[0] - kKclB[0], n, s);
+      for (int sg = 1; sg < 5; ++sg) {
+        launch_kcl_build(f64, ys, op->kdtf.p, op->kst.p, kKclA[sg - 1] - kKclB[sg - 1], n, s);
+        rhs_into(op, op->kst.p, op->k.p, kind, dof, gv);
+        launch_kcl_accum(f64, ys, op->kdtf.p, op->kerr.p, op->k.p, ht, kKclB[sg], kKclBhat[sg] - kKclB[sg], n, s);
+      }
+      launch_weighted_err(f64, op->kbak.p, ys, op->kerr.p, ctl.atol, ctl.rtol, n, part, nblk, res, s);
+      double ssum = 0.0;
+      RF_HIP(hipMemcpyAsync(&ssum, res, 8, hipMemcpyDeviceToHost, s));
+      sync(op);
+      const double e = std::sqrt(ssum / (double)std::max<int64_t>(n, 1));
+      if (std::isfinite(e) && e <= 1.0) {
+        t_rel += ht;
+        ++acc;
+        h = ht * factor(ctl, e, prev, false);
+        prev = std::max(e, kKclErrFloor);
+        hmin = std::min(hmin, h);
+        hmax = std::max(hmax, h);
+      } else {
+        ++rej;
+        launch_copy(ys, op->kbak.p, (int64_t)(w * n), s);
+        h = ht * factor(ctl, std::isfinite(e) ? e : kKclNonfinite, prev, true);
+      }
+      if (h < h_floor) {
+        st = {acc, rej, hmin, hmax};
+        throw Error{RFHIP_ECOLLAPSE, "GPU KCL: step size collapsed below " + std::to_string(ctl.min_step_factor) +
+                                         "*dt after " + std::to_string(acc) + " accepted, " + std::to_string(rej) +
+                                         " rejected substeps"};
+      }
+    }
+    emit(fr + 1);
+  }
+  st = {acc, rej, hmin, hmax};
+}
+
+// ---- Krylov (propagator.rs:186-373) ---------------------------------------
+
+// Arnoldi with CGS2 on the device, Hessenberg on the host. aug: the ETD
+// augmentation [[A, b], [0, 0]] with the scalar component carried here.
+int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, double tol, bool aug, double *out) {
+  const int64_t n = op->n_dof;
+  RF_REQUIRE(max_dim >= 1 && max_dim <= kMaxKrylov, "Krylov dimension must be in [1, 64]");
+  hipStream_t s = op->stream;
+  double nrm2 = 0.0;
+  for (int64_t i = 0; i < n; ++i) nrm2 += v_host[i] * v_host[i];
+  if (aug) nrm2 += 1.0;
+  const double beta = std::sqrt(nrm2);
+  if (beta == 0.0) {
+    std::fill(out, out + n, 0.0);
+    return 0;
+  }
+  if (op->kb_cap < max_dim) {
+    op->kb_basis.alloc(8 * (size_t)(max_dim + 1) * n);
+    op->kb_cap = max_dim;
+  }
+  op->kb_w.alloc(8 * n);
+  op->kb_x.alloc(8 * n);
+  op->kb_out.alloc(8 * n);
+  op->kb_proj.alloc(8 * (kMaxKrylov + 8));
+  const int nblk = krylov_dot_blocks(n);
+  op->kb_part.alloc(8 * (size_t)nblk * (kMaxKrylov + 1));
+  double *basis = op->kb_basis.as<double>(), *w = op->kb_w.as<double>();
+  double *proj = op->kb_proj.as<double>(), *part = op->kb_part.as<double>();
+  double *norm_dev = proj + kMaxKrylov;
+  const bool f32op = op->form == RFHIP_FORM_PLAIN_F32;
+  if (f32op) {
+    op->kst.alloc(4 * n);
+    op->k.alloc(4 * n);
+  }
+  RF_HIP(hipMemcpyAsync(op->kb_x.p, v_host, 8 * n, hipMemcpyHostToDevice, s));
+  launch_scale(op->kb_x.as<double>(), basis, n, 1.0 / beta, s);
+  std::vector<double> H((size_t)max_dim * max_dim, 0.0), sc(max_dim + 1, 0.0), pj(kMaxKrylov);
+  sc[0] = aug ? 1.0 / beta : 0.0;
+  int dim = max_dim;
+  std::vector<double> th, eth;
+  for (int j = 0; j < max_dim; ++j) {
+    const double *bj = basis + (int64_t)j * n;
+    // matvec (the reference GPU path drops to f32 through cast kernels, gpu/operator.rs:1329-1353)
+    if (f32op) {
+      launch_cast_d2f(bj, op->kst.as<float>(), n, s);
+      launch_plain_apply(op, op->kst.p, op->k.p, s);
+      launch_cast_f2d(op->k.as<float>(), w, n, s);
+    } else if (is_fused(op->form)) {
+      launch_fused_apply(op, bj, w, s);
+    } else {
+      launch_plain_apply(op, bj, w, s);
+    }
+    if (aug && sc[j] != 0.0) launch_axpy(w, op->src.as<double>(), sc[j], n, s);
+    double ws = 0.0;  // augmented scalar of w (zero row of the augmented matrix)
+    const int cols = j + 1;
+    double norm2 = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+      launch_multi_dot(basis, n, cols, w, part, nblk, proj, s);
+      RF_HIP(hipMemcpyAsync(pj.data(), proj, 8 * cols, hipMemcpyDeviceToHost, s));
+      sync(op);
+      double dscal = 0.0;
+      for (int i = 0; i < cols; ++i) {
+        if (aug) pj[i] += sc[i] * ws;
+        H[(size_t)i * max_dim + j] += pj[i];
+        dscal += pj[i] * sc[i];
+      }
+      if (aug) RF_HIP(hipMemcpyAsync(proj, pj.data(), 8 * cols, hipMemcpyHostToDevice, s));
+      launch_axpy_basis_norm(w, basis, n, cols, proj, part, nblk, norm_dev, s);
+      if (aug) ws -= dscal;
+      if (pass == 1) {
+        RF_HIP(hipMemcpyAsync(&norm2, norm_dev, 8, hipMemcpyDeviceToHost, s));
+        sync(op);
+      }
+    }
+    const double hnext = std::sqrt(norm2 + ws * ws);
+    const int d = j + 1;
+    if (hnext < kArnoldiBreakdown || d == max_dim) {
+      dim = d;
+      break;
+    }
+    if (tol > 0.0) {
+      th.assign((size_t)d * d, 0.0);
+      eth.assign((size_t)d * d, 0.0);
+      for (int a = 0; a < d; ++a)
+        for (int b = 0; b < d; ++b) th[(size_t)a * d + b] = t * H[(size_t)a * max_dim + b];
+      expm_host(th.data(), d, eth.data());
+      if (hnext * std::fabs(eth[(size_t)(d - 1) * d]) < tol) {
+        dim = d;
+        break;
+      }
+    }
+    H[(size_t)(j + 1) * max_dim + j] = hnext;
+    launch_scale(w, basis + (int64_t)(j + 1) * n, n, 1.0 / hnext, s);
+    sc[j + 1] = ws / hnext;
+  }
+  th.assign((size_t)dim * dim, 0.0);
+  eth.assign((size_t)dim * dim, 0.0);
+  for (int a = 0; a < dim; ++a)
+    for (int b = 0; b < dim; ++b) th[(size_t)a * dim + b] = t * H[(size_t)a * max_dim + b];
+  expm_host(th.data(), dim, eth.data());
+  for (int i = 0; i < dim; ++i) pj[i] = beta * eth[(size_t)i * dim];
+  RF_HIP(hipMemcpyAsync(proj, pj.data(), 8 * dim, hipMemcpyHostToDevice, s));
+  launch_lincomb(basis, n, dim, proj, op->kb_out.as<double>(), s);
+  RF_HIP(hipMemcpyAsync(out, op->kb_out.p, 8 * n, hipMemcpyDeviceToHost, s));
+  sync(op);
+  return dim;
+}
+
+}  // namespace
+
+// propagator.rs:69-114: infinity-norm scaling to <= 0.5, Taylor 18, squaring.
+// Kept step for step (and with the reference's zero-skip products) so the
+// Hessenberg exponential matches the CPU path it replaces.
+void expm_host(const double *a, int n, double *out) {
+  const size_t nn = (size_t)n * n;
+  double norm = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double row = 0.0;
+    for (int j = 0; j < n; ++j) row += std::fabs(a[(size_t)i * n + j]);
+    norm = std::max(norm, row);
+  }
+  int sq = 0;
+  if (norm > 0.5) sq = (int)std::max<long>(0, (long)std::ceil(std::log2(norm)) + 1);
+  double scale = 1.0;
+  for (int q = 0; q < sq; ++q) scale *= 2.0;
+  std::vector<double> b(nn), res(nn, 0.0), term(nn, 0.0), t2(nn), tmp(nn);
+  for (size_t k = 0; k < nn; ++k) b[k] = a[k] / scale;
+  for (int i = 0; i < n; ++i) res[(size_t)i * n + i] = term[(size_t)i * n + i] = 1.0;
+  auto mm = [n](const std::vector<double> &x, const std::vector<double> &y, std::vector<double> &o) {
+    std::fill(o.begin(), o.end(), 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < n; ++k) {
+        const double xik = x[(size_t)i * n + k];
+        if (xik == 0.0) continue;
+        for (int j = 0; j < n; ++j) o[(size_t)i * n + j] += xik * y[(size_t)k * n + j];
+      }
+  };
+  for (int k = 1; k <= 18; ++k) {
+    mm(term, b, t2);
+    const double inv = 1.0 / (double)k;
+    for (size_t x = 0; x < nn; ++x) t2[x] *= inv;
+    for (size_t x = 0; x < nn; ++x) res[x] += t2[x];
+    term.swap(t2);
+  }
+  for (int q = 0; q < sq; ++q) {
+    mm(res, res, tmp);
+    res.swap(tmp);
+  }
+  std::copy(res.begin(), res.end(), out);
+}
+
+}  // namespace rfhip
+
+using namespace rfhip;
+
+extern "C" {
+
+size_t rfhip_last_error(char *buf, size_t len) {
+  if (buf && len) {
+    const size_t c = std::min(len - 1, g_last_error.size());
+    std::memcpy(buf, g_last_error.data(), c);
+    buf[c] = 0;
+  }
+  return g_last_error.size();
+}
+
+int rfhip_ctx_create(rfhip_ctx **out) {
+  return guarded([&] {
+    RF_REQUIRE(out, "null output handle");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) throw Error{RFHIP_ENODEV, "no HIP GPU device found"};
+    auto ctx = std::make_unique<rfhip_ctx>();
+    ctx->device = 0;
+    RF_HIP(hipSetDevice(0));
+    hipDeviceProp_t prop;
+    RF_HIP(hipGetDeviceProperties(&prop, 0));
+    ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    *out = ctx.release();
+  });
+}
+
+void rfhip_ctx_destroy(rfhip_ctx *ctx) { delete ctx; }
+
+int rfhip_ctx_device_name(const rfhip_ctx *ctx, char *buf, size_t len) {
+  return guarded([&] {
+    RF_REQUIRE(ctx && buf && len, "bad arguments");
+    const size_t c = std::min(len - 1, ctx->name.size());
+    std::memcpy(buf, ctx->name.data(), c);
+    buf[c] = 0;
+  });
+}
+
+int rfhip_ctx_has_fp64(const rfhip_ctx *ctx) { return ctx ? 1 : 0; }
+
+int rfhip_op_create(rfhip_ctx *ctx, const rfhip_op_desc *d, int form, rfhip_op **out) {
+  return guarded([&] {
+    RF_REQUIRE(ctx && d && out, "null argument");
+    RF_REQUIRE(form >= RFHIP_FORM_PLAIN_F32 && form <= RFHIP_FORM_FUSED_F64C, "unknown form");
+    RF_REQUIRE(d->n_elem >= 1, "empty operator");
+    const int p = order_of_np(d->np);
+    RF_REQUIRE(p >= 1 && d->nfp == (p + 1) * (p + 2) / 2, "inconsistent Np / Nfp");
+    if (!(d->np == 4 || d->np == 10 || d->np == 20 || d->np == 35))
+      throw Error{RFHIP_EUNSUPPORTED, "orders 1..4 are compiled (Np 4, 10, 20, 35)"};
+    RF_REQUIRE(d->diff_r && d->diff_s && d->diff_t && d->lift && d->face_nodes && d->jinv && d->inv_eps && d->inv_mu &&
+                   d->sigma_eps && d->sigma_mu && d->face_normal && d->face_fscale && d->face_neighbor &&
+                   d->face_nbr_local && d->face_port && d->face_perm,
+               "missing operator array");
+    RF_REQUIRE(6 * (int64_t)d->np * d->n_elem < ((int64_t)1 << 40), "operator too large");
+    RF_HIP(hipSetDevice(ctx->device));
+    auto op = std::make_unique<rfhip_op>();
+    op->ctx = ctx;
+    op->form = form;
+    op->np = d->np;
+    op->nfp = d->nfp;
+    op->n_elem = d->n_elem;
+    op->n_dof = 6 * (int64_t)d->np * d->n_elem;
+    op->alpha = d->flux_alpha;
+    RF_HIP(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    op->port_host.assign(d->face_port, d->face_port + 4 * d->n_elem);
+    op->dev = DevOp{};
+    op->dev.np = d->np;
+    op->dev.nfp = d->nfp;
+    op->dev.n_elem = d->n_elem;
+    op->dev.alpha = d->flux_alpha;
+    if (is_fused(form))
+      build_fused(op.get(), *d);
+    else
+      build_plain(op.get(), *d);
+    sync(op.get());
+    *out = op.release();
+  });
+}
+
+void rfhip_op_destroy(rfhip_op *op) {
+  if (!op) return;
+  if (op->stream) {
+    (void)hipStreamSynchronize(op->stream);
+    (void)hipStreamDestroy(op->stream);
+  }
+  delete op;
+}
+
+int64_t rfhip_op_n_dof(const rfhip_op *op) { return op ? op->n_dof : -1; }
+int rfhip_op_form(const rfhip_op *op) { return op ? op->form : -1; }
+
+int rfhip_op_apply(rfhip_op *op, const float *y, float *dy) {
+  return guarded([&] {
+    RF_REQUIRE(op && y && dy, "null argument");
+    const int64_t n = op->n_dof;
+    hipStream_t s = op->stream;
+    if (op->form == RFHIP_FORM_PLAIN_F32) {
+      RF_HIP(hipMemcpyAsync(op->y[0].p, y, 4 * n, hipMemcpyHostToDevice, s));
+      launch_plain_apply(op, op->y[0].p, op->k.p, s);
+      RF_HIP(hipMemcpyAsync(dy, op->k.p, 4 * n, hipMemcpyDeviceToHost, s));
+      sync(op);
+      return;
+    }
+    op->kst.alloc(8 * n);
+    op->kb_x.alloc(8 * n);
+    RF_HIP(hipMemcpyAsync(op->kb_x.p, y, 4 * n, hipMemcpyHostToDevice, s));
+    launch_cast_f2d(op->kb_x.as<float>(), op->kst.as<double>(), n, s);
+    if (is_fused(op->form))
+      launch_fused_apply(op, op->kst.as<double>(), op->k.as<double>(), s);
+    else
+      launch_plain_apply(op, op->kst.p, op->k.p, s);
+    launch_cast_d2f(op->k.as<double>(), op->kb_x.as<float>(), n, s);
+    RF_HIP(hipMemcpyAsync(dy, op->kb_x.p, 4 * n, hipMemcpyDeviceToHost, s));
+    sync(op);
+  });
+}
+
+int rfhip_op_apply_f64(rfhip_op *op, const double *y, double *dy) {
+  return guarded([&] {
+    RF_REQUIRE(op && y && dy, "null argument");
+    const int64_t n = op->n_dof;
+    hipStream_t s = op->stream;
+    if (op->form == RFHIP_FORM_PLAIN_F32) {
+      op->kb_x.alloc(8 * n);
+      RF_HIP(hipMemcpyAsync(op->kb_x.p, y, 8 * n, hipMemcpyHostToDevice, s));
+      launch_cast_d2f(op->kb_x.as<double>(), op->y[0].as<float>(), n, s);
+      launch_plain_apply(op, op->y[0].p, op->k.p, s);
+      launch_cast_f2d(op->k.as<float>(), op->kb_x.as<double>(), n, s);
+      RF_HIP(hipMemcpyAsync(dy, op->kb_x.p, 8 * n, hipMemcpyDeviceToHost, s));
+      sync(op);
+      return;
+    }
+    op->kst.alloc(8 * n);
+    RF_HIP(hipMemcpyAsync(op->kst.p, y, 8 * n, hipMemcpyHostToDevice, s));
+    if (is_fused(op->form))
+      launch_fused_apply(op, op->kst.as<double>(), op->k.as<double>(), s);
+    else
+      launch_plain_apply(op, op->kst.p, op->k.p, s);
+    RF_HIP(hipMemcpyAsync(dy, op->k.p, 8 * n, hipMemcpyDeviceToHost, s));
+    sync(op);
+  });
+}
+
+int rfhip_op_transient(rfhip_op *op, const float *y0, float dt, int64_t steps, int64_t substeps, int kind,
+                       int64_t dof, const double *src, const float *g, int64_t n_g, int record, float *out) {
+  return guarded([&] {
+    RF_REQUIRE(op && y0 && out, "null argument");
+    RF_REQUIRE(steps >= 0, "negative step count");
+    substeps = std::max<int64_t>(1, substeps);
+    check_drive(op, kind, dof, src, n_g, steps * substeps, g);
+    load_source(op, kind, src);
+    state_from_host_f32(op, y0);
+    const int64_t n = op->n_dof;
+    transient_impl<float>(op, steps, substeps, (double)dt, kind, dof, g, record != 0, [&](int64_t row) {
+      if (row == 0)
+        std::memcpy(out, y0, sizeof(float) * n);
+      else
+        state_to_host_f32(op, row < 0 ? out : out + row * n);
+    });
+  });
+}
+
+int rfhip_op_transient_f64(rfhip_op *op, const double *y0, double dt, int64_t steps, int64_t substeps, int kind,
+                           int64_t dof, const double *src, const double *g, int64_t n_g, int record, double *out) {
+  return guarded([&] {
+    RF_REQUIRE(op && y0 && out, "null argument");
+    RF_REQUIRE(steps >= 0, "negative step count");
+    substeps = std::max<int64_t>(1, substeps);
+    check_drive(op, kind, dof, src, n_g, steps * substeps, g);
+    load_source(op, kind, src);
+    state_from_host_f64(op, y0);
+    const int64_t n = op->n_dof;
+    transient_impl<double>(op, steps, substeps, dt, kind, dof, g, record != 0, [&](int64_t row) {
+      if (row == 0)
+        std::memcpy(out, y0, sizeof(double) * n);
+      else
+        state_to_host_f64(op, row < 0 ? out : out + row * n);
+    });
+  });
+}
+
+int rfhip_op_transient_kcl(rfhip_op *op, const float *y0, float dt, int64_t steps, int kind, int64_t dof,
+                           const double *src, const float *g, int64_t n_g, const rfhip_controller *ctl, float *out,
+                           rfhip_kcl_stats *stats) {
+  return guarded([&] {
+    RF_REQUIRE(op && y0 && out && ctl, "null argument");
+    check_drive(op, kind, dof, src, n_g, steps, g);
+    load_source(op, kind, src);
+    const int64_t n = op->n_dof;
+    hipStream_t s = op->stream;
+    const bool f64 = f64_regs(op->form);
+    op->kb_w.alloc(8 * n);  // KCL state register
+    op->kb_x.alloc(8 * n);
+    void *ys = op->kb_w.p;
+    RF_HIP(hipMemcpyAsync(op->kb_x.p, y0, 4 * n, hipMemcpyHostToDevice, s));
+    if (f64)
+      launch_cast_f2d(op->kb_x.as<float>(), op->kb_w.as<double>(), n, s);
+    else
+      launch_copy(ys, op->kb_x.p, 4 * n, s);
+    rfhip_kcl_stats st{};
+    try {
+      kcl_impl<float>(op, ys, (double)dt, steps, kind, dof, g, *ctl, st, [&](int64_t row) {
+        if (f64) {
+          launch_cast_d2f(op->kb_w.as<double>(), op->kb_x.as<float>(), n, s);
+          RF_HIP(hipMemcpyAsync(out + row * n, op->kb_x.p, 4 * n, hipMemcpyDeviceToHost, s));
+        } else {
+          RF_HIP(hipMemcpyAsync(out + row * n, ys, 4 * n, hipMemcpyDeviceToHost, s));
+        }
+        sync(op);
+      });
+    } catch (const Error &) {
+      if (stats) *stats = st;
+      throw;
+    }
+    if (stats) *stats = st;
+  });
+}
+
+int rfhip_op_transient_kcl_f64(rfhip_op *op, const double *y0, double dt, int64_t steps, int kind, int64_t dof,
+                               const double *src, const double *g, int64_t n_g, const rfhip_controller *ctl,
+                               double *out, rfhip_kcl_stats *stats) {
+  return guarded([&] {
+    RF_REQUIRE(op && y0 && out && ctl, "null argument");
+    check_drive(op, kind, dof, src, n_g, steps, g);
+    load_source(op, kind, src);
+    const int64_t n = op->n_dof;
+    hipStream_t s = op->stream;
+    const bool f64 = f64_regs(op->form);
+    op->kb_w.alloc(8 * n);
+    op->kb_x.alloc(8 * n);
+    void *ys = op->kb_w.p;
+    RF_HIP(hipMemcpyAsync(op->kb_x.p, y0, 8 * n, hipMemcpyHostToDevice, s));
+    if (f64)
+      launch_copy(ys, op->kb_x.p, 8 * n, s);
+    else
+      launch_cast_d2f(op->kb_x.as<double>(), op->kb_w.as<float>(), n, s);
+    rfhip_kcl_stats st{};
+    try {
+      kcl_impl<double>(op, ys, dt, steps, kind, dof, g, *ctl, st, [&](int64_t row) {
+        if (f64) {
+          RF_HIP(hipMemcpyAsync(out + row * n, ys, 8 * n, hipMemcpyDeviceToHost, s));
+        } else {
+          launch_cast_f2d(op->kb_w.as<float>(), op->kb_x.as<double>(), n, s);
+          RF_HIP(hipMemcpyAsync(out + row * n, op->kb_x.p, 8 * n, hipMemcpyDeviceToHost, s));
+        }
+        sync(op);
+      });
+    } catch (const Error &) {
+      if (stats) *stats = st;
+      throw;
+    }
+    if (stats) *stats = st;
+  });
+}
+
+int rfhip_op_expmv(rfhip_op *op, const double *v, double t, int32_t max_dim, double tol, double *out,
+                   int32_t *dim_used) {
+  return guarded([&] {
+    RF_REQUIRE(op && v && out, "null argument");
+    const int dim = krylov_impl(op, v, t, max_dim, tol, false, out);
+    if (dim_used) *dim_used = dim;
+  });
+}
+
+int rfhip_op_etd_step(rfhip_op *op, const double *y, const double *b, double h, int32_t max_dim, double tol,
+                      double *out, int32_t *dim_used) {
+  return guarded([&] {
+    RF_REQUIRE(op && y && b && out, "null argument");
+    upload(op->src, b, op->n_dof, op->stream);
+    const int dim = krylov_impl(op, y, h, max_dim, tol, true, out);
+    if (dim_used) *dim_used = dim;
+  });
+}
+
+int rfhip_op_state_load(rfhip_op *op, const double *y0) {
+  return guarded([&] {
+    RF_REQUIRE(op && y0, "null argument");
+    state_from_host_f64(op, y0);
+    sync(op);
+  });
+}
+
+int rfhip_op_state_store(rfhip_op *op, double *y) {
+  return guarded([&] {
+    RF_REQUIRE(op && y, "null argument");
+    state_to_host_f64(op, y);
+  });
+}
+
+int rfhip_op_state_lserk(rfhip_op *op, double dt, int64_t nsteps) {
+  return guarded([&] {
+    RF_REQUIRE(op, "null argument");
+    zero_p(op);
+    for (int64_t k = 0; k < nsteps; ++k) lserk_step(op, dt, RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
+    sync(op);
+  });
+}
+
+int rfhip_op_state_lserk_timed(rfhip_op *op, double dt, int64_t nsteps, double *total_ms, double *stage_ms_avg) {
+  return guarded([&] {
+    RF_REQUIRE(op && nsteps >= 1, "bad arguments");
+    hipStream_t s = op->stream;
+    std::vector<hipEvent_t> ev((size_t)10 * nsteps + 2, nullptr);
+    struct Cleanup {
+      std::vector<hipEvent_t> &v;
+      ~Cleanup() {
+        for (auto e : v)
+          if (e) (void)hipEventDestroy(e);
+      }
+    } cleanup{ev};
+    for (auto &e : ev) RF_HIP(hipEventCreate(&e));
+    zero_p(op);
+    RF_HIP(hipEventRecord(ev[10 * nsteps], s));
+    for (int64_t k = 0; k < nsteps; ++k) lserk_step(op, dt, RFHIP_DRIVE_FREE, 0, 0.0, &ev[10 * k]);
+    RF_HIP(hipEventRecord(ev[10 * nsteps + 1], s));
+    sync(op);
+    float ms = 0.0f;
+    double kern = 0.0;
+    for (int64_t q = 0; q < 5 * nsteps; ++q) {
+      RF_HIP(hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]));
+      kern += ms;
+    }
+    RF_HIP(hipEventElapsedTime(&ms, ev[10 * nsteps], ev[10 * nsteps + 1]));
+    if (total_ms) *total_ms = ms;
+    if (stage_ms_avg) *stage_ms_avg = kern / (double)(5 * nsteps);
+  });
+}
+
+int rfhip_op_state_energy(rfhip_op *op, const double *mass_ref, const double *det, double *energy) {
+  return guarded([&] {
+    RF_REQUIRE(op && mass_ref && det && energy, "null argument");
+    hipStream_t s = op->stream;
+    const int np = op->np;
+    const int64_t K = op->n_elem;
+    const double *y = state_f64_device(op);
+    DevBuf m, dd, ie, im, part;
+    upload(m, mass_ref, (size_t)np * np, s);
+    upload(dd, det, (size_t)K, s);
+    // materials as f64 per element from the form's data
+    std::vector<double> hie(3 * K), him(3 * K);
+    if (is_fused(op->form)) {
+      std::vector<int32_t> mat(K);
+      std::vector<double> tab(op->mat_tab.bytes / 8);
+      RF_HIP(hipMemcpyAsync(mat.data(), op->mat.p, 4 * K, hipMemcpyDeviceToHost, s));
+      RF_HIP(hipMemcpyAsync(tab.data(), op->mat_tab.p, op->mat_tab.bytes, hipMemcpyDeviceToHost, s));
+      sync(op);
+      for (int64_t e = 0; e < K; ++e)
+        for (int c = 0; c < 3; ++c) {
+          hie[3 * e + c] = tab[12 * mat[e] + c];
+          him[3 * e + c] = tab[12 * mat[e] + 3 + c];
+        }
+    } else if (op->form == RFHIP_FORM_PLAIN_F64) {
+      RF_HIP(hipMemcpyAsync(hie.data(), op->inv_eps.p, 24 * K, hipMemcpyDeviceToHost, s));
+      RF_HIP(hipMemcpyAsync(him.data(), op->inv_mu.p, 24 * K, hipMemcpyDeviceToHost, s));
+      sync(op);
+    } else {
+      std::vector<float> fe(3 * K), fmu(3 * K);
+      RF_HIP(hipMemcpyAsync(fe.data(), op->inv_eps.p, 12 * K, hipMemcpyDeviceToHost, s));
+      RF_HIP(hipMemcpyAsync(fmu.data(), op->inv_mu.p, 12 * K, hipMemcpyDeviceToHost, s));
+      sync(op);
+      for (int64_t q = 0; q < 3 * K; ++q) {
+        hie[q] = fe[q];
+        him[q] = fmu[q];
+      }
+    }
+    upload(ie, hie.data(), hie.size(), s);
+    upload(im, him.data(), him.size(), s);
+    const int nblk = 1024;
+    part.alloc(8 * (nblk + 1));
+    launch_energy(y, m.as<double>(), dd.as<double>(), ie.as<double>(), im.as<double>(), np, K, part.as<double>(),
+                  nblk, part.as<double>() + nblk, s);
+    double half = 0.0;
+    RF_HIP(hipMemcpyAsync(&half, part.as<double>() + nblk, 8, hipMemcpyDeviceToHost, s));
+    sync(op);
+    *energy = 0.5 * half;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,274 @@
+// Fused DG stage kernel, FP64 arithmetic.
+//
+// One launch per LSERK4 stage does what the reference GPU path spreads over
+// two kernels (gpu/apply.cl:17-209 + gpu/lserk.cl:8-20): volume curl,
+// upwind flux, LIFT, materials, held source and the low-storage update
+// p <- a p + dt k, y <- y + b p. y is double-buffered (read y_in, write
+// y_out) because neighbouring elements read y_in's face traces during the
+// same stage.
+//
+// Per-element HBM traffic is the state itself plus 92 bytes of geometry:
+//  * J^-1 (9 doubles). Face normals and fscale are not stored: with
+//    g_f = -grad(lambda_f) = {-row2, -row0, -row1, row0+row1+row2} of J^-1
+//    for the four local faces, n = g/|g| and fscale = 2 area/|det J| = |g|,
+//    so 0.5*fscale*n = 0.5*g (rhs.rs:829, topology.rs:56-73).
+//  * one int32 per face: neighbour, its local face and the orientation of
+//    the shared triangle, which index a small table of neighbour nodes
+//    (replaces face_perm + face_nodes double indirection, apply.cl:151-152).
+//  * a material index into a table of distinct (1/eps, 1/mu, sigma/eps,
+//    sigma*/mu) rows.
+//
+// Work mapping: a block holds EPB = 256/Np elements, one thread per
+// (element, node). The block's state is staged in LDS with coalesced loads;
+// D^T and LIFT^T are read from L1 with consecutive nodes on consecutive
+// lanes.
+#include "rfhip_impl.hpp"
+
+namespace rfhip {
+namespace {
+
+constexpr int kFusedWG = 256;
+
+__device__ __forceinline__ uint16_t bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+template <bool CIN>
+__device__ __forceinline__ double load_state(const FusedIO &io, int64_t i) {
+  if constexpr (CIN) {
+    return (double)static_cast<const float *>(io.yin)[i] + (double)bf16_to_f(static_cast<const uint16_t *>(io.yin_lo)[i]);
+  } else {
+    return static_cast<const double *>(io.yin)[i];
+  }
+}
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// EPI: 0 = apply (dy = A y, FP64), 1 = LSERK stage with FP64 registers,
+// 2 = LSERK stage with compressed registers (y f32+bf16, p f32).
+template <int NP, int NFP, bool CIN, int EPI>
+__global__ void __launch_bounds__(kFusedWG) k_fused(DevOp d, FusedIO io, double sa, double sb, double dt) {
+  constexpr int EPB = kFusedWG / NP;
+  constexpr int NF4 = 4 * NFP;
+  __shared__ double su[EPB * NP * 6];
+  __shared__ double sf[EPB * NF4 * 6];
+  const int tid = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int64_t rem = d.n_elem - e0;
+  const int nel = rem < EPB ? (int)rem : EPB;
+  {
+    const int64_t g0 = e0 * NP * 6;
+    const int cnt = nel * NP * 6;
+    for (int q = tid; q < cnt; q += kFusedWG) su[q] = load_state<CIN>(io, g0 + q);
+  }
+  __syncthreads();
+  const int le = tid / NP, i = tid - (tid / NP) * NP;
+  const bool active = tid < nel * NP;
+  const int64_t e = e0 + le;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  double ji[9];
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) ji[q] = d.jinv64[e * 9 + q];
+    // Volume: r[k][c] = sum_j D_k[i][j] u[j][c] for all six components.
+    double r[3][6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
+    const double *ub = su + le * NP * 6;
+#pragma unroll 4
+    for (int j = 0; j < NP; ++j) {
+      double u[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) u[c] = ub[j * 6 + c];
+      const double d0 = d.dT[(0 * NP + j) * NP + i];
+      const double d1 = d.dT[(1 * NP + j) * NP + i];
+      const double d2 = d.dT[(2 * NP + j) * NP + i];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        r[0][c] = fma(d0, u[c], r[0][c]);
+        r[1][c] = fma(d1, u[c], r[1][c]);
+        r[2][c] = fma(d2, u[c], r[2][c]);
+      }
+    }
+    // Physical derivative d(f_c)/dx_p = sum_k J^-1[k][p] r[k][c]; curl.
+    auto pd = [&](int p, int c) { return ji[0 * 3 + p] * r[0][c] + ji[1 * 3 + p] * r[1][c] + ji[2 * 3 + p] * r[2][c]; };
+    acc[0] = pd(1, 5) - pd(2, 4);
+    acc[1] = pd(2, 3) - pd(0, 5);
+    acc[2] = pd(0, 4) - pd(1, 3);
+    acc[3] = -(pd(1, 2) - pd(2, 1));
+    acc[4] = -(pd(2, 0) - pd(0, 2));
+    acc[5] = -(pd(0, 1) - pd(1, 0));
+
+    // Surface flux per face node, scaled by 0.5*fscale (folded into g).
+    for (int fm = i; fm < NF4; fm += NP) {
+      const int f = fm / NFP, m = fm - f * NFP;
+      double g[3];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        const double r0 = ji[0 * 3 + x], r1 = ji[1 * 3 + x], r2 = ji[2 * 3 + x];
+        g[x] = f == 0 ? -r2 : f == 1 ? -r0 : f == 2 ? -r1 : (r0 + r1 + r2);
+      }
+      const double g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+      const double inv_g = 1.0 / sqrt(g2);
+      const int code = d.fcode[e * 4 + f];
+      const int vi = d.face_nodes[f * NFP + m];
+      double em[3], hm[3], je[3], jh[3];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        em[x] = ub[vi * 6 + x];
+        hm[x] = ub[vi * 6 + 3 + x];
+      }
+      double a = d.alpha;
+      if (code >= 0) {
+        const int64_t nb = code >> 5;
+        const int nlf = (code >> 3) & 3, o = code & 7;
+        const int vj = d.nbr_node[((f * 4 + nlf) * 6 + o) * NFP + m];
+        const int64_t nbb = (nb * NP + vj) * 6;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          je[x] = em[x] - load_state<CIN>(io, nbb + x);
+          jh[x] = hm[x] - load_state<CIN>(io, nbb + 3 + x);
+        }
+      } else if (code == kCodePec) {
+        // PEC ghost: [E] = 2 E_t, [H] = 2 (n.H) n (rhs.rs:1387-1402).
+        const double ig2 = inv_g * inv_g;
+        const double edg = (g[0] * em[0] + g[1] * em[1] + g[2] * em[2]) * ig2;
+        const double hdg = (g[0] * hm[0] + g[1] * hm[1] + g[2] * hm[2]) * ig2;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          je[x] = 2.0 * (em[x] - edg * g[x]);
+          jh[x] = 2.0 * hdg * g[x];
+        }
+      } else {
+        // Port: characteristic flux against a zero incident field (rhs.rs:1380-1386).
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          je[x] = em[x];
+          jh[x] = hm[x];
+        }
+        a = 1.0;
+      }
+      // coef n x [.] = 0.5 g x [.];  coef n x (n x [.]) = 0.5 g x (g x [.]) / |g|.
+      double gxh[3], gxe[3], gge[3], ggh[3];
+      cross3(g, jh, gxh);
+      cross3(g, je, gxe);
+      cross3(g, gxe, gge);
+      cross3(g, gxh, ggh);
+      const double ha = 0.5 * a * inv_g;
+      double *out = sf + (le * NF4 + fm) * 6;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        out[x] = -0.5 * gxh[x] + ha * gge[x];
+        out[3 + x] = 0.5 * gxe[x] + ha * ggh[x];
+      }
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  {
+    const double *fb = sf + le * NF4 * 6;
+#pragma unroll 4
+    for (int fm = 0; fm < NF4; ++fm) {
+      const double w = d.liftT[fm * NP + i];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc[c] = fma(w, fb[fm * 6 + c], acc[c]);
+    }
+  }
+  const double *mt = d.mat_tab + 12 * d.mat[e];
+  const double *ub = su + (le * NP + i) * 6;
+  double kv[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    kv[c] = mt[c] * acc[c] - mt[6 + c] * ub[c];
+    kv[3 + c] = mt[3 + c] * acc[3 + c] - mt[9 + c] * ub[3 + c];
+  }
+  const int64_t base = (e * NP + i) * 6;
+  if (io.src_kind == RFHIP_DRIVE_POINT) {
+    const int64_t off = io.src_dof - base;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (off == c) kv[c] += io.g;
+  } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) kv[c] += io.src_vec[base + c] * io.g;
+  }
+  if constexpr (EPI == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) io.dy[base + c] = kv[c];
+  } else if constexpr (EPI == 1) {
+    double *pp = static_cast<double *>(io.p);
+    double *yo = static_cast<double *>(io.yout);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double pn = sa * pp[base + c] + dt * kv[c];
+      pp[base + c] = pn;
+      yo[base + c] = ub[c] + sb * pn;
+    }
+  } else {
+    float *pp = static_cast<float *>(io.p);
+    float *yh = static_cast<float *>(io.yout);
+    uint16_t *yl = static_cast<uint16_t *>(io.yout_lo);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const float pn = (float)(sa * (double)pp[base + c] + dt * kv[c]);
+      pp[base + c] = pn;
+      const double yn = ub[c] + sb * (double)pn;
+      const float h = (float)yn;
+      yh[base + c] = h;
+      yl[base + c] = bf16_rne((float)(yn - (double)h));
+    }
+  }
+}
+
+template <int NP>
+struct Nfp {
+  static constexpr int v = NP == 4 ? 3 : NP == 10 ? 6 : NP == 20 ? 10 : 15;
+};
+
+template <int NP, bool CIN, int EPI>
+void fused_np(const rfhip_op *op, const FusedIO &io, double a, double b, double dt, hipStream_t s) {
+  constexpr int EPB = kFusedWG / NP;
+  const int64_t groups = (op->n_elem + EPB - 1) / EPB;
+  hipLaunchKernelGGL((k_fused<NP, Nfp<NP>::v, CIN, EPI>), dim3((unsigned)groups), dim3(kFusedWG), 0, s, op->dev, io,
+                     a, b, dt);
+}
+
+template <bool CIN, int EPI>
+void fused_dispatch(const rfhip_op *op, const FusedIO &io, double a, double b, double dt, hipStream_t s) {
+  switch (op->np) {
+    case 4: fused_np<4, CIN, EPI>(op, io, a, b, dt, s); break;
+    case 10: fused_np<10, CIN, EPI>(op, io, a, b, dt, s); break;
+    case 20: fused_np<20, CIN, EPI>(op, io, a, b, dt, s); break;
+    case 35: fused_np<35, CIN, EPI>(op, io, a, b, dt, s); break;
+    default: throw Error{RFHIP_EUNSUPPORTED, "order not compiled (Np must be 4, 10, 20 or 35)"};
+  }
+  RF_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_fused_lserk(const rfhip_op *op, const FusedIO &io, double a, double b, double dt, hipStream_t s) {
+  if (op->form == RFHIP_FORM_FUSED_F64C)
+    fused_dispatch<true, 2>(op, io, a, b, dt, s);
+  else
+    fused_dispatch<false, 1>(op, io, a, b, dt, s);
+}
+
+void launch_fused_apply(const rfhip_op *op, const double *y, double *dy, hipStream_t s) {
+  FusedIO io{};
+  io.yin = y;
+  io.dy = dy;
+  io.src_kind = RFHIP_DRIVE_FREE;
+  fused_dispatch<false, 0>(op, io, 0.0, 0.0, 0.0, s);
+}
+
+}  // namespace rfhip

@@ -1,0 +1,201 @@
+// Internal declarations shared by the rfhip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rfhip.h"
+
+namespace rfhip {
+
+// ---- errors ------------------------------------------------------------
+
+void set_error(const std::string &msg);
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define RF_HIP(call)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw ::rfhip::Error{RFHIP_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define RF_REQUIRE(cond, msg)                                \
+  do {                                                       \
+    if (!(cond)) throw ::rfhip::Error{RFHIP_EINVAL, (msg)}; \
+  } while (0)
+
+// ---- device memory -----------------------------------------------------
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t nbytes) {
+    if (nbytes <= bytes && p) return;
+    release();
+    if (nbytes == 0) nbytes = 8;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error{RFHIP_ENOMEM, "hipMalloc of " + std::to_string(nbytes) + " bytes failed: " + hipGetErrorString(e)};
+    }
+    bytes = nbytes;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+template <class T>
+void upload(DevBuf &b, const T *host, size_t count, hipStream_t s) {
+  b.alloc(sizeof(T) * count);
+  RF_HIP(hipMemcpyAsync(b.p, host, sizeof(T) * count, hipMemcpyHostToDevice, s));
+}
+
+// ---- operator ----------------------------------------------------------
+
+// Per-face code of the fused kernels, one int32 per (element, face):
+//   interior: (neighbour << 5) | (neighbour_local_face << 3) | orientation
+//   PEC wall: -1;  port p: -2 - p.
+// orientation indexes the six vertex correspondences between the two
+// copies of the shared triangle (see the neighbour-node table).
+constexpr int kCodePec = -1;
+
+struct Stage {
+  double a, b;
+};
+
+// Launch parameters and device pointers of one operator, passed by value.
+struct DevOp {
+  int np, nfp;
+  int64_t n_elem;
+  double alpha;
+  // plain form (precision of the form)
+  const void *dr, *ds, *dt, *lift;  // Np*Np, Np*4Nfp (row-major)
+  const int *face_nodes;            // 4*Nfp
+  const void *jinv, *inv_eps, *inv_mu, *sig_eps, *sig_mu, *normal, *fscale;
+  const int *nbr, *nbr_lf, *port, *perm;
+  // fused forms (FP64)
+  const double *dT;        // 3*Np*Np, transposed: dT[(k*Np + j)*Np + i] = D_k[i][j]
+  const double *liftT;     // 4Nfp*Np: liftT[fm*Np + i] = LIFT[i][fm]
+  const double *jinv64;    // 9K
+  const int *fcode;        // 4K
+  const int *mat;          // K, index into the material table
+  const double *mat_tab;   // n_mat*12: inv_eps3, inv_mu3, sig_eps3, sig_mu3
+  const int *nbr_node;     // 4 (nlf) * 6 (orientation) * Nfp: neighbour volume node
+};
+
+struct rfop_state;
+
+}  // namespace rfhip
+
+struct rfhip_ctx {
+  int device = 0;
+  std::string name;
+};
+
+struct rfhip_op {
+  rfhip_ctx *ctx = nullptr;
+  int form = 0;
+  int np = 0, nfp = 0;
+  int64_t n_elem = 0, n_dof = 0;
+  double alpha = 1.0;
+  hipStream_t stream = nullptr;
+  // host copies used by the host-side logic
+  std::vector<int32_t> port_host;  // 4K
+  // device operator data
+  rfhip::DevBuf dr, ds, dt, lift, face_nodes, jinv, inv_eps, inv_mu, sig_eps, sig_mu, normal, fscale, nbr,
+      nbr_lf, port, perm;
+  rfhip::DevBuf dT, liftT, jinv64, fcode, mat, mat_tab, nbr_node;
+  // state registers (precision depends on form)
+  rfhip::DevBuf y[2], ylo[2], p, k, src;
+  int cur = 0;  // index of the current state buffer for fused forms
+  // KCL / Krylov / scratch buffers
+  rfhip::DevBuf kst, kdtf, kerr, kbak, red, red_host_dummy;
+  rfhip::DevBuf kb_basis, kb_w, kb_proj, kb_part, kb_out, kb_src, kb_x;
+  int kb_cap = 0;
+  rfhip::DevOp dev{};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace rfhip {
+
+// Kernel launchers (implemented in the .hip files).
+void launch_plain_apply(const rfhip_op *op, const void *y, void *dy, hipStream_t s);
+void launch_plain_stage(const rfhip_op *op, void *p, const void *k, void *y, double a, double b, double dt,
+                        hipStream_t s);
+void launch_add_point(const rfhip_op *op, void *k, int64_t dof, double val, hipStream_t s);
+void launch_add_vec(const rfhip_op *op, void *k, const void *src, double g, hipStream_t s);
+
+// Fused forms. y_in/y_out: FP64 AoS (F64) or hi/lo pairs (F64C).
+struct FusedIO {
+  const void *yin;
+  const void *yin_lo;  // bf16 lo part (F64C), else null
+  void *yout;
+  void *yout_lo;
+  void *p;             // double (F64) or float (F64C)
+  double *dy;          // apply-only output
+  int src_kind;
+  int64_t src_dof;
+  const double *src_vec;
+  double g;
+};
+void launch_fused_lserk(const rfhip_op *op, const FusedIO &io, double a, double b, double dt, hipStream_t s);
+void launch_fused_apply(const rfhip_op *op, const double *y, double *dy, hipStream_t s);
+
+// Conversions between boundary AoS and the form's storage.
+void launch_f64_to_c(const double *src, float *hi, uint16_t *lo, int64_t n, hipStream_t s);
+void launch_c_to_f64(const float *hi, const uint16_t *lo, double *dst, int64_t n, hipStream_t s);
+void launch_cast_d2f(const double *src, float *dst, int64_t n, hipStream_t s);
+void launch_cast_f2d(const float *src, double *dst, int64_t n, hipStream_t s);
+void launch_fill(void *dst, int64_t nbytes, hipStream_t s);
+
+// KCL vector kernels on T = float or double (is_f64 selects).
+void launch_kcl_stage0(bool f64, void *y, void *dtf, void *err, const void *k, double dt, double b0, double e0,
+                       int64_t n, hipStream_t s);
+void launch_kcl_build(bool f64, const void *y, const void *dtf, void *stage, double amb, int64_t n, hipStream_t s);
+void launch_kcl_accum(bool f64, void *y, void *dtf, void *err, const void *k, double dt, double b, double ew,
+                      int64_t n, hipStream_t s);
+void launch_copy(void *dst, const void *src, int64_t nbytes, hipStream_t s);
+// sum((err/(atol + rtol*max|yo|,|yn|))^2) into *out (device double), with a
+// non-finite y_new entry flagged as +inf.
+void launch_weighted_err(bool f64, const void *yo, const void *yn, const void *err, double atol, double rtol,
+                         int64_t n, double *partials, int nblk, double *out, hipStream_t s);
+int weighted_err_blocks(int64_t n);
+
+// Krylov (FP64).
+int krylov_dot_blocks(int64_t n);
+void launch_multi_dot(const double *basis, int64_t n, int cols, const double *w, double *partials, int nblk,
+                      double *proj, hipStream_t s);
+void launch_axpy_basis_norm(double *w, const double *basis, int64_t n, int cols, const double *proj,
+                            double *partials, int nblk, double *norm2, hipStream_t s);
+void launch_scale(const double *w, double *dst, int64_t n, double inv, hipStream_t s);
+void launch_lincomb(const double *basis, int64_t n, int dim, const double *coef, double *out, hipStream_t s);
+void launch_axpy(double *w, const double *x, double a, int64_t n, hipStream_t s);
+void launch_energy(const double *y, const double *mass, const double *det, const double *inv_eps,
+                   const double *inv_mu, int np, int64_t K, double *partials, int nblk, double *out,
+                   hipStream_t s);
+
+// Host-side dense exponential (propagator.rs:69-114).
+void expm_host(const double *a, int n, double *out);
+
+}  // namespace rfhip

@@ -1,0 +1,342 @@
+// Streaming kernels around the operator: register conversions, the KCL
+// stage updates and error norm (gpu/kcl.cl:12-112), the Krylov CGS2
+// building blocks (gpu/expmv.cl:34-165 restructured: one multi-dot pass per
+// CGS2 sweep instead of one work-group per basis row) and the field energy
+// (rhs.rs:1509-1544). All HBM-bound; grid-stride loops over 256-thread
+// blocks.
+#include "rfhip_impl.hpp"
+
+namespace rfhip {
+namespace {
+
+constexpr int kBS = 256;
+
+inline unsigned grid_for(int64_t n, int64_t cap = 8192) {
+  int64_t g = (n + kBS - 1) / kBS;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+__device__ __forceinline__ uint16_t bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__global__ void __launch_bounds__(kBS) k_f64_to_c(const double *__restrict__ s, float *__restrict__ hi,
+                                                  uint16_t *__restrict__ lo, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    const double v = s[i];
+    const float h = (float)v;
+    hi[i] = h;
+    lo[i] = bf16_rne((float)(v - (double)h));
+  }
+}
+__global__ void __launch_bounds__(kBS) k_c_to_f64(const float *__restrict__ hi, const uint16_t *__restrict__ lo,
+                                                  double *__restrict__ d, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS)
+    d[i] = (double)hi[i] + (double)bf16_to_f(lo[i]);
+}
+__global__ void __launch_bounds__(kBS) k_d2f(const double *__restrict__ s, float *__restrict__ d, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) d[i] = (float)s[i];
+}
+__global__ void __launch_bounds__(kBS) k_f2d(const float *__restrict__ s, double *__restrict__ d, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) d[i] = (double)s[i];
+}
+__global__ void __launch_bounds__(kBS) k_fill0(uint32_t *__restrict__ d, int64_t nwords) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBS) d[i] = 0u;
+}
+__global__ void __launch_bounds__(kBS) k_copy(uint32_t *__restrict__ d, const uint32_t *__restrict__ s, int64_t nw) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < nw; i += (int64_t)gridDim.x * kBS) d[i] = s[i];
+}
+
+// ---- KCL RK4(3)5[2R+]C (explicit_adaptive.rs:84-152) -------------------
+
+template <typename T>
+__global__ void __launch_bounds__(kBS) k_kcl0(T *__restrict__ y, T *__restrict__ dtf, T *__restrict__ err,
+                                              const T *__restrict__ k, T dt, T b0, T e0, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    const T f = dt * k[i];
+    dtf[i] = f;
+    y[i] += b0 * f;
+    err[i] = e0 * f;
+  }
+}
+template <typename T>
+__global__ void __launch_bounds__(kBS) k_kcl_build(const T *__restrict__ y, const T *__restrict__ dtf,
+                                                   T *__restrict__ st, T amb, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS)
+    st[i] = y[i] + amb * dtf[i];
+}
+template <typename T>
+__global__ void __launch_bounds__(kBS) k_kcl_accum(T *__restrict__ y, T *__restrict__ dtf, T *__restrict__ err,
+                                                   const T *__restrict__ k, T dt, T b, T ew, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    const T f = dt * k[i];
+    dtf[i] = f;
+    y[i] += b * f;
+    err[i] += ew * f;
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  // wave64 shuffle reduction, then across the block's waves.
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += sh[q];
+  __syncthreads();
+  return t;  // valid on thread 0
+}
+
+// Per-block sum of (err / (atol + rtol*max(|yo|, |yn|)))^2; a non-finite y_new
+// makes the block report +inf (session.rs:160-171 returns INFINITY).
+template <typename T>
+__global__ void __launch_bounds__(kBS) k_werr(const T *__restrict__ yo, const T *__restrict__ yn,
+                                              const T *__restrict__ er, double atol, double rtol, int64_t n,
+                                              double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    const double b = (double)yn[i];
+    if (!isfinite(b)) {
+      acc = INFINITY;
+      continue;
+    }
+    const double r = (double)er[i] / (atol + rtol * fmax(fabs((double)yo[i]), fabs(b)));
+    acc += r * r;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Sums nblk partials into out[0] with one block (fixed order: deterministic).
+__global__ void __launch_bounds__(kBS) k_finish_sum(const double *__restrict__ part, int nblk, double *out) {
+  __shared__ double sh[kBS / 64];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kBS) acc += part[i];
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[0] = t;
+}
+
+// ---- Krylov CGS2 pieces (propagator.rs:231-271) -------------------------
+
+constexpr int kMaxCols = 64;
+
+// partials[b*cols + i] = sum over block b's slice of basis_i . w. Columns
+// are taken eight at a time so w is read ceil(cols/8) times, the basis once.
+constexpr int kDotTile = 8;
+__global__ void __launch_bounds__(kBS) k_multi_dot(const double *__restrict__ basis, int64_t n, int cols,
+                                                   const double *__restrict__ w, double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk;
+  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+  for (int c0 = 0; c0 < cols; c0 += kDotTile) {
+    double acc[kDotTile];
+#pragma unroll
+    for (int t = 0; t < kDotTile; ++t) acc[t] = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kBS) {
+      const double wi = w[i];
+#pragma unroll
+      for (int t = 0; t < kDotTile; ++t)
+        if (c0 + t < cols) acc[t] += basis[(int64_t)(c0 + t) * n + i] * wi;
+    }
+#pragma unroll
+    for (int t = 0; t < kDotTile; ++t) {
+      if (c0 + t >= cols) break;
+      const double s = block_sum(acc[t], sh);
+      if (threadIdx.x == 0) part[(int64_t)blockIdx.x * cols + c0 + t] = s;
+    }
+  }
+}
+__global__ void __launch_bounds__(kBS) k_dot_finish(const double *__restrict__ part, int nblk, int cols,
+                                                    double *__restrict__ proj) {
+  __shared__ double sh[kBS / 64];
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kBS) acc += part[(int64_t)b * cols + c];
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) proj[c] = t;
+}
+// w -= sum_i proj[i]*basis_i, then per-block partial ||w||^2.
+__global__ void __launch_bounds__(kBS) k_axpy_basis_norm(double *__restrict__ w, const double *__restrict__ basis,
+                                                         int64_t n, int cols, const double *__restrict__ proj,
+                                                         double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  __shared__ double sp[kMaxCols];
+  for (int c = threadIdx.x; c < cols; c += kBS) sp[c] = proj[c];
+  __syncthreads();
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    double wi = w[i];
+    for (int c = 0; c < cols; ++c) wi -= sp[c] * basis[(int64_t)c * n + i];
+    w[i] = wi;
+    acc += wi * wi;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+__global__ void __launch_bounds__(kBS) k_scale(const double *__restrict__ w, double *__restrict__ d, int64_t n,
+                                               double inv) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) d[i] = w[i] * inv;
+}
+__global__ void __launch_bounds__(kBS) k_lincomb(const double *__restrict__ basis, int64_t n, int dim,
+                                                 const double *__restrict__ coef, double *__restrict__ out) {
+  __shared__ double sc[kMaxCols];
+  for (int c = threadIdx.x; c < dim; c += kBS) sc[c] = coef[c];
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    double acc = 0.0;
+    for (int c = 0; c < dim; ++c) acc += sc[c] * basis[(int64_t)c * n + i];
+    out[i] = acc;
+  }
+}
+__global__ void __launch_bounds__(kBS) k_axpy(double *__restrict__ w, const double *__restrict__ x, double a,
+                                              int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) w[i] += a * x[i];
+}
+
+// 0.5 * sum_e |det J_e| sum_c w_c f_c^T M f_c, per-block partials
+// (rhs.rs:1509-1544). One thread per element.
+__global__ void __launch_bounds__(kBS) k_energy(const double *__restrict__ y, const double *__restrict__ mass,
+                                                const double *__restrict__ det, const double *__restrict__ ie,
+                                                const double *__restrict__ im, int np, int64_t K,
+                                                double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  double acc = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * kBS + threadIdx.x; e < K; e += (int64_t)gridDim.x * kBS) {
+    const double *ye = y + e * np * 6;
+    double w[6];
+    for (int c = 0; c < 3; ++c) {
+      w[c] = 1.0 / ie[3 * e + c];
+      w[3 + c] = 1.0 / im[3 * e + c];
+    }
+    double s = 0.0;
+    for (int a = 0; a < np; ++a)
+      for (int b = 0; b < np; ++b) {
+        const double m = mass[a * np + b];
+        for (int c = 0; c < 6; ++c) s += w[c] * m * ye[a * 6 + c] * ye[b * 6 + c];
+      }
+    acc += fabs(det[e]) * s;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+}  // namespace
+
+void launch_f64_to_c(const double *src, float *hi, uint16_t *lo, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_f64_to_c, dim3(grid_for(n)), dim3(kBS), 0, s, src, hi, lo, n);
+  RF_HIP(hipGetLastError());
+}
+void launch_c_to_f64(const float *hi, const uint16_t *lo, double *dst, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_c_to_f64, dim3(grid_for(n)), dim3(kBS), 0, s, hi, lo, dst, n);
+  RF_HIP(hipGetLastError());
+}
+void launch_cast_d2f(const double *src, float *dst, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_d2f, dim3(grid_for(n)), dim3(kBS), 0, s, src, dst, n);
+  RF_HIP(hipGetLastError());
+}
+void launch_cast_f2d(const float *src, double *dst, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_f2d, dim3(grid_for(n)), dim3(kBS), 0, s, src, dst, n);
+  RF_HIP(hipGetLastError());
+}
+void launch_fill(void *dst, int64_t nbytes, hipStream_t s) {
+  const int64_t nw = nbytes / 4;
+  hipLaunchKernelGGL(k_fill0, dim3(grid_for(nw)), dim3(kBS), 0, s, static_cast<uint32_t *>(dst), nw);
+  RF_HIP(hipGetLastError());
+}
+void launch_copy(void *dst, const void *src, int64_t nbytes, hipStream_t s) {
+  const int64_t nw = nbytes / 4;
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(nw)), dim3(kBS), 0, s, static_cast<uint32_t *>(dst),
+                     static_cast<const uint32_t *>(src), nw);
+  RF_HIP(hipGetLastError());
+}
+
+void launch_kcl_stage0(bool f64, void *y, void *dtf, void *err, const void *k, double dt, double b0, double e0,
+                       int64_t n, hipStream_t s) {
+  if (f64)
+    hipLaunchKernelGGL(k_kcl0<double>, dim3(grid_for(n)), dim3(kBS), 0, s, (double *)y, (double *)dtf,
+                       (double *)err, (const double *)k, dt, b0, e0, n);
+  else
+    hipLaunchKernelGGL(k_kcl0<float>, dim3(grid_for(n)), dim3(kBS), 0, s, (float *)y, (float *)dtf, (float *)err,
+                       (const float *)k, (float)dt, (float)b0, (float)e0, n);
+  RF_HIP(hipGetLastError());
+}
+void launch_kcl_build(bool f64, const void *y, const void *dtf, void *stage, double amb, int64_t n, hipStream_t s) {
+  if (f64)
+    hipLaunchKernelGGL(k_kcl_build<double>, dim3(grid_for(n)), dim3(kBS), 0, s, (const double *)y,
+                       (const double *)dtf, (double *)stage, amb, n);
+  else
+    hipLaunchKernelGGL(k_kcl_build<float>, dim3(grid_for(n)), dim3(kBS), 0, s, (const float *)y, (const float *)dtf,
+                       (float *)stage, (float)amb, n);
+  RF_HIP(hipGetLastError());
+}
+void launch_kcl_accum(bool f64, void *y, void *dtf, void *err, const void *k, double dt, double b, double ew,
+                      int64_t n, hipStream_t s) {
+  if (f64)
+    hipLaunchKernelGGL(k_kcl_accum<double>, dim3(grid_for(n)), dim3(kBS), 0, s, (double *)y, (double *)dtf,
+                       (double *)err, (const double *)k, dt, b, ew, n);
+  else
+    hipLaunchKernelGGL(k_kcl_accum<float>, dim3(grid_for(n)), dim3(kBS), 0, s, (float *)y, (float *)dtf,
+                       (float *)err, (const float *)k, (float)dt, (float)b, (float)ew, n);
+  RF_HIP(hipGetLastError());
+}
+
+int weighted_err_blocks(int64_t n) { return (int)grid_for(n, 2048); }
+
+void launch_weighted_err(bool f64, const void *yo, const void *yn, const void *err, double atol, double rtol,
+                         int64_t n, double *partials, int nblk, double *out, hipStream_t s) {
+  if (f64)
+    hipLaunchKernelGGL(k_werr<double>, dim3(nblk), dim3(kBS), 0, s, (const double *)yo, (const double *)yn,
+                       (const double *)err, atol, rtol, n, partials);
+  else
+    hipLaunchKernelGGL(k_werr<float>, dim3(nblk), dim3(kBS), 0, s, (const float *)yo, (const float *)yn,
+                       (const float *)err, atol, rtol, n, partials);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, partials, nblk, out);
+  RF_HIP(hipGetLastError());
+}
+
+int krylov_dot_blocks(int64_t n) { return (int)grid_for(n, 1024); }
+
+void launch_multi_dot(const double *basis, int64_t n, int cols, const double *w, double *partials, int nblk,
+                      double *proj, hipStream_t s) {
+  RF_REQUIRE(cols <= kMaxCols, "Krylov dimension above 64");
+  hipLaunchKernelGGL(k_multi_dot, dim3(nblk), dim3(kBS), 0, s, basis, n, cols, w, partials);
+  hipLaunchKernelGGL(k_dot_finish, dim3(cols), dim3(kBS), 0, s, partials, nblk, cols, proj);
+  RF_HIP(hipGetLastError());
+}
+void launch_axpy_basis_norm(double *w, const double *basis, int64_t n, int cols, const double *proj,
+                            double *partials, int nblk, double *norm2, hipStream_t s) {
+  RF_REQUIRE(cols <= kMaxCols, "Krylov dimension above 64");
+  hipLaunchKernelGGL(k_axpy_basis_norm, dim3(nblk), dim3(kBS), 0, s, w, basis, n, cols, proj, partials);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, partials, nblk, norm2);
+  RF_HIP(hipGetLastError());
+}
+void launch_scale(const double *w, double *dst, int64_t n, double inv, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kBS), 0, s, w, dst, n, inv);
+  RF_HIP(hipGetLastError());
+}
+void launch_lincomb(const double *basis, int64_t n, int dim, const double *coef, double *out, hipStream_t s) {
+  RF_REQUIRE(dim <= kMaxCols, "Krylov dimension above 64");
+  hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n)), dim3(kBS), 0, s, basis, n, dim, coef, out);
+  RF_HIP(hipGetLastError());
+}
+void launch_axpy(double *w, const double *x, double a, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(kBS), 0, s, w, x, a, n);
+  RF_HIP(hipGetLastError());
+}
+void launch_energy(const double *y, const double *mass, const double *det, const double *ie, const double *im, int np,
+                   int64_t K, double *partials, int nblk, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy, dim3(nblk), dim3(kBS), 0, s, y, mass, det, ie, im, np, K, partials);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, partials, nblk, out);
+  RF_HIP(hipGetLastError());
+}
+
+}  // namespace rfhip

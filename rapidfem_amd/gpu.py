@@ -1,0 +1,221 @@
+"""GpuContext / GpuOperator over the rfhip C ABI.
+
+Mirrors rapidfem_td::gpu::{GpuContext, GpuOperator}
+(crates/rapidfem-td/src/gpu/mod.rs:33-135, gpu/operator.rs:205-1486): same
+method names and argument meaning; errors raise RfhipError where the
+reference returns Err(String), and length mismatches (asserts in the
+reference, gpu/operator.rs:518,552) raise as well.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .maxwell import MaxwellOperator
+
+
+@dataclass
+class Controller:
+    """session.rs:130-155 Controller with its defaults (constants.rs:268-299)."""
+
+    atol: float = 1e-8
+    rtol: float = 1e-4
+    safety: float = 0.9
+    growth_limit: float = 5.0
+    shrink_limit: float = 0.2
+    pi_alpha: float = 0.7 / 4.0
+    pi_beta: float = 0.4 / 4.0
+    min_step_factor: float = 1e-10
+
+    def c(self) -> L.ControllerC:
+        return L.ControllerC(self.atol, self.rtol, self.safety, self.growth_limit, self.shrink_limit,
+                             self.pi_alpha, self.pi_beta, self.min_step_factor)
+
+
+@dataclass
+class KclStats:
+    """session.rs:218-223 KclStats (h in operator time here)."""
+
+    accepted: int
+    rejected: int
+    h_min: float
+    h_max: float
+
+
+class Drive:
+    """session.rs:107-114: Free, Point(dof) or Vector(pattern)."""
+
+    def __init__(self, kind: int, dof: int = 0, vec: np.ndarray | None = None):
+        self.kind, self.dof, self.vec = kind, dof, vec
+
+    @classmethod
+    def free(cls):
+        return cls(L.DRIVE_FREE)
+
+    @classmethod
+    def point(cls, dof: int):
+        return cls(L.DRIVE_POINT, int(dof))
+
+    @classmethod
+    def vector(cls, b):
+        return cls(L.DRIVE_VECTOR, 0, np.ascontiguousarray(b, np.float64))
+
+
+class GpuContext:
+    """gpu/mod.rs:33-65: the first HIP device."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        L.check(L.lib.rfhip_ctx_create(C.byref(h)))
+        self._h = h
+        buf = C.create_string_buffer(256)
+        L.check(L.lib.rfhip_ctx_device_name(self._h, buf, 256))
+        self.device_name = buf.value.decode()
+        self.fp64 = bool(L.lib.rfhip_ctx_has_fp64(self._h))
+
+    @classmethod
+    def new(cls) -> "GpuContext":
+        return cls()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            L.lib.rfhip_ctx_destroy(self._h)
+            self._h = None
+
+
+class GpuOperator:
+    """gpu/operator.rs:78-153 GpuOperator; form selects the kernel family
+    ('plain_f32' = the reference's structure and precision, 'plain_f64',
+    'fused_f64', 'fused_f64c')."""
+
+    def __init__(self, ctx: GpuContext, op: MaxwellOperator | None, form: str = "fused_f64", *, desc=None,
+                 mass=None, det=None):
+        self.ctx = ctx  # keep the context alive
+        self.form = form
+        if op is not None:
+            d, keep = op.desc()
+            mass, det = op.re.mass, op.det
+        else:
+            d, keep = desc, None
+        h = C.c_void_p()
+        L.check(L.lib.rfhip_op_create(ctx._h, C.byref(d), L.FORMS[form], C.byref(h)))
+        del keep
+        self._h = h
+        self._n = int(L.lib.rfhip_op_n_dof(self._h))
+        self.n_elem = int(d.n_elem)
+        self._mass = None if mass is None else np.ascontiguousarray(mass, np.float64)
+        self._det = None if det is None else np.ascontiguousarray(det, np.float64)
+
+    @classmethod
+    def from_desc(cls, ctx: GpuContext, desc: L.OpDesc, form: str = "fused_f64", mass=None, det=None):
+        """From raw rfhip_op_desc data (e.g. another host's MaxwellOperator)."""
+        return cls(ctx, None, form, desc=desc, mass=mass, det=det)
+
+    @classmethod
+    def new(cls, ctx: GpuContext, op: MaxwellOperator, form: str = "fused_f64") -> "GpuOperator":
+        return cls(ctx, op, form)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            L.lib.rfhip_op_destroy(self._h)
+            self._h = None
+
+    def n_dof(self) -> int:
+        return self._n
+
+    def _vec(self, a, dt):
+        a = np.ascontiguousarray(a, dt)
+        if a.shape != (self._n,):
+            raise ValueError(f"state length mismatch: {a.shape} vs ({self._n},)")
+        return a
+
+    # gpu/operator.rs:513-523
+    def apply(self, ctx: GpuContext, y) -> np.ndarray:
+        y = self._vec(y, np.float32)
+        dy = np.empty_like(y)
+        L.check(L.lib.rfhip_op_apply(self._h, y, dy))
+        return dy
+
+    def apply_f64(self, y) -> np.ndarray:
+        y = self._vec(y, np.float64)
+        dy = np.empty_like(y)
+        L.check(L.lib.rfhip_op_apply_f64(self._h, y, dy))
+        return dy
+
+    # gpu/operator.rs:906-953
+    def transient(self, ctx, y0, dt, steps, substeps=1, drive: Drive | None = None, g=None, record=True,
+                  precision="f32") -> np.ndarray:
+        drive = drive or Drive.free()
+        f64 = precision == "f64"
+        dtp = np.float64 if f64 else np.float32
+        y0 = self._vec(y0, dtp)
+        substeps = max(1, int(substeps))
+        gg = None if g is None else np.ascontiguousarray(g, dtp)
+        n_g = 0 if gg is None else gg.shape[0]
+        out = np.empty(((steps + 1) if record else 1) * self._n, dtp)
+        fn = L.lib.rfhip_op_transient_f64 if f64 else L.lib.rfhip_op_transient
+        L.check(fn(self._h, y0, dt, int(steps), substeps, drive.kind, drive.dof, L.ptr(drive.vec), L.ptr(gg),
+                   n_g, 1 if record else 0, out))
+        return out.reshape(steps + 1, self._n) if record else out
+
+    # gpu/operator.rs:823-893
+    def transient_kcl(self, ctx, y0, dt, steps, drive: Drive | None = None, g=None, ctl: Controller | None = None,
+                      precision="f32"):
+        drive = drive or Drive.free()
+        ctl = ctl or Controller()
+        f64 = precision == "f64"
+        dtp = np.float64 if f64 else np.float32
+        y0 = self._vec(y0, dtp)
+        gg = None if g is None else np.ascontiguousarray(g, dtp)
+        n_g = 0 if gg is None else gg.shape[0]
+        out = np.empty((steps + 1) * self._n, dtp)
+        st = L.KclStatsC()
+        cc = ctl.c()
+        fn = L.lib.rfhip_op_transient_kcl_f64 if f64 else L.lib.rfhip_op_transient_kcl
+        L.check(fn(self._h, y0, dt, int(steps), drive.kind, drive.dof, L.ptr(drive.vec), L.ptr(gg), n_g,
+                   C.byref(cc), out, C.byref(st)))
+        return out.reshape(steps + 1, self._n), KclStats(st.accepted, st.rejected, st.h_min, st.h_max)
+
+    # gpu/operator.rs:1175-1197 (CPU algorithm propagator.rs:186-326)
+    def expmv(self, ctx, v, t, m, tol=0.0):
+        v = self._vec(v, np.float64)
+        out = np.empty_like(v)
+        dim = C.c_int32()
+        L.check(L.lib.rfhip_op_expmv(self._h, v, float(t), int(m), float(tol), out, C.byref(dim)))
+        self.last_dim = dim.value
+        return out
+
+    # gpu/operator.rs:1267-1298 (CPU algorithm propagator.rs:338-373)
+    def etd_step(self, ctx, y, b, h, m, tol=0.0):
+        y = self._vec(y, np.float64)
+        b = self._vec(b, np.float64)
+        out = np.empty_like(y)
+        dim = C.c_int32()
+        L.check(L.lib.rfhip_op_etd_step(self._h, y, b, float(h), int(m), float(tol), out, C.byref(dim)))
+        self.last_dim = dim.value
+        return out
+
+    # ---- device-resident state (no reference counterpart) ----
+    def state_load(self, y0):
+        L.check(L.lib.rfhip_op_state_load(self._h, self._vec(y0, np.float64)))
+
+    def state_store(self) -> np.ndarray:
+        out = np.empty(self._n)
+        L.check(L.lib.rfhip_op_state_store(self._h, out))
+        return out
+
+    def state_lserk(self, dt, nsteps):
+        L.check(L.lib.rfhip_op_state_lserk(self._h, float(dt), int(nsteps)))
+
+    def state_lserk_timed(self, dt, nsteps):
+        tot, stage = C.c_double(), C.c_double()
+        L.check(L.lib.rfhip_op_state_lserk_timed(self._h, float(dt), int(nsteps), C.byref(tot), C.byref(stage)))
+        return tot.value, stage.value
+
+    def state_energy(self) -> float:
+        e = C.c_double()
+        L.check(L.lib.rfhip_op_state_energy(self._h, self._mass, self._det, C.byref(e)))
+        return e.value

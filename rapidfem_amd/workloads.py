@@ -1,0 +1,109 @@
+"""Synthetic workloads: the meshes, materials and initial fields the
+benchmark and the parity tests run on.
+
+  cavity_te101  6x6x6 jittered Kuhn box (1296 tets, +-10% h), N=2, TE101
+                E_y = sin(pi x) sin(pi z), vacuum, PEC, upwind.
+  pulse_box     2x1x1 box, nx x ny x nz cells x 6 tets (60x30x30 = 324,000
+                tets at full size), N=3, eps_r=4 on tets whose centroid has
+                x in [0.8, 1.2], E_x = exp(-|r-c|^2/(2 s^2)) sin(2 pi 5 x),
+                s=0.05, c=(0.4,0.5,0.5), PEC, upwind.
+  modes_box     n^3 jittered unit box (55^3*6 = 998,250 tets at full size),
+                N=4, sum of 16 PEC cavity modes with N(0,1) amplitudes.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .maxwell import MaxwellOperator, structured_box, structured_box_jittered
+
+
+def te101_field(op: MaxwellOperator) -> np.ndarray:
+    x = op.node_coords()
+    y = np.zeros((op.n_elem, op.re.n_nodes, 6))
+    y[..., 1] = np.sin(math.pi * x[..., 0]) * np.sin(math.pi * x[..., 2])
+    return y.reshape(-1)
+
+
+def cavity_te101(n=6, order=2, amplitude=0.1, seed=1, flux_alpha=1.0) -> tuple[MaxwellOperator, np.ndarray]:
+    mesh = structured_box_jittered(n, n, n, 1.0, 1.0, 1.0, amplitude, seed)
+    op = MaxwellOperator(mesh, order, flux_alpha)
+    return op, te101_field(op)
+
+
+def pulse_box(nx=60, ny=30, nz=30, order=3, eps_r=4.0, flux_alpha=1.0) -> tuple[MaxwellOperator, np.ndarray]:
+    mesh = structured_box(nx, ny, nz, 2.0, 1.0, 1.0)
+    cx = mesh.nodes[mesh.tets].mean(axis=1)[:, 0]
+    eps = np.ones((mesh.n_tets, 3))
+    eps[(cx >= 0.8) & (cx <= 1.2)] = eps_r
+    op = MaxwellOperator(mesh, order, flux_alpha, materials={"eps": eps})
+    x = op.node_coords()
+    r2 = (x[..., 0] - 0.4) ** 2 + (x[..., 1] - 0.5) ** 2 + (x[..., 2] - 0.5) ** 2
+    y = np.zeros((op.n_elem, op.re.n_nodes, 6))
+    y[..., 0] = np.exp(-r2 / (2 * 0.05 ** 2)) * np.sin(2 * math.pi * 5 * x[..., 0])
+    return op, y.reshape(-1)
+
+
+def cavity_modes_field(op: MaxwellOperator, n_modes=16, seed=0, lengths=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """Sum of PEC box cavity modes (standing E, H = 0) with N(0,1) amplitudes.
+
+    Mode (m, n, p): E = (A cos(kx x) sin(ky y) sin(kz z), B sin cos sin,
+    C sin sin cos) with k = pi (m/lx, n/ly, p/lz) and (A, B, C) . k = 0, so
+    E is divergence-free and its tangential part vanishes on every wall.
+    """
+    rng = np.random.default_rng(seed)
+    lx, ly, lz = lengths
+    x = op.node_coords()
+    y = np.zeros((op.n_elem, op.re.n_nodes, 6))
+    cands = [(m, n, p) for m in range(0, 4) for n in range(0, 4) for p in range(0, 4)
+             if (m > 0) + (n > 0) + (p > 0) >= 2]
+    picks = rng.choice(len(cands), size=n_modes, replace=False)
+    for idx in picks:
+        m, n, p = cands[idx]
+        k = np.array([math.pi * m / lx, math.pi * n / ly, math.pi * p / lz])
+        pol = rng.normal(size=3)
+        pol -= k * (pol @ k) / (k @ k)
+        pol /= np.linalg.norm(pol)
+        a = rng.normal()
+        cx, sx = np.cos(k[0] * x[..., 0]), np.sin(k[0] * x[..., 0])
+        cy, sy = np.cos(k[1] * x[..., 1]), np.sin(k[1] * x[..., 1])
+        cz, sz = np.cos(k[2] * x[..., 2]), np.sin(k[2] * x[..., 2])
+        y[..., 0] += a * pol[0] * cx * sy * sz
+        y[..., 1] += a * pol[1] * sx * cy * sz
+        y[..., 2] += a * pol[2] * sx * sy * cz
+    return y.reshape(-1)
+
+
+def modes_box(n=55, order=4, amplitude=0.1, seed=3, flux_alpha=1.0) -> tuple[MaxwellOperator, np.ndarray]:
+    mesh = structured_box_jittered(n, n, n, 1.0, 1.0, 1.0, amplitude, seed)
+    op = MaxwellOperator(mesh, order, flux_alpha)
+    return op, cavity_modes_field(op, 16, seed)
+
+
+# Algorithmic work per DOF-stage of the fused stage kernel: flops (volume
+# 36 Np^2, metric+curl 114 Np, flux 78*4Nfp, LIFT 48 Np Nfp, materials
+# 18 Np, per element, plus 5 for the update, per DOF) and HBM bytes
+# (read y, write y, read p, write p + per-element geometry).
+def flops_per_dof_stage(order: int) -> float:
+    np_ = (order + 1) * (order + 2) * (order + 3) // 6
+    nfp = (order + 1) * (order + 2) // 2
+    per_elem = 36 * np_ ** 2 + 114 * np_ + 78 * 4 * nfp + 48 * np_ * nfp + 18 * np_
+    return per_elem / (6 * np_) + 5.0
+
+
+def bytes_per_dof_stage(order: int, form: str) -> float:
+    np_ = (order + 1) * (order + 2) * (order + 3) // 6
+    if form == "fused_f64c":
+        state = 6 + 6 + 4 + 4  # y (f32+bf16) read+write, p (f32) read+write
+        geom = 9 * 8 + 4 * 4 + 4  # J^-1 f64, 4 face codes, material index
+    elif form == "fused_f64":
+        state = 4 * 8
+        geom = 9 * 8 + 4 * 4 + 4
+    elif form == "plain_f64":
+        state = 7 * 8  # apply: read y, write k; stage: read p, k, y, write p, y
+        geom = (9 + 12 + 12 + 4) * 8 + (4 + 4 + 4) * 4 + 4 * ((order + 1) * (order + 2) // 2) * 4
+    else:  # plain_f32
+        state = 7 * 4
+        geom = (9 + 12 + 12 + 4) * 4 + (4 + 4 + 4) * 4 + 4 * ((order + 1) * (order + 2) // 2) * 4
+    return state + geom / (6 * np_)
