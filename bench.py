@@ -1,0 +1,177 @@
+"""Benchmark: DGTD DOF-stage updates/s of the LSERK4 stepper on one MI355X.
+
+Workload: the 2x1x1 box of 60x30x30 cells x 6 tets (324,000 tets), order
+N=3 (Np=20, 38.88M DOFs), eps_r=4 block at x in [0.8, 1.2], Gaussian-pulse
+E_x, PEC walls, upwind flux, LSERK4 at the CFL step. A step is one LSERK4
+step (5 stages) over the whole mesh with the state resident in HBM.
+
+Reported: the chosen form (fused_f64c: FP64 arithmetic, f32+bf16 state and
+f32 residual registers) and the plain FP64 form (the reference kernel
+structure) on the same state; the chosen form's relative L2 deviation from
+the FP64 fused form after the timed steps; the dominant kernel's roofline
+from HIP-event timing; and the CPU oracle (C restatement of the reference
+CPU path) timed on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "DGTD DOF-stage updates/s + %HBM/MFMA peak, chosen vs plain at 1e-6 rel L2"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_stage_pulse_box_traffic.json")
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=40)
+    ap.add_argument("--cells", type=int, nargs=3, default=[60, 30, 30])
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--form", default="fused_f64c")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    return ap.parse_args()
+
+
+def cpu_baseline(op, y0, dt, steps):
+    """Oracle LSERK4 on the full mesh for `steps` steps, all host threads."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    oop = O.Operator(op.mesh.nodes, op.mesh.tets, op.order, op.flux_alpha,
+                     eps=1.0 / op.inv_eps)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oop.lserk(y0, dt, steps)
+    t = time.perf_counter() - t0
+    rate = oop.n_dof * 5 * steps / t
+    return {"value": rate, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C restatement of the reference CPU path, OpenMP) LSERK4, {steps} steps on the "
+                      f"full {op.n_elem}-tet mesh, {t:.1f} s (operator build {t_build:.1f} s excluded)"}
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(the multi-GPU sharding is a design in DESIGN.md, not built)"}))
+        return 2
+    import numpy as np
+
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+
+    nx, ny, nz = args.cells
+    t0 = time.perf_counter()
+    op, y0 = W.pulse_box(nx, ny, nz, args.order)
+    log(f"operator build {time.perf_counter() - t0:.1f}s: K={op.n_elem} N={args.order} n_dof={op.n_dof}")
+    ctx = R.GpuContext()
+    log(f"device {ctx.device_name}")
+    sess = R.TdSession(op, 1.0, "fused_f64", ctx)
+    t0 = time.perf_counter()
+    dt = sess.cfl_dt()
+    log(f"cfl_dt {dt:.6e} ({time.perf_counter() - t0:.1f}s, rho={sess.rho:.4e})")
+    ref = sess.gop
+    n = op.n_dof
+    K, Wm = args.steps, args.warmup
+
+    # Reference FP64 trajectory for the error check: warmup + K steps.
+    ref.state_load(y0)
+    ref.state_lserk(dt, Wm + K)
+    y_ref = ref.state_store()
+
+    chosen = R.GpuOperator(ctx, op, args.form)
+    chosen.state_load(y0)
+    chosen.state_lserk(dt, Wm)
+    t0 = time.perf_counter()
+    total_ms, stage_ms = chosen.state_lserk_timed(dt, K)
+    wall = time.perf_counter() - t0
+    y_ch = chosen.state_store()
+    err = float(np.linalg.norm(y_ch - y_ref) / np.linalg.norm(y_ref))
+    ms_per_step = total_ms / K
+    value = n * 5 * K / (total_ms * 1e-3)
+    log(f"{args.form}: {value:.4e} DOF-stage/s, {ms_per_step:.3f} ms/step (wall {wall:.2f}s), "
+        f"stage kernel {stage_ms:.4f} ms, rel L2 vs fused_f64 after {Wm + K} steps {err:.3e}")
+
+    # Plain FP64 form (reference structure), same state and dt.
+    plain = R.GpuOperator(ctx, op, "plain_f64")
+    plain.state_load(y0)
+    plain.state_lserk(dt, 2)
+    ptot, papply = plain.state_lserk_timed(dt, args.plain_steps)
+    pvalue = n * 5 * args.plain_steps / (ptot * 1e-3)
+    log(f"plain_f64: {pvalue:.4e} DOF-stage/s, {ptot / args.plain_steps:.3f} ms/step, apply kernel {papply:.4f} ms")
+    del plain
+
+    # Fused FP64 form for reference.
+    ref.state_load(y0)
+    ftot, fstage = ref.state_lserk_timed(dt, max(10, K // 4))
+    fvalue = n * 5 * max(10, K // 4) / (ftot * 1e-3)
+    log(f"fused_f64: {fvalue:.4e} DOF-stage/s, stage kernel {fstage:.4f} ms")
+
+    # Roofline of the dominant kernel (the fused stage kernel).
+    bpd = W.bytes_per_dof_stage(args.order, args.form)
+    fpd = W.flops_per_dof_stage(args.order)
+    gbs = bpd * n / (stage_ms * 1e-3) / 1e9
+    tfl = fpd * n / (stage_ms * 1e-3) / 1e12
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        try:
+            tr = json.load(open(TRAFFIC_FILE))
+            if tr.get("form") == args.form and tr.get("n_dof") == n:
+                traffic = tr.get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes_per_launch": bpd * n,
+                "fp64": {"achieved_tflops": round(tfl, 2), "peak_tflops": FP64_PEAK_TFLOPS,
+                         "frac": round(tfl / FP64_PEAK_TFLOPS, 4), "flops_per_dof_stage": round(fpd, 1)}}
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(op, y0, dt, args.cpu_steps)
+            log(f"cpu baseline {cpu['value']:.4e} DOF-stage/s on {cpu['cores']} threads")
+        except Exception as e:  # the oracle is optional on a box without gcc
+            log(f"cpu baseline failed: {e}")
+    out = {
+        "metric": METRIC, "value": value, "unit": "DOF-stage/s", "n_gpus": 1, "steps": K, "warmup": Wm,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"pulse_box {nx}x{ny}x{nz} cells x 6 tets (2x1x1 box), N={args.order}, eps_r=4 "
+                               f"block, Gaussian E_x, PEC, upwind, LSERK4 at cfl_dt",
+                   "n_elem": op.n_elem, "n_dof": n, "order": args.order, "dt": dt, "form": args.form,
+                   "registers": "y f32+bf16, p f32" if args.form == "fused_f64c" else "f64"},
+        "plain": {"form": "plain_f64", "value": pvalue, "ms_per_step": ptot / args.plain_steps},
+        "fused_f64": {"value": fvalue},
+        "speedup_vs_plain": value / pvalue,
+        "rel_l2_vs_f64": err, "rel_l2_steps": Wm + K,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

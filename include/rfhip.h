@@ -173,6 +173,17 @@ int rfhip_op_etd_step(rfhip_op *op, const double *y, const double *b,
                       double h, int32_t max_dim, double tol, double *out,
                       int32_t *dim_used);
 
+/* ---- kernel variants (no reference counterpart) --------------------------
+ * The fused forms have several kernel variants per order (work per thread,
+ * block shape). rfhip_op_autotune times each on the operator's own mesh with
+ * scratch registers (the state is not modified) and keeps the fastest;
+ * chosen/ms_best may be NULL. rfhip_op_set_variant pins one. Plain forms have
+ * a single variant (0). */
+int rfhip_op_n_variants(const rfhip_op *op);
+int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best);
+int rfhip_op_set_variant(rfhip_op *op, int variant);
+int rfhip_op_variant(const rfhip_op *op);
+
 /* ---- device-resident runs (no reference counterpart) -------------------
  * For callers that keep the state in HBM between calls (benchmarks, long
  * runs without per-frame recording): load once, step, read back. */

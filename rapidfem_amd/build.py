@@ -66,4 +66,6 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
     print(build(verbose="-v" in sys.argv))

@@ -8,7 +8,7 @@
 #include <functional>
 #include <map>
 #include <memory>
-#include <unordered_map>
+#include <type_traits>
 
 #include "rfhip_impl.hpp"
 
@@ -37,6 +37,7 @@ constexpr double kKclErrFloor = 1e-12;   // constants.rs KCL_ERR_FLOOR
 constexpr double kKclNonfinite = 10.0;   // constants.rs KCL_NONFINITE_ERR
 constexpr double kArnoldiBreakdown = 1e-12;  // constants.rs:62
 constexpr int kMaxKrylov = 64;
+constexpr int kFusedVariants = 3;  // rfhip_fused.hip launch_stage
 
 template <class F>
 int guarded(F &&f) {
@@ -204,8 +205,10 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
           nbr_node[at] = m2 < 0 ? 0 : d.face_nodes[g * nfp + m2];
         }
   // Per-face codes, and a check that normal*fscale is the J^-1 gradient.
-  std::vector<int32_t> fcode(4 * K);
+  std::vector<int32_t> fcode(4 * K), mat(K);
   double worst = 0.0;
+  std::vector<double> tab;
+  std::map<std::array<double, 12>, int> seen;
   for (int64_t e = 0; e < K; ++e) {
     const double *ji = d.jinv + 9 * e;
     for (int f = 0; f < 4; ++f) {
@@ -219,12 +222,8 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
       for (int x = 0; x < 3; ++x)
         worst = std::max(worst, std::fabs(g[x] - d.face_fscale[q] * d.face_normal[3 * q + x]) / gn);
       const int nb = d.face_neighbor[q];
-      if (d.face_port[q] >= 0 && nb < 0) {
-        fcode[q] = -2 - d.face_port[q];
-        continue;
-      }
       if (nb < 0) {
-        fcode[q] = kCodePec;
+        fcode[q] = d.face_port[q] >= 0 ? -2 - d.face_port[q] : kCodePec;
         continue;
       }
       RF_REQUIRE(nb < K && d.face_nbr_local[q] >= 0 && d.face_nbr_local[q] < 4, "bad neighbour data");
@@ -237,17 +236,10 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
           ok = cand[((size_t)(f * 4 + g2) * 6 + o) * nfp + m] == d.face_perm[q * nfp + m];
         if (ok) found = o;
       }
-      if (found < 0) throw Error{RFHIP_EUNSUPPORTED, "face permutation is not a triangle symmetry (non-conforming face)"};
+      if (found < 0)
+        throw Error{RFHIP_EUNSUPPORTED, "face permutation is not a triangle symmetry (non-conforming face)"};
       fcode[q] = (int32_t)((nb << 5) | (g2 << 3) | found);
     }
-  }
-  if (worst > 1e-9)
-    throw Error{RFHIP_EUNSUPPORTED, "face normal*fscale is not the J^-1 face gradient (rel " + std::to_string(worst) + ")"};
-  // Material table.
-  std::vector<int32_t> mat(K);
-  std::vector<double> tab;
-  std::map<std::array<double, 12>, int> seen;
-  for (int64_t e = 0; e < K; ++e) {
     std::array<double, 12> row;
     for (int c = 0; c < 3; ++c) {
       row[c] = d.inv_eps[3 * e + c];
@@ -262,16 +254,18 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
     }
     mat[e] = it->second;
   }
+  if (worst > 1e-9)
+    throw Error{RFHIP_EUNSUPPORTED, "face normal*fscale is not the J^-1 face gradient (rel " + std::to_string(worst) + ")"};
   // Transposed reference matrices.
-  std::vector<double> dT(3 * np * np), liftT(nf4 * np);
+  std::vector<double> dT(3 * np * np), liftc((size_t)nf4 * np);
   const double *D[3] = {d.diff_r, d.diff_s, d.diff_t};
   for (int k = 0; k < 3; ++k)
     for (int j = 0; j < np; ++j)
       for (int i = 0; i < np; ++i) dT[(k * np + j) * np + i] = D[k][i * np + j];
   for (int fm = 0; fm < nf4; ++fm)
-    for (int i = 0; i < np; ++i) liftT[fm * np + i] = d.lift[i * nf4 + fm];
+    for (int i = 0; i < np; ++i) liftc[(size_t)fm * np + i] = d.lift[(size_t)i * nf4 + fm];
   upload(op->dT, dT.data(), dT.size(), s);
-  upload(op->liftT, liftT.data(), liftT.size(), s);
+  upload(op->liftc, liftc.data(), liftc.size(), s);
   upload(op->jinv64, d.jinv, 9 * (size_t)K, s);
   upload(op->fcode, fcode.data(), fcode.size(), s);
   upload(op->mat, mat.data(), mat.size(), s);
@@ -279,7 +273,7 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   upload(op->nbr_node, nbr_node.data(), nbr_node.size(), s);
   upload(op->face_nodes, d.face_nodes, 4 * (size_t)nfp, s);
   sync(op);
-  const size_t n = op->n_dof;
+  const size_t n = op->n_vec;
   if (op->form == RFHIP_FORM_FUSED_F64) {
     op->y[0].alloc(8 * n);
     op->y[1].alloc(8 * n);
@@ -292,27 +286,80 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
     op->p.alloc(4 * n);
   }
   op->k.alloc(8 * n);
-  DevOp &dv = op->dev;
-  dv.dT = op->dT.as<double>();
-  dv.liftT = op->liftT.as<double>();
-  dv.jinv64 = op->jinv64.as<double>();
-  dv.fcode = op->fcode.as<int>();
-  dv.mat = op->mat.as<int>();
-  dv.mat_tab = op->mat_tab.as<double>();
-  dv.nbr_node = op->nbr_node.as<int>();
-  dv.face_nodes = op->face_nodes.as<int>();
+  StageArgs &a = op->sa;
+  a = StageArgs{};
+  a.n_elem = K;
+  a.alpha = d.flux_alpha;
+  a.dT = op->dT.as<double>();
+  a.lift = op->liftc.as<double>();
+  a.jinv = op->jinv64.as<double>();
+  a.fcode = op->fcode.as<int>();
+  a.mat = op->mat.as<int>();
+  a.mat_tab = op->mat_tab.as<double>();
+  a.face_nodes = op->face_nodes.as<int>();
+  a.nbr_node = op->nbr_node.as<int>();
+}
+
+// ---- vectors at the boundary ---------------------------------------------
+//
+// "Layout" vectors are f64 AoS vectors on the device. "Register" vectors
+// are what the KCL registers and the apply kernels work on: f64 AoS, except
+// f32 AoS for PLAIN_F32.
+
+int64_t nvec(const rfhip_op *op) { return op->n_vec; }
+
+void h2d_layout_f64(rfhip_op *op, const double *host, double *dev) {
+  RF_HIP(hipMemcpyAsync(dev, host, 8 * op->n_dof, hipMemcpyHostToDevice, op->stream));
+}
+void d2h_layout_f64(rfhip_op *op, const double *dev, double *host) {
+  RF_HIP(hipMemcpyAsync(host, dev, 8 * op->n_dof, hipMemcpyDeviceToHost, op->stream));
+  sync(op);
+}
+void h2d_reg_f64(rfhip_op *op, const double *host, void *dev) {
+  if (op->form != RFHIP_FORM_PLAIN_F32) {
+    h2d_layout_f64(op, host, static_cast<double *>(dev));
+    return;
+  }
+  op->aos.alloc(8 * op->n_dof);
+  RF_HIP(hipMemcpyAsync(op->aos.p, host, 8 * op->n_dof, hipMemcpyHostToDevice, op->stream));
+  launch_cast_d2f(op->aos.as<double>(), static_cast<float *>(dev), op->n_dof, op->stream);
+}
+void d2h_reg_f64(rfhip_op *op, const void *dev, double *host) {
+  if (op->form != RFHIP_FORM_PLAIN_F32) {
+    d2h_layout_f64(op, static_cast<const double *>(dev), host);
+    return;
+  }
+  op->aos.alloc(8 * op->n_dof);
+  launch_cast_f2d(static_cast<const float *>(dev), op->aos.as<double>(), op->n_dof, op->stream);
+  RF_HIP(hipMemcpyAsync(host, op->aos.p, 8 * op->n_dof, hipMemcpyDeviceToHost, op->stream));
+  sync(op);
+}
+std::vector<double> widen(const float *x, int64_t n) { return std::vector<double>(x, x + n); }
+void narrow(const std::vector<double> &x, float *out) {
+  for (size_t i = 0; i < x.size(); ++i) out[i] = (float)x[i];
 }
 
 // ---- stepping -----------------------------------------------------------
 
-// k = A*x (+ b*g) for the KCL / Krylov paths; x and k in the register
-// precision of the form (f32 for PLAIN_F32, f64 otherwise).
+void fill_source(rfhip_op *op, StageArgs &a, int kind, int64_t dof, double g) {
+  a.src_kind = kind;
+  a.g = g;
+  a.src_vec = op->src.as<double>();
+  a.src_dof = dof;
+}
+
+// k = A*x (+ b*g) on register vectors.
 void rhs_into(rfhip_op *op, const void *x, void *k, int kind, int64_t dof, double g) {
   hipStream_t s = op->stream;
-  if (is_fused(op->form))
-    launch_fused_apply(op, static_cast<const double *>(x), static_cast<double *>(k), s);
-  else
-    launch_plain_apply(op, x, k, s);
+  if (is_fused(op->form)) {
+    StageArgs a = op->sa;
+    a.yin = x;
+    a.dy = static_cast<double *>(k);
+    fill_source(op, a, kind, dof, g);
+    launch_stage(op->order, op->form, 0, a, s);
+    return;
+  }
+  launch_plain_apply(op, x, k, s);
   if (kind == RFHIP_DRIVE_POINT)
     launch_add_point(op, k, dof, g, s);
   else if (kind == RFHIP_DRIVE_VECTOR)
@@ -326,17 +373,17 @@ void lserk_step(rfhip_op *op, double h, int kind, int64_t dof, double g, hipEven
   for (int st = 0; st < 5; ++st) {
     if (stage_ev) RF_HIP(hipEventRecord(stage_ev[2 * st], s));
     if (is_fused(op->form)) {
-      FusedIO io{};
-      io.yin = op->y[op->cur].p;
-      io.yin_lo = op->ylo[op->cur].p;
-      io.yout = op->y[1 - op->cur].p;
-      io.yout_lo = op->ylo[1 - op->cur].p;
-      io.p = op->p.p;
-      io.src_kind = kind;
-      io.src_dof = dof;
-      io.src_vec = op->src.as<double>();
-      io.g = g;
-      launch_fused_lserk(op, io, kLsA[st], kLsB[st], h, s);
+      StageArgs a = op->sa;
+      a.yin = op->y[op->cur].p;
+      a.yin_lo = op->ylo[op->cur].p;
+      a.yout = op->y[1 - op->cur].p;
+      a.yout_lo = op->ylo[1 - op->cur].p;
+      a.p = op->p.p;
+      a.sa = kLsA[st];
+      a.sb = kLsB[st];
+      a.dt = h;
+      fill_source(op, a, kind, dof, g);
+      launch_stage(op->order, op->form, 1, a, s);
       if (stage_ev) RF_HIP(hipEventRecord(stage_ev[2 * st + 1], s));
       op->cur = 1 - op->cur;
     } else {
@@ -353,78 +400,42 @@ void lserk_step(rfhip_op *op, double h, int kind, int64_t dof, double g, hipEven
 
 void zero_p(rfhip_op *op) {
   const size_t w = (op->form == RFHIP_FORM_PLAIN_F32 || op->form == RFHIP_FORM_FUSED_F64C) ? 4 : 8;
-  launch_fill(op->p.p, (int64_t)(w * op->n_dof), op->stream);
+  launch_fill(op->p.p, (int64_t)(w * nvec(op)), op->stream);
 }
 
 // Load an f64 host state into the form's state registers.
 void state_from_host_f64(rfhip_op *op, const double *y0) {
-  const int64_t n = op->n_dof;
-  hipStream_t s = op->stream;
   op->cur = 0;
-  if (op->form == RFHIP_FORM_PLAIN_F32) {
-    op->kst.alloc(8 * n);
-    RF_HIP(hipMemcpyAsync(op->kst.p, y0, 8 * n, hipMemcpyHostToDevice, s));
-    launch_cast_d2f(op->kst.as<double>(), op->y[0].as<float>(), n, s);
-  } else if (op->form == RFHIP_FORM_FUSED_F64C) {
-    RF_HIP(hipMemcpyAsync(op->k.p, y0, 8 * n, hipMemcpyHostToDevice, s));
-    launch_f64_to_c(op->k.as<double>(), op->y[0].as<float>(), op->ylo[0].as<uint16_t>(), n, s);
-  } else {
-    RF_HIP(hipMemcpyAsync(op->y[0].p, y0, 8 * n, hipMemcpyHostToDevice, s));
-  }
-}
-void state_from_host_f32(rfhip_op *op, const float *y0) {
-  const int64_t n = op->n_dof;
-  hipStream_t s = op->stream;
-  op->cur = 0;
-  if (op->form == RFHIP_FORM_PLAIN_F32) {
-    RF_HIP(hipMemcpyAsync(op->y[0].p, y0, 4 * n, hipMemcpyHostToDevice, s));
-    return;
-  }
-  op->kst.alloc(8 * n);
-  float *tmp = op->kst.as<float>();
-  RF_HIP(hipMemcpyAsync(tmp, y0, 4 * n, hipMemcpyHostToDevice, s));
-  double *dst = op->form == RFHIP_FORM_FUSED_F64C ? op->k.as<double>() : op->y[0].as<double>();
-  launch_cast_f2d(tmp, dst, n, s);
-  if (op->form == RFHIP_FORM_FUSED_F64C)
-    launch_f64_to_c(dst, op->y[0].as<float>(), op->ylo[0].as<uint16_t>(), n, s);
-}
-// Current state as f64 on the device (may stage through op->k).
-const double *state_f64_device(rfhip_op *op) {
-  const int64_t n = op->n_dof;
-  hipStream_t s = op->stream;
-  if (op->form == RFHIP_FORM_PLAIN_F32) {
-    op->kst.alloc(8 * n);
-    launch_cast_f2d(op->y[0].as<float>(), op->kst.as<double>(), n, s);
-    return op->kst.as<double>();
-  }
   if (op->form == RFHIP_FORM_FUSED_F64C) {
-    launch_c_to_f64(op->y[op->cur].as<float>(), op->ylo[op->cur].as<uint16_t>(), op->k.as<double>(), n, s);
+    h2d_layout_f64(op, y0, op->k.as<double>());
+    launch_f64_to_c(op->k.as<double>(), op->y[0].as<float>(), op->ylo[0].as<uint16_t>(), nvec(op), op->stream);
+  } else {
+    h2d_reg_f64(op, y0, op->y[0].p);
+  }
+}
+// Current state as an f64 layout vector on the device.
+const double *state_layout_f64(rfhip_op *op) {
+  if (op->form == RFHIP_FORM_FUSED_F64C) {
+    launch_c_to_f64(op->y[op->cur].as<float>(), op->ylo[op->cur].as<uint16_t>(), op->k.as<double>(), nvec(op),
+                    op->stream);
     return op->k.as<double>();
+  }
+  if (op->form == RFHIP_FORM_PLAIN_F32) {
+    op->kst.alloc(8 * op->n_dof);
+    launch_cast_f2d(op->y[0].as<float>(), op->kst.as<double>(), op->n_dof, op->stream);
+    return op->kst.as<double>();
   }
   return op->y[is_fused(op->form) ? op->cur : 0].as<double>();
 }
-void state_to_host_f64(rfhip_op *op, double *out) {
-  const double *src = state_f64_device(op);
-  RF_HIP(hipMemcpyAsync(out, src, 8 * op->n_dof, hipMemcpyDeviceToHost, op->stream));
-  sync(op);
-}
-void state_to_host_f32(rfhip_op *op, float *out) {
-  const int64_t n = op->n_dof;
-  if (op->form == RFHIP_FORM_PLAIN_F32) {
-    RF_HIP(hipMemcpyAsync(out, op->y[0].p, 4 * n, hipMemcpyDeviceToHost, op->stream));
-    sync(op);
-    return;
-  }
-  const double *src = state_f64_device(op);
-  op->kb_x.alloc(4 * n);
-  launch_cast_d2f(src, op->kb_x.as<float>(), n, op->stream);
-  RF_HIP(hipMemcpyAsync(out, op->kb_x.p, 4 * n, hipMemcpyDeviceToHost, op->stream));
-  sync(op);
-}
+void state_to_host_f64(rfhip_op *op, double *out) { d2h_layout_f64(op, state_layout_f64(op), out); }
+// Current state as an f64 AoS vector on the device.
+const double *state_aos_f64(rfhip_op *op) { return state_layout_f64(op); }
 
 void load_source(rfhip_op *op, int kind, const double *src) {
   if (kind != RFHIP_DRIVE_VECTOR) return;
-  upload(op->src, src, op->n_dof, op->stream);
+  op->src.alloc(8 * nvec(op));
+  h2d_layout_f64(op, src, op->src.as<double>());
+  sync(op);
 }
 
 template <class G>
@@ -454,11 +465,13 @@ double factor(const rfhip_controller &c, double err_norm, double prev, bool reje
   return std::min(std::max(f, c.shrink_limit), c.growth_limit);
 }
 
-// KCL transient on registers y=op->y[0]-like buffer `ys` (f32 or f64).
+// KCL RK4(3)5[2R+]C transient with the PI controller (gpu/operator.rs:823-893
+// structure, session.rs:781-827 control logic) on the register vector ys.
+// The controller runs on the host; each substep reads back one double.
 template <class G>
 void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_t dof, const G *g,
               const rfhip_controller &ctl, rfhip_kcl_stats &st, const std::function<void(int64_t)> &emit) {
-  const int64_t n = op->n_dof;
+  const int64_t n = nvec(op);
   const bool f64 = f64_regs(op->form);
   const size_t w = f64 ? 8 : 4;
   hipStream_t s = op->stream;
@@ -466,7 +479,7 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
   op->kdtf.alloc(w * n);
   op->kerr.alloc(w * n);
   op->kbak.alloc(w * n);
-  if (!is_fused(op->form)) op->k.alloc(w * n);
+  for (DevBuf *b : {&op->kst, &op->kdtf, &op->kerr, &op->kbak}) launch_fill(b->p, (int64_t)b->bytes, s);
   const int nblk = weighted_err_blocks(n);
   op->red.alloc(8 * (nblk + 1));
   double *part = op->red.as<double>();
@@ -493,7 +506,7 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
       double ssum = 0.0;
       RF_HIP(hipMemcpyAsync(&ssum, res, 8, hipMemcpyDeviceToHost, s));
       sync(op);
-      const double e = std::sqrt(ssum / (double)std::max<int64_t>(n, 1));
+      const double e = std::sqrt(ssum / (double)std::max<int64_t>(op->n_dof, 1));
       if (std::isfinite(e) && e <= 1.0) {
         t_rel += ht;
         ++acc;
@@ -521,17 +534,18 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
 // ---- Krylov (propagator.rs:186-373) ---------------------------------------
 
 // Arnoldi with CGS2 on the device, Hessenberg on the host. aug: the ETD
-// augmentation [[A, b], [0, 0]] with the scalar component carried here.
+// augmentation [[A, b], [0, 0]] with the scalar component carried here and
+// b = op->src (a layout vector).
 int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, double tol, bool aug, double *out) {
-  const int64_t n = op->n_dof;
+  const int64_t n = nvec(op);
   RF_REQUIRE(max_dim >= 1 && max_dim <= kMaxKrylov, "Krylov dimension must be in [1, 64]");
   hipStream_t s = op->stream;
   double nrm2 = 0.0;
-  for (int64_t i = 0; i < n; ++i) nrm2 += v_host[i] * v_host[i];
+  for (int64_t i = 0; i < op->n_dof; ++i) nrm2 += v_host[i] * v_host[i];
   if (aug) nrm2 += 1.0;
   const double beta = std::sqrt(nrm2);
   if (beta == 0.0) {
-    std::fill(out, out + n, 0.0);
+    std::fill(out, out + op->n_dof, 0.0);
     return 0;
   }
   if (op->kb_cap < max_dim) {
@@ -542,6 +556,7 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
   op->kb_x.alloc(8 * n);
   op->kb_out.alloc(8 * n);
   op->kb_proj.alloc(8 * (kMaxKrylov + 8));
+  launch_fill(op->kb_w.p, 8 * n, s);
   const int nblk = krylov_dot_blocks(n);
   op->kb_part.alloc(8 * (size_t)nblk * (kMaxKrylov + 1));
   double *basis = op->kb_basis.as<double>(), *w = op->kb_w.as<double>();
@@ -552,7 +567,7 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
     op->kst.alloc(4 * n);
     op->k.alloc(4 * n);
   }
-  RF_HIP(hipMemcpyAsync(op->kb_x.p, v_host, 8 * n, hipMemcpyHostToDevice, s));
+  h2d_layout_f64(op, v_host, op->kb_x.as<double>());
   launch_scale(op->kb_x.as<double>(), basis, n, 1.0 / beta, s);
   std::vector<double> H((size_t)max_dim * max_dim, 0.0), sc(max_dim + 1, 0.0), pj(kMaxKrylov);
   sc[0] = aug ? 1.0 / beta : 0.0;
@@ -565,10 +580,8 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
       launch_cast_d2f(bj, op->kst.as<float>(), n, s);
       launch_plain_apply(op, op->kst.p, op->k.p, s);
       launch_cast_f2d(op->k.as<float>(), w, n, s);
-    } else if (is_fused(op->form)) {
-      launch_fused_apply(op, bj, w, s);
     } else {
-      launch_plain_apply(op, bj, w, s);
+      rhs_into(op, bj, w, RFHIP_DRIVE_FREE, 0, 0.0);
     }
     if (aug && sc[j] != 0.0) launch_axpy(w, op->src.as<double>(), sc[j], n, s);
     double ws = 0.0;  // augmented scalar of w (zero row of the augmented matrix)
@@ -621,8 +634,7 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
   for (int i = 0; i < dim; ++i) pj[i] = beta * eth[(size_t)i * dim];
   RF_HIP(hipMemcpyAsync(proj, pj.data(), 8 * dim, hipMemcpyHostToDevice, s));
   launch_lincomb(basis, n, dim, proj, op->kb_out.as<double>(), s);
-  RF_HIP(hipMemcpyAsync(out, op->kb_out.p, 8 * n, hipMemcpyDeviceToHost, s));
-  sync(op);
+  d2h_layout_f64(op, op->kb_out.as<double>(), out);
   return dim;
 }
 
@@ -673,6 +685,48 @@ void expm_host(const double *a, int n, double *out) {
 
 using namespace rfhip;
 
+namespace {
+
+template <class T, class G>
+int kcl_entry(rfhip_op *op, const T *y0, double dt, int64_t steps, int kind, int64_t dof, const double *src,
+              const G *g, int64_t n_g, const rfhip_controller *ctl, T *out, rfhip_kcl_stats *stats) {
+  return guarded([&] {
+    RF_REQUIRE(op && y0 && out && ctl, "null argument");
+    RF_REQUIRE(steps >= 0, "negative step count");
+    check_drive(op, kind, dof, src, n_g, steps, g);
+    load_source(op, kind, src);
+    const int64_t n = op->n_dof;
+    op->kb_w.alloc(8 * nvec(op));  // the KCL state register
+    launch_fill(op->kb_w.p, 8 * nvec(op), op->stream);
+    void *ys = op->kb_w.p;
+    std::vector<double> row(n);
+    if constexpr (std::is_same<T, double>::value) {
+      h2d_reg_f64(op, y0, ys);
+    } else {
+      row = widen(y0, n);
+      h2d_reg_f64(op, row.data(), ys);
+    }
+    rfhip_kcl_stats st{};
+    try {
+      kcl_impl<G>(op, ys, dt, steps, kind, dof, g, *ctl, st, [&](int64_t r) {
+        if constexpr (std::is_same<T, double>::value) {
+          d2h_reg_f64(op, ys, out + r * n);
+        } else {
+          d2h_reg_f64(op, ys, row.data());
+          narrow(row, out + r * n);
+        }
+      });
+    } catch (const Error &) {
+      if (stats) *stats = st;
+      throw;
+    }
+    if (stats) *stats = st;
+  });
+}
+
+}  // namespace
+
+
 extern "C" {
 
 size_t rfhip_last_error(char *buf, size_t len) {
@@ -719,8 +773,7 @@ int rfhip_op_create(rfhip_ctx *ctx, const rfhip_op_desc *d, int form, rfhip_op *
     RF_REQUIRE(d->n_elem >= 1, "empty operator");
     const int p = order_of_np(d->np);
     RF_REQUIRE(p >= 1 && d->nfp == (p + 1) * (p + 2) / 2, "inconsistent Np / Nfp");
-    if (!(d->np == 4 || d->np == 10 || d->np == 20 || d->np == 35))
-      throw Error{RFHIP_EUNSUPPORTED, "orders 1..4 are compiled (Np 4, 10, 20, 35)"};
+    if (p > 4) throw Error{RFHIP_EUNSUPPORTED, "orders 1..4 are compiled (Np 4, 10, 20, 35)"};
     RF_REQUIRE(d->diff_r && d->diff_s && d->diff_t && d->lift && d->face_nodes && d->jinv && d->inv_eps && d->inv_mu &&
                    d->sigma_eps && d->sigma_mu && d->face_normal && d->face_fscale && d->face_neighbor &&
                    d->face_nbr_local && d->face_port && d->face_perm,
@@ -730,10 +783,12 @@ int rfhip_op_create(rfhip_ctx *ctx, const rfhip_op_desc *d, int form, rfhip_op *
     auto op = std::make_unique<rfhip_op>();
     op->ctx = ctx;
     op->form = form;
+    op->order = p;
     op->np = d->np;
     op->nfp = d->nfp;
     op->n_elem = d->n_elem;
     op->n_dof = 6 * (int64_t)d->np * d->n_elem;
+    op->n_vec = op->n_dof;
     op->alpha = d->flux_alpha;
     RF_HIP(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
     op->port_host.assign(d->face_port, d->face_port + 4 * d->n_elem);
@@ -775,43 +830,24 @@ int rfhip_op_apply(rfhip_op *op, const float *y, float *dy) {
       sync(op);
       return;
     }
-    op->kst.alloc(8 * n);
-    op->kb_x.alloc(8 * n);
-    RF_HIP(hipMemcpyAsync(op->kb_x.p, y, 4 * n, hipMemcpyHostToDevice, s));
-    launch_cast_f2d(op->kb_x.as<float>(), op->kst.as<double>(), n, s);
-    if (is_fused(op->form))
-      launch_fused_apply(op, op->kst.as<double>(), op->k.as<double>(), s);
-    else
-      launch_plain_apply(op, op->kst.p, op->k.p, s);
-    launch_cast_d2f(op->k.as<double>(), op->kb_x.as<float>(), n, s);
-    RF_HIP(hipMemcpyAsync(dy, op->kb_x.p, 4 * n, hipMemcpyDeviceToHost, s));
-    sync(op);
+    const auto yd = widen(y, n);
+    std::vector<double> out(n);
+    op->kst.alloc(8 * nvec(op));
+    h2d_reg_f64(op, yd.data(), op->kst.p);
+    rhs_into(op, op->kst.p, op->k.p, RFHIP_DRIVE_FREE, 0, 0.0);
+    d2h_reg_f64(op, op->k.p, out.data());
+    narrow(out, dy);
   });
 }
 
 int rfhip_op_apply_f64(rfhip_op *op, const double *y, double *dy) {
   return guarded([&] {
     RF_REQUIRE(op && y && dy, "null argument");
-    const int64_t n = op->n_dof;
-    hipStream_t s = op->stream;
-    if (op->form == RFHIP_FORM_PLAIN_F32) {
-      op->kb_x.alloc(8 * n);
-      RF_HIP(hipMemcpyAsync(op->kb_x.p, y, 8 * n, hipMemcpyHostToDevice, s));
-      launch_cast_d2f(op->kb_x.as<double>(), op->y[0].as<float>(), n, s);
-      launch_plain_apply(op, op->y[0].p, op->k.p, s);
-      launch_cast_f2d(op->k.as<float>(), op->kb_x.as<double>(), n, s);
-      RF_HIP(hipMemcpyAsync(dy, op->kb_x.p, 8 * n, hipMemcpyDeviceToHost, s));
-      sync(op);
-      return;
-    }
-    op->kst.alloc(8 * n);
-    RF_HIP(hipMemcpyAsync(op->kst.p, y, 8 * n, hipMemcpyHostToDevice, s));
-    if (is_fused(op->form))
-      launch_fused_apply(op, op->kst.as<double>(), op->k.as<double>(), s);
-    else
-      launch_plain_apply(op, op->kst.p, op->k.p, s);
-    RF_HIP(hipMemcpyAsync(dy, op->k.p, 8 * n, hipMemcpyDeviceToHost, s));
-    sync(op);
+    const size_t w = op->form == RFHIP_FORM_PLAIN_F32 ? 4 : 8;
+    op->kst.alloc(w * nvec(op));
+    h2d_reg_f64(op, y, op->kst.p);
+    rhs_into(op, op->kst.p, op->k.p, RFHIP_DRIVE_FREE, 0, 0.0);
+    d2h_reg_f64(op, op->k.p, dy);
   });
 }
 
@@ -823,13 +859,17 @@ int rfhip_op_transient(rfhip_op *op, const float *y0, float dt, int64_t steps, i
     substeps = std::max<int64_t>(1, substeps);
     check_drive(op, kind, dof, src, n_g, steps * substeps, g);
     load_source(op, kind, src);
-    state_from_host_f32(op, y0);
     const int64_t n = op->n_dof;
-    transient_impl<float>(op, steps, substeps, (double)dt, kind, dof, g, record != 0, [&](int64_t row) {
-      if (row == 0)
+    const auto y0d = widen(y0, n);
+    state_from_host_f64(op, y0d.data());
+    std::vector<double> row(n);
+    transient_impl<float>(op, steps, substeps, (double)dt, kind, dof, g, record != 0, [&](int64_t r) {
+      if (r == 0) {
         std::memcpy(out, y0, sizeof(float) * n);
-      else
-        state_to_host_f32(op, row < 0 ? out : out + row * n);
+        return;
+      }
+      state_to_host_f64(op, row.data());
+      narrow(row, r < 0 ? out : out + r * n);
     });
   });
 }
@@ -842,13 +882,13 @@ int rfhip_op_transient_f64(rfhip_op *op, const double *y0, double dt, int64_t st
     substeps = std::max<int64_t>(1, substeps);
     check_drive(op, kind, dof, src, n_g, steps * substeps, g);
     load_source(op, kind, src);
-    state_from_host_f64(op, y0);
     const int64_t n = op->n_dof;
-    transient_impl<double>(op, steps, substeps, dt, kind, dof, g, record != 0, [&](int64_t row) {
-      if (row == 0)
+    state_from_host_f64(op, y0);
+    transient_impl<double>(op, steps, substeps, dt, kind, dof, g, record != 0, [&](int64_t r) {
+      if (r == 0)
         std::memcpy(out, y0, sizeof(double) * n);
       else
-        state_to_host_f64(op, row < 0 ? out : out + row * n);
+        state_to_host_f64(op, r < 0 ? out : out + r * n);
     });
   });
 }
@@ -856,75 +896,13 @@ int rfhip_op_transient_f64(rfhip_op *op, const double *y0, double dt, int64_t st
 int rfhip_op_transient_kcl(rfhip_op *op, const float *y0, float dt, int64_t steps, int kind, int64_t dof,
                            const double *src, const float *g, int64_t n_g, const rfhip_controller *ctl, float *out,
                            rfhip_kcl_stats *stats) {
-  return guarded([&] {
-    RF_REQUIRE(op && y0 && out && ctl, "null argument");
-    check_drive(op, kind, dof, src, n_g, steps, g);
-    load_source(op, kind, src);
-    const int64_t n = op->n_dof;
-    hipStream_t s = op->stream;
-    const bool f64 = f64_regs(op->form);
-    op->kb_w.alloc(8 * n);  // KCL state register
-    op->kb_x.alloc(8 * n);
-    void *ys = op->kb_w.p;
-    RF_HIP(hipMemcpyAsync(op->kb_x.p, y0, 4 * n, hipMemcpyHostToDevice, s));
-    if (f64)
-      launch_cast_f2d(op->kb_x.as<float>(), op->kb_w.as<double>(), n, s);
-    else
-      launch_copy(ys, op->kb_x.p, 4 * n, s);
-    rfhip_kcl_stats st{};
-    try {
-      kcl_impl<float>(op, ys, (double)dt, steps, kind, dof, g, *ctl, st, [&](int64_t row) {
-        if (f64) {
-          launch_cast_d2f(op->kb_w.as<double>(), op->kb_x.as<float>(), n, s);
-          RF_HIP(hipMemcpyAsync(out + row * n, op->kb_x.p, 4 * n, hipMemcpyDeviceToHost, s));
-        } else {
-          RF_HIP(hipMemcpyAsync(out + row * n, ys, 4 * n, hipMemcpyDeviceToHost, s));
-        }
-        sync(op);
-      });
-    } catch (const Error &) {
-      if (stats) *stats = st;
-      throw;
-    }
-    if (stats) *stats = st;
-  });
+  return kcl_entry<float, float>(op, y0, (double)dt, steps, kind, dof, src, g, n_g, ctl, out, stats);
 }
 
 int rfhip_op_transient_kcl_f64(rfhip_op *op, const double *y0, double dt, int64_t steps, int kind, int64_t dof,
                                const double *src, const double *g, int64_t n_g, const rfhip_controller *ctl,
                                double *out, rfhip_kcl_stats *stats) {
-  return guarded([&] {
-    RF_REQUIRE(op && y0 && out && ctl, "null argument");
-    check_drive(op, kind, dof, src, n_g, steps, g);
-    load_source(op, kind, src);
-    const int64_t n = op->n_dof;
-    hipStream_t s = op->stream;
-    const bool f64 = f64_regs(op->form);
-    op->kb_w.alloc(8 * n);
-    op->kb_x.alloc(8 * n);
-    void *ys = op->kb_w.p;
-    RF_HIP(hipMemcpyAsync(op->kb_x.p, y0, 8 * n, hipMemcpyHostToDevice, s));
-    if (f64)
-      launch_copy(ys, op->kb_x.p, 8 * n, s);
-    else
-      launch_cast_d2f(op->kb_x.as<double>(), op->kb_w.as<float>(), n, s);
-    rfhip_kcl_stats st{};
-    try {
-      kcl_impl<double>(op, ys, dt, steps, kind, dof, g, *ctl, st, [&](int64_t row) {
-        if (f64) {
-          RF_HIP(hipMemcpyAsync(out + row * n, ys, 8 * n, hipMemcpyDeviceToHost, s));
-        } else {
-          launch_cast_f2d(op->kb_w.as<float>(), op->kb_x.as<double>(), n, s);
-          RF_HIP(hipMemcpyAsync(out + row * n, op->kb_x.p, 8 * n, hipMemcpyDeviceToHost, s));
-        }
-        sync(op);
-      });
-    } catch (const Error &) {
-      if (stats) *stats = st;
-      throw;
-    }
-    if (stats) *stats = st;
-  });
+  return kcl_entry<double, double>(op, y0, dt, steps, kind, dof, src, g, n_g, ctl, out, stats);
 }
 
 int rfhip_op_expmv(rfhip_op *op, const double *v, double t, int32_t max_dim, double tol, double *out,
@@ -940,9 +918,81 @@ int rfhip_op_etd_step(rfhip_op *op, const double *y, const double *b, double h, 
                       double *out, int32_t *dim_used) {
   return guarded([&] {
     RF_REQUIRE(op && y && b && out, "null argument");
-    upload(op->src, b, op->n_dof, op->stream);
+    op->src.alloc(8 * nvec(op));
+    h2d_layout_f64(op, b, op->src.as<double>());
     const int dim = krylov_impl(op, y, h, max_dim, tol, true, out);
     if (dim_used) *dim_used = dim;
+  });
+}
+
+int rfhip_op_n_variants(const rfhip_op *op) { return op && is_fused(op->form) ? kFusedVariants : 1; }
+
+int rfhip_op_variant(const rfhip_op *op) { return op && is_fused(op->form) ? op->sa.variant : 0; }
+
+int rfhip_op_set_variant(rfhip_op *op, int variant) {
+  return guarded([&] {
+    RF_REQUIRE(op, "null argument");
+    RF_REQUIRE(variant >= 0 && variant < rfhip_op_n_variants(op), "variant out of range");
+    if (is_fused(op->form)) op->sa.variant = variant;
+  });
+}
+
+int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best) {
+  return guarded([&] {
+    RF_REQUIRE(op, "null argument");
+    if (!is_fused(op->form)) {
+      if (chosen) *chosen = 0;
+      if (ms_best) *ms_best = 0.0;
+      return;
+    }
+    hipStream_t s = op->stream;
+    const int64_t n = nvec(op);
+    const bool c = op->form == RFHIP_FORM_FUSED_F64C;
+    DevBuf yo, ylo, pp;
+    yo.alloc((c ? 4 : 8) * n);
+    ylo.alloc(2 * n);
+    pp.alloc((c ? 4 : 8) * n);
+    launch_fill(pp.p, (int64_t)pp.bytes, s);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    RF_HIP(hipEventCreate(&e0));
+    RF_HIP(hipEventCreate(&e1));
+    int best = 0;
+    double best_ms = 1e300;
+    try {
+      for (int v = 0; v < kFusedVariants; ++v) {
+        StageArgs a = op->sa;
+        a.variant = v;
+        a.yin = op->y[op->cur].p;
+        a.yin_lo = op->ylo[op->cur].p;
+        a.yout = yo.p;
+        a.yout_lo = ylo.p;
+        a.p = pp.p;
+        a.sa = kLsA[1];
+        a.sb = kLsB[1];
+        a.dt = 1e-3;
+        a.src_kind = RFHIP_DRIVE_FREE;
+        for (int w = 0; w < 2; ++w) launch_stage(op->order, op->form, 1, a, s);
+        RF_HIP(hipEventRecord(e0, s));
+        for (int w = 0; w < 5; ++w) launch_stage(op->order, op->form, 1, a, s);
+        RF_HIP(hipEventRecord(e1, s));
+        sync(op);
+        float ms = 0.0f;
+        RF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms / 5.0 < best_ms) {
+          best_ms = ms / 5.0;
+          best = v;
+        }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    op->sa.variant = best;
+    if (chosen) *chosen = best;
+    if (ms_best) *ms_best = best_ms;
   });
 }
 
@@ -1006,16 +1056,13 @@ int rfhip_op_state_energy(rfhip_op *op, const double *mass_ref, const double *de
     hipStream_t s = op->stream;
     const int np = op->np;
     const int64_t K = op->n_elem;
-    const double *y = state_f64_device(op);
-    DevBuf m, dd, ie, im, part;
-    upload(m, mass_ref, (size_t)np * np, s);
-    upload(dd, det, (size_t)K, s);
-    // materials as f64 per element from the form's data
+    const double *y = state_aos_f64(op);
+    // per-element 1/eps and 1/mu in f64
     std::vector<double> hie(3 * K), him(3 * K);
     if (is_fused(op->form)) {
       std::vector<int32_t> mat(K);
       std::vector<double> tab(op->mat_tab.bytes / 8);
-      RF_HIP(hipMemcpyAsync(mat.data(), op->mat.p, 4 * K, hipMemcpyDeviceToHost, s));
+      RF_HIP(hipMemcpyAsync(mat.data(), op->mat.p, 4 * mat.size(), hipMemcpyDeviceToHost, s));
       RF_HIP(hipMemcpyAsync(tab.data(), op->mat_tab.p, op->mat_tab.bytes, hipMemcpyDeviceToHost, s));
       sync(op);
       for (int64_t e = 0; e < K; ++e)
@@ -1037,6 +1084,9 @@ int rfhip_op_state_energy(rfhip_op *op, const double *mass_ref, const double *de
         him[q] = fmu[q];
       }
     }
+    DevBuf m, dd, ie, im, part;
+    upload(m, mass_ref, (size_t)np * np, s);
+    upload(dd, det, (size_t)K, s);
     upload(ie, hie.data(), hie.size(), s);
     upload(im, him.data(), him.size(), s);
     const int nblk = 1024;

@@ -22,6 +22,8 @@
 // (element, node). The block's state is staged in LDS with coalesced loads;
 // D^T and LIFT^T are read from L1 with consecutive nodes on consecutive
 // lanes.
+#include <cstdlib>
+
 #include "rfhip_impl.hpp"
 
 namespace rfhip {
@@ -37,7 +39,7 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 template <bool CIN>
-__device__ __forceinline__ double load_state(const FusedIO &io, int64_t i) {
+__device__ __forceinline__ double load_state(const StageArgs &io, int64_t i) {
   if constexpr (CIN) {
     return (double)static_cast<const float *>(io.yin)[i] + (double)bf16_to_f(static_cast<const uint16_t *>(io.yin_lo)[i]);
   } else {
@@ -54,7 +56,9 @@ __device__ __forceinline__ void cross3(const double *a, const double *b, double 
 // EPI: 0 = apply (dy = A y, FP64), 1 = LSERK stage with FP64 registers,
 // 2 = LSERK stage with compressed registers (y f32+bf16, p f32).
 template <int NP, int NFP, bool CIN, int EPI>
-__global__ void __launch_bounds__(kFusedWG) k_fused(DevOp d, FusedIO io, double sa, double sb, double dt) {
+__global__ void __launch_bounds__(kFusedWG) k_fused(StageArgs io) {
+  const StageArgs &d = io;
+  const double sa = io.sa, sb = io.sb, dt = io.dt;
   constexpr int EPB = kFusedWG / NP;
   constexpr int NF4 = 4 * NFP;
   __shared__ double su[EPB * NP * 6];
@@ -76,7 +80,7 @@ __global__ void __launch_bounds__(kFusedWG) k_fused(DevOp d, FusedIO io, double 
   double ji[9];
   if (active) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) ji[q] = d.jinv64[e * 9 + q];
+    for (int q = 0; q < 9; ++q) ji[q] = d.jinv[e * 9 + q];
     // Volume: r[k][c] = sum_j D_k[i][j] u[j][c] for all six components.
     double r[3][6];
 #pragma unroll
@@ -178,7 +182,7 @@ __global__ void __launch_bounds__(kFusedWG) k_fused(DevOp d, FusedIO io, double 
     const double *fb = sf + le * NF4 * 6;
 #pragma unroll 4
     for (int fm = 0; fm < NF4; ++fm) {
-      const double w = d.liftT[fm * NP + i];
+      const double w = d.lift[fm * NP + i];
 #pragma unroll
       for (int c = 0; c < 6; ++c) acc[c] = fma(w, fb[fm * 6 + c], acc[c]);
     }
@@ -229,26 +233,261 @@ __global__ void __launch_bounds__(kFusedWG) k_fused(DevOp d, FusedIO io, double 
   }
 }
 
+// Register-blocked variant: each thread owns NB nodes of one element, so
+// every state value and flux value read from LDS feeds NB rows of D and
+// LIFT. The face flux is first computed into registers, then written over
+// the state's LDS region (the two are never live together), which keeps the
+// block's LDS at max(state, flux) instead of their sum. Own node values stay
+// in registers for the epilogue.
+template <int NP, int NFP, int NB, bool CIN, int EPI>
+__global__ void __launch_bounds__(kFusedWG) k_fused2(StageArgs io) {
+  const StageArgs &d = io;
+  constexpr int TPE = (NP + NB - 1) / NB;  // threads per element
+  constexpr int EPB = kFusedWG / TPE;      // elements per block
+  constexpr int NF4 = 4 * NFP;
+  constexpr int FPT = (NF4 + TPE - 1) / TPE;  // face nodes per thread
+  constexpr int SU = EPB * NP * 6, SF = EPB * NF4 * 6;
+  __shared__ double smem[SU > SF ? SU : SF];
+  const int tid = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int64_t rem = d.n_elem - e0;
+  const int nel = rem < EPB ? (int)rem : EPB;
+  {
+    const int64_t g0 = e0 * NP * 6;
+    const int cnt = nel * NP * 6;
+    for (int q = tid; q < cnt; q += kFusedWG) smem[q] = load_state<CIN>(io, g0 + q);
+  }
+  __syncthreads();
+  const int le = tid / TPE, t = tid - (tid / TPE) * TPE;
+  const bool active = tid < nel * TPE;
+  const int64_t e = e0 + le;
+  int node[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) node[b] = t + b * TPE;
+  double acc[NB][6], own[NB][6], flx[FPT][6];
+  double ji[9];
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) ji[q] = d.jinv[e * 9 + q];
+    const double *ub = smem + le * NP * 6;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) own[b][c] = node[b] < NP ? ub[node[b] * 6 + c] : 0.0;
+    // Volume: r[b][k][c] = sum_j D_k[node_b][j] u[j][c].
+    double r[NB][3][6];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) r[b][k][c] = 0.0;
+#pragma unroll 2
+    for (int j = 0; j < NP; ++j) {
+      double u[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) u[c] = ub[j * 6 + c];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int ib = node[b] < NP ? node[b] : 0;
+        const double d0 = d.dT[(0 * NP + j) * NP + ib];
+        const double d1 = d.dT[(1 * NP + j) * NP + ib];
+        const double d2 = d.dT[(2 * NP + j) * NP + ib];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          r[b][0][c] = fma(d0, u[c], r[b][0][c]);
+          r[b][1][c] = fma(d1, u[c], r[b][1][c]);
+          r[b][2][c] = fma(d2, u[c], r[b][2][c]);
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      auto pd = [&](int p, int c) {
+        return ji[0 * 3 + p] * r[b][0][c] + ji[1 * 3 + p] * r[b][1][c] + ji[2 * 3 + p] * r[b][2][c];
+      };
+      acc[b][0] = pd(1, 5) - pd(2, 4);
+      acc[b][1] = pd(2, 3) - pd(0, 5);
+      acc[b][2] = pd(0, 4) - pd(1, 3);
+      acc[b][3] = -(pd(1, 2) - pd(2, 1));
+      acc[b][4] = -(pd(2, 0) - pd(0, 2));
+      acc[b][5] = -(pd(0, 1) - pd(1, 0));
+    }
+    // Face flux into registers, scaled by 0.5*fscale (folded into g).
+#pragma unroll
+    for (int s2 = 0; s2 < FPT; ++s2) {
+      const int fm = t + s2 * TPE;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) flx[s2][c] = 0.0;
+      if (fm >= NF4) continue;
+      const int f = fm / NFP, m = fm - f * NFP;
+      double g[3];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        const double r0 = ji[0 * 3 + x], r1 = ji[1 * 3 + x], r2 = ji[2 * 3 + x];
+        g[x] = f == 0 ? -r2 : f == 1 ? -r0 : f == 2 ? -r1 : (r0 + r1 + r2);
+      }
+      const double inv_g = 1.0 / sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+      const int code = d.fcode[e * 4 + f];
+      const int vi = d.face_nodes[f * NFP + m];
+      double em[3], hm[3], je[3], jh[3];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        em[x] = ub[vi * 6 + x];
+        hm[x] = ub[vi * 6 + 3 + x];
+      }
+      double a = d.alpha;
+      if (code >= 0) {
+        const int64_t nb = code >> 5;
+        const int nlf = (code >> 3) & 3, o = code & 7;
+        const int vj = d.nbr_node[((f * 4 + nlf) * 6 + o) * NFP + m];
+        const int64_t nbb = (nb * NP + vj) * 6;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          je[x] = em[x] - load_state<CIN>(io, nbb + x);
+          jh[x] = hm[x] - load_state<CIN>(io, nbb + 3 + x);
+        }
+      } else if (code == kCodePec) {
+        const double ig2 = inv_g * inv_g;
+        const double edg = (g[0] * em[0] + g[1] * em[1] + g[2] * em[2]) * ig2;
+        const double hdg = (g[0] * hm[0] + g[1] * hm[1] + g[2] * hm[2]) * ig2;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          je[x] = 2.0 * (em[x] - edg * g[x]);
+          jh[x] = 2.0 * hdg * g[x];
+        }
+      } else {
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          je[x] = em[x];
+          jh[x] = hm[x];
+        }
+        a = 1.0;
+      }
+      double gxh[3], gxe[3], gge[3], ggh[3];
+      cross3(g, jh, gxh);
+      cross3(g, je, gxe);
+      cross3(g, gxe, gge);
+      cross3(g, gxh, ggh);
+      const double ha = 0.5 * a * inv_g;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        flx[s2][x] = -0.5 * gxh[x] + ha * gge[x];
+        flx[s2][3 + x] = 0.5 * gxe[x] + ha * ggh[x];
+      }
+    }
+  }
+  __syncthreads();  // every thread is done reading the state in LDS
+  if (active) {
+#pragma unroll
+    for (int s2 = 0; s2 < FPT; ++s2) {
+      const int fm = t + s2 * TPE;
+      if (fm >= NF4) continue;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) smem[(le * NF4 + fm) * 6 + c] = flx[s2][c];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  {
+    const double *fb = smem + le * NF4 * 6;
+#pragma unroll 2
+    for (int fm = 0; fm < NF4; ++fm) {
+      double F[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) F[c] = fb[fm * 6 + c];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int ib = node[b] < NP ? node[b] : 0;
+        const double w = d.lift[fm * NP + ib];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[b][c] = fma(w, F[c], acc[b][c]);
+      }
+    }
+  }
+  const double *mt = d.mat_tab + 12 * d.mat[e];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (node[b] >= NP) continue;
+    double kv[6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      kv[c] = mt[c] * acc[b][c] - mt[6 + c] * own[b][c];
+      kv[3 + c] = mt[3 + c] * acc[b][3 + c] - mt[9 + c] * own[b][3 + c];
+    }
+    const int64_t base = (e * NP + node[b]) * 6;
+    if (io.src_kind == RFHIP_DRIVE_POINT) {
+      const int64_t off = io.src_dof - base;
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        if (off == c) kv[c] += io.g;
+    } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) kv[c] += io.src_vec[base + c] * io.g;
+    }
+    if constexpr (EPI == 0) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) io.dy[base + c] = kv[c];
+    } else if constexpr (EPI == 1) {
+      double *pp = static_cast<double *>(io.p);
+      double *yo = static_cast<double *>(io.yout);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const double pn = io.sa * pp[base + c] + io.dt * kv[c];
+        pp[base + c] = pn;
+        yo[base + c] = own[b][c] + io.sb * pn;
+      }
+    } else {
+      float *pp = static_cast<float *>(io.p);
+      float *yh = static_cast<float *>(io.yout);
+      uint16_t *yl = static_cast<uint16_t *>(io.yout_lo);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const float pn = (float)(io.sa * (double)pp[base + c] + io.dt * kv[c]);
+        pp[base + c] = pn;
+        const double yn = own[b][c] + io.sb * (double)pn;
+        const float h = (float)yn;
+        yh[base + c] = h;
+        yl[base + c] = bf16_rne((float)(yn - (double)h));
+      }
+    }
+  }
+}
+
 template <int NP>
 struct Nfp {
   static constexpr int v = NP == 4 ? 3 : NP == 10 ? 6 : NP == 20 ? 10 : 15;
 };
 
+// Kernel variant per launch: 0 = one node per thread, 1/2 = NB nodes per
+// thread (k_fused2). Chosen by the operator (autotuned or set).
+int g_variant_override = -1;
+
 template <int NP, bool CIN, int EPI>
-void fused_np(const rfhip_op *op, const FusedIO &io, double a, double b, double dt, hipStream_t s) {
-  constexpr int EPB = kFusedWG / NP;
-  const int64_t groups = (op->n_elem + EPB - 1) / EPB;
-  hipLaunchKernelGGL((k_fused<NP, Nfp<NP>::v, CIN, EPI>), dim3((unsigned)groups), dim3(kFusedWG), 0, s, op->dev, io,
-                     a, b, dt);
+void fused_np(int variant, const StageArgs &a, hipStream_t s) {
+  constexpr int F = Nfp<NP>::v;
+  if (variant == 0) {
+    constexpr int EPB = kFusedWG / NP;
+    const int64_t groups = (a.n_elem + EPB - 1) / EPB;
+    hipLaunchKernelGGL((k_fused<NP, F, CIN, EPI>), dim3((unsigned)groups), dim3(kFusedWG), 0, s, a);
+  } else if (variant == 1) {
+    constexpr int EPB = kFusedWG / ((NP + 1) / 2);
+    const int64_t groups = (a.n_elem + EPB - 1) / EPB;
+    hipLaunchKernelGGL((k_fused2<NP, F, 2, CIN, EPI>), dim3((unsigned)groups), dim3(kFusedWG), 0, s, a);
+  } else {
+    constexpr int EPB = kFusedWG / NP;
+    const int64_t groups = (a.n_elem + EPB - 1) / EPB;
+    hipLaunchKernelGGL((k_fused2<NP, F, 1, CIN, EPI>), dim3((unsigned)groups), dim3(kFusedWG), 0, s, a);
+  }
 }
 
 template <bool CIN, int EPI>
-void fused_dispatch(const rfhip_op *op, const FusedIO &io, double a, double b, double dt, hipStream_t s) {
-  switch (op->np) {
-    case 4: fused_np<4, CIN, EPI>(op, io, a, b, dt, s); break;
-    case 10: fused_np<10, CIN, EPI>(op, io, a, b, dt, s); break;
-    case 20: fused_np<20, CIN, EPI>(op, io, a, b, dt, s); break;
-    case 35: fused_np<35, CIN, EPI>(op, io, a, b, dt, s); break;
+void fused_dispatch(int np, int variant, const StageArgs &a, hipStream_t s) {
+  switch (np) {
+    case 4: fused_np<4, CIN, EPI>(variant, a, s); break;
+    case 10: fused_np<10, CIN, EPI>(variant, a, s); break;
+    case 20: fused_np<20, CIN, EPI>(variant, a, s); break;
+    case 35: fused_np<35, CIN, EPI>(variant, a, s); break;
     default: throw Error{RFHIP_EUNSUPPORTED, "order not compiled (Np must be 4, 10, 20 or 35)"};
   }
   RF_HIP(hipGetLastError());
@@ -256,19 +495,20 @@ void fused_dispatch(const rfhip_op *op, const FusedIO &io, double a, double b, d
 
 }  // namespace
 
-void launch_fused_lserk(const rfhip_op *op, const FusedIO &io, double a, double b, double dt, hipStream_t s) {
-  if (op->form == RFHIP_FORM_FUSED_F64C)
-    fused_dispatch<true, 2>(op, io, a, b, dt, s);
+void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s) {
+  const int np = (order + 1) * (order + 2) * (order + 3) / 6;
+  int variant = a.variant;
+  if (g_variant_override < 0) {
+    const char *env = getenv("RFHIP_FUSED_VARIANT");
+    g_variant_override = env ? atoi(env) : 99;
+  }
+  if (g_variant_override != 99) variant = g_variant_override;
+  if (epi == 0)
+    fused_dispatch<false, 0>(np, variant, a, s);
+  else if (form == RFHIP_FORM_FUSED_F64C)
+    fused_dispatch<true, 2>(np, variant, a, s);
   else
-    fused_dispatch<false, 1>(op, io, a, b, dt, s);
-}
-
-void launch_fused_apply(const rfhip_op *op, const double *y, double *dy, hipStream_t s) {
-  FusedIO io{};
-  io.yin = y;
-  io.dy = dy;
-  io.src_kind = RFHIP_DRIVE_FREE;
-  fused_dispatch<false, 0>(op, io, 0.0, 0.0, 0.0, s);
+    fused_dispatch<false, 1>(np, variant, a, s);
 }
 
 }  // namespace rfhip

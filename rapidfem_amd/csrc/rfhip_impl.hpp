@@ -80,31 +80,42 @@ void upload(DevBuf &b, const T *host, size_t count, hipStream_t s) {
 // copies of the shared triangle (see the neighbour-node table).
 constexpr int kCodePec = -1;
 
-struct Stage {
-  double a, b;
-};
-
-// Launch parameters and device pointers of one operator, passed by value.
+// Device pointers of the plain form, passed by value.
 struct DevOp {
   int np, nfp;
   int64_t n_elem;
   double alpha;
-  // plain form (precision of the form)
-  const void *dr, *ds, *dt, *lift;  // Np*Np, Np*4Nfp (row-major)
+  const void *dr, *ds, *dt, *lift;  // Np*Np, Np*4Nfp (row-major), form precision
   const int *face_nodes;            // 4*Nfp
   const void *jinv, *inv_eps, *inv_mu, *sig_eps, *sig_mu, *normal, *fscale;
   const int *nbr, *nbr_lf, *port, *perm;
-  // fused forms (FP64)
-  const double *dT;        // 3*Np*Np, transposed: dT[(k*Np + j)*Np + i] = D_k[i][j]
-  const double *liftT;     // 4Nfp*Np: liftT[fm*Np + i] = LIFT[i][fm]
-  const double *jinv64;    // 9K
-  const int *fcode;        // 4K
-  const int *mat;          // K, index into the material table
-  const double *mat_tab;   // n_mat*12: inv_eps3, inv_mu3, sig_eps3, sig_mu3
-  const int *nbr_node;     // 4 (nlf) * 6 (orientation) * Nfp: neighbour volume node
 };
 
-struct rfop_state;
+// Arguments of the fused stage kernel (rfhip_fused.hip), passed by value.
+// Vectors use the reference's AoS layout y[(e*Np + node)*6 + comp].
+struct StageArgs {
+  int64_t n_elem;
+  double alpha;
+  int variant;            // kernel variant (rfhip_fused.hip)
+  const double *dT;       // 3*Np*Np: dT[(k*Np + j)*Np + i] = D_k[i][j]
+  const double *lift;     // LIFT column-major: lift[fm*Np + i]
+  const double *jinv;     // 9K
+  const int *fcode;       // 4K
+  const int *mat;         // K
+  const double *mat_tab;  // n_mat * 12: inv_eps3, inv_mu3, sig_eps3, sig_mu3
+  const int *face_nodes;  // 4*Nfp
+  const int *nbr_node;    // [(f*4 + nlf)*6 + orientation][Nfp] -> neighbour volume node
+  const void *yin;        // double, or float hi (compressed)
+  const void *yin_lo;     // bf16 lo (compressed) or null
+  void *yout, *yout_lo;
+  void *p;                // double, or float (compressed)
+  double *dy;             // apply output
+  double sa, sb, dt;      // stage coefficients
+  int src_kind;
+  int64_t src_dof;
+  const double *src_vec;
+  double g;
+};
 
 }  // namespace rfhip
 
@@ -125,16 +136,18 @@ struct rfhip_op {
   // device operator data
   rfhip::DevBuf dr, ds, dt, lift, face_nodes, jinv, inv_eps, inv_mu, sig_eps, sig_mu, normal, fscale, nbr,
       nbr_lf, port, perm;
-  rfhip::DevBuf dT, liftT, jinv64, fcode, mat, mat_tab, nbr_node;
+  rfhip::DevBuf dT, liftc, jinv64, fcode, mat, mat_tab, nbr_node;
+  int64_t n_vec = 0;  // length of device vectors (n_dof)
+  int order = 0;
   // state registers (precision depends on form)
   rfhip::DevBuf y[2], ylo[2], p, k, src;
   int cur = 0;  // index of the current state buffer for fused forms
   // KCL / Krylov / scratch buffers
-  rfhip::DevBuf kst, kdtf, kerr, kbak, red, red_host_dummy;
-  rfhip::DevBuf kb_basis, kb_w, kb_proj, kb_part, kb_out, kb_src, kb_x;
+  rfhip::DevBuf kst, kdtf, kerr, kbak, red, aos;
+  rfhip::DevBuf kb_basis, kb_w, kb_proj, kb_part, kb_out, kb_x;
   int kb_cap = 0;
   rfhip::DevOp dev{};
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  rfhip::StageArgs sa{};  // fused forms: geometry part filled at creation
 };
 
 namespace rfhip {
@@ -146,21 +159,9 @@ void launch_plain_stage(const rfhip_op *op, void *p, const void *k, void *y, dou
 void launch_add_point(const rfhip_op *op, void *k, int64_t dof, double val, hipStream_t s);
 void launch_add_vec(const rfhip_op *op, void *k, const void *src, double g, hipStream_t s);
 
-// Fused forms. y_in/y_out: FP64 AoS (F64) or hi/lo pairs (F64C).
-struct FusedIO {
-  const void *yin;
-  const void *yin_lo;  // bf16 lo part (F64C), else null
-  void *yout;
-  void *yout_lo;
-  void *p;             // double (F64) or float (F64C)
-  double *dy;          // apply-only output
-  int src_kind;
-  int64_t src_dof;
-  const double *src_vec;
-  double g;
-};
-void launch_fused_lserk(const rfhip_op *op, const FusedIO &io, double a, double b, double dt, hipStream_t s);
-void launch_fused_apply(const rfhip_op *op, const double *y, double *dy, hipStream_t s);
+// Fused forms: epi 0 = apply (dy), 1 = LSERK stage (FP64 or compressed
+// registers by form).
+void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s);
 
 // Conversions between boundary AoS and the form's storage.
 void launch_f64_to_c(const double *src, float *hi, uint16_t *lo, int64_t n, hipStream_t s);

@@ -198,6 +198,23 @@ class GpuOperator:
         self.last_dim = dim.value
         return out
 
+    # ---- kernel variants (no reference counterpart) ----
+    def n_variants(self) -> int:
+        return int(L.lib.rfhip_op_n_variants(self._h))
+
+    def autotune(self) -> tuple[int, float]:
+        """Time every kernel variant on this operator and keep the fastest."""
+        v, ms = C.c_int(), C.c_double()
+        L.check(L.lib.rfhip_op_autotune(self._h, C.byref(v), C.byref(ms)))
+        return v.value, ms.value
+
+    def set_variant(self, v: int) -> None:
+        L.check(L.lib.rfhip_op_set_variant(self._h, int(v)))
+
+    @property
+    def variant(self) -> int:
+        return int(L.lib.rfhip_op_variant(self._h))
+
     # ---- device-resident state (no reference counterpart) ----
     def state_load(self, y0):
         L.check(L.lib.rfhip_op_state_load(self._h, self._vec(y0, np.float64)))

@@ -58,3 +58,202 @@ def test_apply_matches_oracle(ctx, oracle, order, form):
     err = rel_l2(got, want)
     print(f"apply order={order} form={form} rel_l2={err:.3e}")
     assert err < tol
+
+
+def _te101(oracle, n=3, order=2, jitter=True, alpha=1.0):
+    xyz, tets = _mesh(oracle, n, jitter)
+    oop = oracle.Operator(xyz, tets, order, alpha)
+    from rapidfem_amd import maxwell
+    mesh = maxwell.Mesh(xyz, tets)
+    x = maxwell.MaxwellOperator(mesh, order, alpha).node_coords()
+    y = np.zeros((oop.K, x.shape[1], 6))
+    y[..., 1] = np.sin(np.pi * x[..., 0]) * np.sin(np.pi * x[..., 2])
+    y[..., 3] = 0.3 * np.sin(2 * np.pi * x[..., 1])  # not an eigenmode: exercise all terms
+    return oop, y.reshape(-1)
+
+
+@pytest.mark.parametrize("form", ALL_FORMS)
+@pytest.mark.parametrize("drive", ["free", "point", "vector"])
+def test_lserk_transient_matches_oracle(ctx, oracle, form, drive):
+    """GpuOperator::transient (gpu/operator.rs:906-953) vs the oracle's LSERK4 run
+    (explicit.rs:100-135): 3 frames x 4 substeps, held source per substep."""
+    from rapidfem_amd import Drive
+
+    oop, y0 = _te101(oracle, 3, 2)
+    g = _gop(ctx, oop, form)
+    n = oop.n_dof
+    steps, sub, dt = 3, 4, 0.08
+    h = dt / sub
+    wave = np.cos(0.7 * np.arange(steps * sub))
+    if drive == "free":
+        d, od, gg = Drive.free(), None, None
+    elif drive == "point":
+        d, od, gg = Drive.point(n // 3 + 1), ("point", n // 3 + 1), wave
+    else:
+        b = np.sin(np.arange(n) * 0.01)
+        d, od, gg = Drive.vector(b), ("vector", b), wave
+    prec = "f32" if form == "plain_f32" else "f64"
+    traj = g.transient(ctx, y0, dt, steps, sub, d, gg, True, precision=prec)
+    want = [y0]
+    y = y0
+    for k in range(steps):
+        y = oop.lserk(y, h, sub, od, None if gg is None else gg[k * sub:(k + 1) * sub])
+        want.append(y)
+    tol = {"plain_f32": 1e-3, "plain_f64": 1e-12, "fused_f64": 1e-12, "fused_f64c": 1e-7}[form]
+    for k in range(steps + 1):
+        err = rel_l2(traj[k], want[k])
+        print(f"transient form={form} drive={drive} frame={k} rel_l2={err:.3e}")
+        assert err < tol
+
+
+@pytest.mark.parametrize("form", FORMS64)
+def test_kcl_adaptive_matches_oracle(ctx, oracle, form):
+    """GpuOperator::transient_kcl vs the oracle's KCL + PI controller: same
+    accept/reject counts and states (session.rs:781-827)."""
+    from rapidfem_amd import Controller
+
+    oop, y0 = _te101(oracle, 3, 2)
+    g = _gop(ctx, oop, form)
+    dt, steps = 0.05, 3
+    ctl = Controller(atol=1e-8, rtol=1e-6)
+    traj, st = g.transient_kcl(ctx, y0, dt, steps, ctl=ctl, precision="f64")
+    octl = oracle.Controller.default()
+    octl.atol, octl.rtol = 1e-8, 1e-6
+    y, acc, rej, hmin, hmax = oop.adaptive(y0, dt, steps, octl)
+    err = rel_l2(traj[-1], y)
+    print(f"kcl form={form}: gpu acc/rej {st.accepted}/{st.rejected}, oracle {acc}/{rej}, rel_l2 {err:.3e}")
+    assert (st.accepted, st.rejected) == (acc, rej)
+    assert err < (1e-7 if form == "fused_f64c" else 1e-11)
+    assert abs(st.h_min - hmin) <= 1e-9 * hmin and abs(st.h_max - hmax) <= 1e-9 * hmax
+
+
+def test_kcl_collapse_is_an_error(ctx, oracle):
+    """A step collapse is reported (session.rs:817-824), not a hang."""
+    from rapidfem_amd import Controller, RfhipError
+
+    oop, y0 = _te101(oracle, 2, 2)
+    g = _gop(ctx, oop, "fused_f64")
+    ctl = Controller(atol=1e-30, rtol=1e-30, min_step_factor=0.5)
+    with pytest.raises(RfhipError) as ei:
+        g.transient_kcl(ctx, y0, 0.05, 2, ctl=ctl, precision="f64")
+    assert ei.value.code == 5
+
+
+@pytest.mark.parametrize("form", ["plain_f32", "plain_f64", "fused_f64"])
+@pytest.mark.parametrize("tol", [0.0, 1e-10])
+def test_expmv_matches_oracle(ctx, oracle, form, tol):
+    """GpuOperator::expmv vs propagator.rs expmv_into (CGS2 Arnoldi, Taylor-18 expm)."""
+    oop, y0 = _te101(oracle, 3, 2)
+    g = _gop(ctx, oop, form)
+    want, wdim = oop.expmv(y0, 0.3, 30, tol)
+    got = g.expmv(ctx, y0, 0.3, 30, tol)
+    err = rel_l2(got, want)
+    print(f"expmv form={form} tol={tol}: dim gpu {g.last_dim} oracle {wdim} rel_l2 {err:.3e}")
+    assert g.last_dim == wdim
+    assert err < (1e-3 if form == "plain_f32" else 1e-10)
+
+
+@pytest.mark.parametrize("form", ["plain_f64", "fused_f64"])
+def test_etd_matches_oracle(ctx, oracle, form):
+    """GpuOperator::etd_step vs propagator.rs etd_step_into (augmented Arnoldi)."""
+    oop, y0 = _te101(oracle, 3, 2)
+    g = _gop(ctx, oop, form)
+    b = np.zeros(oop.n_dof)
+    b[oop.n_dof // 2] = 1.0
+    b[100:160] = 0.1
+    want = oop.etd_step(y0, b, 0.2, 25, 0.0)
+    got = g.etd_step(ctx, y0, b, 0.2, 25, 0.0)
+    err = rel_l2(got, want)
+    print(f"etd form={form}: rel_l2 {err:.3e}")
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("form", FORMS64)
+def test_energy_matches_oracle_and_central_flux_conserves(ctx, oracle, form):
+    """field_energy (rhs.rs:1509) of the resident state matches the oracle
+    before and after 200 LSERK4 steps; with the central flux (M~A skew,
+    rhs.rs:1830) the discrete energy only moves by LSERK4's own O(dt^6)
+    dissipation, far below the upwind flux's loss."""
+    oop, y0 = _te101(oracle, 3, 2, alpha=0.0)
+    g = _gop(ctx, oop, form)
+    g.state_load(y0)
+    e0 = g.state_energy()
+    assert abs(e0 - oop.field_energy(y0)) <= 1e-12 * e0
+    dt = 0.3 * oop.cfl_dt()
+    g.state_lserk(dt, 200)
+    e1 = g.state_energy()
+    e1_or = oop.field_energy(oop.lserk(y0, dt, 200))
+    drift = abs(e1 - e0) / e0
+    g.state_load(y0)
+    g.state_lserk(dt / 2, 400)
+    drift_half = abs(g.state_energy() - e0) / e0
+    print(f"energy form={form}: E0={e0:.12e} E200={e1:.12e} oracle {e1_or:.12e} drift={drift:.3e} "
+          f"drift(dt/2)={drift_half:.3e}")
+    assert abs(e1 - e1_or) <= (1e-8 if form == "fused_f64c" else 1e-11) * e0
+    # A non-skew surface term would leave an O(T) energy change independent of dt.
+    assert drift_half < drift / 16
+    oup, _ = _te101(oracle, 3, 2, alpha=1.0)
+    gu = _gop(ctx, oup, form)
+    gu.state_load(y0)
+    gu.state_lserk(dt, 200)
+    assert (e0 - gu.state_energy()) / e0 > 10 * drift
+
+
+def test_errors_are_status_codes(ctx, oracle):
+    """Length mismatches and bad drives are EINVAL, never a crash
+    (gpu/operator.rs:518,552 asserts become status codes)."""
+    from rapidfem_amd import Drive, RfhipError
+
+    oop, y0 = _te101(oracle, 2, 2)
+    g = _gop(ctx, oop, "fused_f64")
+    with pytest.raises(ValueError):
+        g.apply_f64(y0[:-1])
+    with pytest.raises(RfhipError) as ei:
+        g.transient(ctx, y0, 0.01, 2, 1, Drive.point(oop.n_dof + 5), np.ones(2), precision="f64")
+    assert ei.value.code == 1
+    with pytest.raises(RfhipError) as ei:
+        g.transient(ctx, y0, 0.01, 2, 1, Drive.point(3), np.ones(5), precision="f64")
+    assert ei.value.code == 1
+
+
+def test_single_element_and_ports(ctx, oracle):
+    """K=1 (every face PEC) and characteristic port faces (rhs.rs:1380-1386)."""
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    tets = np.array([[0, 1, 2, 3]], np.int64)
+    for port in (None, np.array([0, -1, 1, -1], np.int32)):
+        oop = oracle.Operator(xyz, tets, 3, 1.0, face_port=port)
+        y = np.cos(np.arange(oop.n_dof) * 0.37)
+        want = oop.apply(y)
+        for form in FORMS64:
+            got = _gop(ctx, oop, form).apply_f64(y)
+            assert rel_l2(got, want) < 1e-12, (form, port)
+    xyz, tets = _mesh(oracle, 3)
+    K = tets.shape[0]
+    fp = -np.ones((K, 4), np.int32)
+    oop0 = oracle.Operator(xyz, tets, 2, 1.0)
+    ex = oop0.export()
+    fp[ex["nbr"] < 0] = 0  # every boundary face is port 0 (absorbing box)
+    oop = oracle.Operator(xyz, tets, 2, 1.0, face_port=fp.reshape(-1))
+    y = np.cos(np.arange(oop.n_dof) * 0.11)
+    want = oop.apply(y)
+    for form in FORMS64:
+        got = _gop(ctx, oop, form).apply_f64(y)
+        assert rel_l2(got, want) < 1e-12, form
+
+
+def test_product_builder_operator_matches_oracle(ctx, oracle):
+    """The package's own host builder + fused kernel vs the oracle on the
+    config-1 cavity (6x6x6 jittered, N=2, TE101), 50 LSERK4 steps."""
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+
+    op, y0 = W.cavity_te101(6, 2)
+    oop = oracle.Operator(op.mesh.nodes, op.mesh.tets, 2, 1.0)
+    dt = 0.016
+    want = oop.lserk(y0, dt, 50)
+    for form in FORMS64:
+        g = R.GpuOperator(ctx, op, form)
+        got = g.transient(ctx, y0, dt, 50, 1, record=False, precision="f64")
+        err = rel_l2(got, want)
+        print(f"builder+{form}: rel_l2 {err:.3e}")
+        assert err < (1e-7 if form == "fused_f64c" else 1e-11)
