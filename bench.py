@@ -76,7 +76,7 @@ def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(the multi-GPU sharding is a design in DESIGN.md, not built)"}))
+                                   "(the element-slab sharding across GPUs is designed, not built)"}))
         return 2
     import numpy as np
 
