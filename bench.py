@@ -48,7 +48,39 @@ def parse():
     ap.add_argument("--form", default="fused_f64c")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--autotune", type=int, default=1, help="pick the fastest kernel variant on this mesh")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="counter-profiling run: a 1 GiB streaming copy (torch.neg, 8 B read + 8 B written "
+                         "per element) as the calibration kernel, then 10 LSERK4 steps of the chosen form at a "
+                         "fixed dt (no CFL search), for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+def traffic_probe(args) -> int:
+    import torch
+
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+
+    n_copy = 1 << 27  # 2^27 f64 = 1 GiB read + 1 GiB written per launch
+    a = torch.ones(n_copy, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    del a, b
+    nx, ny, nz = args.cells
+    op, y0 = W.pulse_box(nx, ny, nz, args.order)
+    ctx = R.GpuContext()
+    g = R.GpuOperator(ctx, op, args.form)
+    if args.autotune:
+        g.autotune()
+    g.state_load(y0)
+    g.state_lserk(0.002, 10)
+    print(json.dumps({"probe": "traffic", "copy_bytes_read": 8 * n_copy, "copy_bytes_written": 8 * n_copy,
+                      "form": args.form, "variant": g.variant, "n_dof": op.n_dof, "stage_launches": 50,
+                      "algorithmic_bytes_per_launch": W.bytes_per_dof_stage(args.order, args.form) * op.n_dof}))
+    return 0
 
 
 def cpu_baseline(op, y0, dt, steps):
@@ -78,6 +110,8 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(the element-slab sharding across GPUs is designed, not built)"}))
         return 2
+    if args.traffic_probe:
+        return traffic_probe(args)
     import numpy as np
 
     import rapidfem_amd as R
@@ -103,6 +137,10 @@ def main() -> int:
     y_ref = ref.state_store()
 
     chosen = R.GpuOperator(ctx, op, args.form)
+    variant = 0
+    if args.autotune:
+        variant, tune_ms = chosen.autotune()
+        log(f"autotune: variant {variant} of {chosen.n_variants()} ({tune_ms:.4f} ms/stage)")
     chosen.state_load(y0)
     chosen.state_lserk(dt, Wm)
     t0 = time.perf_counter()
@@ -162,6 +200,7 @@ def main() -> int:
         "config": {"workload": f"pulse_box {nx}x{ny}x{nz} cells x 6 tets (2x1x1 box), N={args.order}, eps_r=4 "
                                f"block, Gaussian E_x, PEC, upwind, LSERK4 at cfl_dt",
                    "n_elem": op.n_elem, "n_dof": n, "order": args.order, "dt": dt, "form": args.form,
+                   "variant": variant,
                    "registers": "y f32+bf16, p f32" if args.form == "fused_f64c" else "f64"},
         "plain": {"form": "plain_f64", "value": pvalue, "ms_per_step": ptot / args.plain_steps},
         "fused_f64": {"value": fvalue},

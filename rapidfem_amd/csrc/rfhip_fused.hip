@@ -53,6 +53,31 @@ __device__ __forceinline__ void cross3(const double *a, const double *b, double 
   o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+// The six field values of one node starting at element index nb (a multiple
+// of 6): 8-byte f32 pairs + 4-byte bf16 pairs, or 16-byte f64 pairs.
+template <bool CIN>
+__device__ __forceinline__ void load_trace(const StageArgs &io, int64_t nb, double *v) {
+  if constexpr (CIN) {
+    const float2 *hp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + nb);
+    const uint32_t *lp = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + nb);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float2 h = hp[q];
+      const uint32_t l = lp[q];
+      v[2 * q] = (double)h.x + (double)__uint_as_float(l << 16);
+      v[2 * q + 1] = (double)h.y + (double)__uint_as_float(l & 0xFFFF0000u);
+    }
+  } else {
+    const double2 *dp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + nb);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double2 w = dp[q];
+      v[2 * q] = w.x;
+      v[2 * q + 1] = w.y;
+    }
+  }
+}
+
 // EPI: 0 = apply (dy = A y, FP64), 1 = LSERK stage with FP64 registers,
 // 2 = LSERK stage with compressed registers (y f32+bf16, p f32).
 template <int NP, int NFP, bool CIN, int EPI>
@@ -454,30 +479,299 @@ __global__ void __launch_bounds__(kFusedWG) k_fused2(StageArgs io) {
   }
 }
 
+// Latency-oriented variant of k_fused (same thread mapping):
+//  * the face codes and neighbour-node indices of a thread's face nodes are
+//    resolved before anything else (they are a chain of dependent loads),
+//    and with PREV the neighbour traces themselves are requested too, so
+//    their latency overlaps the state load and the volume contraction;
+//  * the state is moved into LDS with 8/16-byte vector loads;
+//  * the unit normal and 0.5*fscale of each face are computed once per
+//    element (four threads) into LDS, not once per face node;
+//  * DLDS: D_k^T and LIFT^T live in LDS and the block walks over element
+//    groups (grid-stride), so the operator matrices are read from L2 once
+//    per block instead of through L1 by every wave.
+template <int NP, int NFP, bool CIN, int EPI, bool DLDS, bool PREV>
+__global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
+  constexpr int EPB = kFusedWG / NP;
+  constexpr int NF4 = 4 * NFP;
+  constexpr int FPT = (NF4 + NP - 1) / NP;  // face nodes per thread
+  constexpr int ND = 3 * NP * NP;
+  constexpr int SD = DLDS ? ND + NF4 * NP : 1;
+  __shared__ __align__(16) double su[EPB * NP * 6];
+  __shared__ __align__(16) double sf[EPB * NF4 * 6];
+  __shared__ __align__(16) double sg[EPB * 16];  // per face: n_x, n_y, n_z, 0.5*fscale
+  __shared__ __align__(16) double sd[SD];
+  const double *dT = io.dT;
+  const double *lift = io.lift;
+  if constexpr (DLDS) {
+    for (int q = threadIdx.x; q < SD; q += kFusedWG) sd[q] = q < ND ? io.dT[q] : io.lift[q - ND];
+    dT = sd;
+    lift = sd + ND;
+  }
+  const int tid = threadIdx.x;
+  const int le = tid / NP, i = tid - (tid / NP) * NP;
+  const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int64_t e0 = grp * EPB;
+    const int64_t rem = io.n_elem - e0;
+    const int nel = rem < EPB ? (int)rem : EPB;
+    const bool active = tid < nel * NP;
+    const int64_t e = e0 + le;
+
+    // 1. neighbour addresses (and with PREV the traces) of this thread's
+    //    face nodes, in flight early.
+    double nv[FPT][6];
+    int64_t nbase[FPT];
+    int fcodes[FPT];
+    if (active) {
+#pragma unroll
+      for (int s = 0; s < FPT; ++s) {
+        const int fm = i + s * NP;
+        fcodes[s] = kCodePec;
+        nbase[s] = 0;
+        if (fm >= NF4) continue;
+        const int f = fm / NFP, m = fm - f * NFP;
+        const int code = io.fcode[e * 4 + f];
+        fcodes[s] = code;
+        if (code < 0) continue;
+        const int64_t nb = code >> 5;
+        const int nlf = (code >> 3) & 3, o = code & 7;
+        const int vj = io.nbr_node[((f * 4 + nlf) * 6 + o) * NFP + m];
+        nbase[s] = (nb * NP + vj) * 6;
+        if constexpr (PREV) load_trace<CIN>(io, nbase[s], nv[s]);
+      }
+    }
+    // 2. the group's state into LDS, two values per lane and load.
+    {
+      const int64_t g0 = e0 * NP * 6;
+      const int cnt2 = nel * NP * 3;
+      for (int q = tid; q < cnt2; q += kFusedWG) {
+        double2 v;
+        if constexpr (CIN) {
+          const float2 h = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + g0)[q];
+          const uint32_t l = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + g0)[q];
+          v.x = (double)h.x + (double)__uint_as_float(l << 16);
+          v.y = (double)h.y + (double)__uint_as_float(l & 0xFFFF0000u);
+        } else {
+          v = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + g0)[q];
+        }
+        reinterpret_cast<double2 *>(su)[q] = v;
+      }
+    }
+    // 3. face geometry from J^-1: g_f = {-row2, -row0, -row1, row0+row1+row2},
+    //    n = g/|g|, 0.5*fscale = 0.5*|g|.
+    double ji[9];
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) ji[q] = io.jinv[e * 9 + q];
+      if (i < 4) {
+        double g[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          const double r0 = ji[x], r1 = ji[3 + x], r2 = ji[6 + x];
+          g[x] = i == 0 ? -r2 : i == 1 ? -r0 : i == 2 ? -r1 : (r0 + r1 + r2);
+        }
+        const double len = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        const double inv = 1.0 / len;
+        double *o = sg + le * 16 + i * 4;
+        o[0] = g[0] * inv;
+        o[1] = g[1] * inv;
+        o[2] = g[2] * inv;
+        o[3] = 0.5 * len;
+      }
+    }
+    __syncthreads();
+
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (active) {
+      const double *ub = su + le * NP * 6;
+      double r[3][6];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
+#pragma unroll 4
+      for (int j = 0; j < NP; ++j) {
+        double u[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) u[c] = ub[j * 6 + c];
+        const double d0 = dT[(0 * NP + j) * NP + i];
+        const double d1 = dT[(1 * NP + j) * NP + i];
+        const double d2 = dT[(2 * NP + j) * NP + i];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          r[0][c] = fma(d0, u[c], r[0][c]);
+          r[1][c] = fma(d1, u[c], r[1][c]);
+          r[2][c] = fma(d2, u[c], r[2][c]);
+        }
+      }
+      auto pd = [&](int p, int c) { return ji[p] * r[0][c] + ji[3 + p] * r[1][c] + ji[6 + p] * r[2][c]; };
+      acc[0] = pd(1, 5) - pd(2, 4);
+      acc[1] = pd(2, 3) - pd(0, 5);
+      acc[2] = pd(0, 4) - pd(1, 3);
+      acc[3] = -(pd(1, 2) - pd(2, 1));
+      acc[4] = -(pd(2, 0) - pd(0, 2));
+      acc[5] = -(pd(0, 1) - pd(1, 0));
+
+      // Upwind flux with unit normal n and c = 0.5*fscale:
+      //   fE = c(-n x [H] + a((n.[E])n - [E])),  fH = c(n x [E] + a((n.[H])n - [H])).
+      // PEC ([E] = 2E_t, [H] = 2(n.H)n): fE = -2ac(E - (n.E)n), fH = 2c n x E.
+#pragma unroll
+      for (int s = 0; s < FPT; ++s) {
+        const int fm = i + s * NP;
+        if (fm >= NF4) continue;
+        const int f = fm / NFP, m = fm - f * NFP;
+        const double *gq = sg + le * 16 + f * 4;
+        const double n0 = gq[0], n1 = gq[1], n2 = gq[2], cf = gq[3];
+        const int vi = io.face_nodes[f * NFP + m];
+        const double *uv = ub + vi * 6;
+        const double e0v = uv[0], e1v = uv[1], e2v = uv[2], h0 = uv[3], h1 = uv[4], h2 = uv[5];
+        double fe0, fe1, fe2, fh0, fh1, fh2;
+        const int code = fcodes[s];
+        if (code == kCodePec) {
+          const double ne = n0 * e0v + n1 * e1v + n2 * e2v;
+          const double ka = -2.0 * io.alpha * cf, k2 = 2.0 * cf;
+          fe0 = ka * (e0v - ne * n0);
+          fe1 = ka * (e1v - ne * n1);
+          fe2 = ka * (e2v - ne * n2);
+          fh0 = k2 * (n1 * e2v - n2 * e1v);
+          fh1 = k2 * (n2 * e0v - n0 * e2v);
+          fh2 = k2 * (n0 * e1v - n1 * e0v);
+        } else {
+          double je0, je1, je2, jh0, jh1, jh2, a;
+          if (code >= 0) {
+            if constexpr (!PREV) load_trace<CIN>(io, nbase[s], nv[s]);
+            je0 = e0v - nv[s][0];
+            je1 = e1v - nv[s][1];
+            je2 = e2v - nv[s][2];
+            jh0 = h0 - nv[s][3];
+            jh1 = h1 - nv[s][4];
+            jh2 = h2 - nv[s][5];
+            a = io.alpha;
+          } else {  // port: zero incident field, fully characteristic
+            je0 = e0v, je1 = e1v, je2 = e2v, jh0 = h0, jh1 = h1, jh2 = h2;
+            a = 1.0;
+          }
+          const double ne = n0 * je0 + n1 * je1 + n2 * je2;
+          const double nh = n0 * jh0 + n1 * jh1 + n2 * jh2;
+          fe0 = cf * (a * (ne * n0 - je0) - (n1 * jh2 - n2 * jh1));
+          fe1 = cf * (a * (ne * n1 - je1) - (n2 * jh0 - n0 * jh2));
+          fe2 = cf * (a * (ne * n2 - je2) - (n0 * jh1 - n1 * jh0));
+          fh0 = cf * (a * (nh * n0 - jh0) + (n1 * je2 - n2 * je1));
+          fh1 = cf * (a * (nh * n1 - jh1) + (n2 * je0 - n0 * je2));
+          fh2 = cf * (a * (nh * n2 - jh2) + (n0 * je1 - n1 * je0));
+        }
+        double2 *out = reinterpret_cast<double2 *>(sf + (le * NF4 + fm) * 6);
+        out[0] = make_double2(fe0, fe1);
+        out[1] = make_double2(fe2, fh0);
+        out[2] = make_double2(fh1, fh2);
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const double *fb = sf + le * NF4 * 6;
+#pragma unroll 4
+      for (int fm = 0; fm < NF4; ++fm) {
+        const double w = lift[fm * NP + i];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] = fma(w, fb[fm * 6 + c], acc[c]);
+      }
+      const double *mt = io.mat_tab + 12 * io.mat[e];
+      const double *ub = su + (le * NP + i) * 6;
+      double kv[6], own[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) own[c] = ub[c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        kv[c] = mt[c] * acc[c] - mt[6 + c] * own[c];
+        kv[3 + c] = mt[3 + c] * acc[3 + c] - mt[9 + c] * own[3 + c];
+      }
+      const int64_t base = (e * NP + i) * 6;
+      if (io.src_kind == RFHIP_DRIVE_POINT) {
+        const int64_t off = io.src_dof - base;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+          if (off == c) kv[c] += io.g;
+      } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) kv[c] += io.src_vec[base + c] * io.g;
+      }
+      if constexpr (EPI == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) io.dy[base + c] = kv[c];
+      } else if constexpr (EPI == 1) {
+        double *pp = static_cast<double *>(io.p);
+        double *yo = static_cast<double *>(io.yout);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          const double pn = io.sa * pp[base + c] + io.dt * kv[c];
+          pp[base + c] = pn;
+          yo[base + c] = own[c] + io.sb * pn;
+        }
+      } else {
+        float *pp = static_cast<float *>(io.p);
+        float *yh = static_cast<float *>(io.yout);
+        uint16_t *yl = static_cast<uint16_t *>(io.yout_lo);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          const float pn = (float)(io.sa * (double)pp[base + c] + io.dt * kv[c]);
+          pp[base + c] = pn;
+          const double yn = own[c] + io.sb * (double)pn;
+          const float h = (float)yn;
+          yh[base + c] = h;
+          yl[base + c] = bf16_rne((float)(yn - (double)h));
+        }
+      }
+    }
+    __syncthreads();  // su, sf and sg are rewritten by the next group
+  }
+}
+
 template <int NP>
 struct Nfp {
   static constexpr int v = NP == 4 ? 3 : NP == 10 ? 6 : NP == 20 ? 10 : 15;
 };
 
-// Kernel variant per launch: 0 = one node per thread, 1/2 = NB nodes per
-// thread (k_fused2). Chosen by the operator (autotuned or set).
+// Blocks of `kernel` resident on the whole device at once (grid-stride
+// launches use this many).
+inline int64_t resident_blocks(const void *kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
+  RF_HIP(hipGetDevice(&dev));
+  RF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  RF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kFusedWG, 0));
+  return (int64_t)(cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+}
+
+// Kernel variant per launch: 0 = k_fused (one node per thread), 1/2 =
+// k_fused2 with 2/1 nodes per thread, 3 = k_fused3 (early neighbour
+// indices), 4 = k_fused3 with early neighbour traces, 5 = k_fused3 with D
+// and LIFT in LDS over a grid-stride loop. Chosen by the
+// operator (autotuned or set).
 int g_variant_override = -1;
 
 template <int NP, bool CIN, int EPI>
 void fused_np(int variant, const StageArgs &a, hipStream_t s) {
   constexpr int F = Nfp<NP>::v;
+  constexpr int EPB1 = kFusedWG / NP;
+  const int64_t groups1 = (a.n_elem + EPB1 - 1) / EPB1;
   if (variant == 0) {
-    constexpr int EPB = kFusedWG / NP;
-    const int64_t groups = (a.n_elem + EPB - 1) / EPB;
-    hipLaunchKernelGGL((k_fused<NP, F, CIN, EPI>), dim3((unsigned)groups), dim3(kFusedWG), 0, s, a);
+    hipLaunchKernelGGL((k_fused<NP, F, CIN, EPI>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
   } else if (variant == 1) {
     constexpr int EPB = kFusedWG / ((NP + 1) / 2);
     const int64_t groups = (a.n_elem + EPB - 1) / EPB;
     hipLaunchKernelGGL((k_fused2<NP, F, 2, CIN, EPI>), dim3((unsigned)groups), dim3(kFusedWG), 0, s, a);
+  } else if (variant == 2) {
+    hipLaunchKernelGGL((k_fused2<NP, F, 1, CIN, EPI>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
+  } else if (variant == 3) {
+    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, false, false>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
+  } else if (variant == 4) {
+    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, false, true>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
   } else {
-    constexpr int EPB = kFusedWG / NP;
-    const int64_t groups = (a.n_elem + EPB - 1) / EPB;
-    hipLaunchKernelGGL((k_fused2<NP, F, 1, CIN, EPI>), dim3((unsigned)groups), dim3(kFusedWG), 0, s, a);
+    static int64_t resident = 0;
+    if (resident == 0)
+      resident = resident_blocks(reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false>));
+    const int64_t grid = groups1 < resident ? groups1 : resident;
+    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
   }
 }
 

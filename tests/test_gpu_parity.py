@@ -257,3 +257,58 @@ def test_product_builder_operator_matches_oracle(ctx, oracle):
         err = rel_l2(got, want)
         print(f"builder+{form}: rel_l2 {err:.3e}")
         assert err < (1e-7 if form == "fused_f64c" else 1e-11)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("form", ["fused_f64", "fused_f64c"])
+def test_every_kernel_variant_matches_oracle(ctx, oracle, order, form):
+    """Every fused kernel variant (the autotuner may pick any) against the
+    oracle: one operator apply and 6 LSERK4 steps with ports, PEC walls,
+    materials and conductivity."""
+    xyz, tets = _mesh(oracle, 3)
+    K = tets.shape[0]
+    eps = np.ones((K, 3))
+    eps[::3] = 4.0
+    sig = np.zeros(K)
+    sig[::5] = 0.3
+    port = np.full(4 * K, -1, np.int32)
+    port[0:40:4] = 0  # local face 0 of the first tets, kept only where it is a boundary face
+    oop = oracle.Operator(xyz, tets, order, 1.0, eps=eps, sigma=sig, face_port=port)
+    n = oop.n_dof
+    y = np.sin(0.3 + 0.017 * np.arange(n))
+    want_apply = oop.apply(y)
+    dt = 0.01
+    want = oop.lserk(y, dt, 6)
+    g = _gop(ctx, oop, form)
+    for v in range(g.n_variants()):
+        g.set_variant(v)
+        assert g.variant == v
+        e_apply = rel_l2(g.apply_f64(y), want_apply)
+        g.state_load(y)
+        g.state_lserk(dt, 6)
+        e_step = rel_l2(g.state_store(), want)
+        print(f"variant order={order} form={form} v={v} apply {e_apply:.3e} lserk {e_step:.3e}", flush=True)
+        assert e_apply < 1e-12
+        assert e_step < (1e-7 if form == "fused_f64c" else 1e-12)
+
+
+@pytest.mark.parametrize("form", ["fused_f64", "fused_f64c"])
+def test_variants_on_a_mesh_larger_than_the_resident_grid(ctx, oracle, form):
+    """18x18x18 cells x 6 = 34,992 tets at N=3: more element groups than
+    blocks resident on the device, so grid-stride variants loop."""
+    xyz, tets = oracle.box_mesh(18, 18, 18, 1.0, 1.0, 1.0, 0.1, 3, True)
+    oop = oracle.Operator(xyz, tets, 3, 1.0)
+    n = oop.n_dof
+    y = np.cos(0.1 + 0.013 * np.arange(n))
+    want_apply = oop.apply(y)
+    want = oop.lserk(y, 0.002, 3)
+    g = _gop(ctx, oop, form)
+    for v in range(g.n_variants()):
+        g.set_variant(v)
+        e_apply = rel_l2(g.apply_f64(y), want_apply)
+        g.state_load(y)
+        g.state_lserk(0.002, 3)
+        e_step = rel_l2(g.state_store(), want)
+        print(f"large-mesh variant form={form} v={v} apply {e_apply:.3e} lserk {e_step:.3e}", flush=True)
+        assert e_apply < 1e-12
+        assert e_step < (1e-7 if form == "fused_f64c" else 1e-12)
