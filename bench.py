@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--form", default="fused_f64c")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--autotune", type=int, default=1, help="pick the fastest kernel variant on this mesh")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="counter-profiling run: a 1 GiB streaming copy (torch.neg, 8 B read + 8 B written "

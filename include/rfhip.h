@@ -131,6 +131,20 @@ int rfhip_op_transient_f64(rfhip_op *op, const double *y0, double dt,
                            const double *g, int64_t n_g, int record,
                            double *out);
 
+/* Explicit transient that records probe DOFs on the device instead of
+ * whole states (session.rs:118-123 Record::Dofs; the reference GPU path
+ * downloads the full state every frame, session.rs:914-921, and picks the
+ * DOFs on the host). rows receives (steps+1)*n_probes values, row 0 taken
+ * from y0, one device-to-host copy at the end; y_final (may be NULL) the
+ * final state. Arguments otherwise as rfhip_op_transient_f64. */
+int rfhip_op_transient_probes_f64(rfhip_op *op, const double *y0, double dt,
+                                  int64_t steps, int64_t substeps,
+                                  int drive_kind, int64_t point_dof,
+                                  const double *src_vec, const double *g,
+                                  int64_t n_g, const int64_t *probes,
+                                  int64_t n_probes, double *rows,
+                                  double *y_final);
+
 /* session.rs:130-140 Controller, all f64. */
 typedef struct {
   double atol, rtol, safety, growth_limit, shrink_limit, pi_alpha, pi_beta,

@@ -74,6 +74,8 @@ SIGNATURES = {
                                      C.c_int, _fp]),
     "rfhip_op_transient_f64": (C.c_int, [_vp, _dp, _d, _i64, _i64, C.c_int, _i64, _vp, _vp, _i64,
                                          C.c_int, _dp]),
+    "rfhip_op_transient_probes_f64": (C.c_int, [_vp, _dp, _d, _i64, _i64, C.c_int, _i64, _vp, _vp, _i64, _lp, _i64,
+                                                _dp, _vp]),
     "rfhip_op_transient_kcl": (C.c_int, [_vp, _fp, C.c_float, _i64, C.c_int, _i64, _vp, _vp, _i64,
                                          C.POINTER(ControllerC), _fp, C.POINTER(KclStatsC)]),
     "rfhip_op_transient_kcl_f64": (C.c_int, [_vp, _dp, _d, _i64, C.c_int, _i64, _vp, _vp, _i64,

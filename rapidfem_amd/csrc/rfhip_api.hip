@@ -37,7 +37,7 @@ constexpr double kKclErrFloor = 1e-12;   // constants.rs KCL_ERR_FLOOR
 constexpr double kKclNonfinite = 10.0;   // constants.rs KCL_NONFINITE_ERR
 constexpr double kArnoldiBreakdown = 1e-12;  // constants.rs:62
 constexpr int kMaxKrylov = 64;
-constexpr int kFusedVariants = 6;  // rfhip_fused.hip launch_stage
+constexpr int kFusedVariants = 7;  // rfhip_fused.hip launch_stage
 
 template <class F>
 int guarded(F &&f) {
@@ -264,8 +264,20 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
       for (int i = 0; i < np; ++i) dT[(k * np + j) * np + i] = D[k][i * np + j];
   for (int fm = 0; fm < nf4; ++fm)
     for (int i = 0; i < np; ++i) liftc[(size_t)fm * np + i] = d.lift[(size_t)i * nf4 + fm];
+  // A-operand fragments of the stacked [Dr; Ds; Dt] (3Np x Np) for the FP64
+  // MFMA 16x16x4 volume kernel: afrag[(mt*KS + ks)*64 + lane] holds row
+  // s = 16 mt + lane%16 (= k*Np + i), column j = 4 ks + lane/16, zero padded.
+  const int mtiles = (3 * np + 15) / 16, ksteps = (np + 3) / 4;
+  std::vector<double> afrag((size_t)mtiles * ksteps * 64, 0.0);
+  for (int mt = 0; mt < mtiles; ++mt)
+    for (int ks = 0; ks < ksteps; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int srow = 16 * mt + l % 16, j = 4 * ks + l / 16;
+        if (srow < 3 * np && j < np) afrag[((size_t)mt * ksteps + ks) * 64 + l] = D[srow / np][(srow % np) * np + j];
+      }
   upload(op->dT, dT.data(), dT.size(), s);
   upload(op->liftc, liftc.data(), liftc.size(), s);
+  upload(op->afrag, afrag.data(), afrag.size(), s);
   upload(op->jinv64, d.jinv, 9 * (size_t)K, s);
   upload(op->fcode, fcode.data(), fcode.size(), s);
   upload(op->mat, mat.data(), mat.size(), s);
@@ -292,6 +304,7 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   a.alpha = d.flux_alpha;
   a.dT = op->dT.as<double>();
   a.lift = op->liftc.as<double>();
+  a.afrag = op->afrag.as<double>();
   a.jinv = op->jinv64.as<double>();
   a.fcode = op->fcode.as<int>();
   a.mat = op->mat.as<int>();
@@ -890,6 +903,40 @@ int rfhip_op_transient_f64(rfhip_op *op, const double *y0, double dt, int64_t st
       else
         state_to_host_f64(op, r < 0 ? out : out + r * n);
     });
+  });
+}
+
+int rfhip_op_transient_probes_f64(rfhip_op *op, const double *y0, double dt, int64_t steps, int64_t substeps,
+                                  int kind, int64_t dof, const double *src, const double *g, int64_t n_g,
+                                  const int64_t *probes, int64_t n_probes, double *rows, double *y_final) {
+  return guarded([&] {
+    RF_REQUIRE(op && y0 && rows, "null argument");
+    RF_REQUIRE(steps >= 0, "negative step count");
+    RF_REQUIRE(n_probes >= 0 && (n_probes == 0 || probes), "bad probe list");
+    for (int64_t q = 0; q < n_probes; ++q)
+      RF_REQUIRE(probes[q] >= 0 && probes[q] < op->n_dof, "probe dof out of range");
+    substeps = std::max<int64_t>(1, substeps);
+    check_drive(op, kind, dof, src, n_g, steps * substeps, g);
+    load_source(op, kind, src);
+    hipStream_t s = op->stream;
+    for (int64_t q = 0; q < n_probes; ++q) rows[q] = y0[probes[q]];
+    if (n_probes > 0) {
+      upload(op->probe_idx, probes, (size_t)n_probes, s);
+      op->probe_rows.alloc(8 * (size_t)std::max<int64_t>(1, steps) * (size_t)n_probes);
+    }
+    state_from_host_f64(op, y0);
+    transient_impl<double>(op, steps, substeps, dt, kind, dof, g, true, [&](int64_t r) {
+      if (r <= 0 || n_probes == 0) return;
+      const int cur = is_fused(op->form) ? op->cur : 0;
+      const int kind_y = op->form == RFHIP_FORM_FUSED_F64C ? 2 : op->form == RFHIP_FORM_PLAIN_F32 ? 1 : 0;
+      launch_gather(op->y[cur].p, op->ylo[cur].as<uint16_t>(), kind_y, op->probe_idx.as<int64_t>(), n_probes,
+                    op->probe_rows.as<double>() + (r - 1) * n_probes, s);
+    });
+    if (n_probes > 0 && steps > 0)
+      RF_HIP(hipMemcpyAsync(rows + n_probes, op->probe_rows.p, 8 * (size_t)steps * (size_t)n_probes,
+                            hipMemcpyDeviceToHost, s));
+    sync(op);
+    if (y_final) state_to_host_f64(op, y_final);
   });
 }
 

@@ -727,6 +727,288 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
   }
 }
 
+// ---- FP64 MFMA volume --------------------------------------------------
+//
+// k_fused4 moves the volume contraction onto the matrix cores: for one
+// field (H, then E) of a block of EPB elements,
+//   R[s][col] = sum_j [Dr; Ds; Dt][s][j] * U[j][col],  s = k*Np + i,
+//   col = 3*element + component,
+// is a (3Np x Np) x (Np x 3 EPB) product done with
+// v_mfma_f64_16x16x4f64 (A fragments precomputed on the host, B read from
+// the state in LDS). The products land in LDS as R[e][i][k][c]; each
+// (element, node) thread then applies J^-1 and forms the curl on the VALU,
+// which meanwhile also does the flux, LIFT and the stage update. Wave w of
+// the block takes output tiles w, w + waves, ...; every MFMA is executed by
+// whole waves (the tile loop is wave-uniform).
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+
+template <int NP>
+struct MfmaCfg {
+  static constexpr int EPB = NP == 35 ? 8 : 16;             // elements per block
+  static constexpr int NT = ((EPB * NP + 63) / 64) * 64;     // threads per block
+  static constexpr int MT = (3 * NP + 15) / 16;              // 16-row tiles of [Dr; Ds; Dt]
+  static constexpr int KS = (NP + 3) / 4;                    // k-steps of 4
+  static constexpr int CH = EPB * 3;                         // columns of one field
+  static constexpr int NTL = (CH + 15) / 16;                 // 16-column tiles
+};
+
+// Unit normal and 0.5*fscale of local face f from J^-1 (rows of ji):
+// g_f = {-row2, -row0, -row1, row0+row1+row2}, n = g/|g|, fscale = |g|.
+__device__ __forceinline__ void face_geom(const double *ji, int f, double *n, double &cf) {
+  double g[3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    const double r0 = ji[x], r1 = ji[3 + x], r2 = ji[6 + x];
+    g[x] = f == 0 ? -r2 : f == 1 ? -r0 : f == 2 ? -r1 : (r0 + r1 + r2);
+  }
+  const double len = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  const double inv = 1.0 / len;
+  n[0] = g[0] * inv;
+  n[1] = g[1] * inv;
+  n[2] = g[2] * inv;
+  cf = 0.5 * len;
+}
+
+// Upwind flux of one face node (rhs.rs:1372-1440) with unit normal n and
+// cf = 0.5*fscale, own trace u and neighbour trace v (interior faces):
+//   fE = cf(-n x [H] + a((n.[E])n - [E])),  fH = cf(n x [E] + a((n.[H])n - [H])),
+// PEC ([E] = 2E_t, [H] = 2(n.H)n): fE = -2a cf(E - (n.E)n), fH = 2cf n x E;
+// port: [.] = own trace, a = 1.
+__device__ __forceinline__ void face_flux(int code, double alpha, const double *n, double cf, const double *u,
+                                          const double *v, double *out) {
+  const double n0 = n[0], n1 = n[1], n2 = n[2];
+  if (code == kCodePec) {
+    const double ne = n0 * u[0] + n1 * u[1] + n2 * u[2];
+    const double ka = -2.0 * alpha * cf, k2 = 2.0 * cf;
+    out[0] = ka * (u[0] - ne * n0);
+    out[1] = ka * (u[1] - ne * n1);
+    out[2] = ka * (u[2] - ne * n2);
+    out[3] = k2 * (n1 * u[2] - n2 * u[1]);
+    out[4] = k2 * (n2 * u[0] - n0 * u[2]);
+    out[5] = k2 * (n0 * u[1] - n1 * u[0]);
+    return;
+  }
+  double j[6], a;
+  if (code >= 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) j[c] = u[c] - v[c];
+    a = alpha;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) j[c] = u[c];
+    a = 1.0;
+  }
+  const double ne = n0 * j[0] + n1 * j[1] + n2 * j[2];
+  const double nh = n0 * j[3] + n1 * j[4] + n2 * j[5];
+  out[0] = cf * (a * (ne * n0 - j[0]) - (n1 * j[5] - n2 * j[4]));
+  out[1] = cf * (a * (ne * n1 - j[1]) - (n2 * j[3] - n0 * j[5]));
+  out[2] = cf * (a * (ne * n2 - j[2]) - (n0 * j[4] - n1 * j[3]));
+  out[3] = cf * (a * (nh * n0 - j[3]) + (n1 * j[2] - n2 * j[1]));
+  out[4] = cf * (a * (nh * n1 - j[4]) + (n2 * j[0] - n0 * j[2]));
+  out[5] = cf * (a * (nh * n2 - j[5]) + (n0 * j[1] - n1 * j[0]));
+}
+
+// Materials, held source and the stage update of one node (rhs.rs:1443-1454,
+// explicit.rs:100-135): k = M^-1(acc) - sigma u (+ b g), then EPI.
+template <int EPI>
+__device__ __forceinline__ void node_epilogue(const StageArgs &io, int64_t base, const double *mt, const double *acc,
+                                              const double *own) {
+  double kv[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    kv[c] = mt[c] * acc[c] - mt[6 + c] * own[c];
+    kv[3 + c] = mt[3 + c] * acc[3 + c] - mt[9 + c] * own[3 + c];
+  }
+  if (io.src_kind == RFHIP_DRIVE_POINT) {
+    const int64_t off = io.src_dof - base;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (off == c) kv[c] += io.g;
+  } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) kv[c] += io.src_vec[base + c] * io.g;
+  }
+  if constexpr (EPI == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) io.dy[base + c] = kv[c];
+  } else if constexpr (EPI == 1) {
+    double *pp = static_cast<double *>(io.p);
+    double *yo = static_cast<double *>(io.yout);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double pn = io.sa * pp[base + c] + io.dt * kv[c];
+      pp[base + c] = pn;
+      yo[base + c] = own[c] + io.sb * pn;
+    }
+  } else {
+    float *pp = static_cast<float *>(io.p);
+    float *yh = static_cast<float *>(io.yout);
+    uint16_t *yl = static_cast<uint16_t *>(io.yout_lo);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const float pn = (float)(io.sa * (double)pp[base + c] + io.dt * kv[c]);
+      pp[base + c] = pn;
+      const double yn = own[c] + io.sb * (double)pn;
+      const float h = (float)yn;
+      yh[base + c] = h;
+      yl[base + c] = bf16_rne((float)(yn - (double)h));
+    }
+  }
+}
+
+template <int NP, int NFP, bool CIN, int EPI>
+__global__ void __launch_bounds__(MfmaCfg<NP>::NT) k_fused4(StageArgs io) {
+  using Cfg = MfmaCfg<NP>;
+  constexpr int EPB = Cfg::EPB, NT = Cfg::NT, MT = Cfg::MT, KS = Cfg::KS, NTL = Cfg::NTL;
+  constexpr int NWAVE = NT / 64;
+  constexpr int NF4 = 4 * NFP;
+  constexpr int FPT = (NF4 + NP - 1) / NP;  // face nodes per thread
+  constexpr int SU = EPB * NP * 6, SF = EPB * NF4 * 6;
+  __shared__ __align__(16) double sx[SU > SF ? SU : SF];  // the state, then the face flux
+  __shared__ __align__(16) double sr[EPB * NP * 9];        // R[e][i][k][c] of one field
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int le = tid / NP, i = tid - (tid / NP) * NP;
+  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int64_t rem = io.n_elem - e0;
+  const int nel = rem < EPB ? (int)rem : EPB;
+  const bool active = tid < nel * NP;
+  const int64_t e = e0 + le;
+
+  // Neighbour traces of this thread's face nodes: requested first.
+  double nv[FPT][6];
+  int fcodes[FPT];
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      fcodes[s] = kCodePec;
+      if (fm >= NF4) continue;
+      const int f = fm / NFP, m = fm - f * NFP;
+      const int code = io.fcode[e * 4 + f];
+      fcodes[s] = code;
+      if (code < 0) continue;
+      const int64_t nb = code >> 5;
+      const int nlf = (code >> 3) & 3, o = code & 7;
+      const int vj = io.nbr_node[((f * 4 + nlf) * 6 + o) * NFP + m];
+      load_trace<CIN>(io, (nb * NP + vj) * 6, nv[s]);
+    }
+  }
+  {
+    const int64_t g0 = e0 * NP * 6;
+    const int cnt2 = nel * NP * 3;
+    for (int q = tid; q < cnt2; q += NT) {
+      double2 v;
+      if constexpr (CIN) {
+        const float2 h = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + g0)[q];
+        const uint32_t l = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + g0)[q];
+        v.x = (double)h.x + (double)__uint_as_float(l << 16);
+        v.y = (double)h.y + (double)__uint_as_float(l & 0xFFFF0000u);
+      } else {
+        v = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + g0)[q];
+      }
+      reinterpret_cast<double2 *>(sx)[q] = v;
+    }
+  }
+  double ji[9];
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) ji[q] = io.jinv[e * 9 + q];
+  }
+  __syncthreads();
+
+  double own[6], acc[6], fl[FPT][6];
+  if (active) {
+    const double *ub = sx + le * NP * 6;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) own[c] = ub[i * 6 + c];
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      if (fm >= NF4) continue;
+      const int f = fm / NFP, m = fm - f * NFP;
+      double n[3], cf;
+      face_geom(ji, f, n, cf);
+      const int vi = io.face_nodes[f * NFP + m];
+      face_flux(fcodes[s], io.alpha, n, cf, ub + vi * 6, nv[s], fl[s]);
+    }
+  }
+
+  // Volume on the matrix cores: field H (half 0, gives dE), then E (dH).
+#pragma unroll 1
+  for (int h = 0; h < 2; ++h) {
+    const int cbase = h == 0 ? 3 : 0;
+    for (int t = wave; t < MT * NTL; t += NWAVE) {
+      const int mt = t / NTL, nt = t - (t / NTL) * NTL;
+      const int col = nt * 16 + (lane & 15);
+      const int ce = col / 3, cc = col - (col / 3) * 3;
+      const bool cv = col < nel * 3;
+      const double *ap = io.afrag + (size_t)mt * KS * 64 + lane;
+      mf_d4 cacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int j = ks * 4 + (lane >> 4);
+        const double a = ap[ks * 64];
+        const double b = (cv && j < NP) ? sx[(ce * NP + j) * 6 + cbase + cc] : 0.0;
+        cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, cacc, 0, 0, 0);
+      }
+      if (cv) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int srow = mt * 16 + (lane >> 4) * 4 + r;
+          if (srow < 3 * NP) {
+            const int k = srow / NP, ii = srow - (srow / NP) * NP;
+            sr[((ce * NP + ii) * 3 + k) * 3 + cc] = cacc[r];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const double *rb = sr + (le * NP + i) * 9;
+      double rv[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) rv[q] = rb[q];
+      // d(p, c) = d(field_c)/dx_p = sum_k J^-1[k][p] R_k[c]
+      auto dd = [&](int p, int c) { return ji[p] * rv[c] + ji[3 + p] * rv[3 + c] + ji[6 + p] * rv[6 + c]; };
+      if (h == 0) {
+        acc[0] = dd(1, 2) - dd(2, 1);
+        acc[1] = dd(2, 0) - dd(0, 2);
+        acc[2] = dd(0, 1) - dd(1, 0);
+      } else {
+        acc[3] = -(dd(1, 2) - dd(2, 1));
+        acc[4] = -(dd(2, 0) - dd(0, 2));
+        acc[5] = -(dd(0, 1) - dd(1, 0));
+      }
+    }
+    __syncthreads();  // sr is rewritten by the next field; after E the state is no longer read
+  }
+
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      if (fm >= NF4) continue;
+      double2 *out = reinterpret_cast<double2 *>(sx + (le * NF4 + fm) * 6);
+      out[0] = make_double2(fl[s][0], fl[s][1]);
+      out[1] = make_double2(fl[s][2], fl[s][3]);
+      out[2] = make_double2(fl[s][4], fl[s][5]);
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  {
+    const double *fb = sx + le * NF4 * 6;
+#pragma unroll 4
+    for (int fm = 0; fm < NF4; ++fm) {
+      const double w = io.lift[fm * NP + i];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc[c] = fma(w, fb[fm * 6 + c], acc[c]);
+    }
+  }
+  node_epilogue<EPI>(io, (e * NP + i) * 6, io.mat_tab + 12 * io.mat[e], acc, own);
+}
+
 template <int NP>
 struct Nfp {
   static constexpr int v = NP == 4 ? 3 : NP == 10 ? 6 : NP == 20 ? 10 : 15;
@@ -745,7 +1027,7 @@ inline int64_t resident_blocks(const void *kernel) {
 // Kernel variant per launch: 0 = k_fused (one node per thread), 1/2 =
 // k_fused2 with 2/1 nodes per thread, 3 = k_fused3 (early neighbour
 // indices), 4 = k_fused3 with early neighbour traces, 5 = k_fused3 with D
-// and LIFT in LDS over a grid-stride loop. Chosen by the
+// and LIFT in LDS over a grid-stride loop, 6 = k_fused4 (FP64 MFMA volume). Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -766,6 +1048,10 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
     hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, false, false>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
   } else if (variant == 4) {
     hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, false, true>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
+  } else if (variant == 6) {
+    using Cfg = MfmaCfg<NP>;
+    const int64_t groups = (a.n_elem + Cfg::EPB - 1) / Cfg::EPB;
+    hipLaunchKernelGGL((k_fused4<NP, F, CIN, EPI>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
   } else {
     static int64_t resident = 0;
     if (resident == 0)

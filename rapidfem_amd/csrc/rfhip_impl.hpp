@@ -99,6 +99,7 @@ struct StageArgs {
   int variant;            // kernel variant (rfhip_fused.hip)
   const double *dT;       // 3*Np*Np: dT[(k*Np + j)*Np + i] = D_k[i][j]
   const double *lift;     // LIFT column-major: lift[fm*Np + i]
+  const double *afrag;    // FP64 MFMA A fragments of [Dr; Ds; Dt] (rfhip_fused.hip k_fused4)
   const double *jinv;     // 9K
   const int *fcode;       // 4K
   const int *mat;         // K
@@ -136,7 +137,7 @@ struct rfhip_op {
   // device operator data
   rfhip::DevBuf dr, ds, dt, lift, face_nodes, jinv, inv_eps, inv_mu, sig_eps, sig_mu, normal, fscale, nbr,
       nbr_lf, port, perm;
-  rfhip::DevBuf dT, liftc, jinv64, fcode, mat, mat_tab, nbr_node;
+  rfhip::DevBuf dT, liftc, afrag, jinv64, fcode, mat, mat_tab, nbr_node;
   int64_t n_vec = 0;  // length of device vectors (n_dof)
   int order = 0;
   // state registers (precision depends on form)
@@ -145,6 +146,7 @@ struct rfhip_op {
   // KCL / Krylov / scratch buffers
   rfhip::DevBuf kst, kdtf, kerr, kbak, red, aos;
   rfhip::DevBuf kb_basis, kb_w, kb_proj, kb_part, kb_out, kb_x;
+  rfhip::DevBuf probe_idx, probe_rows;  // device-side probe recording
   int kb_cap = 0;
   rfhip::DevOp dev{};
   rfhip::StageArgs sa{};  // fused forms: geometry part filled at creation
@@ -169,6 +171,9 @@ void launch_c_to_f64(const float *hi, const uint16_t *lo, double *dst, int64_t n
 void launch_cast_d2f(const double *src, float *dst, int64_t n, hipStream_t s);
 void launch_cast_f2d(const float *src, double *dst, int64_t n, hipStream_t s);
 void launch_fill(void *dst, int64_t nbytes, hipStream_t s);
+// out[q] = y[idx[q]] as f64; kind 0 f64 state, 1 f32, 2 f32 hi + bf16 lo.
+void launch_gather(const void *y, const uint16_t *lo, int kind, const int64_t *idx, int64_t n, double *out,
+                   hipStream_t s);
 
 // KCL vector kernels on T = float or double (is_f64 selects).
 void launch_kcl_stage0(bool f64, void *y, void *dtf, void *err, const void *k, double dt, double b0, double e0,

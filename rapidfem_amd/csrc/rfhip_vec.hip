@@ -34,6 +34,24 @@ __global__ void __launch_bounds__(kBS) k_f64_to_c(const double *__restrict__ s, 
     lo[i] = bf16_rne((float)(v - (double)h));
   }
 }
+// Probe rows (session.rs:118-123 Record::Dofs, recorded on the device):
+// out[q] = y[idx[q]] for a state stored as f64 (kind 0), f32 (1) or
+// f32 hi + bf16 lo (2).
+__global__ void __launch_bounds__(kBS) k_gather(const void *__restrict__ y, const uint16_t *__restrict__ lo, int kind,
+                                                const int64_t *__restrict__ idx, int64_t n, double *__restrict__ out) {
+  for (int64_t q = (int64_t)blockIdx.x * kBS + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBS) {
+    const int64_t i = idx[q];
+    double v;
+    if (kind == 0)
+      v = static_cast<const double *>(y)[i];
+    else if (kind == 1)
+      v = (double)static_cast<const float *>(y)[i];
+    else
+      v = (double)static_cast<const float *>(y)[i] + (double)bf16_to_f(lo[i]);
+    out[q] = v;
+  }
+}
+
 __global__ void __launch_bounds__(kBS) k_c_to_f64(const float *__restrict__ hi, const uint16_t *__restrict__ lo,
                                                   double *__restrict__ d, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS)
@@ -247,6 +265,13 @@ void launch_cast_f2d(const float *src, double *dst, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_f2d, dim3(grid_for(n)), dim3(kBS), 0, s, src, dst, n);
   RF_HIP(hipGetLastError());
 }
+void launch_gather(const void *y, const uint16_t *lo, int kind, const int64_t *idx, int64_t n, double *out,
+                   hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 1024)), dim3(kBS), 0, s, y, lo, kind, idx, n, out);
+  RF_HIP(hipGetLastError());
+}
+
 void launch_fill(void *dst, int64_t nbytes, hipStream_t s) {
   const int64_t nw = nbytes / 4;
   hipLaunchKernelGGL(k_fill0, dim3(grid_for(nw)), dim3(kBS), 0, s, static_cast<uint32_t *>(dst), nw);

@@ -161,6 +161,23 @@ class GpuOperator:
                    n_g, 1 if record else 0, out))
         return out.reshape(steps + 1, self._n) if record else out
 
+    def transient_probes(self, ctx, y0, dt, steps, probes, substeps=1, drive: Drive | None = None, g=None):
+        """Explicit transient recording only the DOFs in `probes`
+        (session.rs:118-123 Record::Dofs) on the device: returns
+        ((steps+1, len(probes)) rows, final state), both f64."""
+        drive = drive or Drive.free()
+        y0 = self._vec(y0, np.float64)
+        substeps = max(1, int(substeps))
+        gg = None if g is None else np.ascontiguousarray(g, np.float64)
+        n_g = 0 if gg is None else gg.shape[0]
+        pr = np.ascontiguousarray(probes, np.int64)
+        rows = np.empty((steps + 1) * max(1, pr.shape[0]))
+        yf = np.empty(self._n)
+        L.check(L.lib.rfhip_op_transient_probes_f64(self._h, y0, float(dt), int(steps), substeps, drive.kind,
+                                                    drive.dof, L.ptr(drive.vec), L.ptr(gg), n_g, pr, pr.shape[0],
+                                                    rows, L.ptr(yf)))
+        return rows[:(steps + 1) * pr.shape[0]].reshape(steps + 1, pr.shape[0]), yf
+
     # gpu/operator.rs:823-893
     def transient_kcl(self, ctx, y0, dt, steps, drive: Drive | None = None, g=None, ctl: Controller | None = None,
                       precision="f32"):

@@ -90,7 +90,10 @@ class TdSession:
             y = self.gop.state_store()
             if not np.all(np.isfinite(y)):
                 return False
-            nf = float(np.sqrt(np.sum(y * y)))
+            with np.errstate(over="ignore", invalid="ignore"):
+                nf = float(np.sqrt(np.sum(y * y)))
+            if not math.isfinite(nf):
+                return False
             if nf > CFL_GROWTH_FACTOR * n0:
                 return False
             rate = (nf / n0) ** (1.0 / CFL_PROBE_STEPS) if nf > 0 else 0.0

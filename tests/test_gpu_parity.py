@@ -312,3 +312,39 @@ def test_variants_on_a_mesh_larger_than_the_resident_grid(ctx, oracle, form):
         print(f"large-mesh variant form={form} v={v} apply {e_apply:.3e} lserk {e_step:.3e}", flush=True)
         assert e_apply < 1e-12
         assert e_step < (1e-7 if form == "fused_f64c" else 1e-12)
+
+
+@pytest.mark.parametrize("form", ALL_FORMS)
+def test_probe_recording_matches_full_record(ctx, oracle, form):
+    """rfhip_op_transient_probes_f64 (probe rows gathered on the device)
+    equals the full-state trajectory at the probe DOFs, frame by frame, and
+    the final state equals the oracle's."""
+    from rapidfem_amd import Drive
+
+    oop, y0 = _te101(oracle, 3, 2)
+    g = _gop(ctx, oop, form)
+    n = oop.n_dof
+    steps, sub, dt = 4, 3, 0.06
+    wave = np.cos(0.5 * np.arange(steps * sub))
+    probes = np.array([0, 5, n // 2, n - 1, 17, 17], np.int64)
+    d = Drive.point(n // 3)
+    traj = g.transient(ctx, y0, dt, steps, sub, d, wave, True, precision="f64")
+    rows, yf = g.transient_probes(ctx, y0, dt, steps, probes, sub, d, wave)
+    assert rows.shape == (steps + 1, probes.size)
+    assert np.array_equal(rows, traj[:, probes])
+    assert np.array_equal(yf, traj[-1])
+    y = y0
+    for k in range(steps):
+        y = oop.lserk(y, dt / sub, sub, ("point", n // 3), wave[k * sub:(k + 1) * sub])
+    tol = {"plain_f32": 1e-3, "plain_f64": 1e-12, "fused_f64": 1e-12, "fused_f64c": 1e-7}[form]
+    assert rel_l2(yf, y) < tol
+
+
+def test_probe_recording_rejects_bad_dofs(ctx, oracle):
+    from rapidfem_amd import RfhipError
+
+    oop, y0 = _te101(oracle, 2, 1)
+    g = _gop(ctx, oop, "fused_f64")
+    with pytest.raises(RfhipError) as ei:
+        g.transient_probes(ctx, y0, 0.01, 2, [0, oop.n_dof])
+    assert ei.value.code == 1
