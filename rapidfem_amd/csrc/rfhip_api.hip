@@ -37,7 +37,7 @@ constexpr double kKclErrFloor = 1e-12;   // constants.rs KCL_ERR_FLOOR
 constexpr double kKclNonfinite = 10.0;   // constants.rs KCL_NONFINITE_ERR
 constexpr double kArnoldiBreakdown = 1e-12;  // constants.rs:62
 constexpr int kMaxKrylov = 64;
-constexpr int kFusedVariants = 7;  // rfhip_fused.hip launch_stage
+constexpr int kFusedVariants = 8;  // rfhip_fused.hip launch_stage
 
 template <class F>
 int guarded(F &&f) {
@@ -277,7 +277,18 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
       }
   upload(op->dT, dT.data(), dT.size(), s);
   upload(op->liftc, liftc.data(), liftc.size(), s);
+  // ... and of LIFT (Np x 4Nfp): lfrag[(mt*KSL + ks)*64 + lane] holds
+  // LIFT[16 mt + lane%16][4 ks + lane/16], zero padded.
+  const int ltiles = (np + 15) / 16, lsteps = (nf4 + 3) / 4;
+  std::vector<double> lfrag((size_t)ltiles * lsteps * 64, 0.0);
+  for (int mt = 0; mt < ltiles; ++mt)
+    for (int ks = 0; ks < lsteps; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int i = 16 * mt + l % 16, fm = 4 * ks + l / 16;
+        if (i < np && fm < nf4) lfrag[((size_t)mt * lsteps + ks) * 64 + l] = d.lift[(size_t)i * nf4 + fm];
+      }
   upload(op->afrag, afrag.data(), afrag.size(), s);
+  upload(op->lfrag, lfrag.data(), lfrag.size(), s);
   upload(op->jinv64, d.jinv, 9 * (size_t)K, s);
   upload(op->fcode, fcode.data(), fcode.size(), s);
   upload(op->mat, mat.data(), mat.size(), s);
@@ -305,6 +316,7 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   a.dT = op->dT.as<double>();
   a.lift = op->liftc.as<double>();
   a.afrag = op->afrag.as<double>();
+  a.lfrag = op->lfrag.as<double>();
   a.jinv = op->jinv64.as<double>();
   a.fcode = op->fcode.as<int>();
   a.mat = op->mat.as<int>();

@@ -739,7 +739,9 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
 // (element, node) thread then applies J^-1 and forms the curl on the VALU,
 // which meanwhile also does the flux, LIFT and the stage update. Wave w of
 // the block takes output tiles w, w + waves, ...; every MFMA is executed by
-// whole waves (the tile loop is wave-uniform).
+// whole waves (the tile loop is wave-uniform). Operand layout of
+// v_mfma_f64_16x16x4f64 (checked on gfx950): lane l supplies A[l%16][l/16]
+// and B[l/16][l%16] and receives C[4r + l/16][l%16] in element r.
 typedef double mf_d4 __attribute__((ext_vector_type(4)));
 
 template <int NP>
@@ -856,7 +858,7 @@ __device__ __forceinline__ void node_epilogue(const StageArgs &io, int64_t base,
   }
 }
 
-template <int NP, int NFP, bool CIN, int EPI>
+template <int NP, int NFP, bool CIN, int EPI, bool LMF>
 __global__ void __launch_bounds__(MfmaCfg<NP>::NT) k_fused4(StageArgs io) {
   using Cfg = MfmaCfg<NP>;
   constexpr int EPB = Cfg::EPB, NT = Cfg::NT, MT = Cfg::MT, KS = Cfg::KS, NTL = Cfg::NTL;
@@ -955,7 +957,7 @@ __global__ void __launch_bounds__(MfmaCfg<NP>::NT) k_fused4(StageArgs io) {
       if (cv) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int srow = mt * 16 + (lane >> 4) * 4 + r;
+          const int srow = mt * 16 + r * 4 + (lane >> 4);  // C[4r + lane/16][lane%16]
           if (srow < 3 * NP) {
             const int k = srow / NP, ii = srow - (srow / NP) * NP;
             sr[((ce * NP + ii) * 3 + k) * 3 + cc] = cacc[r];
@@ -996,8 +998,40 @@ __global__ void __launch_bounds__(MfmaCfg<NP>::NT) k_fused4(StageArgs io) {
     }
   }
   __syncthreads();
-  if (!active) return;
-  {
+  if constexpr (LMF) {
+    // LIFT on the matrix cores: L[i][col] = sum_fm LIFT[i][fm] F[fm][col],
+    // col = 6*element + component, F[e][fm][c] in LDS; L lands in sr as
+    // L[e][i][c].
+    constexpr int MTL = (NP + 15) / 16, KSL = (NF4 + 3) / 4, NTLL = (EPB * 6 + 15) / 16;
+    for (int t = wave; t < MTL * NTLL; t += NWAVE) {
+      const int mt = t / NTLL, nt = t - (t / NTLL) * NTLL;
+      const int col = nt * 16 + (lane & 15);
+      const int ce = col / 6, cc = col - (col / 6) * 6;
+      const bool cv = col < nel * 6;
+      const double *ap = io.lfrag + (size_t)mt * KSL * 64 + lane;
+      mf_d4 cacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < KSL; ++ks) {
+        const int fm = ks * 4 + (lane >> 4);
+        const double a = ap[ks * 64];
+        const double b = (cv && fm < NF4) ? sx[(ce * NF4 + fm) * 6 + cc] : 0.0;
+        cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, cacc, 0, 0, 0);
+      }
+      if (cv) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = mt * 16 + r * 4 + (lane >> 4);
+          if (row < NP) sr[(ce * NP + row) * 6 + cc] = cacc[r];
+        }
+      }
+    }
+    __syncthreads();
+    if (!active) return;
+    const double *lb = sr + (le * NP + i) * 6;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] += lb[c];
+  } else {
+    if (!active) return;
     const double *fb = sx + le * NF4 * 6;
 #pragma unroll 4
     for (int fm = 0; fm < NF4; ++fm) {
@@ -1027,7 +1061,8 @@ inline int64_t resident_blocks(const void *kernel) {
 // Kernel variant per launch: 0 = k_fused (one node per thread), 1/2 =
 // k_fused2 with 2/1 nodes per thread, 3 = k_fused3 (early neighbour
 // indices), 4 = k_fused3 with early neighbour traces, 5 = k_fused3 with D
-// and LIFT in LDS over a grid-stride loop, 6 = k_fused4 (FP64 MFMA volume). Chosen by the
+// and LIFT in LDS over a grid-stride loop, 6 = k_fused4 (FP64 MFMA volume),
+// 7 = k_fused4 with LIFT on MFMA too. Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -1048,10 +1083,13 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
     hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, false, false>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
   } else if (variant == 4) {
     hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, false, true>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
-  } else if (variant == 6) {
+  } else if (variant == 6 || variant == 7) {
     using Cfg = MfmaCfg<NP>;
     const int64_t groups = (a.n_elem + Cfg::EPB - 1) / Cfg::EPB;
-    hipLaunchKernelGGL((k_fused4<NP, F, CIN, EPI>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
+    if (variant == 6)
+      hipLaunchKernelGGL((k_fused4<NP, F, CIN, EPI, false>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_fused4<NP, F, CIN, EPI, true>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
   } else {
     static int64_t resident = 0;
     if (resident == 0)

@@ -54,24 +54,32 @@ def cavity_modes_field(op: MaxwellOperator, n_modes=16, seed=0, lengths=(1.0, 1.
     """
     rng = np.random.default_rng(seed)
     lx, ly, lz = lengths
-    x = op.node_coords()
-    y = np.zeros((op.n_elem, op.re.n_nodes, 6))
     cands = [(m, n, p) for m in range(0, 4) for n in range(0, 4) for p in range(0, 4)
              if (m > 0) + (n > 0) + (p > 0) >= 2]
     picks = rng.choice(len(cands), size=n_modes, replace=False)
+    modes = []
     for idx in picks:
         m, n, p = cands[idx]
         k = np.array([math.pi * m / lx, math.pi * n / ly, math.pi * p / lz])
         pol = rng.normal(size=3)
         pol -= k * (pol @ k) / (k @ k)
         pol /= np.linalg.norm(pol)
-        a = rng.normal()
-        cx, sx = np.cos(k[0] * x[..., 0]), np.sin(k[0] * x[..., 0])
-        cy, sy = np.cos(k[1] * x[..., 1]), np.sin(k[1] * x[..., 1])
-        cz, sz = np.cos(k[2] * x[..., 2]), np.sin(k[2] * x[..., 2])
-        y[..., 0] += a * pol[0] * cx * sy * sz
-        y[..., 1] += a * pol[1] * sx * cy * sz
-        y[..., 2] += a * pol[2] * sx * sy * cz
+        modes.append(((m, n, p), rng.normal() * pol))
+    x = op.node_coords().reshape(-1, 3)
+    y = np.zeros((x.shape[0], 6))
+    step = 1 << 20
+    for a in range(0, x.shape[0], step):
+        xc = x[a:a + step]
+        # cos/sin(pi q x_d / l_d) for q = 0..3 on this chunk, per axis
+        trig = []
+        for d, ld in enumerate((lx, ly, lz)):
+            ang = (math.pi / ld) * xc[:, d]
+            trig.append([(np.cos(q * ang), np.sin(q * ang)) for q in range(4)])
+        for (m, n, p), amp in modes:
+            (cx, sx), (cy, sy), (cz, sz) = trig[0][m], trig[1][n], trig[2][p]
+            y[a:a + step, 0] += amp[0] * cx * sy * sz
+            y[a:a + step, 1] += amp[1] * sx * cy * sz
+            y[a:a + step, 2] += amp[2] * sx * sy * cz
     return y.reshape(-1)
 
 
