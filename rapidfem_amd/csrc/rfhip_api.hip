@@ -37,7 +37,7 @@ constexpr double kKclErrFloor = 1e-12;   // constants.rs KCL_ERR_FLOOR
 constexpr double kKclNonfinite = 10.0;   // constants.rs KCL_NONFINITE_ERR
 constexpr double kArnoldiBreakdown = 1e-12;  // constants.rs:62
 constexpr int kMaxKrylov = 64;
-constexpr int kFusedVariants = 8;  // rfhip_fused.hip launch_stage
+constexpr int kFusedVariants = 9;  // rfhip_fused.hip launch_stage
 
 template <class F>
 int guarded(F &&f) {
@@ -256,6 +256,28 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   }
   if (worst > 1e-9)
     throw Error{RFHIP_EUNSUPPORTED, "face normal*fscale is not the J^-1 face gradient (rel " + std::to_string(worst) + ")"};
+  // Neighbour trace base per (element, face node) for k_fused5: the AoS
+  // index of the neighbour node's first field value, or the face code on a
+  // boundary face (negative). Needs n_dof < 2^31; otherwise left empty and
+  // that kernel variant is not offered.
+  std::vector<int32_t> nbase;
+  if (6 * (int64_t)np * K < ((int64_t)1 << 31)) {
+    nbase.resize((size_t)K * nf4);
+    for (int64_t e = 0; e < K; ++e)
+      for (int f = 0; f < 4; ++f) {
+        const int32_t code = fcode[4 * e + f];
+        for (int m = 0; m < nfp; ++m) {
+          int32_t v = code;
+          if (code >= 0) {
+            const int64_t nb = code >> 5;
+            const int nlf = (code >> 3) & 3, o = code & 7;
+            const int vj = nbr_node[((size_t)(f * 4 + nlf) * 6 + o) * nfp + m];
+            v = (int32_t)((nb * np + vj) * 6);
+          }
+          nbase[(size_t)e * nf4 + f * nfp + m] = v;
+        }
+      }
+  }
   // Transposed reference matrices.
   std::vector<double> dT(3 * np * np), liftc((size_t)nf4 * np);
   const double *D[3] = {d.diff_r, d.diff_s, d.diff_t};
@@ -294,6 +316,7 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   upload(op->mat, mat.data(), mat.size(), s);
   upload(op->mat_tab, tab.data(), tab.size(), s);
   upload(op->nbr_node, nbr_node.data(), nbr_node.size(), s);
+  if (!nbase.empty()) upload(op->nbase, nbase.data(), nbase.size(), s);
   upload(op->face_nodes, d.face_nodes, 4 * (size_t)nfp, s);
   sync(op);
   const size_t n = op->n_vec;
@@ -323,6 +346,7 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   a.mat_tab = op->mat_tab.as<double>();
   a.face_nodes = op->face_nodes.as<int>();
   a.nbr_node = op->nbr_node.as<int>();
+  a.nbase = nbase.empty() ? nullptr : op->nbase.as<int>();
 }
 
 // ---- vectors at the boundary ---------------------------------------------

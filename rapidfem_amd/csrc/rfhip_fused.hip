@@ -23,6 +23,7 @@
 // D^T and LIFT^T are read from L1 with consecutive nodes on consecutive
 // lanes.
 #include <cstdlib>
+#include <type_traits>
 
 #include "rfhip_impl.hpp"
 
@@ -1043,6 +1044,367 @@ __global__ void __launch_bounds__(MfmaCfg<NP>::NT) k_fused4(StageArgs io) {
   node_epilogue<EPI>(io, (e * NP + i) * 6, io.mat_tab + 12 * io.mat[e], acc, own);
 }
 
+// ---- MFMA volume + MFMA LIFT with linear operand addressing -------------
+//
+// k_fused5 runs both dense contractions of a stage on the matrix cores
+// (v_mfma_f64_16x16x4f64, FP64) and keeps every LDS array component-major
+// with the element index innermost, so the MFMA operand and result
+// addresses are linear in the lane index:
+//   state      sS[(c*Np + j)*E2 + e]         (c = 0..5)
+//   products   sR[(c'*3Np + s)*E2 + e]        (one field, s = k*Np + i)
+//   flux       sF[(c'*4Nfp + fm)*E2 + e]      (one field)
+//   lifted     sL[(c'*Np + i)*E2 + e]         (one field, over sR)
+// with column = c'*EPB + e of each GEMM and E2 = EPB + 1 (bank padding).
+// Per element group: volume(H) -> curl(H) -> volume(E) -> curl(E) ->
+// LIFT(E flux) -> LIFT(H flux), each MFMA phase split over the block's
+// waves by output tile, the VALU phases one thread per (element, node).
+template <int NP>
+struct Mfma5Cfg {
+  static constexpr int EPB = NP == 35 ? 8 : 16;
+  static constexpr int E2 = EPB + 1;
+  static constexpr int NT = ((EPB * NP + 63) / 64) * 64;
+  static constexpr int MT = (3 * NP + 15) / 16;  // volume row tiles
+  static constexpr int KS = (NP + 3) / 4;        // volume k-steps
+  static constexpr int NTC = (3 * EPB + 15) / 16;  // column tiles of one field
+};
+
+// Stage update with the old p already in registers (k_fused5 prefetches it).
+template <int EPI>
+__device__ __forceinline__ void node_update(const StageArgs &io, int64_t base, const double *mt, const double *acc,
+                                            const double *own, const double *pold, const double *srcv) {
+  double kv[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    kv[c] = mt[c] * acc[c] - mt[6 + c] * own[c];
+    kv[3 + c] = mt[3 + c] * acc[3 + c] - mt[9 + c] * own[3 + c];
+  }
+  if (io.src_kind == RFHIP_DRIVE_POINT) {
+    const int64_t off = io.src_dof - base;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (off == c) kv[c] += io.g;
+  } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) kv[c] += srcv[c] * io.g;
+  }
+  if constexpr (EPI == 0) {
+    double2 *dy = reinterpret_cast<double2 *>(io.dy + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) dy[q] = make_double2(kv[2 * q], kv[2 * q + 1]);
+  } else if constexpr (EPI == 1) {
+    double pn[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) pn[c] = io.sa * pold[c] + io.dt * kv[c];
+    double2 *pp = reinterpret_cast<double2 *>(static_cast<double *>(io.p) + base);
+    double2 *yo = reinterpret_cast<double2 *>(static_cast<double *>(io.yout) + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      pp[q] = make_double2(pn[2 * q], pn[2 * q + 1]);
+      yo[q] = make_double2(own[2 * q] + io.sb * pn[2 * q], own[2 * q + 1] + io.sb * pn[2 * q + 1]);
+    }
+  } else {
+    float pn[6];
+    float yh[6];
+    uint16_t yl[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      pn[c] = (float)(io.sa * pold[c] + io.dt * kv[c]);
+      const double yn = own[c] + io.sb * (double)pn[c];
+      yh[c] = (float)yn;
+      yl[c] = bf16_rne((float)(yn - (double)yh[c]));
+    }
+    float2 *pp = reinterpret_cast<float2 *>(static_cast<float *>(io.p) + base);
+    float2 *yo = reinterpret_cast<float2 *>(static_cast<float *>(io.yout) + base);
+    uint32_t *lo = reinterpret_cast<uint32_t *>(static_cast<uint16_t *>(io.yout_lo) + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      pp[q] = make_float2(pn[2 * q], pn[2 * q + 1]);
+      yo[q] = make_float2(yh[2 * q], yh[2 * q + 1]);
+      lo[q] = (uint32_t)yl[2 * q] | ((uint32_t)yl[2 * q + 1] << 16);
+    }
+  }
+}
+
+template <int NP, int NFP, bool CIN, int EPI>
+__global__ void __launch_bounds__(Mfma5Cfg<NP>::NT) k_fused5(StageArgs io) {
+  using Cfg = Mfma5Cfg<NP>;
+  constexpr int EPB = Cfg::EPB, E2 = Cfg::E2, NT = Cfg::NT, MT = Cfg::MT, KS = Cfg::KS, NTC = Cfg::NTC;
+  constexpr int NWAVE = NT / 64;
+  constexpr int NF4 = 4 * NFP;
+  constexpr int FPT = (NF4 + NP - 1) / NP;
+  constexpr int MTL = (NP + 15) / 16, KSL = NF4 / 4;  // LIFT tiles (4Nfp is a multiple of 4)
+  constexpr int SS = 6 * NP * E2, SF = 3 * NF4 * E2, SR = 3 * 3 * NP * E2;
+  constexpr bool COLPAD = (3 * EPB) % 16 != 0;  // last column tile partly empty
+  // Fixed row tile per wave when there are enough waves: its A fragments are
+  // loaded once and every MFMA chain of the wave shares them.
+  constexpr bool VFIX = MT <= NWAVE, LFIX = MTL <= NWAVE;
+  __shared__ __align__(16) double sA[SS > SF ? SS : SF];  // state, then one field's flux
+  __shared__ __align__(16) double sB[SR];                 // products, then lifted flux
+  __shared__ __align__(16) double sg[EPB * 16];            // per face: n, 0.5*fscale
+  __shared__ __align__(16) double sm[EPB * 12];            // material row per element
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int le = tid / NP, i = tid - (tid / NP) * NP;
+  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int64_t rem = io.n_elem - e0;
+  const int nel = rem < EPB ? (int)rem : EPB;
+  const bool active = tid < nel * NP;
+  const int64_t e = e0 + le;
+  const int64_t base = (e * NP + i) * 6;
+  const int lcol = lane & 15, lrow = lane >> 4;
+
+  // P0: every global read of the group is issued here (neighbour traces,
+  // state, J^-1, old p, source, material row, A fragments) so the block
+  // waits for one memory latency instead of a chain of them.
+  double nv[FPT][6];
+  int fcodes[FPT];
+  double ji[9];
+  // old p of this node: f32 for the compressed form (kept narrow until use)
+  using PT = typename std::conditional<EPI == 2, float, double>::type;
+  PT pold[6];
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      fcodes[s] = kCodePec;
+      if (fm >= NF4) continue;
+      const int nbv = io.nbase[e * NF4 + fm];
+      fcodes[s] = nbv;
+      if (nbv >= 0) load_trace<CIN>(io, nbv, nv[s]);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) ji[q] = io.jinv[e * 9 + q];
+    if constexpr (EPI == 1) {
+      const double2 *pp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.p) + base);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double2 v = pp[q];
+        pold[2 * q] = v.x;
+        pold[2 * q + 1] = v.y;
+      }
+    } else if constexpr (EPI == 2) {
+      const float2 *pp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.p) + base);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float2 v = pp[q];
+        pold[2 * q] = v.x;
+        pold[2 * q + 1] = v.y;
+      }
+    }
+    for (int q = i; q < 12; q += NP) sm[le * 12 + q] = io.mat_tab[12 * io.mat[e] + q];
+  }
+  double av[VFIX ? KS : 1];
+  if constexpr (VFIX) {
+    const double *ap = io.afrag + (size_t)(wave < MT ? wave : 0) * KS * 64 + lane;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) av[ks] = ap[ks * 64];
+  }
+  {
+    const int64_t g0 = e0 * NP * 6;
+    const int cnt2 = nel * NP * 3;
+    for (int q = tid; q < cnt2; q += NT) {
+      double2 v;
+      if constexpr (CIN) {
+        const float2 h = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + g0)[q];
+        const uint32_t l = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + g0)[q];
+        v.x = (double)h.x + (double)__uint_as_float(l << 16);
+        v.y = (double)h.y + (double)__uint_as_float(l & 0xFFFF0000u);
+      } else {
+        v = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + g0)[q];
+      }
+      const int ee = q / (3 * NP), r2 = q - ee * (3 * NP);
+      const int j = r2 / 3, c = 2 * (r2 - j * 3);
+      sA[(c * NP + j) * E2 + ee] = v.x;
+      sA[((c + 1) * NP + j) * E2 + ee] = v.y;
+    }
+  }
+  if (active && i < 4) {
+    double n[3], cf;
+    face_geom(ji, i, n, cf);
+    double *o = sg + le * 16 + i * 4;
+    o[0] = n[0];
+    o[1] = n[1];
+    o[2] = n[2];
+    o[3] = cf;
+  }
+  __syncthreads();
+
+  double own[6], acc[6], fl[FPT][6];
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) own[c] = sA[(c * NP + i) * E2 + le];
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      if (fm >= NF4) continue;
+      const int f = fm / NFP, m = fm - f * NFP;
+      const double *gq = sg + le * 16 + f * 4;
+      const double n[3] = {gq[0], gq[1], gq[2]};
+      const int vi = io.face_nodes[f * NFP + m];
+      double u[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) u[c] = sA[(c * NP + vi) * E2 + le];
+      face_flux(fcodes[s] >= 0 ? 0 : fcodes[s], io.alpha, n, gq[3], u, nv[s], fl[s]);
+    }
+  }
+
+  // Volume of one field on the matrix cores, R[s][c'*EPB + e] into sB: the
+  // wave keeps one row tile and runs the NTC column tiles as independent
+  // accumulation chains.
+  auto volume = [&](int cbase) {
+    for (int mt = wave; mt < MT; mt += NWAVE) {
+      double a[KS];
+      if constexpr (VFIX) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) a[ks] = av[ks];
+      } else {
+        const double *ap = io.afrag + (size_t)mt * KS * 64 + lane;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) a[ks] = ap[ks * 64];
+      }
+      mf_d4 cacc[NTC];
+#pragma unroll
+      for (int nt = 0; nt < NTC; ++nt) cacc[nt] = mf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const bool jv = NP % 4 == 0 || ks < KS - 1 || ks * 4 + lrow < NP;
+#pragma unroll
+        for (int nt = 0; nt < NTC; ++nt) {
+          const int col = nt * 16 + lcol;
+          const int cp = col / EPB, ce = col - (col / EPB) * EPB;
+          const bool cv = !COLPAD || cp < 3;
+          const double b = (cv && jv) ? sA[((cbase + (cv ? cp : 0)) * NP + ks * 4 + lrow) * E2 + ce] : 0.0;
+          cacc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b, cacc[nt], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int nt = 0; nt < NTC; ++nt) {
+        const int col = nt * 16 + lcol;
+        const int cp = col / EPB, ce = col - (col / EPB) * EPB;
+        if (COLPAD && cp >= 3) continue;
+        double *rp = sB + (cp * 3 * NP + mt * 16 + lrow) * E2 + ce;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (mt * 16 + r * 4 + lrow < 3 * NP) rp[r * 4 * E2] = cacc[nt][r];
+      }
+    }
+  };
+  // d(field_c')/dx_p at this thread's node from sB, then the curl.
+  auto curl = [&](double *out, double sign) {
+    double rv[3][3];  // rv[k][c']
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) rv[k][c] = sB[(c * 3 * NP + k * NP + i) * E2 + le];
+    auto dd = [&](int p, int c) { return ji[p] * rv[0][c] + ji[3 + p] * rv[1][c] + ji[6 + p] * rv[2][c]; };
+    out[0] = sign * (dd(1, 2) - dd(2, 1));
+    out[1] = sign * (dd(2, 0) - dd(0, 2));
+    out[2] = sign * (dd(0, 1) - dd(1, 0));
+  };
+  // LIFT of one field's flux (sA) into sB as L[c'][i][e]: wave w keeps row
+  // tile w % MTL (A fragments in lf) and takes every (NWAVE/MTL)-th column
+  // tile.
+  double lf[LFIX ? KSL : 1];
+  auto load_lift_frags = [&]() {
+    if constexpr (LFIX) {
+      const double *ap = io.lfrag + (size_t)(wave % MTL) * KSL * 64 + lane;
+#pragma unroll
+      for (int ks = 0; ks < KSL; ++ks) lf[ks] = ap[ks * 64];
+    }
+  };
+  auto lift = [&]() {
+    if constexpr (LFIX) {
+      const int mt = wave % MTL;
+      constexpr int WPM = NWAVE / MTL;  // waves per row tile (rounded down)
+      const int widx = wave / MTL;
+      if (widx >= WPM + (mt < NWAVE % MTL ? 1 : 0)) return;
+      const int nwm = WPM + (mt < NWAVE % MTL ? 1 : 0);
+      for (int nt = widx; nt < NTC; nt += nwm) {
+        const int col = nt * 16 + lcol;
+        const int cp = col / EPB, ce = col - (col / EPB) * EPB;
+        const bool cv = !COLPAD || cp < 3;
+        const double *bp = sA + ((cv ? cp : 0) * NF4 + lrow) * E2 + ce;
+        mf_d4 cacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KSL; ++ks) {
+          const double b = cv ? bp[ks * 4 * E2] : 0.0;
+          cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(lf[ks], b, cacc, 0, 0, 0);
+        }
+        if (cv) {
+          double *lp = sB + (cp * NP + mt * 16 + lrow) * E2 + ce;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (mt * 16 + r * 4 + lrow < NP) lp[r * 4 * E2] = cacc[r];
+        }
+      }
+    } else {
+      for (int t = wave; t < MTL * NTC; t += NWAVE) {
+        const int mt = t / NTC, nt = t - (t / NTC) * NTC;
+        const int col = nt * 16 + lcol;
+        const int cp = col / EPB, ce = col - (col / EPB) * EPB;
+        const bool cv = !COLPAD || cp < 3;
+        const double *ap = io.lfrag + (size_t)mt * KSL * 64 + lane;
+        const double *bp = sA + ((cv ? cp : 0) * NF4 + lrow) * E2 + ce;
+        mf_d4 cacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KSL; ++ks) {
+          const double b = cv ? bp[ks * 4 * E2] : 0.0;
+          cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[ks * 64], b, cacc, 0, 0, 0);
+        }
+        if (cv) {
+          double *lp = sB + (cp * NP + mt * 16 + lrow) * E2 + ce;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (mt * 16 + r * 4 + lrow < NP) lp[r * 4 * E2] = cacc[r];
+        }
+      }
+    }
+  };
+  auto write_flux = [&](int cbase) {
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      if (fm >= NF4) continue;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sA[(c * NF4 + fm) * E2 + le] = fl[s][cbase + c];
+    }
+  };
+
+  volume(3);  // H -> dE
+  __syncthreads();
+  if (active) curl(acc, 1.0);
+  __syncthreads();
+  volume(0);  // E -> dH
+  __syncthreads();
+  load_lift_frags();
+  if (active) {
+    curl(acc + 3, -1.0);
+    write_flux(0);  // the state is no longer read
+  }
+  __syncthreads();
+  lift();  // E flux
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] += sB[(c * NP + i) * E2 + le];
+    write_flux(3);
+  }
+  __syncthreads();
+  lift();  // H flux
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) acc[3 + c] += sB[(c * NP + i) * E2 + le];
+  double po[6], srcv[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
+  if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
+  }
+  node_update<EPI>(io, base, sm + le * 12, acc, own, po, srcv);
+}
+
 template <int NP>
 struct Nfp {
   static constexpr int v = NP == 4 ? 3 : NP == 10 ? 6 : NP == 20 ? 10 : 15;
@@ -1062,7 +1424,8 @@ inline int64_t resident_blocks(const void *kernel) {
 // k_fused2 with 2/1 nodes per thread, 3 = k_fused3 (early neighbour
 // indices), 4 = k_fused3 with early neighbour traces, 5 = k_fused3 with D
 // and LIFT in LDS over a grid-stride loop, 6 = k_fused4 (FP64 MFMA volume),
-// 7 = k_fused4 with LIFT on MFMA too. Chosen by the
+// 7 = k_fused4 with LIFT on MFMA too, 8 = k_fused5 (MFMA volume and LIFT,
+// linear operand addressing). Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -1090,6 +1453,10 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
       hipLaunchKernelGGL((k_fused4<NP, F, CIN, EPI, false>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
     else
       hipLaunchKernelGGL((k_fused4<NP, F, CIN, EPI, true>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
+  } else if (variant == 8 && a.nbase) {
+    using Cfg = Mfma5Cfg<NP>;
+    const int64_t groups = (a.n_elem + Cfg::EPB - 1) / Cfg::EPB;
+    hipLaunchKernelGGL((k_fused5<NP, F, CIN, EPI>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
   } else {
     static int64_t resident = 0;
     if (resident == 0)

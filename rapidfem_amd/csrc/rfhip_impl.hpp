@@ -107,6 +107,7 @@ struct StageArgs {
   const double *mat_tab;  // n_mat * 12: inv_eps3, inv_mu3, sig_eps3, sig_mu3
   const int *face_nodes;  // 4*Nfp
   const int *nbr_node;    // [(f*4 + nlf)*6 + orientation][Nfp] -> neighbour volume node
+  const int *nbase;       // [e*4Nfp + fm]: AoS index of the neighbour trace, or the (negative) face code
   const void *yin;        // double, or float hi (compressed)
   const void *yin_lo;     // bf16 lo (compressed) or null
   void *yout, *yout_lo;
@@ -138,7 +139,7 @@ struct rfhip_op {
   // device operator data
   rfhip::DevBuf dr, ds, dt, lift, face_nodes, jinv, inv_eps, inv_mu, sig_eps, sig_mu, normal, fscale, nbr,
       nbr_lf, port, perm;
-  rfhip::DevBuf dT, liftc, afrag, lfrag, jinv64, fcode, mat, mat_tab, nbr_node;
+  rfhip::DevBuf dT, liftc, afrag, lfrag, jinv64, fcode, mat, mat_tab, nbr_node, nbase;
   int64_t n_vec = 0;  // length of device vectors (n_dof)
   int order = 0;
   // state registers (precision depends on form)
