@@ -177,7 +177,7 @@ def main() -> int:
     if os.path.exists(TRAFFIC_FILE):
         try:
             tr = json.load(open(TRAFFIC_FILE))
-            if tr.get("form") == args.form and tr.get("n_dof") == n:
+            if tr.get("form") == args.form and tr.get("n_dof") == n and tr.get("variant") == variant:
                 traffic = tr.get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None

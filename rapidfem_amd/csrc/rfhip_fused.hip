@@ -79,6 +79,64 @@ __device__ __forceinline__ void load_trace(const StageArgs &io, int64_t nb, doub
   }
 }
 
+// Stage update with the old p already in registers (prefetched by the
+// kernels that use it): materials, held source, p and y.
+template <int EPI>
+__device__ __forceinline__ void node_update(const StageArgs &io, int64_t base, const double *mt, const double *acc,
+                                            const double *own, const double *pold, const double *srcv) {
+  double kv[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    kv[c] = mt[c] * acc[c] - mt[6 + c] * own[c];
+    kv[3 + c] = mt[3 + c] * acc[3 + c] - mt[9 + c] * own[3 + c];
+  }
+  if (io.src_kind == RFHIP_DRIVE_POINT) {
+    const int64_t off = io.src_dof - base;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (off == c) kv[c] += io.g;
+  } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) kv[c] += srcv[c] * io.g;
+  }
+  if constexpr (EPI == 0) {
+    double2 *dy = reinterpret_cast<double2 *>(io.dy + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) dy[q] = make_double2(kv[2 * q], kv[2 * q + 1]);
+  } else if constexpr (EPI == 1) {
+    double pn[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) pn[c] = io.sa * pold[c] + io.dt * kv[c];
+    double2 *pp = reinterpret_cast<double2 *>(static_cast<double *>(io.p) + base);
+    double2 *yo = reinterpret_cast<double2 *>(static_cast<double *>(io.yout) + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      pp[q] = make_double2(pn[2 * q], pn[2 * q + 1]);
+      yo[q] = make_double2(own[2 * q] + io.sb * pn[2 * q], own[2 * q + 1] + io.sb * pn[2 * q + 1]);
+    }
+  } else {
+    float pn[6];
+    float yh[6];
+    uint16_t yl[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      pn[c] = (float)(io.sa * pold[c] + io.dt * kv[c]);
+      const double yn = own[c] + io.sb * (double)pn[c];
+      yh[c] = (float)yn;
+      yl[c] = bf16_rne((float)(yn - (double)yh[c]));
+    }
+    float2 *pp = reinterpret_cast<float2 *>(static_cast<float *>(io.p) + base);
+    float2 *yo = reinterpret_cast<float2 *>(static_cast<float *>(io.yout) + base);
+    uint32_t *lo = reinterpret_cast<uint32_t *>(static_cast<uint16_t *>(io.yout_lo) + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      pp[q] = make_float2(pn[2 * q], pn[2 * q + 1]);
+      yo[q] = make_float2(yh[2 * q], yh[2 * q + 1]);
+      lo[q] = (uint32_t)yl[2 * q] | ((uint32_t)yl[2 * q + 1] << 16);
+    }
+  }
+}
+
 // EPI: 0 = apply (dy = A y, FP64), 1 = LSERK stage with FP64 registers,
 // 2 = LSERK stage with compressed registers (y f32+bf16, p f32).
 template <int NP, int NFP, bool CIN, int EPI>
@@ -480,6 +538,24 @@ __global__ void __launch_bounds__(kFusedWG) k_fused2(StageArgs io) {
   }
 }
 
+// Order of element groups over the blocks. With xcd, blocks are taken to
+// run round-robin over the 8 XCDs (block b on XCD b % 8): XCD x walks the
+// contiguous groups [x*chunk, (x+1)*chunk), so the neighbours an element
+// gathers from were mostly loaded by the same XCD and sit in its L2.
+// Requires gridDim.x % 8 == 0 (else the plain order is used).
+struct GroupRange {
+  int64_t first, end, step;
+};
+__device__ __forceinline__ GroupRange group_range(bool xcd, int64_t ngroups) {
+  if (xcd && (gridDim.x & 7u) == 0) {
+    const int64_t chunk = (ngroups + 7) / 8;
+    const int64_t beg = (int64_t)(blockIdx.x & 7u) * chunk;
+    const int64_t end = beg + chunk < ngroups ? beg + chunk : ngroups;
+    return {beg + (int64_t)(blockIdx.x >> 3), end, (int64_t)(gridDim.x >> 3)};
+  }
+  return {(int64_t)blockIdx.x, ngroups, (int64_t)gridDim.x};
+}
+
 // Latency-oriented variant of k_fused (same thread mapping):
 //  * the face codes and neighbour-node indices of a thread's face nodes are
 //    resolved before anything else (they are a chain of dependent loads),
@@ -491,7 +567,7 @@ __global__ void __launch_bounds__(kFusedWG) k_fused2(StageArgs io) {
 //  * DLDS: D_k^T and LIFT^T live in LDS and the block walks over element
 //    groups (grid-stride), so the operator matrices are read from L2 once
 //    per block instead of through L1 by every wave.
-template <int NP, int NFP, bool CIN, int EPI, bool DLDS, bool PREV>
+template <int NP, int NFP, bool CIN, int EPI, bool DLDS, bool PREV, bool XCD = false>
 __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
   constexpr int EPB = kFusedWG / NP;
   constexpr int NF4 = 4 * NFP;
@@ -502,6 +578,7 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
   __shared__ __align__(16) double sf[EPB * NF4 * 6];
   __shared__ __align__(16) double sg[EPB * 16];  // per face: n_x, n_y, n_z, 0.5*fscale
   __shared__ __align__(16) double sd[SD];
+  __shared__ __align__(16) double sm[EPB * 12];  // material row per element
   const double *dT = io.dT;
   const double *lift = io.lift;
   if constexpr (DLDS) {
@@ -512,7 +589,8 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
   const int tid = threadIdx.x;
   const int le = tid / NP, i = tid - (tid / NP) * NP;
   const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;
-  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+  const GroupRange gr = group_range(XCD, ngroups);
+  for (int64_t grp = gr.first; grp < gr.end; grp += gr.step) {
     const int64_t e0 = grp * EPB;
     const int64_t rem = io.n_elem - e0;
     const int nel = rem < EPB ? (int)rem : EPB;
@@ -531,16 +609,47 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
         fcodes[s] = kCodePec;
         nbase[s] = 0;
         if (fm >= NF4) continue;
-        const int f = fm / NFP, m = fm - f * NFP;
-        const int code = io.fcode[e * 4 + f];
-        fcodes[s] = code;
-        if (code < 0) continue;
-        const int64_t nb = code >> 5;
-        const int nlf = (code >> 3) & 3, o = code & 7;
-        const int vj = io.nbr_node[((f * 4 + nlf) * 6 + o) * NFP + m];
-        nbase[s] = (nb * NP + vj) * 6;
+        if (io.nbase) {  // one read instead of the face code -> node table chain
+          const int nbv = io.nbase[e * NF4 + fm];
+          fcodes[s] = nbv >= 0 ? 0 : nbv;
+          if (nbv < 0) continue;
+          nbase[s] = nbv;
+        } else {
+          const int f = fm / NFP, m = fm - f * NFP;
+          const int code = io.fcode[e * 4 + f];
+          fcodes[s] = code;
+          if (code < 0) continue;
+          const int64_t nb = code >> 5;
+          const int nlf = (code >> 3) & 3, o = code & 7;
+          const int vj = io.nbr_node[((f * 4 + nlf) * 6 + o) * NFP + m];
+          nbase[s] = (nb * NP + vj) * 6;
+        }
         if constexpr (PREV) load_trace<CIN>(io, nbase[s], nv[s]);
       }
+    }
+    // old p and the material row, needed only by the epilogue
+    using PT = typename std::conditional<EPI == 2, float, double>::type;
+    PT pold[6];
+    const int64_t base = (e * NP + i) * 6;
+    if (active) {
+      if constexpr (EPI == 1) {
+        const double2 *pp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.p) + base);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const double2 v = pp[q];
+          pold[2 * q] = v.x;
+          pold[2 * q + 1] = v.y;
+        }
+      } else if constexpr (EPI == 2) {
+        const float2 *pp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.p) + base);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const float2 v = pp[q];
+          pold[2 * q] = v.x;
+          pold[2 * q + 1] = v.y;
+        }
+      }
+      for (int q = i; q < 12; q += NP) sm[le * 12 + q] = io.mat_tab[12 * io.mat[e] + q];
     }
     // 2. the group's state into LDS, two values per lane and load.
     {
@@ -677,54 +786,20 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) acc[c] = fma(w, fb[fm * 6 + c], acc[c]);
       }
-      const double *mt = io.mat_tab + 12 * io.mat[e];
       const double *ub = su + (le * NP + i) * 6;
-      double kv[6], own[6];
+      double own[6], po[6], srcv[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) own[c] = ub[c];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        kv[c] = mt[c] * acc[c] - mt[6 + c] * own[c];
-        kv[3 + c] = mt[3 + c] * acc[3 + c] - mt[9 + c] * own[3 + c];
+      for (int c = 0; c < 6; ++c) {
+        own[c] = ub[c];
+        po[c] = (double)pold[c];
       }
-      const int64_t base = (e * NP + i) * 6;
-      if (io.src_kind == RFHIP_DRIVE_POINT) {
-        const int64_t off = io.src_dof - base;
+      if (io.src_kind == RFHIP_DRIVE_VECTOR) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
-          if (off == c) kv[c] += io.g;
-      } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) kv[c] += io.src_vec[base + c] * io.g;
+        for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
       }
-      if constexpr (EPI == 0) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) io.dy[base + c] = kv[c];
-      } else if constexpr (EPI == 1) {
-        double *pp = static_cast<double *>(io.p);
-        double *yo = static_cast<double *>(io.yout);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          const double pn = io.sa * pp[base + c] + io.dt * kv[c];
-          pp[base + c] = pn;
-          yo[base + c] = own[c] + io.sb * pn;
-        }
-      } else {
-        float *pp = static_cast<float *>(io.p);
-        float *yh = static_cast<float *>(io.yout);
-        uint16_t *yl = static_cast<uint16_t *>(io.yout_lo);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          const float pn = (float)(io.sa * (double)pp[base + c] + io.dt * kv[c]);
-          pp[base + c] = pn;
-          const double yn = own[c] + io.sb * (double)pn;
-          const float h = (float)yn;
-          yh[base + c] = h;
-          yl[base + c] = bf16_rne((float)(yn - (double)h));
-        }
-      }
+      node_update<EPI>(io, base, sm + le * 12, acc, own, po, srcv);
     }
-    __syncthreads();  // su, sf and sg are rewritten by the next group
+    __syncthreads();  // su, sf, sg and sm are rewritten by the next group
   }
 }
 
@@ -1068,64 +1143,7 @@ struct Mfma5Cfg {
   static constexpr int NTC = (3 * EPB + 15) / 16;  // column tiles of one field
 };
 
-// Stage update with the old p already in registers (k_fused5 prefetches it).
-template <int EPI>
-__device__ __forceinline__ void node_update(const StageArgs &io, int64_t base, const double *mt, const double *acc,
-                                            const double *own, const double *pold, const double *srcv) {
-  double kv[6];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    kv[c] = mt[c] * acc[c] - mt[6 + c] * own[c];
-    kv[3 + c] = mt[3 + c] * acc[3 + c] - mt[9 + c] * own[3 + c];
-  }
-  if (io.src_kind == RFHIP_DRIVE_POINT) {
-    const int64_t off = io.src_dof - base;
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-      if (off == c) kv[c] += io.g;
-  } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) kv[c] += srcv[c] * io.g;
-  }
-  if constexpr (EPI == 0) {
-    double2 *dy = reinterpret_cast<double2 *>(io.dy + base);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) dy[q] = make_double2(kv[2 * q], kv[2 * q + 1]);
-  } else if constexpr (EPI == 1) {
-    double pn[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) pn[c] = io.sa * pold[c] + io.dt * kv[c];
-    double2 *pp = reinterpret_cast<double2 *>(static_cast<double *>(io.p) + base);
-    double2 *yo = reinterpret_cast<double2 *>(static_cast<double *>(io.yout) + base);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      pp[q] = make_double2(pn[2 * q], pn[2 * q + 1]);
-      yo[q] = make_double2(own[2 * q] + io.sb * pn[2 * q], own[2 * q + 1] + io.sb * pn[2 * q + 1]);
-    }
-  } else {
-    float pn[6];
-    float yh[6];
-    uint16_t yl[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      pn[c] = (float)(io.sa * pold[c] + io.dt * kv[c]);
-      const double yn = own[c] + io.sb * (double)pn[c];
-      yh[c] = (float)yn;
-      yl[c] = bf16_rne((float)(yn - (double)yh[c]));
-    }
-    float2 *pp = reinterpret_cast<float2 *>(static_cast<float *>(io.p) + base);
-    float2 *yo = reinterpret_cast<float2 *>(static_cast<float *>(io.yout) + base);
-    uint32_t *lo = reinterpret_cast<uint32_t *>(static_cast<uint16_t *>(io.yout_lo) + base);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      pp[q] = make_float2(pn[2 * q], pn[2 * q + 1]);
-      yo[q] = make_float2(yh[2 * q], yh[2 * q + 1]);
-      lo[q] = (uint32_t)yl[2 * q] | ((uint32_t)yl[2 * q + 1] << 16);
-    }
-  }
-}
-
-template <int NP, int NFP, bool CIN, int EPI>
+template <int NP, int NFP, bool CIN, int EPI, bool XCD = false>
 __global__ void __launch_bounds__(Mfma5Cfg<NP>::NT) k_fused5(StageArgs io) {
   using Cfg = Mfma5Cfg<NP>;
   constexpr int EPB = Cfg::EPB, E2 = Cfg::E2, NT = Cfg::NT, MT = Cfg::MT, KS = Cfg::KS, NTC = Cfg::NTC;
@@ -1145,7 +1163,9 @@ __global__ void __launch_bounds__(Mfma5Cfg<NP>::NT) k_fused5(StageArgs io) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int le = tid / NP, i = tid - (tid / NP) * NP;
-  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int64_t grp = group_range(XCD, (io.n_elem + EPB - 1) / EPB).first;
+  if (grp >= (io.n_elem + EPB - 1) / EPB) return;  // padding blocks of the XCD order
+  const int64_t e0 = grp * EPB;
   const int64_t rem = io.n_elem - e0;
   const int nel = rem < EPB ? (int)rem : EPB;
   const bool active = tid < nel * NP;
@@ -1425,7 +1445,8 @@ inline int64_t resident_blocks(const void *kernel) {
 // indices), 4 = k_fused3 with early neighbour traces, 5 = k_fused3 with D
 // and LIFT in LDS over a grid-stride loop, 6 = k_fused4 (FP64 MFMA volume),
 // 7 = k_fused4 with LIFT on MFMA too, 8 = k_fused5 (MFMA volume and LIFT,
-// linear operand addressing). Chosen by the
+// linear operand addressing), 9 = variant 5 and 10 = variant 8 with the
+// XCD-aware group order. Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -1453,10 +1474,23 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
       hipLaunchKernelGGL((k_fused4<NP, F, CIN, EPI, false>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
     else
       hipLaunchKernelGGL((k_fused4<NP, F, CIN, EPI, true>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
-  } else if (variant == 8 && a.nbase) {
+  } else if ((variant == 8 || variant == 10) && a.nbase) {
     using Cfg = Mfma5Cfg<NP>;
     const int64_t groups = (a.n_elem + Cfg::EPB - 1) / Cfg::EPB;
-    hipLaunchKernelGGL((k_fused5<NP, F, CIN, EPI>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
+    if (variant == 8)
+      hipLaunchKernelGGL((k_fused5<NP, F, CIN, EPI, false>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_fused5<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))), dim3(Cfg::NT),
+                         0, s, a);
+  } else if (variant == 9) {
+    static int64_t resident = 0;
+    if (resident == 0)
+      resident = resident_blocks(reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, true>));
+    const int64_t cap = 8 * ((groups1 + 7) / 8);
+    int64_t grid = (resident / 8) * 8;
+    if (grid < 8) grid = 8;
+    if (grid > cap) grid = cap;
+    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, true>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
   } else {
     static int64_t resident = 0;
     if (resident == 0)
