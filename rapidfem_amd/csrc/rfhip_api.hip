@@ -37,7 +37,7 @@ constexpr double kKclErrFloor = 1e-12;   // constants.rs KCL_ERR_FLOOR
 constexpr double kKclNonfinite = 10.0;   // constants.rs KCL_NONFINITE_ERR
 constexpr double kArnoldiBreakdown = 1e-12;  // constants.rs:62
 constexpr int kMaxKrylov = 64;
-constexpr int kFusedVariants = 11;  // rfhip_fused.hip launch_stage
+constexpr int kFusedVariants = 12;  // rfhip_fused.hip launch_stage
 
 template <class F>
 int guarded(F &&f) {
@@ -309,8 +309,21 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
         const int i = 16 * mt + l % 16, fm = 4 * ks + l / 16;
         if (i < np && fm < nf4) lfrag[((size_t)mt * lsteps + ks) * 64 + l] = d.lift[(size_t)i * nf4 + fm];
       }
+  // ... and node-interleaved for k_fused7: wave w owns nodes 4w..4w+3 and
+  // its A rows are (k, node) = (row / 4, 4w + row % 4), row 4k + q, k < 3
+  // (row tile 16 = 3 derivatives x 4 nodes + a zero row per node), so one
+  // lane receives all three reference derivatives of its node.
+  const int nwaves7 = (np + 3) / 4;
+  std::vector<double> afrag7((size_t)nwaves7 * ksteps * 64, 0.0);
+  for (int w = 0; w < nwaves7; ++w)
+    for (int ks = 0; ks < ksteps; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int row = l % 16, k = row / 4, node = 4 * w + row % 4, j = 4 * ks + l / 16;
+        if (k < 3 && node < np && j < np) afrag7[((size_t)w * ksteps + ks) * 64 + l] = D[k][node * np + j];
+      }
   upload(op->afrag, afrag.data(), afrag.size(), s);
   upload(op->lfrag, lfrag.data(), lfrag.size(), s);
+  upload(op->afrag7, afrag7.data(), afrag7.size(), s);
   upload(op->jinv64, d.jinv, 9 * (size_t)K, s);
   upload(op->fcode, fcode.data(), fcode.size(), s);
   upload(op->mat, mat.data(), mat.size(), s);
@@ -340,6 +353,7 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   a.lift = op->liftc.as<double>();
   a.afrag = op->afrag.as<double>();
   a.lfrag = op->lfrag.as<double>();
+  a.afrag7 = op->afrag7.as<double>();
   a.jinv = op->jinv64.as<double>();
   a.fcode = op->fcode.as<int>();
   a.mat = op->mat.as<int>();

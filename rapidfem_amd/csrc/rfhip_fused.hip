@@ -1425,6 +1425,225 @@ __global__ void __launch_bounds__(Mfma5Cfg<NP>::NT) k_fused5(StageArgs io) {
   node_update<EPI>(io, base, sm + le * 12, acc, own, po, srcv);
 }
 
+// ---- MFMA volume with the curl in registers -------------------------------
+//
+// k_fused7 maps wave w, lane l of a 16-element block to node 4w + l/16 of
+// element l%16. Its volume A operand interleaves the stacked [Dr; Ds; Dt]
+// by node (row 4k + q <-> derivative k of node 4w + q, a zero row for
+// k = 3), so v_mfma_f64_16x16x4f64 hands each lane, in element r of its
+// accumulator, the three reference derivatives of its own node for one
+// component (one column tile per component, columns = elements). J^-1 and
+// the curl are then applied in registers with no LDS round trip and no
+// barrier between the two fields. The face flux goes straight to LDS and
+// all six components are lifted in one MFMA phase (three barriers per
+// element group in all).
+template <int NP>
+struct Mfma7Cfg {
+  static constexpr int NW = (NP + 3) / 4;  // waves; wave w owns nodes 4w..4w+3
+  static constexpr int NT = NW * 64;
+  static constexpr int KS = (NP + 3) / 4;
+};
+
+template <int NP, int NFP, bool CIN, int EPI, bool XCD>
+__global__ void __launch_bounds__(Mfma7Cfg<NP>::NT) k_fused7(StageArgs io) {
+  using Cfg = Mfma7Cfg<NP>;
+  constexpr int EPB = 16, E2 = 17, NW = Cfg::NW, NT = Cfg::NT, KS = Cfg::KS;
+  constexpr int NF4 = 4 * NFP;
+  constexpr int FPT = (NF4 + NP - 1) / NP;
+  constexpr int MTL = (NP + 15) / 16, KSL = NF4 / 4;
+  constexpr int SS = 6 * NP * E2, SF = 6 * NF4 * E2;
+  constexpr bool LFIX = MTL <= NW;
+  __shared__ __align__(16) double sA[SS];  // state, then the lifted flux L[c][i][e]
+  __shared__ __align__(16) double sF[SF];  // face flux F[c][fm][e]
+  __shared__ __align__(16) double sg[EPB * 16];           // per face: n, 0.5*fscale
+  __shared__ __align__(16) double sm[EPB * 12];           // material row per element
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int el = lane & 15, q = lane >> 4;
+  const int i = 4 * wave + q;
+  const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;
+  const int64_t grp = group_range(XCD, ngroups).first;
+  if (grp >= ngroups) return;  // padding blocks of the XCD order
+  const int64_t e0 = grp * EPB;
+  const int64_t rem = io.n_elem - e0;
+  const int nel = rem < EPB ? (int)rem : EPB;
+  const bool active = i < NP && el < nel;
+  const int64_t e = e0 + el;
+  const int64_t base = (e * NP + i) * 6;
+
+  // P0: all global reads of the group.
+  double nv[FPT][6];
+  int fcodes[FPT];
+  double ji[9];
+  using PT = typename std::conditional<EPI == 2, float, double>::type;
+  PT pold[6];
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      fcodes[s] = kCodePec;
+      if (fm >= NF4) continue;
+      const int nbv = io.nbase[e * NF4 + fm];
+      fcodes[s] = nbv >= 0 ? 0 : nbv;
+      if (nbv >= 0) load_trace<CIN>(io, nbv, nv[s]);
+    }
+  }
+  if (el < nel) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) ji[c] = io.jinv[e * 9 + c];
+  }
+  if (active) {
+    if constexpr (EPI == 1) {
+      const double2 *pp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.p) + base);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double2 v = pp[c];
+        pold[2 * c] = v.x;
+        pold[2 * c + 1] = v.y;
+      }
+    } else if constexpr (EPI == 2) {
+      const float2 *pp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.p) + base);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float2 v = pp[c];
+        pold[2 * c] = v.x;
+        pold[2 * c + 1] = v.y;
+      }
+    }
+  }
+  for (int t = tid; t < nel * 12; t += NT) sm[t] = io.mat_tab[12 * io.mat[e0 + t / 12] + t % 12];
+  double av[KS];
+  {
+    const double *ap = io.afrag7 + (size_t)wave * KS * 64 + lane;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) av[ks] = ap[ks * 64];
+  }
+  {
+    const int64_t g0 = e0 * NP * 6;
+    const int cnt2 = nel * NP * 3;
+    for (int t = tid; t < cnt2; t += NT) {
+      double2 v;
+      if constexpr (CIN) {
+        const float2 h = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + g0)[t];
+        const uint32_t l = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + g0)[t];
+        v.x = (double)h.x + (double)__uint_as_float(l << 16);
+        v.y = (double)h.y + (double)__uint_as_float(l & 0xFFFF0000u);
+      } else {
+        v = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + g0)[t];
+      }
+      const int ee = t / (3 * NP), r2 = t - ee * (3 * NP);
+      const int j = r2 / 3, c = 2 * (r2 - j * 3);
+      sA[(c * NP + j) * E2 + ee] = v.x;
+      sA[((c + 1) * NP + j) * E2 + ee] = v.y;
+    }
+  }
+  if (wave == 0 && q < 4 && el < nel) {  // face q of element el
+    double n[3], cf;
+    face_geom(ji, q, n, cf);
+    double *o = sg + el * 16 + q * 4;
+    o[0] = n[0];
+    o[1] = n[1];
+    o[2] = n[2];
+    o[3] = cf;
+  }
+  __syncthreads();
+
+  // P1: own values, face flux into LDS, and both volume products with their
+  // curls.
+  double own[6], acc[6];
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) own[c] = sA[(c * NP + i) * E2 + el];
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      if (fm >= NF4) continue;
+      const int f = fm / NFP, m = fm - f * NFP;
+      const double *gq = sg + el * 16 + f * 4;
+      const double n[3] = {gq[0], gq[1], gq[2]};
+      const int vi = io.face_nodes[f * NFP + m];
+      double u[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) u[c] = sA[(c * NP + vi) * E2 + el];
+      double fo[6];
+      face_flux(fcodes[s], io.alpha, n, gq[3], u, nv[s], fo);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) sF[(c * NF4 + fm) * E2 + el] = fo[c];
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int cbase = h == 0 ? 3 : 0;
+    mf_d4 cacc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cacc[c] = mf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const bool jv = NP % 4 == 0 || ks < KS - 1 || ks * 4 + q < NP;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double b = jv ? sA[((cbase + c) * NP + ks * 4 + q) * E2 + el] : 0.0;
+        cacc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], b, cacc[c], 0, 0, 0);
+      }
+    }
+    // cacc[c'][k] = d(field_c')/dr_k at node i of element el.
+    auto dd = [&](int p, int c) { return ji[p] * cacc[c][0] + ji[3 + p] * cacc[c][1] + ji[6 + p] * cacc[c][2]; };
+    const double sgn = h == 0 ? 1.0 : -1.0;
+    acc[3 * h + 0] = sgn * (dd(1, 2) - dd(2, 1));
+    acc[3 * h + 1] = sgn * (dd(2, 0) - dd(0, 2));
+    acc[3 * h + 2] = sgn * (dd(0, 1) - dd(1, 0));
+  }
+  __syncthreads();  // the state in LDS is no longer read; the flux is complete
+
+  // P2: LIFT of all six flux components (columns c*16 + e) into sA as L[c][i][e].
+  {
+    const int lcol = lane & 15, lrow = lane >> 4;
+    if constexpr (LFIX) {
+      const int mt = wave % MTL;
+      const int nwm = NW / MTL + (mt < NW % MTL ? 1 : 0);
+      double lf[KSL];
+      const double *ap = io.lfrag + (size_t)mt * KSL * 64 + lane;
+#pragma unroll
+      for (int ks = 0; ks < KSL; ++ks) lf[ks] = ap[ks * 64];
+      for (int nt = wave / MTL; nt < 6; nt += nwm) {
+        const double *bp = sF + (nt * NF4 + lrow) * E2 + lcol;
+        mf_d4 c4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KSL; ++ks) c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(lf[ks], bp[ks * 4 * E2], c4, 0, 0, 0);
+        double *lp = sA + (nt * NP + mt * 16 + lrow) * E2 + lcol;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (mt * 16 + r * 4 + lrow < NP) lp[r * 4 * E2] = c4[r];
+      }
+    } else {
+      for (int t = wave; t < MTL * 6; t += NW) {
+        const int mt = t / 6, nt = t - (t / 6) * 6;
+        const double *ap = io.lfrag + (size_t)mt * KSL * 64 + lane;
+        const double *bp = sF + (nt * NF4 + lrow) * E2 + lcol;
+        mf_d4 c4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KSL; ++ks) c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[ks * 64], bp[ks * 4 * E2], c4, 0, 0, 0);
+        double *lp = sA + (nt * NP + mt * 16 + lrow) * E2 + lcol;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (mt * 16 + r * 4 + lrow < NP) lp[r * 4 * E2] = c4[r];
+      }
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) acc[c] += sA[(c * NP + i) * E2 + el];
+  double po[6], srcv[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
+  if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
+  }
+  node_update<EPI>(io, base, sm + el * 12, acc, own, po, srcv);
+}
+
 template <int NP>
 struct Nfp {
   static constexpr int v = NP == 4 ? 3 : NP == 10 ? 6 : NP == 20 ? 10 : 15;
@@ -1446,7 +1665,8 @@ inline int64_t resident_blocks(const void *kernel) {
 // and LIFT in LDS over a grid-stride loop, 6 = k_fused4 (FP64 MFMA volume),
 // 7 = k_fused4 with LIFT on MFMA too, 8 = k_fused5 (MFMA volume and LIFT,
 // linear operand addressing), 9 = variant 5 and 10 = variant 8 with the
-// XCD-aware group order. Chosen by the
+// XCD-aware group order, 11 = k_fused7 (curl in registers; Np <= 20, else
+// variant 5). Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -1482,6 +1702,10 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
     else
       hipLaunchKernelGGL((k_fused5<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))), dim3(Cfg::NT),
                          0, s, a);
+  } else if (variant == 11 && a.nbase && NP <= 20) {
+    const int64_t groups = (a.n_elem + 15) / 16;
+    hipLaunchKernelGGL((k_fused7<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))),
+                       dim3(Mfma7Cfg<NP>::NT), 0, s, a);
   } else if (variant == 9) {
     static int64_t resident = 0;
     if (resident == 0)

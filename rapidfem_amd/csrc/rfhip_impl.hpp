@@ -101,6 +101,7 @@ struct StageArgs {
   const double *lift;     // LIFT column-major: lift[fm*Np + i]
   const double *afrag;    // FP64 MFMA A fragments of [Dr; Ds; Dt] (rfhip_fused.hip k_fused4)
   const double *lfrag;    // FP64 MFMA A fragments of LIFT (k_fused4 with LIFT on the matrix cores)
+  const double *afrag7;   // node-interleaved A fragments of [Dr; Ds; Dt] (k_fused7)
   const double *jinv;     // 9K
   const int *fcode;       // 4K
   const int *mat;         // K
@@ -139,7 +140,7 @@ struct rfhip_op {
   // device operator data
   rfhip::DevBuf dr, ds, dt, lift, face_nodes, jinv, inv_eps, inv_mu, sig_eps, sig_mu, normal, fscale, nbr,
       nbr_lf, port, perm;
-  rfhip::DevBuf dT, liftc, afrag, lfrag, jinv64, fcode, mat, mat_tab, nbr_node, nbase;
+  rfhip::DevBuf dT, liftc, afrag, lfrag, afrag7, jinv64, fcode, mat, mat_tab, nbr_node, nbase;
   int64_t n_vec = 0;  // length of device vectors (n_dof)
   int order = 0;
   // state registers (precision depends on form)
