@@ -145,6 +145,30 @@ int rfhip_op_transient_probes_f64(rfhip_op *op, const double *y0, double dt,
                                   int64_t n_probes, double *rows,
                                   double *y_final);
 
+/* Port modal extraction on the device (rhs.rs:1211-1242
+ * port_modal_projections, recorded per frame as TdSession::port_signals,
+ * session.rs:1041-1054, does from full downloaded states). Port `port` is
+ * given by its n_faces (element, local face) pairs, the mode's transverse
+ * profiles at each face node (profiles: n_faces*Nfp*6 doubles, e_t xyz then
+ * h_t xyz, face-node order of face_nodes) and the quadrature weights
+ * 2*area*face_node_weight (weights: n_faces*Nfp). Setting a port again
+ * replaces it; ports are numbered 0..rfhip_op_n_ports()-1 (unset ones are
+ * empty and record 0). */
+int rfhip_op_set_port(rfhip_op *op, int32_t port, int64_t n_faces,
+                      const int32_t *elem, const int32_t *local_face,
+                      const double *profiles, const double *weights);
+int rfhip_op_n_ports(const rfhip_op *op);
+/* Explicit transient recording (P_e, P_h) of every port per frame on the
+ * device: signals receives (steps+1)*n_ports*2 doubles (row 0 from y0),
+ * y_final (may be NULL) the final state. Arguments otherwise as
+ * rfhip_op_transient_f64. */
+int rfhip_op_transient_ports_f64(rfhip_op *op, const double *y0, double dt,
+                                 int64_t steps, int64_t substeps,
+                                 int drive_kind, int64_t point_dof,
+                                 const double *src_vec, const double *g,
+                                 int64_t n_g, double *signals,
+                                 double *y_final);
+
 /* session.rs:130-140 Controller, all f64. */
 typedef struct {
   double atol, rtol, safety, growth_limit, shrink_limit, pi_alpha, pi_beta,

@@ -307,3 +307,21 @@ def err_norm(ctl, y_old, y_new, err):
 
 def factor(ctl, e, prev, reject):
     return float(lib().orc_factor(C.byref(ctl), float(e), float(prev), 1 if reject else 0))
+
+
+def port_projections(y, dof, prof, w):
+    """rhs.rs:1211-1242 port_modal_projections for one port given as
+    per-face-node entries: dof[q] = AoS index of the node's first field
+    value, prof[q] = (e_t, h_t), w[q] = 2*area*face-node weight. Returns
+    (P_e, P_h)."""
+    y = np.asarray(y, np.float64)
+    dof = np.asarray(dof, np.int64).reshape(-1)
+    prof = np.asarray(prof, np.float64).reshape(-1, 6)
+    w = np.asarray(w, np.float64).reshape(-1)
+    ef = y[dof[:, None] + np.arange(3)]
+    hf = y[dof[:, None] + 3 + np.arange(3)]
+    e_dot = float(np.sum(w * np.sum(ef * prof[:, :3], axis=1)))
+    h_dot = float(np.sum(w * np.sum(hf * prof[:, 3:], axis=1)))
+    e_norm = float(np.sum(w * np.sum(prof[:, :3] ** 2, axis=1)))
+    h_norm = float(np.sum(w * np.sum(prof[:, 3:] ** 2, axis=1)))
+    return (e_dot / e_norm if e_norm > 0 else 0.0, h_dot / h_norm if h_norm > 0 else 0.0)

@@ -855,6 +855,7 @@ int rfhip_op_create(rfhip_ctx *ctx, const rfhip_op_desc *d, int form, rfhip_op *
     op->alpha = d->flux_alpha;
     RF_HIP(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
     op->port_host.assign(d->face_port, d->face_port + 4 * d->n_elem);
+    op->face_nodes_host.assign(d->face_nodes, d->face_nodes + 4 * d->nfp);
     op->dev = DevOp{};
     op->dev.np = d->np;
     op->dev.nfp = d->nfp;
@@ -985,6 +986,90 @@ int rfhip_op_transient_probes_f64(rfhip_op *op, const double *y0, double dt, int
     if (n_probes > 0 && steps > 0)
       RF_HIP(hipMemcpyAsync(rows + n_probes, op->probe_rows.p, 8 * (size_t)steps * (size_t)n_probes,
                             hipMemcpyDeviceToHost, s));
+    sync(op);
+    if (y_final) state_to_host_f64(op, y_final);
+  });
+}
+
+int rfhip_op_set_port(rfhip_op *op, int32_t port, int64_t n_faces, const int32_t *elem, const int32_t *local_face,
+                      const double *profiles, const double *weights) {
+  return guarded([&] {
+    RF_REQUIRE(op, "null argument");
+    RF_REQUIRE(port >= 0 && port < 65536, "port index out of range");
+    RF_REQUIRE(n_faces >= 0 && (n_faces == 0 || (elem && local_face && profiles && weights)), "bad port arrays");
+    const int nfp = op->nfp, np = op->np;
+    rfhip_op::PortEntries pe;
+    pe.dof.resize((size_t)n_faces * nfp);
+    pe.prof.assign(profiles, profiles + (size_t)n_faces * nfp * 6);
+    pe.w.assign(weights, weights + (size_t)n_faces * nfp);
+    for (int64_t q = 0; q < n_faces; ++q) {
+      RF_REQUIRE(elem[q] >= 0 && elem[q] < op->n_elem && local_face[q] >= 0 && local_face[q] < 4,
+                 "port face out of range");
+      for (int m = 0; m < nfp; ++m)
+        pe.dof[(size_t)q * nfp + m] = ((int64_t)elem[q] * np + op->face_nodes_host[local_face[q] * nfp + m]) * 6;
+    }
+    for (size_t q = 0; q < pe.w.size(); ++q) {
+      const double *pr = &pe.prof[6 * q];
+      pe.e_norm += pe.w[q] * (pr[0] * pr[0] + pr[1] * pr[1] + pr[2] * pr[2]);
+      pe.h_norm += pe.w[q] * (pr[3] * pr[3] + pr[4] * pr[4] + pr[5] * pr[5]);
+    }
+    if ((int)op->ports.size() <= port) op->ports.resize(port + 1);
+    op->ports[port] = std::move(pe);
+    op->ports_dirty = true;
+  });
+}
+
+int rfhip_op_n_ports(const rfhip_op *op) { return op ? (int)op->ports.size() : -1; }
+
+int rfhip_op_transient_ports_f64(rfhip_op *op, const double *y0, double dt, int64_t steps, int64_t substeps, int kind,
+                                 int64_t dof, const double *src, const double *g, int64_t n_g, double *signals,
+                                 double *y_final) {
+  return guarded([&] {
+    RF_REQUIRE(op && y0 && signals, "null argument");
+    RF_REQUIRE(steps >= 0, "negative step count");
+    substeps = std::max<int64_t>(1, substeps);
+    check_drive(op, kind, dof, src, n_g, steps * substeps, g);
+    load_source(op, kind, src);
+    hipStream_t s = op->stream;
+    const int np_ = (int)op->ports.size();
+    if (np_ > 0 && op->ports_dirty) {
+      std::vector<int64_t> dofs, off(np_ + 1, 0);
+      std::vector<double> prof, w, norm(2 * np_);
+      for (int p = 0; p < np_; ++p) {
+        const auto &pe = op->ports[p];
+        dofs.insert(dofs.end(), pe.dof.begin(), pe.dof.end());
+        prof.insert(prof.end(), pe.prof.begin(), pe.prof.end());
+        w.insert(w.end(), pe.w.begin(), pe.w.end());
+        off[p + 1] = (int64_t)dofs.size();
+        norm[2 * p] = pe.e_norm;
+        norm[2 * p + 1] = pe.h_norm;
+      }
+      if (dofs.empty()) {  // keep the device pointers valid
+        dofs.push_back(0);
+        prof.assign(6, 0.0);
+        w.push_back(0.0);
+      }
+      upload(op->port_dof, dofs.data(), dofs.size(), s);
+      upload(op->port_prof, prof.data(), prof.size(), s);
+      upload(op->port_w, w.data(), w.size(), s);
+      upload(op->port_off, off.data(), off.size(), s);
+      upload(op->port_norm, norm.data(), norm.size(), s);
+      sync(op);
+      op->ports_dirty = false;
+    }
+    const size_t row = 2 * (size_t)np_;
+    if (np_ > 0) op->port_rows.alloc(8 * row * (size_t)(steps + 1));
+    state_from_host_f64(op, y0);
+    transient_impl<double>(op, steps, substeps, dt, kind, dof, g, true, [&](int64_t r) {
+      if (r < 0 || np_ == 0) return;
+      const int cur = is_fused(op->form) ? op->cur : 0;
+      const int kind_y = op->form == RFHIP_FORM_FUSED_F64C ? 2 : op->form == RFHIP_FORM_PLAIN_F32 ? 1 : 0;
+      launch_port_proj(op->y[cur].p, op->ylo[cur].as<uint16_t>(), kind_y, op->port_dof.as<int64_t>(),
+                       op->port_prof.as<double>(), op->port_w.as<double>(), op->port_off.as<int64_t>(),
+                       op->port_norm.as<double>(), np_, op->port_rows.as<double>() + r * row, s);
+    });
+    if (np_ > 0)
+      RF_HIP(hipMemcpyAsync(signals, op->port_rows.p, 8 * row * (size_t)(steps + 1), hipMemcpyDeviceToHost, s));
     sync(op);
     if (y_final) state_to_host_f64(op, y_final);
   });

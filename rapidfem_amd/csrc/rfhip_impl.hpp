@@ -137,6 +137,7 @@ struct rfhip_op {
   hipStream_t stream = nullptr;
   // host copies used by the host-side logic
   std::vector<int32_t> port_host;  // 4K
+  std::vector<int32_t> face_nodes_host;  // 4*Nfp
   // device operator data
   rfhip::DevBuf dr, ds, dt, lift, face_nodes, jinv, inv_eps, inv_mu, sig_eps, sig_mu, normal, fscale, nbr,
       nbr_lf, port, perm;
@@ -150,6 +151,15 @@ struct rfhip_op {
   rfhip::DevBuf kst, kdtf, kerr, kbak, red, aos;
   rfhip::DevBuf kb_basis, kb_w, kb_proj, kb_part, kb_out, kb_x;
   rfhip::DevBuf probe_idx, probe_rows;  // device-side probe recording
+  // port modal extraction: host-side per-port entries, device copies
+  struct PortEntries {
+    std::vector<int64_t> dof;
+    std::vector<double> prof, w;
+    double e_norm = 0.0, h_norm = 0.0;
+  };
+  std::vector<PortEntries> ports;
+  rfhip::DevBuf port_dof, port_prof, port_w, port_off, port_norm, port_rows;
+  bool ports_dirty = true;
   int kb_cap = 0;
   rfhip::DevOp dev{};
   rfhip::StageArgs sa{};  // fused forms: geometry part filled at creation
@@ -174,6 +184,10 @@ void launch_c_to_f64(const float *hi, const uint16_t *lo, double *dst, int64_t n
 void launch_cast_d2f(const double *src, float *dst, int64_t n, hipStream_t s);
 void launch_cast_f2d(const float *src, double *dst, int64_t n, hipStream_t s);
 void launch_fill(void *dst, int64_t nbytes, hipStream_t s);
+// Port modal projections (P_e, P_h) of every port into out[2p], out[2p+1].
+void launch_port_proj(const void *y, const uint16_t *lo, int kind, const int64_t *dof, const double *prof,
+                      const double *w, const int64_t *off, const double *norm, int n_ports, double *out,
+                      hipStream_t s);
 // out[q] = y[idx[q]] as f64; kind 0 f64 state, 1 f32, 2 f32 hi + bf16 lo.
 void launch_gather(const void *y, const uint16_t *lo, int kind, const int64_t *idx, int64_t n, double *out,
                    hipStream_t s);

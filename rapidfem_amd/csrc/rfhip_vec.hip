@@ -52,6 +52,51 @@ __global__ void __launch_bounds__(kBS) k_gather(const void *__restrict__ y, cons
   }
 }
 
+// Port modal projections (rhs.rs:1211-1242) of the current state, one block
+// per port: out[2p] = sum_w w (E.e_t) / e_norm, out[2p+1] = sum_w w (H.h_t) /
+// h_norm over the port's face-node entries [off[p], off[p+1]). Entry q has
+// the AoS index of its node's first field value, the profile (e_t, h_t) and
+// the weight 2 * area * face-node weight.
+__global__ void __launch_bounds__(kBS) k_port_proj(const void *__restrict__ y, const uint16_t *__restrict__ lo,
+                                                   int kind, const int64_t *__restrict__ dof,
+                                                   const double *__restrict__ prof, const double *__restrict__ w,
+                                                   const int64_t *__restrict__ off, const double *__restrict__ norm,
+                                                   double *__restrict__ out) {
+  __shared__ double red[2][kBS];
+  const int p = blockIdx.x;
+  double se = 0.0, sh = 0.0;
+  for (int64_t q = off[p] + threadIdx.x; q < off[p + 1]; q += kBS) {
+    const int64_t b = dof[q];
+    double v[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      if (kind == 0)
+        v[c] = static_cast<const double *>(y)[b + c];
+      else if (kind == 1)
+        v[c] = (double)static_cast<const float *>(y)[b + c];
+      else
+        v[c] = (double)static_cast<const float *>(y)[b + c] + (double)bf16_to_f(lo[b + c]);
+    }
+    const double *pr = prof + 6 * q;
+    se += w[q] * (v[0] * pr[0] + v[1] * pr[1] + v[2] * pr[2]);
+    sh += w[q] * (v[3] * pr[3] + v[4] * pr[4] + v[5] * pr[5]);
+  }
+  red[0][threadIdx.x] = se;
+  red[1][threadIdx.x] = sh;
+  __syncthreads();
+  for (int st = kBS / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + st];
+      red[1][threadIdx.x] += red[1][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * p] = norm[2 * p] > 0.0 ? red[0][0] / norm[2 * p] : 0.0;
+    out[2 * p + 1] = norm[2 * p + 1] > 0.0 ? red[1][0] / norm[2 * p + 1] : 0.0;
+  }
+}
+
 __global__ void __launch_bounds__(kBS) k_c_to_f64(const float *__restrict__ hi, const uint16_t *__restrict__ lo,
                                                   double *__restrict__ d, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS)
@@ -269,6 +314,15 @@ void launch_gather(const void *y, const uint16_t *lo, int kind, const int64_t *i
                    hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 1024)), dim3(kBS), 0, s, y, lo, kind, idx, n, out);
+  RF_HIP(hipGetLastError());
+}
+
+void launch_port_proj(const void *y, const uint16_t *lo, int kind, const int64_t *dof, const double *prof,
+                      const double *w, const int64_t *off, const double *norm, int n_ports, double *out,
+                      hipStream_t s) {
+  if (n_ports <= 0) return;
+  hipLaunchKernelGGL(k_port_proj, dim3((unsigned)n_ports), dim3(kBS), 0, s, y, lo, kind, dof, prof, w, off, norm,
+                     out);
   RF_HIP(hipGetLastError());
 }
 

@@ -178,6 +178,36 @@ class GpuOperator:
                                                     rows, L.ptr(yf)))
         return rows[:(steps + 1) * pr.shape[0]].reshape(steps + 1, pr.shape[0]), yf
 
+    # ---- port modal extraction on the device (rhs.rs:1211-1242) ----
+    def set_port(self, port: int, elem, local_face, profiles, weights) -> None:
+        """Port `port`: (element, local face) pairs, (e_t, h_t) profiles per
+        face node (n_faces, Nfp, 6) and weights 2*area*face-node weight
+        (n_faces, Nfp)."""
+        elem = np.ascontiguousarray(elem, np.int32)
+        lf = np.ascontiguousarray(local_face, np.int32)
+        prof = np.ascontiguousarray(profiles, np.float64).reshape(-1)
+        w = np.ascontiguousarray(weights, np.float64).reshape(-1)
+        L.check(L.lib.rfhip_op_set_port(self._h, int(port), elem.shape[0], elem, lf, prof, w))
+
+    @property
+    def n_ports(self) -> int:
+        return int(L.lib.rfhip_op_n_ports(self._h))
+
+    def transient_ports(self, ctx, y0, dt, steps, substeps=1, drive: Drive | None = None, g=None):
+        """Explicit transient recording (P_e, P_h) of every port per frame on
+        the device: ((steps+1, n_ports, 2) signals, final state)."""
+        drive = drive or Drive.free()
+        y0 = self._vec(y0, np.float64)
+        substeps = max(1, int(substeps))
+        gg = None if g is None else np.ascontiguousarray(g, np.float64)
+        n_g = 0 if gg is None else gg.shape[0]
+        npo = self.n_ports
+        sig = np.zeros(max(1, (steps + 1) * npo * 2))
+        yf = np.empty(self._n)
+        L.check(L.lib.rfhip_op_transient_ports_f64(self._h, y0, float(dt), int(steps), substeps, drive.kind,
+                                                   drive.dof, L.ptr(drive.vec), L.ptr(gg), n_g, sig, L.ptr(yf)))
+        return sig[:(steps + 1) * npo * 2].reshape(steps + 1, npo, 2), yf
+
     # gpu/operator.rs:823-893
     def transient_kcl(self, ctx, y0, dt, steps, drive: Drive | None = None, g=None, ctl: Controller | None = None,
                       precision="f32"):

@@ -213,3 +213,39 @@ class MaxwellOperator:
             setattr(d, k, v.ctypes.data_as(C.c_void_p))
         d.flux_alpha = self.flux_alpha
         return d, list(arrs.values())
+
+
+def port_face_data(op: MaxwellOperator, faces, profile):
+    """Arrays of a port for GpuOperator.set_port, as MaxwellOperator's
+    PortData holds them (rhs.rs:319-329): `faces` is a list of (element,
+    local face) boundary faces, `profile(xyz) -> (e_t, h_t)` the mode's
+    transverse profiles at points xyz (n, 3). Weights are 2*area times the
+    face-node weight (rhs.rs:1224-1228), area = fscale*|det J|/2."""
+    faces = np.asarray(faces, np.int64).reshape(-1, 2)
+    re = op.re
+    nfp = re.n_face_nodes
+    x = op.node_coords()
+    elem = faces[:, 0].astype(np.int32)
+    lf = faces[:, 1].astype(np.int32)
+    pts = x[faces[:, 0][:, None], re.face_nodes[faces[:, 1]]]  # (n_faces, Nfp, 3)
+    et, ht = profile(pts.reshape(-1, 3))
+    prof = np.concatenate([np.asarray(et, np.float64).reshape(-1, 3), np.asarray(ht, np.float64).reshape(-1, 3)],
+                          axis=1).reshape(len(faces), nfp, 6)
+    area = op.fscale[faces[:, 0], faces[:, 1]] * np.abs(op.det[faces[:, 0]]) / 2.0
+    w = 2.0 * area[:, None] * re.face_node_weights[faces[:, 1]]
+    return elem, lf, prof, w
+
+
+def boundary_faces_on_plane(op: MaxwellOperator, axis: int, value: float, tol: float = 1e-9):
+    """(element, local face) of every boundary face lying in the plane
+    x[axis] = value."""
+    x = op.node_coords()
+    re = op.re
+    out = []
+    for e in np.nonzero((op.nbr < 0).any(axis=1))[0]:
+        for f in range(4):
+            if op.nbr[e, f] >= 0:
+                continue
+            if np.all(np.abs(x[e, re.face_nodes[f], axis] - value) < tol):
+                out.append((int(e), f))
+    return out

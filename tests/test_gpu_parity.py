@@ -348,3 +348,46 @@ def test_probe_recording_rejects_bad_dofs(ctx, oracle):
     with pytest.raises(RfhipError) as ei:
         g.transient_probes(ctx, y0, 0.01, 2, [0, oop.n_dof])
     assert ei.value.code == 1
+
+
+@pytest.mark.parametrize("form", ALL_FORMS)
+def test_port_signals_on_device_match_oracle(ctx, oracle, form):
+    """rfhip_op_transient_ports_f64: (P_e, P_h) of two ports recorded per
+    frame on the device against the oracle's projections (rhs.rs:1211-1242)
+    of the oracle trajectory; ports on the x = 0 and x = 1 walls, point
+    drive."""
+    from rapidfem_amd import Drive, maxwell
+
+    oop, y0 = _te101(oracle, 3, 2)
+    xyz, tets = _mesh(oracle, 3)
+    mop = maxwell.MaxwellOperator(maxwell.Mesh(xyz, tets), 2, 1.0)
+    g = _gop(ctx, oop, form)
+
+    def prof(x):
+        et = np.stack([0 * x[:, 0], np.sin(np.pi * x[:, 2]), 0 * x[:, 0]], axis=1)
+        ht = np.stack([0 * x[:, 0], 0 * x[:, 0], np.sin(np.pi * x[:, 2])], axis=1)
+        return et, ht
+
+    entries = []
+    for port, val in enumerate((0.0, 1.0)):
+        faces = maxwell.boundary_faces_on_plane(mop, 0, val)
+        elem, lf, pr, w = maxwell.port_face_data(mop, faces, prof)
+        g.set_port(port, elem, lf, pr, w)
+        dof = ((elem[:, None].astype(np.int64) * mop.re.n_nodes + mop.re.face_nodes[lf]) * 6).reshape(-1)
+        entries.append((dof, pr, w))
+    assert g.n_ports == 2
+    n = oop.n_dof
+    steps, sub, dt = 4, 3, 0.06
+    wave = np.cos(0.5 * np.arange(steps * sub))
+    sig, yf = g.transient_ports(ctx, y0, dt, steps, sub, Drive.point(n // 3), wave)
+    assert sig.shape == (steps + 1, 2, 2)
+    y = y0
+    tol = {"plain_f32": 1e-3, "plain_f64": 1e-11, "fused_f64": 1e-11, "fused_f64c": 1e-6}[form]
+    for k in range(steps + 1):
+        if k > 0:
+            y = oop.lserk(y, dt / sub, sub, ("point", n // 3), wave[(k - 1) * sub:k * sub])
+        for p, (dof, pr, w) in enumerate(entries):
+            want = np.array(oracle.port_projections(y, dof, pr, w))
+            scale = max(np.abs(want).max(), 1e-3)
+            assert np.abs(sig[k, p] - want).max() < tol * scale, (k, p, sig[k, p], want)
+    assert rel_l2(yf, y) < (1e-3 if form == "plain_f32" else 1e-6)

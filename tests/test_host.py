@@ -196,3 +196,30 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_port_face_data_weights_and_projection(oracle):
+    """Port extraction arrays (rhs.rs:319-329, 1211-1242): the weights of the
+    x = 0 wall of the unit box sum to its area 1, and a state equal to the
+    profile projects to P_e = P_h = 1 (closed forms)."""
+    from rapidfem_amd.maxwell import MaxwellOperator, boundary_faces_on_plane, port_face_data, structured_box_jittered
+
+    mesh = structured_box_jittered(3, 3, 3, 1.0, 1.0, 1.0, 0.1, 4)
+    op = MaxwellOperator(mesh, 2)
+    faces = boundary_faces_on_plane(op, 0, 0.0)
+    assert len(faces) == 2 * 9
+
+    def prof(x):
+        et = np.stack([0 * x[:, 0], np.sin(np.pi * x[:, 2]), 0 * x[:, 0]], axis=1)
+        ht = np.stack([0 * x[:, 0], 0 * x[:, 0], np.sin(np.pi * x[:, 1])], axis=1)
+        return et, ht
+
+    elem, lf, pr, w = port_face_data(op, faces, prof)
+    assert abs(w.sum() - 1.0) < 1e-12
+    y = np.zeros(op.n_dof)
+    nfp = op.re.n_face_nodes
+    dof = ((elem[:, None].astype(np.int64) * op.re.n_nodes + op.re.face_nodes[lf]) * 6).reshape(-1)
+    y[dof[:, None] + np.arange(6)] = pr.reshape(-1, 6)
+    pe, ph = oracle.port_projections(y, dof, pr, w)
+    assert abs(pe - 1.0) < 1e-14 and abs(ph - 1.0) < 1e-14
+    assert pr.shape == (len(faces), nfp, 6)
