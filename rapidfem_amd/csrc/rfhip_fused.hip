@@ -567,15 +567,18 @@ __device__ __forceinline__ GroupRange group_range(bool xcd, int64_t ngroups) {
 //  * DLDS: D_k^T and LIFT^T live in LDS and the block walks over element
 //    groups (grid-stride), so the operator matrices are read from L2 once
 //    per block instead of through L1 by every wave.
-template <int NP, int NFP, bool CIN, int EPI, bool DLDS, bool PREV, bool XCD = false>
-__global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
+template <int NP, int NFP, bool CIN, int EPI, bool DLDS, bool PREV, bool XCD = false, bool SPF = false, int UNR = 4>
+__global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3))) k_fused3(StageArgs io) {
   constexpr int EPB = kFusedWG / NP;
   constexpr int NF4 = 4 * NFP;
   constexpr int FPT = (NF4 + NP - 1) / NP;  // face nodes per thread
   constexpr int ND = 3 * NP * NP;
   constexpr int SD = DLDS ? ND + NF4 * NP : 1;
   __shared__ __align__(16) double su[EPB * NP * 6];
-  __shared__ __align__(16) double sf[EPB * NF4 * 6];
+  // per-element flux stride padded by two doubles: the elements of a wave
+  // then start on different LDS banks
+  constexpr int SFS = NF4 * 6 + 2;
+  __shared__ __align__(16) double sf[EPB * SFS];
   __shared__ __align__(16) double sg[EPB * 16];  // per face: n_x, n_y, n_z, 0.5*fscale
   __shared__ __align__(16) double sd[SD];
   __shared__ __align__(16) double sm[EPB * 12];  // material row per element
@@ -590,12 +593,55 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
   const int le = tid / NP, i = tid - (tid / NP) * NP;
   const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;
   const GroupRange gr = group_range(XCD, ngroups);
+  // SPF: the state of a block's next group is read into registers while the
+  // current group computes, and stored to LDS at the next group's start.
+  constexpr int PPT = (EPB * NP * 3 + kFusedWG - 1) / kFusedWG;  // state pairs per thread
+  using HT = typename std::conditional<CIN, float2, double2>::type;
+  HT sh[SPF ? PPT : 1];
+  uint32_t sl[SPF ? PPT : 1];
+  auto prefetch_state = [&](int64_t g) {
+    const int64_t rg = io.n_elem - g * EPB;
+    const int cnt = (rg < EPB ? (int)rg : EPB) * NP * 3;
+    const int64_t gb = g * EPB * NP * 6;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int q = tid + k * kFusedWG;
+      if (q < cnt) {
+        if constexpr (CIN) {
+          sh[k] = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + gb)[q];
+          sl[k] = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + gb)[q];
+        } else {
+          sh[k] = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + gb)[q];
+        }
+      }
+    }
+  };
+  if constexpr (SPF) {
+    if (gr.first < gr.end) prefetch_state(gr.first);
+  }
   for (int64_t grp = gr.first; grp < gr.end; grp += gr.step) {
     const int64_t e0 = grp * EPB;
     const int64_t rem = io.n_elem - e0;
     const int nel = rem < EPB ? (int)rem : EPB;
     const bool active = tid < nel * NP;
     const int64_t e = e0 + le;
+    if constexpr (SPF) {
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        const int q = tid + k * kFusedWG;
+        if (q < nel * NP * 3) {
+          double2 v;
+          if constexpr (CIN) {
+            v.x = (double)sh[k].x + (double)__uint_as_float(sl[k] << 16);
+            v.y = (double)sh[k].y + (double)__uint_as_float(sl[k] & 0xFFFF0000u);
+          } else {
+            v = sh[k];
+          }
+          reinterpret_cast<double2 *>(su)[q] = v;
+        }
+      }
+      if (grp + gr.step < gr.end) prefetch_state(grp + gr.step);
+    }
 
     // 1. neighbour addresses (and with PREV the traces) of this thread's
     //    face nodes, in flight early.
@@ -652,7 +698,7 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
       for (int q = i; q < 12; q += NP) sm[le * 12 + q] = io.mat_tab[12 * io.mat[e] + q];
     }
     // 2. the group's state into LDS, two values per lane and load.
-    {
+    if constexpr (!SPF) {
       const int64_t g0 = e0 * NP * 6;
       const int cnt2 = nel * NP * 3;
       for (int q = tid; q < cnt2; q += kFusedWG) {
@@ -700,7 +746,7 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
       for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
-#pragma unroll 4
+#pragma unroll UNR
       for (int j = 0; j < NP; ++j) {
         double u[6];
 #pragma unroll
@@ -771,7 +817,7 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
           fh1 = cf * (a * (nh * n1 - jh1) + (n2 * je0 - n0 * je2));
           fh2 = cf * (a * (nh * n2 - jh2) + (n0 * je1 - n1 * je0));
         }
-        double2 *out = reinterpret_cast<double2 *>(sf + (le * NF4 + fm) * 6);
+        double2 *out = reinterpret_cast<double2 *>(sf + le * SFS + fm * 6);
         out[0] = make_double2(fe0, fe1);
         out[1] = make_double2(fe2, fh0);
         out[2] = make_double2(fh1, fh2);
@@ -779,8 +825,8 @@ __global__ void __launch_bounds__(kFusedWG) k_fused3(StageArgs io) {
     }
     __syncthreads();
     if (active) {
-      const double *fb = sf + le * NF4 * 6;
-#pragma unroll 4
+      const double *fb = sf + le * SFS;
+#pragma unroll UNR
       for (int fm = 0; fm < NF4; ++fm) {
         const double w = lift[fm * NP + i];
 #pragma unroll
@@ -1666,7 +1712,8 @@ inline int64_t resident_blocks(const void *kernel) {
 // 7 = k_fused4 with LIFT on MFMA too, 8 = k_fused5 (MFMA volume and LIFT,
 // linear operand addressing), 9 = variant 5 and 10 = variant 8 with the
 // XCD-aware group order, 11 = k_fused7 (curl in registers; Np <= 20, else
-// variant 5). Chosen by the
+// variant 5), 12/13 = variant 5 with the next group's state read ahead
+// (loops unrolled by 4 / 2). Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -1706,6 +1753,22 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
     const int64_t groups = (a.n_elem + 15) / 16;
     hipLaunchKernelGGL((k_fused7<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))),
                        dim3(Mfma7Cfg<NP>::NT), 0, s, a);
+  } else if (variant == 12 || variant == 13) {
+    // variant 5 with the next group's state read ahead (12), and with loops
+    // unrolled by 2 instead of 4 (13)
+    const void *kf = variant == 12
+                         ? reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 4>)
+                         : reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 2>);
+    static int64_t resident[2] = {0, 0};
+    int64_t &res = resident[variant - 12];
+    if (res == 0) res = resident_blocks(kf);
+    const int64_t grid = groups1 < res ? groups1 : res;
+    if (variant == 12)
+      hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 4>), dim3((unsigned)grid),
+                         dim3(kFusedWG), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 2>), dim3((unsigned)grid),
+                         dim3(kFusedWG), 0, s, a);
   } else if (variant == 9) {
     static int64_t resident = 0;
     if (resident == 0)
