@@ -538,6 +538,23 @@ __global__ void __launch_bounds__(kFusedWG) k_fused2(StageArgs io) {
   }
 }
 
+// Unit normal and 0.5*fscale of local face f from J^-1 (rows of ji):
+// g_f = {-row2, -row0, -row1, row0+row1+row2}, n = g/|g|, fscale = |g|.
+__device__ __forceinline__ void face_geom(const double *ji, int f, double *n, double &cf) {
+  double g[3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    const double r0 = ji[x], r1 = ji[3 + x], r2 = ji[6 + x];
+    g[x] = f == 0 ? -r2 : f == 1 ? -r0 : f == 2 ? -r1 : (r0 + r1 + r2);
+  }
+  const double len = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  const double inv = 1.0 / len;
+  n[0] = g[0] * inv;
+  n[1] = g[1] * inv;
+  n[2] = g[2] * inv;
+  cf = 0.5 * len;
+}
+
 // Order of element groups over the blocks. With xcd, blocks are taken to
 // run round-robin over the 8 XCDs (block b on XCD b % 8): XCD x walks the
 // contiguous groups [x*chunk, (x+1)*chunk), so the neighbours an element
@@ -567,7 +584,8 @@ __device__ __forceinline__ GroupRange group_range(bool xcd, int64_t ngroups) {
 //  * DLDS: D_k^T and LIFT^T live in LDS and the block walks over element
 //    groups (grid-stride), so the operator matrices are read from L2 once
 //    per block instead of through L1 by every wave.
-template <int NP, int NFP, bool CIN, int EPI, bool DLDS, bool PREV, bool XCD = false, bool SPF = false, int UNR = 4>
+template <int NP, int NFP, bool CIN, int EPI, bool DLDS, bool PREV, bool XCD = false, bool SPF = false, int UNR = 4,
+          bool LATE = false>
 __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3))) k_fused3(StageArgs io) {
   constexpr int EPB = kFusedWG / NP;
   constexpr int NF4 = 4 * NFP;
@@ -640,7 +658,9 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
           reinterpret_cast<double2 *>(su)[q] = v;
         }
       }
-      if (grp + gr.step < gr.end) prefetch_state(grp + gr.step);
+      // LATE: issued after this group's own reads instead (vector loads
+      // retire in issue order, so a wait on J^-1 would otherwise wait on it)
+      if (!LATE && grp + gr.step < gr.end) prefetch_state(grp + gr.step);
     }
 
     // 1. neighbour addresses (and with PREV the traces) of this thread's
@@ -720,6 +740,11 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
     if (active) {
 #pragma unroll
       for (int q = 0; q < 9; ++q) ji[q] = io.jinv[e * 9 + q];
+    }
+    if constexpr (SPF && LATE) {
+      if (grp + gr.step < gr.end) prefetch_state(grp + gr.step);
+    }
+    if (active && !LATE) {
       if (i < 4) {
         double g[3];
 #pragma unroll
@@ -777,8 +802,20 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
         const int fm = i + s * NP;
         if (fm >= NF4) continue;
         const int f = fm / NFP, m = fm - f * NFP;
-        const double *gq = sg + le * 16 + f * 4;
-        const double n0 = gq[0], n1 = gq[1], n2 = gq[2], cf = gq[3];
+        double n0, n1, n2, cf;
+        if constexpr (LATE) {  // geometry here, so the group start need not wait for J^-1
+          double nn[3];
+          face_geom(ji, f, nn, cf);
+          n0 = nn[0];
+          n1 = nn[1];
+          n2 = nn[2];
+        } else {
+          const double *gq = sg + le * 16 + f * 4;
+          n0 = gq[0];
+          n1 = gq[1];
+          n2 = gq[2];
+          cf = gq[3];
+        }
         const int vi = io.face_nodes[f * NFP + m];
         const double *uv = ub + vi * 6;
         const double e0v = uv[0], e1v = uv[1], e2v = uv[2], h0 = uv[3], h1 = uv[4], h2 = uv[5];
@@ -875,23 +912,6 @@ struct MfmaCfg {
   static constexpr int CH = EPB * 3;                         // columns of one field
   static constexpr int NTL = (CH + 15) / 16;                 // 16-column tiles
 };
-
-// Unit normal and 0.5*fscale of local face f from J^-1 (rows of ji):
-// g_f = {-row2, -row0, -row1, row0+row1+row2}, n = g/|g|, fscale = |g|.
-__device__ __forceinline__ void face_geom(const double *ji, int f, double *n, double &cf) {
-  double g[3];
-#pragma unroll
-  for (int x = 0; x < 3; ++x) {
-    const double r0 = ji[x], r1 = ji[3 + x], r2 = ji[6 + x];
-    g[x] = f == 0 ? -r2 : f == 1 ? -r0 : f == 2 ? -r1 : (r0 + r1 + r2);
-  }
-  const double len = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-  const double inv = 1.0 / len;
-  n[0] = g[0] * inv;
-  n[1] = g[1] * inv;
-  n[2] = g[2] * inv;
-  cf = 0.5 * len;
-}
 
 // Upwind flux of one face node (rhs.rs:1372-1440) with unit normal n and
 // cf = 0.5*fscale, own trace u and neighbour trace v (interior faces):
@@ -1713,7 +1733,8 @@ inline int64_t resident_blocks(const void *kernel) {
 // linear operand addressing), 9 = variant 5 and 10 = variant 8 with the
 // XCD-aware group order, 11 = k_fused7 (curl in registers; Np <= 20, else
 // variant 5), 12/13 = variant 5 with the next group's state read ahead
-// (loops unrolled by 4 / 2). Chosen by the
+// (loops unrolled by 4 / 2), 14 = variant 13 with the read-ahead issued
+// after the group's own reads and the face geometry formed in the flux. Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -1753,6 +1774,14 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
     const int64_t groups = (a.n_elem + 15) / 16;
     hipLaunchKernelGGL((k_fused7<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))),
                        dim3(Mfma7Cfg<NP>::NT), 0, s, a);
+  } else if (variant == 14) {
+    static int64_t res = 0;
+    if (res == 0)
+      res = resident_blocks(
+          reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 2, true>));
+    const int64_t grid = groups1 < res ? groups1 : res;
+    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 2, true>), dim3((unsigned)grid),
+                       dim3(kFusedWG), 0, s, a);
   } else if (variant == 12 || variant == 13) {
     // variant 5 with the next group's state read ahead (12), and with loops
     // unrolled by 2 instead of 4 (13)
