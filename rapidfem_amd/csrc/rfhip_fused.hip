@@ -1710,10 +1710,584 @@ __global__ void __launch_bounds__(Mfma7Cfg<NP>::NT) k_fused7(StageArgs io) {
   node_update<EPI>(io, base, sm + el * 12, acc, own, po, srcv);
 }
 
+// Raw field values of one node as stored (f32 pairs + bf16 pairs, or f64
+// pairs), converted to f64 only where they are used, so a load needs no
+// wait at the place it is issued.
+template <bool CIN>
+struct TraceRaw;
+template <>
+struct TraceRaw<true> {
+  float2 h[3];
+  uint32_t l[3];
+};
+template <>
+struct TraceRaw<false> {
+  double2 d[3];
+};
+
+template <bool CIN>
+__device__ __forceinline__ void load_trace_raw(const StageArgs &io, int64_t nb, TraceRaw<CIN> &r) {
+  if constexpr (CIN) {
+    const float2 *hp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + nb);
+    const uint32_t *lp = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + nb);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      r.h[q] = hp[q];
+      r.l[q] = lp[q];
+    }
+  } else {
+    const double2 *dp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + nb);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) r.d[q] = dp[q];
+  }
+}
+
+template <bool CIN>
+__device__ __forceinline__ void trace_f64(const TraceRaw<CIN> &r, double *v) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if constexpr (CIN) {
+      v[2 * q] = (double)r.h[q].x + (double)__uint_as_float(r.l[q] << 16);
+      v[2 * q + 1] = (double)r.h[q].y + (double)__uint_as_float(r.l[q] & 0xFFFF0000u);
+    } else {
+      v[2 * q] = r.d[q].x;
+      v[2 * q + 1] = r.d[q].y;
+    }
+  }
+}
+
+#ifndef RF_K8_WPE
+#define RF_K8_WPE 3  // waves per SIMD the persistent kernel is compiled for
+#endif
+
+// Persistent k_fused7: each block walks over element groups and reads the
+// next group's state, neighbour-table entries and material indices into
+// registers while the current group computes, so a group starts without
+// waiting on HBM. The group's own J^-1, old p, material rows and neighbour
+// traces are requested at its start, ahead of the read-ahead (vector loads
+// retire in issue order), and are first used after the barrier that
+// publishes the state. All of these loads are unconditional (clamped to a
+// valid index on inactive lanes and boundary faces) so that no conversion
+// or wait is forced where they are issued. Face geometry is formed in the
+// flux; the face-node table lives in LDS.
+template <int NP, int NFP, bool CIN, int EPI>
+__global__ void __launch_bounds__(Mfma7Cfg<NP>::NT) __attribute__((amdgpu_waves_per_eu(RF_K8_WPE)))
+k_fused8(StageArgs io) {
+  using Cfg = Mfma7Cfg<NP>;
+  constexpr int EPB = 16, E2 = 17, NW = Cfg::NW, NT = Cfg::NT, KS = Cfg::KS;
+  constexpr int NF4 = 4 * NFP;
+  constexpr int FPT = (NF4 + NP - 1) / NP;
+  constexpr int MTL = (NP + 15) / 16, KSL = NF4 / 4;
+  constexpr int SS = 6 * NP * E2, SF = 6 * NF4 * E2;
+  constexpr bool LFIX = MTL <= NW;
+  constexpr int PPT = (EPB * NP * 3 + NT - 1) / NT;  // state pairs per thread
+  constexpr int MPT = (EPB * 12 + NT - 1) / NT;  // material entries per thread
+  __shared__ __align__(16) double sA[SS];  // state, then the lifted flux L[c][i][e]
+  __shared__ __align__(16) double sF[SF];  // face flux F[c][fm][e]
+  __shared__ __align__(16) double sm[EPB * 12];
+  __shared__ __align__(16) double sL[MTL * KSL * 64];  // LIFT A fragments
+  __shared__ int sfn[NF4];
+  const int wave = threadIdx.x >> 6;
+  const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;
+  for (int t = threadIdx.x; t < NF4; t += NT) sfn[t] = io.face_nodes[t];
+  for (int t = threadIdx.x; t < MTL * KSL * 64; t += NT) sL[t] = io.lfrag[t];
+  // Lane ids are recomputed from an opaque copy of threadIdx.x where they
+  // are used inside the group loop, so the compiler does not hoist
+  // per-lane 64-bit addresses out of it and hold them in registers.
+  auto lane_ids = [&](int &tid, int &lane, int &el, int &q, int &i) {
+    int tv = threadIdx.x;
+    asm volatile("" : "+v"(tv));
+    tid = tv;
+    lane = tv & 63;
+    el = lane & 15;
+    q = lane >> 4;
+    i = 4 * wave + q;
+  };
+  int tid, lane, el, q, i;
+  lane_ids(tid, lane, el, q, i);
+
+  double av[KS];
+  {
+    const double *ap = io.afrag7 + (size_t)wave * KS * 64 + lane;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) av[ks] = ap[ks * 64];
+  }
+  using HT = typename std::conditional<CIN, float2, double2>::type;
+  HT sh[PPT];
+  uint32_t sl[PPT];
+  int nbn[FPT];
+  int mnext[MPT];
+  auto read_ahead = [&](int64_t g) {
+    int tid, lane, el, q, i;
+    lane_ids(tid, lane, el, q, i);
+    const int64_t rg = io.n_elem - g * EPB;
+    const int ng = rg < EPB ? (int)rg : EPB;
+    const int64_t gb = g * EPB * NP * 6;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int t0 = tid + k * NT;
+      const int t = t0 < ng * NP * 3 ? t0 : ng * NP * 3 - 1;
+      if constexpr (CIN) {
+        sh[k] = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + gb)[t];
+        sl[k] = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + gb)[t];
+      } else {
+        sh[k] = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + gb)[t];
+      }
+    }
+    const int64_t eg = g * EPB + (el < ng ? el : ng - 1);
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      const int v = io.nbase[eg * NF4 + (fm < NF4 ? fm : NF4 - 1)];
+      nbn[s] = (i < NP && fm < NF4) ? v : kCodePec;
+    }
+#pragma unroll
+    for (int k = 0; k < MPT; ++k) {
+      const int t = tid + k * NT, te = t / 12;
+      mnext[k] = io.mat[g * EPB + (te < ng ? te : ng - 1)];
+    }
+  };
+  // The read-ahead state of a group with `ng` elements into sA, transposed
+  // to component-major [c][node][element].
+  auto write_state = [&](int ng) {
+    int tid, lane, el, q, i;
+    lane_ids(tid, lane, el, q, i);
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int t = tid + k * NT;
+      if (t < ng * NP * 3) {
+        double vx, vy;
+        if constexpr (CIN) {
+          vx = (double)sh[k].x + (double)__uint_as_float(sl[k] << 16);
+          vy = (double)sh[k].y + (double)__uint_as_float(sl[k] & 0xFFFF0000u);
+        } else {
+          vx = sh[k].x;
+          vy = sh[k].y;
+        }
+        const int ee = t / (3 * NP), r2 = t - ee * (3 * NP);
+        const int j = r2 / 3, c = 2 * (r2 - j * 3);
+        sA[(c * NP + j) * E2 + ee] = vx;
+        sA[((c + 1) * NP + j) * E2 + ee] = vy;
+      }
+    }
+  };
+  // Every load of the read-ahead is unconditional (the last group reads its
+  // own data again), so the compiler can count outstanding loads exactly
+  // and never has to drain them all at a merge point. The launch holds at
+  // most ngroups blocks.
+  read_ahead(blockIdx.x);
+  {
+    const int64_t r0 = io.n_elem - (int64_t)blockIdx.x * EPB;
+    write_state(r0 < EPB ? (int)r0 : EPB);
+  }
+
+#pragma unroll 1
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    lane_ids(tid, lane, el, q, i);
+    const int64_t e0 = grp * EPB;
+    const int64_t rem = io.n_elem - e0;
+    const int nel = rem < EPB ? (int)rem : EPB;
+    const bool active = i < NP && el < nel;
+    const int64_t e = e0 + (el < nel ? el : nel - 1);  // clamped: loads stay in range
+    const int64_t base = (e * NP + (i < NP ? i : NP - 1)) * 6;
+
+    // Group start: this group's reads, then the next group's read-ahead.
+    double ji[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) ji[c] = io.jinv[e * 9 + c];
+    TraceRaw<CIN> nr[FPT];
+    int fcodes[FPT];
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      fcodes[s] = nbn[s] >= 0 ? 0 : nbn[s];
+      load_trace_raw<CIN>(io, nbn[s] >= 0 ? nbn[s] : 0, nr[s]);
+    }
+    using PT = typename std::conditional<EPI == 2, float, double>::type;
+    PT pold[6];
+    if constexpr (EPI == 1) {
+      const double2 *pp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.p) + base);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double2 v = pp[c];
+        pold[2 * c] = v.x;
+        pold[2 * c + 1] = v.y;
+      }
+    } else if constexpr (EPI == 2) {
+      const float2 *pp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.p) + base);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float2 v = pp[c];
+        pold[2 * c] = v.x;
+        pold[2 * c + 1] = v.y;
+      }
+    }
+    double mrow[MPT];
+#pragma unroll
+    for (int k = 0; k < MPT; ++k) mrow[k] = io.mat_tab[12 * mnext[k] + (tid + k * NT) % 12];
+    read_ahead(grp + gridDim.x < ngroups ? grp + gridDim.x : grp);
+    __syncthreads();
+
+    // Face flux into LDS, then the volume products with their curls (MFMA,
+    // no global reads).
+    double own[6], acc[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) own[c] = sA[(c * NP + (i < NP ? i : 0)) * E2 + el];
+    if (active) {
+#pragma unroll
+      for (int s = 0; s < FPT; ++s) {
+        const int fm = i + s * NP;
+        if (fm >= NF4) continue;
+        const int f = fm / NFP, m = fm - f * NFP;
+        double n[3], cf;
+        face_geom(ji, f, n, cf);
+        const int vi = sfn[fm];
+        double u[6], nv[6], fo[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) u[c] = sA[(c * NP + vi) * E2 + el];
+        trace_f64<CIN>(nr[s], nv);
+        face_flux(fcodes[s], io.alpha, n, cf, u, nv, fo);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sF[(c * NF4 + fm) * E2 + el] = fo[c];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int cbase = h == 0 ? 3 : 0;
+      mf_d4 cacc[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) cacc[c] = mf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const bool jv = NP % 4 == 0 || ks < KS - 1 || ks * 4 + q < NP;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double b = jv ? sA[((cbase + c) * NP + ks * 4 + q) * E2 + el] : 0.0;
+          cacc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], b, cacc[c], 0, 0, 0);
+        }
+      }
+      auto dd = [&](int p, int c) { return ji[p] * cacc[c][0] + ji[3 + p] * cacc[c][1] + ji[6 + p] * cacc[c][2]; };
+      const double sgn = h == 0 ? 1.0 : -1.0;
+      acc[3 * h + 0] = sgn * (dd(1, 2) - dd(2, 1));
+      acc[3 * h + 1] = sgn * (dd(2, 0) - dd(0, 2));
+      acc[3 * h + 2] = sgn * (dd(0, 1) - dd(1, 0));
+    }
+#pragma unroll
+    for (int k = 0; k < MPT; ++k)
+      if (tid + k * NT < nel * 12) sm[tid + k * NT] = mrow[k];
+    __syncthreads();  // state reads done, flux and materials complete
+
+    // LIFT of all six flux components into sA as L[c][i][e].
+    {
+      const int lcol = lane & 15, lrow = lane >> 4;
+      if constexpr (LFIX) {
+        const int mt = wave % MTL;
+        const int nwm = NW / MTL + (mt < NW % MTL ? 1 : 0);
+        const double *ap = sL + mt * KSL * 64 + lane;
+        for (int nt = wave / MTL; nt < 6; nt += nwm) {
+          const double *bp = sF + (nt * NF4 + lrow) * E2 + lcol;
+          mf_d4 c4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int ks = 0; ks < KSL; ++ks)
+            c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[ks * 64], bp[ks * 4 * E2], c4, 0, 0, 0);
+          double *lp = sA + (nt * NP + mt * 16 + lrow) * E2 + lcol;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (mt * 16 + r * 4 + lrow < NP) lp[r * 4 * E2] = c4[r];
+        }
+      } else {
+        for (int t = wave; t < MTL * 6; t += NW) {
+          const int mt = t / 6, nt = t - (t / 6) * 6;
+          const double *ap = sL + mt * KSL * 64 + lane;
+          const double *bp = sF + (nt * NF4 + lrow) * E2 + lcol;
+          mf_d4 c4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int ks = 0; ks < KSL; ++ks)
+            c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[ks * 64], bp[ks * 4 * E2], c4, 0, 0, 0);
+          double *lp = sA + (nt * NP + mt * 16 + lrow) * E2 + lcol;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (mt * 16 + r * 4 + lrow < NP) lp[r * 4 * E2] = c4[r];
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc[c] += sA[(c * NP + i) * E2 + el];
+    }
+    __syncthreads();  // L is read; sA takes the next group's state
+    // Written before this group's global stores, so waiting for the
+    // read-ahead never waits for stores.
+    if (grp + gridDim.x < ngroups) {
+      const int64_t rn = io.n_elem - (grp + gridDim.x) * EPB;
+      write_state(rn < EPB ? (int)rn : EPB);
+    }
+    if (active) {
+      double po[6], srcv[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
+      if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
+      }
+      node_update<EPI>(io, base, sm + el * 12, acc, own, po, srcv);
+    }
+  }
+}
+
+// k_fused9: the k_fused3 layout (one thread per element node, D and LIFT in
+// LDS, VALU contractions) with the group's memory traffic pipelined:
+//   * the next group's state, neighbour-table entries and material indices
+//     are read into registers while the current group computes (the state
+//     goes to LDS after the group's last LDS reads);
+//   * the group's J^-1, neighbour traces, old p and material rows are all
+//     requested at its start, J^-1 first, then the read-ahead; the volume
+//     contraction (which needs J^-1 only) runs while the traces arrive;
+//   * every load is unconditional, with indices clamped to valid entries on
+//     inactive lanes and boundary faces, so no conversion or wait is forced
+//     where a load is issued and the compiler counts outstanding loads
+//     exactly;
+//   * face geometry is formed in the flux and the face-node table is in LDS.
+template <int NP, int NFP, bool CIN, int EPI, int UNR>
+__global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3))) k_fused9(StageArgs io) {
+  constexpr int EPB = kFusedWG / NP;
+  constexpr int NF4 = 4 * NFP;
+  constexpr int FPT = (NF4 + NP - 1) / NP;
+  constexpr int ND = 3 * NP * NP;
+  constexpr int SD = ND + NF4 * NP;
+  constexpr int SFS = NF4 * 6 + 2;
+  constexpr int PPT = (EPB * NP * 3 + kFusedWG - 1) / kFusedWG;
+  __shared__ __align__(16) double su[EPB * NP * 6];
+  __shared__ __align__(16) double sf[EPB * SFS];
+  __shared__ __align__(16) double sd[SD];
+  __shared__ __align__(16) double sm[EPB * 12];
+  __shared__ int sfn[NF4];
+  for (int q = threadIdx.x; q < SD; q += kFusedWG) sd[q] = q < ND ? io.dT[q] : io.lift[q - ND];
+  for (int q = threadIdx.x; q < NF4; q += kFusedWG) sfn[q] = io.face_nodes[q];
+  const double *dT = sd;
+  const double *lift = sd + ND;
+  const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;
+  // Lane ids come from an opaque copy of threadIdx.x wherever the group loop
+  // uses them, so per-lane addresses are not hoisted out of it and held in
+  // registers.
+  auto lane_ids = [&](int &tid, int &le, int &i) {
+    int tv = threadIdx.x;
+    asm volatile("" : "+v"(tv));
+    tid = tv;
+    le = tv / NP;
+    i = tv - le * NP;
+  };
+
+  using HT = typename std::conditional<CIN, float2, double2>::type;
+  HT sh[PPT];
+  uint32_t sl[PPT];
+  int nbn[FPT];
+  int mnext;
+  auto read_ahead = [&](int64_t g) {
+    int tid, le, i;
+    lane_ids(tid, le, i);
+    const int64_t rg = io.n_elem - g * EPB;
+    const int ng = rg < EPB ? (int)rg : EPB;
+    const int64_t gb = g * EPB * NP * 6;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int q0 = tid + k * kFusedWG;
+      const int q = q0 < ng * NP * 3 ? q0 : ng * NP * 3 - 1;
+      if constexpr (CIN) {
+        sh[k] = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + gb)[q];
+        sl[k] = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + gb)[q];
+      } else {
+        sh[k] = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + gb)[q];
+      }
+    }
+    const int64_t eg = g * EPB + (le < ng ? le : ng - 1);
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      const int v = io.nbase[eg * NF4 + (fm < NF4 ? fm : NF4 - 1)];
+      nbn[s] = fm < NF4 ? v : kCodePec;
+    }
+    mnext = io.mat[eg];
+  };
+  auto write_state = [&](int ng) {
+    int tid, le, i;
+    lane_ids(tid, le, i);
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int q = tid + k * kFusedWG;
+      if (q < ng * NP * 3) {
+        double2 v;
+        if constexpr (CIN) {
+          v.x = (double)sh[k].x + (double)__uint_as_float(sl[k] << 16);
+          v.y = (double)sh[k].y + (double)__uint_as_float(sl[k] & 0xFFFF0000u);
+        } else {
+          v = sh[k];
+        }
+        reinterpret_cast<double2 *>(su)[q] = v;
+      }
+    }
+  };
+  // The launch holds at most ngroups blocks; the last group of a block reads
+  // itself ahead again, which keeps every read-ahead load unconditional.
+  read_ahead(blockIdx.x);
+  {
+    const int64_t r0 = io.n_elem - (int64_t)blockIdx.x * EPB;
+    write_state(r0 < EPB ? (int)r0 : EPB);
+  }
+
+#pragma unroll 1
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    int tid, le, i;
+    lane_ids(tid, le, i);
+    const int64_t e0 = grp * EPB;
+    const int64_t rem = io.n_elem - e0;
+    const int nel = rem < EPB ? (int)rem : EPB;
+    const bool active = tid < nel * NP;
+    const int64_t e = e0 + (le < nel ? le : nel - 1);
+    const int64_t base = (e * NP + i) * 6;
+
+    double ji[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) ji[q] = io.jinv[e * 9 + q];
+    TraceRaw<CIN> nr[FPT];
+    int fcodes[FPT];
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      fcodes[s] = nbn[s] >= 0 ? 0 : nbn[s];
+      load_trace_raw<CIN>(io, nbn[s] >= 0 ? nbn[s] : 0, nr[s]);
+    }
+    // material row entry i of element le (threads with i < 12)
+    const double mrow = io.mat_tab[12 * mnext + (i < 12 ? i : 0)];
+    read_ahead(grp + gridDim.x < ngroups ? grp + gridDim.x : grp);
+    __syncthreads();  // the group's state is in LDS
+    // Needed only after the flux; issued here, behind the traces, to keep
+    // them out of the registers of the volume contraction.
+    using PT = typename std::conditional<EPI == 2, float, double>::type;
+    PT pold[6];
+    if constexpr (EPI == 1) {
+      const double2 *pp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.p) + base);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double2 v = pp[q];
+        pold[2 * q] = v.x;
+        pold[2 * q + 1] = v.y;
+      }
+    } else if constexpr (EPI == 2) {
+      const float2 *pp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.p) + base);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float2 v = pp[q];
+        pold[2 * q] = v.x;
+        pold[2 * q + 1] = v.y;
+      }
+    }
+
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const double *ub = su + le * NP * 6;
+    if (active) {
+      double r[3][6];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
+#pragma unroll UNR
+      for (int j = 0; j < NP; ++j) {
+        double u[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) u[c] = ub[j * 6 + c];
+        const double d0 = dT[(0 * NP + j) * NP + i];
+        const double d1 = dT[(1 * NP + j) * NP + i];
+        const double d2 = dT[(2 * NP + j) * NP + i];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          r[0][c] = fma(d0, u[c], r[0][c]);
+          r[1][c] = fma(d1, u[c], r[1][c]);
+          r[2][c] = fma(d2, u[c], r[2][c]);
+        }
+      }
+      auto pd = [&](int p, int c) { return ji[p] * r[0][c] + ji[3 + p] * r[1][c] + ji[6 + p] * r[2][c]; };
+      acc[0] = pd(1, 5) - pd(2, 4);
+      acc[1] = pd(2, 3) - pd(0, 5);
+      acc[2] = pd(0, 4) - pd(1, 3);
+      acc[3] = -(pd(1, 2) - pd(2, 1));
+      acc[4] = -(pd(2, 0) - pd(0, 2));
+      acc[5] = -(pd(0, 1) - pd(1, 0));
+      // The curl is formed here, before the flux: left to itself the
+      // compiler sinks it past the flux and keeps the 18 reference
+      // derivatives live there.
+      asm volatile("" ::"v"(acc[0]), "v"(acc[1]), "v"(acc[2]), "v"(acc[3]), "v"(acc[4]), "v"(acc[5]) : "memory");
+
+#pragma unroll
+      for (int s = 0; s < FPT; ++s) {
+        const int fm = i + s * NP;
+        if (fm >= NF4) continue;
+        const int f = fm / NFP;
+        double n[3], cf;
+        face_geom(ji, f, n, cf);
+        double u[6], nv[6], fo[6];
+        const double *uv = ub + sfn[fm] * 6;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) u[c] = uv[c];
+        trace_f64<CIN>(nr[s], nv);
+        face_flux(fcodes[s], io.alpha, n, cf, u, nv, fo);
+        double2 *out = reinterpret_cast<double2 *>(sf + le * SFS + fm * 6);
+        out[0] = make_double2(fo[0], fo[1]);
+        out[1] = make_double2(fo[2], fo[3]);
+        out[2] = make_double2(fo[4], fo[5]);
+      }
+      if (i < 12) sm[le * 12 + i] = mrow;
+    }
+    __syncthreads();  // flux and materials complete
+    double own[6];
+    if (active) {
+      const double *fb = sf + le * SFS;
+#pragma unroll UNR
+      for (int fm = 0; fm < NF4; ++fm) {
+        const double w = lift[fm * NP + i];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] = fma(w, fb[fm * 6 + c], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) own[c] = ub[i * 6 + c];
+    }
+    __syncthreads();  // su and sf are read; su takes the next group's state
+    if (grp + gridDim.x < ngroups) {
+      const int64_t rn = io.n_elem - (grp + gridDim.x) * EPB;
+      write_state(rn < EPB ? (int)rn : EPB);
+    }
+    // The remaining read-ahead registers are taken here, before this
+    // group's stores: a wait placed after the (lane-predicated) stores would
+    // be counted as if they had not been issued and would wait on them.
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) asm volatile("" : "+v"(nbn[s]));
+    asm volatile("" : "+v"(mnext));
+    if (active) {
+      double po[6], srcv[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
+      if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
+      }
+      node_update<EPI>(io, base, sm + le * 12, acc, own, po, srcv);
+    }
+  }
+}
+
 template <int NP>
 struct Nfp {
   static constexpr int v = NP == 4 ? 3 : NP == 10 ? 6 : NP == 20 ? 10 : 15;
 };
+
+// Blocks of `kernel` (launched with `nt` threads) resident on the whole
+// device at once.
+inline int64_t resident_blocks_nt(const void *kernel, int nt) {
+  int dev = 0, cus = 0, per_cu = 0;
+  RF_HIP(hipGetDevice(&dev));
+  RF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  RF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, 0));
+  return (int64_t)(cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+}
 
 // Blocks of `kernel` resident on the whole device at once (grid-stride
 // launches use this many).
@@ -1734,7 +2308,10 @@ inline int64_t resident_blocks(const void *kernel) {
 // XCD-aware group order, 11 = k_fused7 (curl in registers; Np <= 20, else
 // variant 5), 12/13 = variant 5 with the next group's state read ahead
 // (loops unrolled by 4 / 2), 14 = variant 13 with the read-ahead issued
-// after the group's own reads and the face geometry formed in the flux. Chosen by the
+// after the group's own reads and the face geometry formed in the flux,
+// 15 = k_fused8 (persistent k_fused7 with read-ahead; Np <= 20),
+// 16 / 17 = k_fused9 (k_fused3 layout with the memory traffic pipelined;
+// loops unrolled by 2 / 4). Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -1770,6 +2347,26 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
     else
       hipLaunchKernelGGL((k_fused5<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))), dim3(Cfg::NT),
                          0, s, a);
+  } else if ((variant == 16 || variant == 17) && a.nbase) {
+    const void *kf = variant == 16 ? reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 2>)
+                                   : reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 4>);
+    static int64_t resident[2] = {0, 0};
+    int64_t &res = resident[variant - 16];
+    if (res == 0) res = resident_blocks(kf);
+    const int64_t grid = groups1 < res ? groups1 : res;
+    if (grid > 0) {
+      if (variant == 16)
+        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 4>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+    }
+  } else if (variant == 15 && a.nbase && NP <= 20) {
+    static int64_t res = 0;
+    if (res == 0) res = resident_blocks_nt(reinterpret_cast<const void *>(&k_fused8<NP, F, CIN, EPI>), Mfma7Cfg<NP>::NT);
+    const int64_t groups = (a.n_elem + 15) / 16;
+    const int64_t grid = groups < res ? groups : res;
+    if (grid > 0)
+      hipLaunchKernelGGL((k_fused8<NP, F, CIN, EPI>), dim3((unsigned)grid), dim3(Mfma7Cfg<NP>::NT), 0, s, a);
   } else if (variant == 11 && a.nbase && NP <= 20) {
     const int64_t groups = (a.n_elem + 15) / 16;
     hipLaunchKernelGGL((k_fused7<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))),
