@@ -221,6 +221,11 @@ int rfhip_op_n_variants(const rfhip_op *op);
 int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best);
 int rfhip_op_set_variant(rfhip_op *op, int variant);
 int rfhip_op_variant(const rfhip_op *op);
+/* Test aid (no reference counterpart): fills the shared memory (LDS) of
+ * every CU with NaN on the operator's stream and waits, so a kernel variant
+ * that reads LDS it did not write fails its parity check instead of
+ * passing on values an earlier launch left behind. */
+int rfhip_op_poison_lds(rfhip_op *op);
 
 /* ---- device-resident runs (no reference counterpart) -------------------
  * For callers that keep the state in HBM between calls (benchmarks, long

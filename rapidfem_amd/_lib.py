@@ -88,6 +88,7 @@ SIGNATURES = {
     "rfhip_op_n_variants": (C.c_int, [_vp]),
     "rfhip_op_autotune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(_d)]),
     "rfhip_op_set_variant": (C.c_int, [_vp, C.c_int]),
+    "rfhip_op_poison_lds": (C.c_int, [_vp]),
     "rfhip_op_variant": (C.c_int, [_vp]),
     "rfhip_op_state_load": (C.c_int, [_vp, _dp]),
     "rfhip_op_state_store": (C.c_int, [_vp, _dp]),

@@ -1119,6 +1119,14 @@ int rfhip_op_set_variant(rfhip_op *op, int variant) {
   });
 }
 
+int rfhip_op_poison_lds(rfhip_op *op) {
+  return guarded([&] {
+    RF_REQUIRE(op, "null argument");
+    launch_poison_lds(op->stream);
+    sync(op);
+  });
+}
+
 int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best) {
   return guarded([&] {
     RF_REQUIRE(op, "null argument");

@@ -2053,8 +2053,12 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
   constexpr int EPB = kFusedWG / NP;
   constexpr int NF4 = 4 * NFP;
   constexpr int FPT = (NF4 + NP - 1) / NP;
-  constexpr int ND = 3 * NP * NP;
-  constexpr int SD = ND + NF4 * NP;
+  // D_k as [k][i][j] (row stride DS) and LIFT as [i][fm] (row stride LS):
+  // a 16-byte LDS read then returns two consecutive j (or fm) entries of a
+  // thread's row; the padded strides spread the rows over the banks.
+  constexpr int DS = NP + (NP & 1) + 2, LS = NF4 + 2;
+  constexpr int ND = 3 * NP * DS;
+  constexpr int SD = ND + NP * LS;
   constexpr int SFS = NF4 * 6 + 2;
   constexpr int PPT = (EPB * NP * 3 + kFusedWG - 1) / kFusedWG;
   __shared__ __align__(16) double su[EPB * NP * 6];
@@ -2062,7 +2066,17 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
   __shared__ __align__(16) double sd[SD];
   __shared__ __align__(16) double sm[EPB * 12];
   __shared__ int sfn[NF4];
-  for (int q = threadIdx.x; q < SD; q += kFusedWG) sd[q] = q < ND ? io.dT[q] : io.lift[q - ND];
+  for (int q = threadIdx.x; q < SD; q += kFusedWG) {
+    double v = 0.0;
+    if (q < ND) {
+      const int k = q / (NP * DS), r = q - k * NP * DS, i = r / DS, j = r - i * DS;
+      if (j < NP) v = io.dT[(k * NP + j) * NP + i];
+    } else {
+      const int i = (q - ND) / LS, fm = (q - ND) - i * LS;
+      if (fm < NF4) v = io.lift[fm * NP + i];
+    }
+    sd[q] = v;
+  }
   for (int q = threadIdx.x; q < NF4; q += kFusedWG) sfn[q] = io.face_nodes[q];
   const double *dT = sd;
   const double *lift = sd + ND;
@@ -2156,8 +2170,14 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
       fcodes[s] = nbn[s] >= 0 ? 0 : nbn[s];
       load_trace_raw<CIN>(io, nbn[s] >= 0 ? nbn[s] : 0, nr[s]);
     }
-    // material row entry i of element le (threads with i < 12)
-    const double mrow = io.mat_tab[12 * mnext + (i < 12 ? i : 0)];
+    // material row entries i, i + Np, ... (< 12) of element le
+    constexpr int MR = (12 + NP - 1) / NP;
+    double mrow[MR];
+#pragma unroll
+    for (int k = 0; k < MR; ++k) {
+      const int q = i + k * NP;
+      mrow[k] = io.mat_tab[12 * mnext + (q < 12 ? q : 0)];
+    }
     read_ahead(grp + gridDim.x < ngroups ? grp + gridDim.x : grp);
     __syncthreads();  // the group's state is in LDS
     // Needed only after the flux; issued here, behind the traces, to keep
@@ -2190,21 +2210,27 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
       for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
-#pragma unroll UNR
-      for (int j = 0; j < NP; ++j) {
+      const double *dr = dT + i * DS;
+      auto vol_step = [&](int j, double d0, double d1, double d2) {
         double u[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) u[c] = ub[j * 6 + c];
-        const double d0 = dT[(0 * NP + j) * NP + i];
-        const double d1 = dT[(1 * NP + j) * NP + i];
-        const double d2 = dT[(2 * NP + j) * NP + i];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
           r[0][c] = fma(d0, u[c], r[0][c]);
           r[1][c] = fma(d1, u[c], r[1][c]);
           r[2][c] = fma(d2, u[c], r[2][c]);
         }
+      };
+#pragma unroll UNR
+      for (int j = 0; j < NP - 1; j += 2) {
+        const double2 a0 = *reinterpret_cast<const double2 *>(dr + j);
+        const double2 a1 = *reinterpret_cast<const double2 *>(dr + NP * DS + j);
+        const double2 a2 = *reinterpret_cast<const double2 *>(dr + 2 * NP * DS + j);
+        vol_step(j, a0.x, a1.x, a2.x);
+        vol_step(j + 1, a0.y, a1.y, a2.y);
       }
+      if constexpr (NP & 1) vol_step(NP - 1, dr[NP - 1], dr[NP * DS + NP - 1], dr[2 * NP * DS + NP - 1]);
       auto pd = [&](int p, int c) { return ji[p] * r[0][c] + ji[3 + p] * r[1][c] + ji[6 + p] * r[2][c]; };
       acc[0] = pd(1, 5) - pd(2, 4);
       acc[1] = pd(2, 3) - pd(0, 5);
@@ -2235,17 +2261,22 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
         out[1] = make_double2(fo[2], fo[3]);
         out[2] = make_double2(fo[4], fo[5]);
       }
-      if (i < 12) sm[le * 12 + i] = mrow;
+#pragma unroll
+      for (int k = 0; k < MR; ++k)
+        if (i + k * NP < 12) sm[le * 12 + i + k * NP] = mrow[k];
     }
     __syncthreads();  // flux and materials complete
     double own[6];
     if (active) {
       const double *fb = sf + le * SFS;
+      const double *lr = lift + i * LS;
 #pragma unroll UNR
-      for (int fm = 0; fm < NF4; ++fm) {
-        const double w = lift[fm * NP + i];
+      for (int fm = 0; fm < NF4; fm += 2) {  // NF4 = 4 Nfp is even
+        const double2 w = *reinterpret_cast<const double2 *>(lr + fm);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) acc[c] = fma(w, fb[fm * 6 + c], acc[c]);
+        for (int c = 0; c < 6; ++c) acc[c] = fma(w.x, fb[fm * 6 + c], acc[c]);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] = fma(w.y, fb[fm * 6 + 6 + c], acc[c]);
       }
 #pragma unroll
       for (int c = 0; c < 6; ++c) own[c] = ub[i * 6 + c];
@@ -2311,7 +2342,7 @@ inline int64_t resident_blocks(const void *kernel) {
 // after the group's own reads and the face geometry formed in the flux,
 // 15 = k_fused8 (persistent k_fused7 with read-ahead; Np <= 20),
 // 16 / 17 = k_fused9 (k_fused3 layout with the memory traffic pipelined;
-// loops unrolled by 2 / 4). Chosen by the
+// loops unrolled by 2 / 4 nodes). Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -2348,17 +2379,17 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
       hipLaunchKernelGGL((k_fused5<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))), dim3(Cfg::NT),
                          0, s, a);
   } else if ((variant == 16 || variant == 17) && a.nbase) {
-    const void *kf = variant == 16 ? reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 2>)
-                                   : reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 4>);
+    const void *kf = variant == 16 ? reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 1>)
+                                   : reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 2>);
     static int64_t resident[2] = {0, 0};
     int64_t &res = resident[variant - 16];
     if (res == 0) res = resident_blocks(kf);
     const int64_t grid = groups1 < res ? groups1 : res;
     if (grid > 0) {
       if (variant == 16)
-        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
       else
-        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 4>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
     }
   } else if (variant == 15 && a.nbase && NP <= 20) {
     static int64_t res = 0;

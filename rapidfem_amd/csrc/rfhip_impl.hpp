@@ -184,6 +184,7 @@ void launch_c_to_f64(const float *hi, const uint16_t *lo, double *dst, int64_t n
 void launch_cast_d2f(const double *src, float *dst, int64_t n, hipStream_t s);
 void launch_cast_f2d(const float *src, double *dst, int64_t n, hipStream_t s);
 void launch_fill(void *dst, int64_t nbytes, hipStream_t s);
+void launch_poison_lds(hipStream_t s);
 // Port modal projections (P_e, P_h) of every port into out[2p], out[2p+1].
 void launch_port_proj(const void *y, const uint16_t *lo, int kind, const int64_t *dof, const double *prof,
                       const double *w, const int64_t *off, const double *norm, int n_ports, double *out,

@@ -326,6 +326,26 @@ void launch_port_proj(const void *y, const uint16_t *lo, int kind, const int64_t
   RF_HIP(hipGetLastError());
 }
 
+// Fills the LDS of every resident block with a NaN pattern (volatile stores
+// the compiler keeps). Launched with four 40 KiB blocks per CU it covers the
+// whole 160 KiB of each CU, so a later kernel that reads LDS it did not
+// write sees NaN instead of what an earlier kernel happened to leave there.
+__global__ void __launch_bounds__(256) k_poison_lds() {
+  __shared__ double pad[5120];
+  volatile double *v = pad;
+  const double nan = __longlong_as_double(0x7ff8dead7ff8deadll);
+  for (int q = threadIdx.x; q < 5120; q += 256) v[q] = nan;
+  __syncthreads();
+}
+
+void launch_poison_lds(hipStream_t s) {
+  int dev = 0, cus = 0;
+  RF_HIP(hipGetDevice(&dev));
+  RF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  hipLaunchKernelGGL(k_poison_lds, dim3((unsigned)(8 * (cus > 0 ? cus : 1))), dim3(256), 0, s);
+  RF_HIP(hipGetLastError());
+}
+
 void launch_fill(void *dst, int64_t nbytes, hipStream_t s) {
   const int64_t nw = nbytes / 4;
   hipLaunchKernelGGL(k_fill0, dim3(grid_for(nw)), dim3(kBS), 0, s, static_cast<uint32_t *>(dst), nw);

@@ -262,6 +262,11 @@ class GpuOperator:
     def variant(self) -> int:
         return int(L.lib.rfhip_op_variant(self._h))
 
+    def poison_lds(self) -> None:
+        """Fill every CU's LDS with NaN (a test aid: a kernel that reads LDS
+        it did not write then fails its parity check)."""
+        L.check(L.lib.rfhip_op_poison_lds(self._h))
+
     # ---- device-resident state (no reference counterpart) ----
     def state_load(self, y0):
         L.check(L.lib.rfhip_op_state_load(self._h, self._vec(y0, np.float64)))

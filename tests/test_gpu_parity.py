@@ -283,7 +283,11 @@ def test_every_kernel_variant_matches_oracle(ctx, oracle, order, form):
     for v in range(g.n_variants()):
         g.set_variant(v)
         assert g.variant == v
+        # NaN in every CU's LDS first: a variant that reads shared memory it
+        # did not write cannot pass on what the previous variant left there.
+        g.poison_lds()
         e_apply = rel_l2(g.apply_f64(y), want_apply)
+        g.poison_lds()
         g.state_load(y)
         g.state_lserk(dt, 6)
         e_step = rel_l2(g.state_store(), want)
@@ -305,7 +309,9 @@ def test_variants_on_a_mesh_larger_than_the_resident_grid(ctx, oracle, form):
     g = _gop(ctx, oop, form)
     for v in range(g.n_variants()):
         g.set_variant(v)
+        g.poison_lds()
         e_apply = rel_l2(g.apply_f64(y), want_apply)
+        g.poison_lds()
         g.state_load(y)
         g.state_lserk(0.002, 3)
         e_step = rel_l2(g.state_store(), want)
