@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--autotune", type=int, default=1, help="pick the fastest kernel variant on this mesh")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="pin this stage-kernel variant instead of autotuning (fused forms)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="counter-profiling run: a 1 GiB streaming copy (torch.neg, 8 B read + 8 B written "
                          "per element) as the calibration kernel, then 10 LSERK4 steps of the chosen form at a "
@@ -73,7 +75,9 @@ def traffic_probe(args) -> int:
     op, y0 = W.pulse_box(nx, ny, nz, args.order)
     ctx = R.GpuContext()
     g = R.GpuOperator(ctx, op, args.form)
-    if args.autotune:
+    if args.variant >= 0:
+        g.set_variant(args.variant)
+    elif args.autotune:
         g.autotune()
     g.state_load(y0)
     g.state_lserk(0.002, 10)
@@ -138,7 +142,11 @@ def main() -> int:
 
     chosen = R.GpuOperator(ctx, op, args.form)
     variant = 0
-    if args.autotune:
+    if args.variant >= 0:
+        chosen.set_variant(args.variant)
+        variant = chosen.variant
+        log(f"variant {variant} pinned")
+    elif args.autotune:
         variant, tune_ms = chosen.autotune()
         log(f"autotune: variant {variant} of {chosen.n_variants()} ({tune_ms:.4f} ms/stage)")
     chosen.state_load(y0)
@@ -177,8 +185,8 @@ def main() -> int:
     if os.path.exists(TRAFFIC_FILE):
         try:
             tr = json.load(open(TRAFFIC_FILE))
-            if tr.get("form") == args.form and tr.get("n_dof") == n and tr.get("variant") == variant:
-                traffic = tr.get("bytes_per_launch")
+            if tr.get("form") == args.form and tr.get("n_dof") == n:
+                traffic = tr.get("variants", {}).get(str(variant), {}).get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

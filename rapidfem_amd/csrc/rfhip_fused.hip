@@ -1756,6 +1756,21 @@ __device__ __forceinline__ void trace_f64(const TraceRaw<CIN> &r, double *v) {
   }
 }
 
+// Marks the registers of a raw trace as read here (an empty asm that takes
+// them as inputs): forces the wait for their load at this point on every
+// path, so no later write to those registers has to wait for it.
+template <bool CIN>
+__device__ __forceinline__ void touch_trace(const TraceRaw<CIN> &r) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if constexpr (CIN) {
+      asm volatile("" ::"v"(r.h[q].x), "v"(r.h[q].y), "v"(r.l[q]));
+    } else {
+      asm volatile("" ::"v"(r.d[q].x), "v"(r.d[q].y));
+    }
+  }
+}
+
 #ifndef RF_K8_WPE
 #define RF_K8_WPE 3  // waves per SIMD the persistent kernel is compiled for
 #endif
@@ -2048,7 +2063,7 @@ k_fused8(StageArgs io) {
 //     where a load is issued and the compiler counts outstanding loads
 //     exactly;
 //   * face geometry is formed in the flux and the face-node table is in LDS.
-template <int NP, int NFP, bool CIN, int EPI, int UNR>
+template <int NP, int NFP, bool CIN, int EPI, int UNR, bool MLIFT = false>
 __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3))) k_fused9(StageArgs io) {
   constexpr int EPB = kFusedWG / NP;
   constexpr int NF4 = 4 * NFP;
@@ -2266,7 +2281,83 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
         if (i + k * NP < 12) sm[le * 12 + i + k * NP] = mrow[k];
     }
     __syncthreads();  // flux and materials complete
+    // Every register loaded for this group is taken here on all lanes: left
+    // to lanes that are active only, a lane-predicated branch would leave the
+    // loads pending on its skip path, and the next group start would wait for
+    // all outstanding loads before reusing those registers.
+#pragma unroll
+    for (int q = 0; q < 9; ++q) asm volatile("" ::"v"(ji[q]));
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) touch_trace<CIN>(nr[s]);
+    if constexpr (EPI != 0) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) asm volatile("" ::"v"(pold[c]));
+    }
+#pragma unroll
+    for (int k = 0; k < MR; ++k) asm volatile("" ::"v"(mrow[k]));
     double own[6];
+    if constexpr (MLIFT) {
+      // LIFT on the matrix cores: L[i][(e,c)] = sum_fm LIFT[i][fm] F[fm][(e,c)]
+      // as 16x16x4 tiles over rows i (MTL tiles) and columns 6e + c (NTL
+      // tiles), the A operand from the LDS copy of LIFT and B from the flux.
+      // L goes back to LDS over the flux (after all tiles have read it) and
+      // each thread takes its node's six values.
+      constexpr int MTL = (NP + 15) / 16, NCOL = EPB * 6, NTL = (NCOL + 15) / 16, KSL = NF4 / 4;
+      constexpr int TPW = (MTL * NTL + 3) / 4;  // tiles per wave
+      const int lane = tid & 63, wave = tid >> 6;
+      const int lc = lane & 15, lr = lane >> 4;
+      mf_d4 ct[TPW];
+#pragma unroll
+      for (int k = 0; k < TPW; ++k) {
+        const int t = wave + 4 * k;
+        ct[k] = mf_d4{0.0, 0.0, 0.0, 0.0};
+        if (t < MTL * NTL) {
+          const int mt = t / NTL, nt = t - (t / NTL) * NTL;
+          const int row = 16 * mt + lc, n = 16 * nt + lc;
+          const double *ap = lift + (row < NP ? row : 0) * LS + lr;
+          const int ne = n < NCOL ? n / 6 : 0, nc = n - (n / 6) * 6;
+          const double *bp = sf + ne * SFS + lr * 6 + nc;
+          const bool av = row < NP, bv = n < NCOL;
+          mf_d4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int ks = 0; ks < KSL; ++ks) {
+            const double x = av ? ap[4 * ks] : 0.0;
+            const double y = bv ? bp[24 * ks] : 0.0;
+            if (ks & 1)
+              c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c1, 0, 0, 0);
+            else
+              c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c0, 0, 0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) ct[k][r] = c0[r] + c1[r];
+        }
+      }
+      __syncthreads();  // every tile has read the flux; L takes its place
+#pragma unroll
+      for (int k = 0; k < TPW; ++k) {
+        const int t = wave + 4 * k;
+        if (t < MTL * NTL) {
+          const int mt = t / NTL, nt = t - (t / NTL) * NTL;
+          const int n = 16 * nt + lc;
+          if (n < NCOL) {
+            const int ne = n / 6, nc = n - ne * 6;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = 16 * mt + 4 * r + lr;
+              if (row < NP) sf[(ne * NP + row) * 6 + nc] = ct[k][r];
+            }
+          }
+        }
+      }
+      __syncthreads();  // L complete
+      if (active) {
+        const double *lp = sf + (le * NP + i) * 6;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] += lp[c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) own[c] = ub[i * 6 + c];
+      }
+    } else {
     if (active) {
       const double *fb = sf + le * SFS;
       const double *lr = lift + i * LS;
@@ -2280,6 +2371,7 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
       }
 #pragma unroll
       for (int c = 0; c < 6; ++c) own[c] = ub[i * 6 + c];
+    }
     }
     __syncthreads();  // su and sf are read; su takes the next group's state
     if (grp + gridDim.x < ngroups) {
@@ -2342,7 +2434,8 @@ inline int64_t resident_blocks(const void *kernel) {
 // after the group's own reads and the face geometry formed in the flux,
 // 15 = k_fused8 (persistent k_fused7 with read-ahead; Np <= 20),
 // 16 / 17 = k_fused9 (k_fused3 layout with the memory traffic pipelined;
-// loops unrolled by 2 / 4 nodes). Chosen by the
+// loops unrolled by 2 / 4 nodes), 18 / 19 = k_fused9 with the LIFT on the
+// matrix cores. Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -2390,6 +2483,19 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
         hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
       else
         hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+    }
+  } else if ((variant == 18 || variant == 19) && a.nbase) {
+    const void *kf = variant == 18 ? reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 1, true>)
+                                   : reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 2, true>);
+    static int64_t resident[2] = {0, 0};
+    int64_t &res = resident[variant - 18];
+    if (res == 0) res = resident_blocks(kf);
+    const int64_t grid = groups1 < res ? groups1 : res;
+    if (grid > 0) {
+      if (variant == 18)
+        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 1, true>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2, true>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
     }
   } else if (variant == 15 && a.nbase && NP <= 20) {
     static int64_t res = 0;
