@@ -37,7 +37,7 @@ constexpr double kKclErrFloor = 1e-12;   // constants.rs KCL_ERR_FLOOR
 constexpr double kKclNonfinite = 10.0;   // constants.rs KCL_NONFINITE_ERR
 constexpr double kArnoldiBreakdown = 1e-12;  // constants.rs:62
 constexpr int kMaxKrylov = 64;
-constexpr int kFusedVariants = 20;  // rfhip_fused.hip launch_stage
+constexpr int kFusedVariants = 22;  // rfhip_fused.hip launch_stage
 
 template <class F>
 int guarded(F &&f) {
@@ -350,6 +350,7 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   a.n_elem = K;
   a.alpha = d.flux_alpha;
   a.dT = op->dT.as<double>();
+  a.dsparse = d_fits_sparse_pattern(op->order, dT.data()) ? 1 : 0;
   a.lift = op->liftc.as<double>();
   a.afrag = op->afrag.as<double>();
   a.lfrag = op->lfrag.as<double>();

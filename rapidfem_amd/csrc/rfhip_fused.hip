@@ -22,6 +22,7 @@
 // (element, node). The block's state is staged in LDS with coalesced loads;
 // D^T and LIFT^T are read from L1 with consecutive nodes on consecutive
 // lanes.
+#include <cmath>
 #include <cstdlib>
 #include <type_traits>
 
@@ -2397,6 +2398,446 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
   }
 }
 
+// ---- sparse volume contraction (k_fused10) ------------------------------
+//
+// The equispaced nodal D_k are sparse. With barycentric multi-indices b (row
+// node) and a (column node), dL_a/dlambda_m at node b can be nonzero only if
+// a_m >= 1 and b_q >= a_q for every q != m (L_a is a product of 1-D factors
+// that vanish at the nodes below a); D_r = d/dlambda_2 - d/dlambda_1 and
+// likewise s (lambda_3) and t (lambda_4). k_fused10 gives each of a block's 4
+// waves a fixed group of nodes (one lane per element and node slot), so a
+// wave visits only the columns some node of its group needs, with the column
+// set known at compile time. The groups balance the waves' column counts;
+// they are the output of gen_node_groups.py (beside this file), which also
+// states the counts. The operator build checks that every D_k entry outside
+// the pattern is zero to round-off before this kernel may run.
+template <int NP>
+struct NodeGroups;
+template <>
+struct NodeGroups<4> {
+  static constexpr int RPW = 1;
+  static constexpr int g[4][1] = {{1}, {0}, {2}, {3}};
+};
+template <>
+struct NodeGroups<10> {
+  static constexpr int RPW = 3;
+  static constexpr int g[4][3] = {{2, 6, 7}, {1, 3, 4}, {5, 8, 9}, {0, -1, -1}};
+};
+template <>
+struct NodeGroups<20> {
+  static constexpr int RPW = 5;
+  static constexpr int g[4][5] = {{0, 3, 11, 12, 19}, {4, 7, 10, 15, 18}, {1, 13, 14, 16, 17}, {2, 5, 6, 8, 9}};
+};
+template <>
+struct NodeGroups<35> {
+  static constexpr int RPW = 9;
+  static constexpr int g[4][9] = {{0, 2, 5, 16, 17, 18, 24, 25, 30},
+                                  {3, 4, 7, 8, 9, 11, 13, 19, 23},
+                                  {1, 6, 10, 12, 14, 22, 31, 32, 33},
+                                  {15, 20, 21, 26, 27, 28, 29, 34, -1}};
+};
+
+// Barycentric multi-index (p - i - j - k, i, j, k) of every node, in node order
+// (i, then j, then k ascending).
+template <int NP>
+struct NodeBary {
+  int b[NP][4];
+  constexpr NodeBary() : b{} {
+    const int p = NP == 4 ? 1 : NP == 10 ? 2 : NP == 20 ? 3 : 4;
+    int n = 0;
+    for (int i = 0; i <= p; ++i)
+      for (int j = 0; j <= p - i; ++j)
+        for (int k = 0; k <= p - i - j; ++k) {
+          b[n][0] = p - i - j - k;
+          b[n][1] = i;
+          b[n][2] = j;
+          b[n][3] = k;
+          ++n;
+        }
+  }
+};
+
+__host__ __device__ constexpr bool d_may_be_nonzero(const int *b, const int *a, int m) {
+  if (a[m] < 1) return false;
+  for (int q = 0; q < 4; ++q)
+    if (q != m && b[q] < a[q]) return false;
+  return true;
+}
+
+// Bit j set: column j of D_k (k = 0 r, 1 s, 2 t) may be nonzero in row i.
+template <int NP>
+__host__ __device__ constexpr uint64_t d_row_mask(int i, int k) {
+  constexpr NodeBary<NP> nb{};
+  uint64_t m = 0;
+  for (int j = 0; j < NP; ++j)
+    if (d_may_be_nonzero(nb.b[i], nb.b[j], k + 1) || d_may_be_nonzero(nb.b[i], nb.b[j], 0)) m |= 1ull << j;
+  return m;
+}
+
+template <int NP>
+__host__ __device__ constexpr uint64_t d_wave_mask(int w, int k) {
+  uint64_t m = 0;
+  for (int s = 0; s < NodeGroups<NP>::RPW; ++s)
+    if (NodeGroups<NP>::g[w][s] >= 0) m |= d_row_mask<NP>(NodeGroups<NP>::g[w][s], k);
+  return m;
+}
+
+// r[k][c] = sum_j D_k[i][j] u[j][c] over the columns of wave W's pattern
+// (dT in LDS as [k][j][i]); the first product of each k initialises it.
+#ifndef SVB
+#define SVB 2
+#endif
+template <int NP, int W>
+__device__ __forceinline__ void sparse_volume(const double *ub, const double *dT, int i, double (&r)[3][6]) {
+  constexpr uint64_t m0 = d_wave_mask<NP>(W, 0), m1 = d_wave_mask<NP>(W, 1), m2 = d_wave_mask<NP>(W, 2);
+  constexpr uint64_t mu = m0 | m1 | m2;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    if (!((mu >> j) & 1)) continue;
+    const double2 *up = reinterpret_cast<const double2 *>(ub + j * 6);
+    const double2 u01 = up[0], u23 = up[1], u45 = up[2];
+    const double u[6] = {u01.x, u01.y, u23.x, u23.y, u45.x, u45.y};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const uint64_t mk = k == 0 ? m0 : k == 1 ? m1 : m2;
+      if (!((mk >> j) & 1)) continue;
+      const double d = dT[(k * NP + j) * NP + i];
+      const bool first = (mk & ((1ull << j) - 1)) == 0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) r[k][c] = first ? d * u[c] : fma(d, u[c], r[k][c]);
+    }
+    // Bound the scheduler's look-ahead to two visited columns: unbounded,
+    // it hoists every column's LDS reads and spills.
+    if (__builtin_popcountll(mu & ((2ull << j) - 1)) % SVB == 0) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// The same contraction with the wave's column masks as wave-uniform runtime
+// values (scalar registers): one code path for all waves, the skipped
+// columns being scalar branches.
+template <int NP>
+__device__ __forceinline__ void sparse_volume_rt(const double *ub, const double *dT, int i, double (&r)[3][6],
+                                                 uint64_t m0, uint64_t m1, uint64_t m2) {
+  const uint64_t mu = m0 | m1 | m2;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
+  // Not unrolled: unrolled, the scheduler hoists the reads of many columns
+  // past the scalar branches and the kernel spills.
+#pragma unroll 1
+  for (int j = 0; j < NP; ++j) {
+    if ((mu >> j) & 1) {
+      const double2 *up = reinterpret_cast<const double2 *>(ub + j * 6);
+      const double2 u01 = up[0], u23 = up[1], u45 = up[2];
+      const double u[6] = {u01.x, u01.y, u23.x, u23.y, u45.x, u45.y};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const uint64_t mk = k == 0 ? m0 : k == 1 ? m1 : m2;
+        if ((mk >> j) & 1) {
+          const double d = dT[(k * NP + j) * NP + i];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) r[k][c] = fma(d, u[c], r[k][c]);
+        }
+      }
+    }
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ uint64_t wave_mask_rt(int wave, int k) {
+  return wave == 0 ? d_wave_mask<NP>(0, k)
+         : wave == 1 ? d_wave_mask<NP>(1, k)
+         : wave == 2 ? d_wave_mask<NP>(2, k)
+                     : d_wave_mask<NP>(3, k);
+}
+
+// The volume term of node i: the sparse contraction for wave W's columns,
+// then J^-1 and the curl, so that only the six curl values leave the
+// wave-specific code (its 18 reference derivatives stay inside).
+template <int NP, int W>
+__device__ __forceinline__ void sparse_curl(const double *ub, const double *dT, int i, const double *ji,
+                                            double *acc) {
+  double r[3][6];
+  sparse_volume<NP, W>(ub, dT, i, r);
+  auto pd = [&](int p, int c) { return ji[p] * r[0][c] + ji[3 + p] * r[1][c] + ji[6 + p] * r[2][c]; };
+  acc[0] = pd(1, 5) - pd(2, 4);
+  acc[1] = pd(2, 3) - pd(0, 5);
+  acc[2] = pd(0, 4) - pd(1, 3);
+  acc[3] = -(pd(1, 2) - pd(2, 1));
+  acc[4] = -(pd(2, 0) - pd(0, 2));
+  acc[5] = -(pd(0, 1) - pd(1, 0));
+}
+
+// k_fused10: k_fused9 with node-major lanes (wave w holds node group w of
+// every element of the block) and the sparse volume contraction above.
+template <int NP, int NFP, bool CIN, int EPI, int UNR>
+__global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3))) k_fused10(StageArgs io) {
+  constexpr int RPW = NodeGroups<NP>::RPW;  // node slots per wave
+  constexpr int EPB = 64 / RPW;              // elements per block (one lane per element and slot)
+  constexpr int NF4 = 4 * NFP;
+  constexpr int FPT = (NF4 + NP - 1) / NP;
+  // D_k as [k][j][i] (a wave reads one column entry for each of its nodes)
+  // and LIFT as [i][fm] (row stride LS, two fm per 16-byte read). The state
+  // of an element is padded to SUS doubles so a wave's elements spread over
+  // the LDS banks.
+  constexpr int LS = NF4 + 2, SUS = NP * 6 + 2;
+  constexpr int ND = 3 * NP * NP;
+  constexpr int SD = ND + NP * LS;
+  constexpr int SFS = NF4 * 6 + 2;
+  constexpr int PPT = (EPB * NP * 3 + kFusedWG - 1) / kFusedWG;
+  __shared__ __align__(16) double su[EPB * SUS];
+  __shared__ __align__(16) double sf[EPB * SFS];
+  __shared__ __align__(16) double sd[SD];
+  __shared__ __align__(16) double sm[EPB * 12];
+  __shared__ int sfn[NF4];
+  __shared__ int sng[4 * RPW];
+  for (int q = threadIdx.x; q < SD; q += kFusedWG) {
+    double v = 0.0;
+    if (q < ND) {
+      v = io.dT[q];
+    } else {
+      const int i = (q - ND) / LS, fm = (q - ND) - i * LS;
+      if (fm < NF4) v = io.lift[fm * NP + i];
+    }
+    sd[q] = v;
+  }
+  for (int q = threadIdx.x; q < NF4; q += kFusedWG) sfn[q] = io.face_nodes[q];
+  for (int q = threadIdx.x; q < 4 * RPW; q += kFusedWG) sng[q] = NodeGroups<NP>::g[q / RPW][q % RPW];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  __syncthreads();  // lane_ids reads the node-group table from here on
+  const double *dT = sd;
+  const double *lift = sd + ND;
+  const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;
+  // Lane ids come from an opaque copy of threadIdx.x wherever the group loop
+  // uses them, so per-lane addresses are not hoisted out of it and held in
+  // registers.
+  // Lane l of wave w: slot l / EPB of the wave's node group (node i, or -1
+  // for an empty slot or a spare lane) and element l % EPB.
+  auto lane_ids = [&](int &tid, int &le, int &i) {
+    int tv = threadIdx.x;
+    asm volatile("" : "+v"(tv));
+    tid = tv;
+    const int l = tv & 63, slot = l / EPB;
+    le = l - slot * EPB;
+    const int g = slot < RPW ? sng[wave * RPW + slot] : -1;
+    i = (g >= 0 && g < NP) ? g : -1;  // only a valid node index ever becomes an address
+  };
+
+  using HT = typename std::conditional<CIN, float2, double2>::type;
+  HT sh[PPT];
+  uint32_t sl[PPT];
+  int nbn[FPT];
+  int mnext;
+  auto read_ahead = [&](int64_t g) {
+    int tid, le, i;
+    lane_ids(tid, le, i);
+    const int64_t rg = io.n_elem - g * EPB;
+    const int ng = rg < EPB ? (int)rg : EPB;
+    const int64_t gb = g * EPB * NP * 6;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int q0 = tid + k * kFusedWG;
+      const int q = q0 < ng * NP * 3 ? q0 : ng * NP * 3 - 1;
+      if constexpr (CIN) {
+        sh[k] = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + gb)[q];
+        sl[k] = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + gb)[q];
+      } else {
+        sh[k] = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + gb)[q];
+      }
+    }
+    const int64_t eg = g * EPB + (le < ng ? le : ng - 1);
+    const int ic = i >= 0 ? i : 0;
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = ic + s * NP;
+      const int v = io.nbase[eg * NF4 + (fm < NF4 ? fm : NF4 - 1)];
+      nbn[s] = (i >= 0 && fm < NF4) ? v : kCodePec;
+    }
+    mnext = io.mat[eg];
+  };
+  auto write_state = [&](int ng) {
+    int tid, le, i;
+    lane_ids(tid, le, i);
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int q = tid + k * kFusedWG;
+      if (q < ng * NP * 3) {
+        double2 v;
+        if constexpr (CIN) {
+          v.x = (double)sh[k].x + (double)__uint_as_float(sl[k] << 16);
+          v.y = (double)sh[k].y + (double)__uint_as_float(sl[k] & 0xFFFF0000u);
+        } else {
+          v = sh[k];
+        }
+        const int ee = q / (3 * NP), r2 = q - ee * (3 * NP);
+        reinterpret_cast<double2 *>(su + ee * SUS)[r2] = v;
+      }
+    }
+  };
+  // The launch holds at most ngroups blocks; the last group of a block reads
+  // itself ahead again, which keeps every read-ahead load unconditional.
+  read_ahead(blockIdx.x);
+  {
+    const int64_t r0 = io.n_elem - (int64_t)blockIdx.x * EPB;
+    write_state(r0 < EPB ? (int)r0 : EPB);
+  }
+
+#pragma unroll 1
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    int tid, le, i;
+    lane_ids(tid, le, i);
+    const int64_t e0 = grp * EPB;
+    const int64_t rem = io.n_elem - e0;
+    const int nel = rem < EPB ? (int)rem : EPB;
+    const bool active = i >= 0 && le < nel;
+    const int ic = i >= 0 ? i : 0;  // clamped: loads stay in range
+    const int64_t e = e0 + (le < nel ? le : nel - 1);
+    const int64_t base = (e * NP + ic) * 6;
+
+    double ji[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) ji[q] = io.jinv[e * 9 + q];
+    TraceRaw<CIN> nr[FPT];
+    int fcodes[FPT];
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      fcodes[s] = nbn[s] >= 0 ? 0 : nbn[s];
+      load_trace_raw<CIN>(io, nbn[s] >= 0 ? nbn[s] : 0, nr[s]);
+    }
+    // material row entries i, i + Np, ... (< 12) of element le
+    constexpr int MR = (12 + NP - 1) / NP;
+    double mrow[MR];
+#pragma unroll
+    for (int k = 0; k < MR; ++k) {
+      const int q = ic + k * NP;
+      mrow[k] = io.mat_tab[12 * mnext + (q < 12 ? q : 0)];
+    }
+    read_ahead(grp + gridDim.x < ngroups ? grp + gridDim.x : grp);
+    __syncthreads();  // the group's state is in LDS
+    // Needed only after the flux; issued here, behind the traces, to keep
+    // them out of the registers of the volume contraction.
+    using PT = typename std::conditional<EPI == 2, float, double>::type;
+    PT pold[6];
+    if constexpr (EPI == 1) {
+      const double2 *pp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.p) + base);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double2 v = pp[q];
+        pold[2 * q] = v.x;
+        pold[2 * q + 1] = v.y;
+      }
+    } else if constexpr (EPI == 2) {
+      const float2 *pp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.p) + base);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float2 v = pp[q];
+        pold[2 * q] = v.x;
+        pold[2 * q + 1] = v.y;
+      }
+    }
+
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const double *ub = su + le * SUS;
+    if (active) {
+      {
+        double r[3][6];
+        // wave-uniform column masks: each wave visits its own column set
+        sparse_volume_rt<NP>(ub, dT, i, r, wave_mask_rt<NP>(wave, 0), wave_mask_rt<NP>(wave, 1),
+                             wave_mask_rt<NP>(wave, 2));
+        auto pd = [&](int p, int c) { return ji[p] * r[0][c] + ji[3 + p] * r[1][c] + ji[6 + p] * r[2][c]; };
+        acc[0] = pd(1, 5) - pd(2, 4);
+        acc[1] = pd(2, 3) - pd(0, 5);
+        acc[2] = pd(0, 4) - pd(1, 3);
+        acc[3] = -(pd(1, 2) - pd(2, 1));
+        acc[4] = -(pd(2, 0) - pd(0, 2));
+        acc[5] = -(pd(0, 1) - pd(1, 0));
+      }
+      // The curl is formed here, before the flux: left to itself the
+      // compiler sinks it past the flux and keeps the 18 reference
+      // derivatives live there.
+      asm volatile("" ::"v"(acc[0]), "v"(acc[1]), "v"(acc[2]), "v"(acc[3]), "v"(acc[4]), "v"(acc[5]) : "memory");
+
+#pragma unroll
+      for (int s = 0; s < FPT; ++s) {
+        const int fm = i + s * NP;
+        if (fm >= NF4) continue;
+        const int f = fm / NFP;
+        double n[3], cf;
+        face_geom(ji, f, n, cf);
+        double u[6], nv[6], fo[6];
+        const double *uv = ub + sfn[fm] * 6;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) u[c] = uv[c];
+        trace_f64<CIN>(nr[s], nv);
+        face_flux(fcodes[s], io.alpha, n, cf, u, nv, fo);
+        double2 *out = reinterpret_cast<double2 *>(sf + le * SFS + fm * 6);
+        out[0] = make_double2(fo[0], fo[1]);
+        out[1] = make_double2(fo[2], fo[3]);
+        out[2] = make_double2(fo[4], fo[5]);
+      }
+#pragma unroll
+      for (int k = 0; k < MR; ++k)
+        if (i + k * NP < 12) sm[le * 12 + i + k * NP] = mrow[k];
+    }
+    __syncthreads();  // flux and materials complete
+    // Every register loaded for this group is taken here on all lanes: left
+    // to lanes that are active only, a lane-predicated branch would leave the
+    // loads pending on its skip path, and the next group start would wait for
+    // all outstanding loads before reusing those registers.
+#pragma unroll
+    for (int q = 0; q < 9; ++q) asm volatile("" ::"v"(ji[q]));
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) touch_trace<CIN>(nr[s]);
+    if constexpr (EPI != 0) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) asm volatile("" ::"v"(pold[c]));
+    }
+#pragma unroll
+    for (int k = 0; k < MR; ++k) asm volatile("" ::"v"(mrow[k]));
+    double own[6];
+    if (active) {
+      const double *fb = sf + le * SFS;
+      const double *lr = lift + i * LS;
+#pragma unroll UNR
+      for (int fm = 0; fm < NF4; fm += 2) {  // NF4 = 4 Nfp is even
+        const double2 w = *reinterpret_cast<const double2 *>(lr + fm);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] = fma(w.x, fb[fm * 6 + c], acc[c]);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] = fma(w.y, fb[fm * 6 + 6 + c], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) own[c] = ub[i * 6 + c];
+    }
+    __syncthreads();  // su and sf are read; su takes the next group's state
+    if (grp + gridDim.x < ngroups) {
+      const int64_t rn = io.n_elem - (grp + gridDim.x) * EPB;
+      write_state(rn < EPB ? (int)rn : EPB);
+    }
+    // The remaining read-ahead registers are taken here, before this
+    // group's stores: a wait placed after the (lane-predicated) stores would
+    // be counted as if they had not been issued and would wait on them.
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) asm volatile("" : "+v"(nbn[s]));
+    asm volatile("" : "+v"(mnext));
+    if (active) {
+      double po[6], srcv[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
+      if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
+      }
+      node_update<EPI>(io, base, sm + le * 12, acc, own, po, srcv);
+    }
+  }
+}
+
 template <int NP>
 struct Nfp {
   static constexpr int v = NP == 4 ? 3 : NP == 10 ? 6 : NP == 20 ? 10 : 15;
@@ -2435,7 +2876,8 @@ inline int64_t resident_blocks(const void *kernel) {
 // 15 = k_fused8 (persistent k_fused7 with read-ahead; Np <= 20),
 // 16 / 17 = k_fused9 (k_fused3 layout with the memory traffic pipelined;
 // loops unrolled by 2 / 4 nodes), 18 / 19 = k_fused9 with the LIFT on the
-// matrix cores. Chosen by the
+// matrix cores, 20 / 21 = k_fused10 (sparse volume, LIFT unrolled by 2 / 4;
+// only when the operator's D_k fit the equispaced pattern). Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -2483,6 +2925,21 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
         hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
       else
         hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+    }
+  } else if ((variant == 20 || variant == 21) && a.nbase && a.dsparse) {
+    const void *kf = variant == 20 ? reinterpret_cast<const void *>(&k_fused10<NP, F, CIN, EPI, 1>)
+                                   : reinterpret_cast<const void *>(&k_fused10<NP, F, CIN, EPI, 2>);
+    static int64_t resident[2] = {0, 0};
+    int64_t &res = resident[variant - 20];
+    if (res == 0) res = resident_blocks(kf);
+    const int64_t epb = 64 / NodeGroups<NP>::RPW;
+    const int64_t groups = (a.n_elem + epb - 1) / epb;
+    const int64_t grid = groups < res ? groups : res;
+    if (grid > 0) {
+      if (variant == 20)
+        hipLaunchKernelGGL((k_fused10<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_fused10<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
     }
   } else if ((variant == 18 || variant == 19) && a.nbase) {
     const void *kf = variant == 18 ? reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 1, true>)
@@ -2563,6 +3020,28 @@ void fused_dispatch(int np, int variant, const StageArgs &a, hipStream_t s) {
 }
 
 }  // namespace
+
+template <int NP>
+static bool d_fits_pattern_np(const double *dT) {
+  for (int k = 0; k < 3; ++k) {
+    double mx = 0.0;
+    for (int q = 0; q < NP * NP; ++q) mx = std::fmax(mx, std::fabs(dT[k * NP * NP + q]));
+    for (int j = 0; j < NP; ++j)
+      for (int i = 0; i < NP; ++i)
+        if (!((d_row_mask<NP>(i, k) >> j) & 1) && std::fabs(dT[(k * NP + j) * NP + i]) > 1e-13 * mx) return false;
+  }
+  return true;
+}
+
+bool d_fits_sparse_pattern(int order, const double *dT) {
+  switch (order) {
+    case 1: return d_fits_pattern_np<4>(dT);
+    case 2: return d_fits_pattern_np<10>(dT);
+    case 3: return d_fits_pattern_np<20>(dT);
+    case 4: return d_fits_pattern_np<35>(dT);
+    default: return false;
+  }
+}
 
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s) {
   const int np = (order + 1) * (order + 2) * (order + 3) / 6;

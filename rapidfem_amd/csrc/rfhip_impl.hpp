@@ -109,6 +109,7 @@ struct StageArgs {
   const int *face_nodes;  // 4*Nfp
   const int *nbr_node;    // [(f*4 + nlf)*6 + orientation][Nfp] -> neighbour volume node
   const int *nbase;       // [e*4Nfp + fm]: AoS index of the neighbour trace, or the (negative) face code
+  int dsparse;            // D_k zero outside the equispaced sparsity pattern (k_fused10 may run)
   const void *yin;        // double, or float hi (compressed)
   const void *yin_lo;     // bf16 lo (compressed) or null
   void *yout, *yout_lo;
@@ -177,6 +178,9 @@ void launch_add_vec(const rfhip_op *op, void *k, const void *src, double g, hipS
 // Fused forms: epi 0 = apply (dy), 1 = LSERK stage (FP64 or compressed
 // registers by form).
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s);
+// True when every entry of dT (3*Np*Np, [k][j][i]) outside the equispaced
+// nodal sparsity pattern is zero to round-off (|x| <= 1e-13 * max |D_k|).
+bool d_fits_sparse_pattern(int order, const double *dT);
 
 // Conversions between boundary AoS and the form's storage.
 void launch_f64_to_c(const double *src, float *hi, uint16_t *lo, int64_t n, hipStream_t s);
