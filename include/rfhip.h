@@ -104,9 +104,28 @@ typedef struct {
 /* GpuOperator::new (gpu/operator.rs:207-396). form: RFHIP_FORM_*. */
 int rfhip_op_create(rfhip_ctx *ctx, const rfhip_op_desc *desc, int form,
                     rfhip_op **out);
+/* GpuOperator::new for an operator with Debye dispersive elements, which the
+ * reference GPU path refuses (gpu/operator.rs:211-215): MaxwellOperator's
+ * DispersiveElem list (rhs.rs:469-487, resolved at rhs.rs:963-985), slot q
+ * being element disp_elem[q] with Pdot = a*P + g*E (a = -1/tau,
+ * g = (eps_s - eps_inf)/tau, dispersive.rs:50-53) and 1/eps_inf. The state
+ * then appends P[(q*Np + node)*3 + c] after the 6*Np*K [E,H] values
+ * (rhs.rs:490-500) and n_dof grows by 3*Np per slot; every form, apply,
+ * transient, KCL and Krylov path carries it (rhs.rs:1295-1324,1456-1472).
+ * The element's own inv_eps in desc is used as given: the reference expects
+ * 1/eps_inf there. RFHIP_EINVAL for an element out of range or listed twice
+ * (rhs.rs:970-974). n_disp = 0 is rfhip_op_create. */
+int rfhip_op_create_dispersive(rfhip_ctx *ctx, const rfhip_op_desc *desc,
+                               int form, int64_t n_disp,
+                               const int64_t *disp_elem, const double *disp_a,
+                               const double *disp_g,
+                               const double *disp_inv_eps_inf, rfhip_op **out);
 void rfhip_op_destroy(rfhip_op *op);
-/* GpuOperator::n_dof (gpu/operator.rs:430). */
+/* GpuOperator::n_dof (gpu/operator.rs:430); MaxwellOperator::n_dof
+ * (rhs.rs:1003-1010) with the appended P block. */
 int64_t rfhip_op_n_dof(const rfhip_op *op);
+/* MaxwellOperator::n_dispersive (rhs.rs:1021-1024). */
+int64_t rfhip_op_n_dispersive(const rfhip_op *op);
 int rfhip_op_form(const rfhip_op *op);
 
 /* GpuOperator::apply (gpu/operator.rs:513-523): dy = A*y, f32 in/out. */

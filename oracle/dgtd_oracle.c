@@ -432,6 +432,11 @@ struct orc_op {
   int *nbr_lf, *port, *perm;         /* 4K, 4K, 4K*nfp */
   double *inv_eps, *inv_mu, *sig_eps, *sig_mu; /* K*3 */
   double alpha;
+  /* rhs.rs:469-487 DispersiveElem, one per P-block slot, and the element ->
+   * slot map (-1: not dispersive); empty when the operator is not dispersive */
+  int64_t n_disp;
+  int64_t *disp_elem, *disp_slot;
+  double *disp_a, *disp_g, *disp_iei;
 };
 
 /* rhs.rs:39-85 element_curl */
@@ -669,10 +674,50 @@ void orc_op_free(orc_op *op) {
   free(op->normal); free(op->fscale); free(op->nbr); free(op->nbr_lf);
   free(op->port); free(op->perm);
   free(op->inv_eps); free(op->inv_mu); free(op->sig_eps); free(op->sig_mu);
+  free(op->disp_elem); free(op->disp_slot); free(op->disp_a); free(op->disp_g);
+  free(op->disp_iei);
   free(op);
 }
 
-int64_t orc_op_n_dof(const orc_op *op) { return 6 * (int64_t)op->np * op->K; }
+/* rhs.rs:1003-1010 n_dof: the [E,H] block plus 3*Np per Debye element. */
+int64_t orc_op_n_dof(const orc_op *op) {
+  return 6 * (int64_t)op->np * op->K + 3 * (int64_t)op->np * op->n_disp;
+}
+
+/* rhs.rs:963-985 resolve the Debye elements: slot s in list order, with
+ * dispersive.rs:50-53 relaxation_coeffs a = -1/tau, g = (eps_s - eps_inf)/tau
+ * and 1/eps_inf. Returns -1 for an element out of range or listed twice. The
+ * element's own eps is left as given: the reference expects it to be eps_inf. */
+int orc_op_set_dispersive(orc_op *op, int64_t n, const int64_t *elem,
+                          const double *eps_inf, const double *eps_static,
+                          const double *tau) {
+  int64_t *slot = malloc(sizeof(int64_t) * op->K);
+  for (int64_t e = 0; e < op->K; ++e) slot[e] = -1;
+  for (int64_t s = 0; s < n; ++s) {
+    if (elem[s] < 0 || elem[s] >= op->K || slot[elem[s]] >= 0) {
+      free(slot);
+      return -1;
+    }
+    slot[elem[s]] = s;
+  }
+  free(op->disp_elem); free(op->disp_slot); free(op->disp_a); free(op->disp_g);
+  free(op->disp_iei);
+  op->n_disp = n;
+  op->disp_slot = slot;
+  op->disp_elem = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  op->disp_a = malloc(sizeof(double) * (n > 0 ? n : 1));
+  op->disp_g = malloc(sizeof(double) * (n > 0 ? n : 1));
+  op->disp_iei = malloc(sizeof(double) * (n > 0 ? n : 1));
+  for (int64_t s = 0; s < n; ++s) {
+    op->disp_elem[s] = elem[s];
+    op->disp_a[s] = -1.0 / tau[s];
+    op->disp_g[s] = (eps_static[s] - eps_inf[s]) / tau[s];
+    op->disp_iei[s] = 1.0 / eps_inf[s];
+  }
+  return 0;
+}
+
+int64_t orc_op_n_dispersive(const orc_op *op) { return op->n_disp; }
 int64_t orc_op_n_elem(const orc_op *op) { return op->K; }
 int orc_op_order(const orc_op *op) { return op->p; }
 
@@ -770,6 +815,16 @@ static void apply_element_(const orc_op *op, int64_t e, const double *y,
       out[node * 6 + c] = ie[c] * de[node * 3 + c] - se[c] * ee[node * 3 + c];
       out[node * 6 + 3 + c] = im[c] * dh[node * 3 + c] - sm[c] * hh[node * 3 + c];
     }
+  /* rhs.rs:1456-1472 Debye element: the polarisation current -Pdot/eps_inf
+   * on the E rows, Pdot = a*P + g*E with P from the appended block */
+  const int64_t slot = op->n_disp ? op->disp_slot[e] : -1;
+  if (slot >= 0) {
+    const double a = op->disp_a[slot], g = op->disp_g[slot], iei = op->disp_iei[slot];
+    const double *P = y + 6 * (int64_t)np * op->K + slot * 3 * np;
+    for (int node = 0; node < np; ++node)
+      for (int c = 0; c < 3; ++c)
+        out[node * 6 + c] -= iei * (a * P[node * 3 + c] + g * ee[node * 3 + c]);
+  }
 }
 
 /* rhs.rs:1262-1294 apply_into: elements dealt out in contiguous runs of
@@ -791,6 +846,17 @@ void orc_apply(const orc_op *op, const double *y, double *dy) {
       for (int64_t e = r * chunk; e < e1; ++e) apply_element_(op, e, y, &dy[e * np * 6], s);
     }
     free(s);
+  }
+  /* rhs.rs:1295-1324 the appended polarisation block: dP = a*P + g*E per
+   * Debye element, no spatial coupling */
+  const int64_t eh = 6 * (int64_t)np * K;
+#pragma omp parallel for num_threads(nt)
+  for (int64_t slot = 0; slot < op->n_disp; ++slot) {
+    const double a = op->disp_a[slot], g = op->disp_g[slot];
+    const int64_t eb = op->disp_elem[slot] * np * 6, pb = eh + slot * 3 * np;
+    for (int node = 0; node < np; ++node)
+      for (int c = 0; c < 3; ++c)
+        dy[pb + node * 3 + c] = a * y[pb + node * 3 + c] + g * y[eb + node * 6 + c];
   }
 }
 

@@ -48,6 +48,8 @@ def _load() -> C.CDLL:
         "orc_op_n_elem": (i64, [vp]),
         "orc_op_order": (C.c_int, [vp]),
         "orc_op_export": (None, [vp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp]),
+        "orc_op_set_dispersive": (C.c_int, [vp, i64, _lp, _dp, _dp, _dp]),
+        "orc_op_n_dispersive": (i64, [vp]),
         "orc_op_set_threads": (None, [C.c_int]),
         "orc_apply": (None, [vp, _dp, _dp]),
         "orc_field_energy": (d, [vp, _dp]),
@@ -136,10 +138,14 @@ def element_curl(p: int, tet: np.ndarray, field: np.ndarray) -> np.ndarray:
 
 
 class Operator:
-    """rhs.rs MaxwellOperator (non-dispersive) restated in C."""
+    """rhs.rs MaxwellOperator restated in C.
+
+    dispersive: optional (elems, eps_inf, eps_static, tau), one entry per
+    Debye element (rhs.rs:782-788 OperatorOptions::dispersive); the state
+    then carries the appended polarisation block (rhs.rs:490-500)."""
 
     def __init__(self, xyz, tets, order, flux_alpha=1.0, eps=None, mu=None,
-                 sigma=None, sigma_m=None, face_port=None):
+                 sigma=None, sigma_m=None, face_port=None, dispersive=None):
         L = lib()
         self.xyz = np.ascontiguousarray(xyz, np.float64)
         self.tets = np.ascontiguousarray(tets, np.int64)
@@ -162,6 +168,17 @@ class Operator:
         self._h = C.c_void_p(h)
         self.order = order
         self.K = K
+        if dispersive is not None:
+            de, einf, es, tau = dispersive
+            de = np.ascontiguousarray(de, np.int64)
+            n = de.shape[0]
+            rc = L.orc_op_set_dispersive(self._h, n, de,
+                                         np.ascontiguousarray(np.broadcast_to(einf, (n,)), np.float64),
+                                         np.ascontiguousarray(np.broadcast_to(es, (n,)), np.float64),
+                                         np.ascontiguousarray(np.broadcast_to(tau, (n,)), np.float64))
+            if rc != 0:
+                raise ValueError("dispersive element out of range or listed twice")
+        self.n_disp = int(L.orc_op_n_dispersive(self._h))
         self.n_dof = int(L.orc_op_n_dof(self._h))
         self.flux_alpha = float(flux_alpha)
 

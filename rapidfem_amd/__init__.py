@@ -7,12 +7,12 @@ GpuContext / GpuOperator / TdSession GPU routes and the operator builder.
 """
 from ._lib import RfhipError  # noqa: F401  (fails loudly if librfhip.so is missing)
 from .gpu import Controller, Drive, GpuContext, GpuOperator, KclStats  # noqa: F401
-from .maxwell import (ElemMaterial, MaxwellOperator, Mesh, ReferenceElement, flux_alpha,  # noqa: F401
+from .maxwell import (DebyeMaterial, ElemMaterial, MaxwellOperator, Mesh, ReferenceElement, flux_alpha,  # noqa: F401
                       structured_box, structured_box_jittered)
 from .session import GaussianPulse, TdSession  # noqa: F401
 
 __all__ = [
-    "RfhipError", "Controller", "Drive", "GpuContext", "GpuOperator", "KclStats", "ElemMaterial",
+    "RfhipError", "Controller", "Drive", "GpuContext", "GpuOperator", "KclStats", "DebyeMaterial", "ElemMaterial",
     "MaxwellOperator", "Mesh", "ReferenceElement", "flux_alpha", "structured_box", "structured_box_jittered",
     "GaussianPulse", "TdSession",
 ]

@@ -65,6 +65,9 @@ SIGNATURES = {
     "rfhip_ctx_device_name": (C.c_int, [_vp, C.c_char_p, C.c_size_t]),
     "rfhip_ctx_has_fp64": (C.c_int, [_vp]),
     "rfhip_op_create": (C.c_int, [_vp, C.POINTER(OpDesc), C.c_int, C.POINTER(_vp)]),
+    "rfhip_op_create_dispersive": (C.c_int, [_vp, C.POINTER(OpDesc), C.c_int, _i64, _lp, _dp, _dp, _dp,
+                                             C.POINTER(_vp)]),
+    "rfhip_op_n_dispersive": (_i64, [_vp]),
     "rfhip_op_destroy": (None, [_vp]),
     "rfhip_op_n_dof": (_i64, [_vp]),
     "rfhip_op_form": (C.c_int, [_vp]),

@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "librfhip.so")
 
-HIP_SOURCES = ["rfhip_api.hip", "rfhip_plain.hip", "rfhip_fused.hip", "rfhip_vec.hip"]
+HIP_SOURCES = ["rfhip_api.hip", "rfhip_plain.hip", "rfhip_fused.hip", "rfhip_vec.hip", "rfhip_debye.hip"]
 CXX_SOURCES = ["host_builder.cpp"]
 
 ARCH = os.environ.get("RFHIP_ARCH", "gfx950")

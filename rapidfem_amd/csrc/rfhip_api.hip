@@ -412,6 +412,13 @@ void fill_source(rfhip_op *op, StageArgs &a, int kind, int64_t dof, double g) {
   a.src_dof = dof;
 }
 
+// k = A*x for the plain forms: the element kernel for [E,H], then the Debye
+// rows (rhs.rs:1262-1325).
+void plain_apply_full(rfhip_op *op, const void *x, void *k, hipStream_t s) {
+  launch_plain_apply(op, x, k, s);
+  launch_debye_apply(op, x, k, op->form == RFHIP_FORM_PLAIN_F32, RFHIP_DRIVE_FREE, 0, nullptr, 0.0, s);
+}
+
 // k = A*x (+ b*g) on register vectors.
 void rhs_into(rfhip_op *op, const void *x, void *k, int kind, int64_t dof, double g) {
   hipStream_t s = op->stream;
@@ -421,9 +428,11 @@ void rhs_into(rfhip_op *op, const void *x, void *k, int kind, int64_t dof, doubl
     a.dy = static_cast<double *>(k);
     fill_source(op, a, kind, dof, g);
     launch_stage(op->order, op->form, 0, a, s);
+    // the fused kernel drives [E,H]; the Debye kernel drives the P rows
+    launch_debye_apply(op, x, k, false, kind, dof, a.src_vec, g, s);
     return;
   }
-  launch_plain_apply(op, x, k, s);
+  plain_apply_full(op, x, k, s);
   if (kind == RFHIP_DRIVE_POINT)
     launch_add_point(op, k, dof, g, s);
   else if (kind == RFHIP_DRIVE_VECTOR)
@@ -449,9 +458,10 @@ void lserk_step(rfhip_op *op, double h, int kind, int64_t dof, double g, hipEven
       fill_source(op, a, kind, dof, g);
       launch_stage(op->order, op->form, 1, a, s);
       if (stage_ev) RF_HIP(hipEventRecord(stage_ev[2 * st + 1], s));
+      launch_debye_stage(op, a, s);
       op->cur = 1 - op->cur;
     } else {
-      launch_plain_apply(op, op->y[0].p, op->k.p, s);
+      plain_apply_full(op, op->y[0].p, op->k.p, s);
       if (stage_ev) RF_HIP(hipEventRecord(stage_ev[2 * st + 1], s));
       if (kind == RFHIP_DRIVE_POINT)
         launch_add_point(op, op->k.p, dof, g, s);
@@ -642,7 +652,7 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
     // matvec (the reference GPU path drops to f32 through cast kernels, gpu/operator.rs:1329-1353)
     if (f32op) {
       launch_cast_d2f(bj, op->kst.as<float>(), n, s);
-      launch_plain_apply(op, op->kst.p, op->k.p, s);
+      plain_apply_full(op, op->kst.p, op->k.p, s);
       launch_cast_f2d(op->k.as<float>(), w, n, s);
     } else {
       rhs_into(op, bj, w, RFHIP_DRIVE_FREE, 0, 0.0);
@@ -831,8 +841,16 @@ int rfhip_ctx_device_name(const rfhip_ctx *ctx, char *buf, size_t len) {
 int rfhip_ctx_has_fp64(const rfhip_ctx *ctx) { return ctx ? 1 : 0; }
 
 int rfhip_op_create(rfhip_ctx *ctx, const rfhip_op_desc *d, int form, rfhip_op **out) {
+  return rfhip_op_create_dispersive(ctx, d, form, 0, nullptr, nullptr, nullptr, nullptr, out);
+}
+
+int rfhip_op_create_dispersive(rfhip_ctx *ctx, const rfhip_op_desc *d, int form, int64_t n_disp,
+                               const int64_t *disp_elem, const double *disp_a, const double *disp_g,
+                               const double *disp_inv_eps_inf, rfhip_op **out) {
   return guarded([&] {
     RF_REQUIRE(ctx && d && out, "null argument");
+    RF_REQUIRE(n_disp >= 0, "negative number of dispersive elements");
+    RF_REQUIRE(n_disp == 0 || (disp_elem && disp_a && disp_g && disp_inv_eps_inf), "missing dispersive array");
     RF_REQUIRE(form >= RFHIP_FORM_PLAIN_F32 && form <= RFHIP_FORM_FUSED_F64C, "unknown form");
     RF_REQUIRE(d->n_elem >= 1, "empty operator");
     const int p = order_of_np(d->np);
@@ -851,10 +869,32 @@ int rfhip_op_create(rfhip_ctx *ctx, const rfhip_op_desc *d, int form, rfhip_op *
     op->np = d->np;
     op->nfp = d->nfp;
     op->n_elem = d->n_elem;
-    op->n_dof = 6 * (int64_t)d->np * d->n_elem;
+    // rhs.rs:963-985: one P-block slot per listed element, in list order;
+    // an element listed twice or out of range is refused (rhs.rs:970-974)
+    std::vector<double> coef(3 * (size_t)n_disp);  // uploaded below; lives until the final sync
+    if (n_disp > 0) {
+      std::vector<char> seen((size_t)d->n_elem, 0);
+      for (int64_t q = 0; q < n_disp; ++q) {
+        const int64_t e = disp_elem[q];
+        RF_REQUIRE(e >= 0 && e < d->n_elem, "dispersive element index out of range");
+        RF_REQUIRE(!seen[(size_t)e], "element listed twice as dispersive");
+        seen[(size_t)e] = 1;
+        RF_REQUIRE(std::isfinite(disp_a[q]) && std::isfinite(disp_g[q]) && std::isfinite(disp_inv_eps_inf[q]),
+                   "non-finite dispersive coefficient");
+        coef[3 * q] = disp_a[q];
+        coef[3 * q + 1] = disp_g[q];
+        coef[3 * q + 2] = disp_inv_eps_inf[q];
+      }
+      op->n_disp = n_disp;
+    }
+    op->n_dof = 6 * (int64_t)d->np * d->n_elem + 3 * (int64_t)d->np * n_disp;
     op->n_vec = op->n_dof;
     op->alpha = d->flux_alpha;
     RF_HIP(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    if (n_disp > 0) {
+      upload(op->disp_elem, disp_elem, (size_t)n_disp, op->stream);
+      upload(op->disp_coef, coef.data(), coef.size(), op->stream);
+    }
     op->port_host.assign(d->face_port, d->face_port + 4 * d->n_elem);
     op->face_nodes_host.assign(d->face_nodes, d->face_nodes + 4 * d->nfp);
     op->dev = DevOp{};
@@ -881,6 +921,7 @@ void rfhip_op_destroy(rfhip_op *op) {
 }
 
 int64_t rfhip_op_n_dof(const rfhip_op *op) { return op ? op->n_dof : -1; }
+int64_t rfhip_op_n_dispersive(const rfhip_op *op) { return op ? op->n_disp : -1; }
 int rfhip_op_form(const rfhip_op *op) { return op ? op->form : -1; }
 
 int rfhip_op_apply(rfhip_op *op, const float *y, float *dy) {
@@ -890,7 +931,7 @@ int rfhip_op_apply(rfhip_op *op, const float *y, float *dy) {
     hipStream_t s = op->stream;
     if (op->form == RFHIP_FORM_PLAIN_F32) {
       RF_HIP(hipMemcpyAsync(op->y[0].p, y, 4 * n, hipMemcpyHostToDevice, s));
-      launch_plain_apply(op, op->y[0].p, op->k.p, s);
+      plain_apply_full(op, op->y[0].p, op->k.p, s);
       RF_HIP(hipMemcpyAsync(dy, op->k.p, 4 * n, hipMemcpyDeviceToHost, s));
       sync(op);
       return;

@@ -144,6 +144,10 @@ struct rfhip_op {
       nbr_lf, port, perm;
   rfhip::DevBuf dT, liftc, afrag, lfrag, afrag7, jinv64, fcode, mat, mat_tab, nbr_node, nbase;
   int64_t n_vec = 0;  // length of device vectors (n_dof)
+  // Debye dispersive block: slot -> element and (a, g, 1/eps_inf); the P
+  // block follows the 6 Np K values of [E,H] in every state vector
+  int64_t n_disp = 0;
+  rfhip::DevBuf disp_elem, disp_coef;
   int order = 0;
   // state registers (precision depends on form)
   rfhip::DevBuf y[2], ylo[2], p, k, src;
@@ -178,6 +182,13 @@ void launch_add_vec(const rfhip_op *op, void *k, const void *src, double g, hipS
 // Fused forms: epi 0 = apply (dy), 1 = LSERK stage (FP64 or compressed
 // registers by form).
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s);
+// Debye block after an applied derivative (dy = A y; f32: plain FP32 form),
+// with the drive of the P rows, and after a fused LSERK4 stage
+// (rfhip_debye.hip). No-ops without Debye elements.
+void launch_debye_apply(const rfhip_op *op, const void *y, void *dy, bool f32, int kind, int64_t dof,
+                        const double *vec, double g, hipStream_t s);
+void launch_debye_stage(const rfhip_op *op, const StageArgs &a, hipStream_t s);
+
 // True when every entry of dT (3*Np*Np, [k][j][i]) outside the equispaced
 // nodal sparsity pattern is zero to round-off (|x| <= 1e-13 * max |D_k|).
 bool d_fits_sparse_pattern(int order, const double *dT);

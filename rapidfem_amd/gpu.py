@@ -92,7 +92,7 @@ class GpuOperator:
     'fused_f64', 'fused_f64c')."""
 
     def __init__(self, ctx: GpuContext, op: MaxwellOperator | None, form: str = "fused_f64", *, desc=None,
-                 mass=None, det=None):
+                 mass=None, det=None, dispersive=None):
         self.ctx = ctx  # keep the context alive
         self.form = form
         if op is not None:
@@ -101,7 +101,16 @@ class GpuOperator:
         else:
             d, keep = desc, None
         h = C.c_void_p()
-        L.check(L.lib.rfhip_op_create(ctx._h, C.byref(d), L.FORMS[form], C.byref(h)))
+        if op is not None and getattr(op, "n_dispersive", 0):
+            dispersive = (op.disp_elem, op.disp_a, op.disp_g, op.disp_inv_eps_inf)
+        if dispersive is not None:
+            de, da, dg, di = (np.ascontiguousarray(dispersive[0], np.int64),
+                              *(np.ascontiguousarray(np.broadcast_to(v, (len(dispersive[0]),)), np.float64)
+                                for v in dispersive[1:]))
+            L.check(L.lib.rfhip_op_create_dispersive(ctx._h, C.byref(d), L.FORMS[form], de.shape[0], de, da, dg,
+                                                     di, C.byref(h)))
+        else:
+            L.check(L.lib.rfhip_op_create(ctx._h, C.byref(d), L.FORMS[form], C.byref(h)))
         del keep
         self._h = h
         self._n = int(L.lib.rfhip_op_n_dof(self._h))
@@ -110,9 +119,11 @@ class GpuOperator:
         self._det = None if det is None else np.ascontiguousarray(det, np.float64)
 
     @classmethod
-    def from_desc(cls, ctx: GpuContext, desc: L.OpDesc, form: str = "fused_f64", mass=None, det=None):
-        """From raw rfhip_op_desc data (e.g. another host's MaxwellOperator)."""
-        return cls(ctx, None, form, desc=desc, mass=mass, det=det)
+    def from_desc(cls, ctx: GpuContext, desc: L.OpDesc, form: str = "fused_f64", mass=None, det=None,
+                  dispersive=None):
+        """From raw rfhip_op_desc data (e.g. another host's MaxwellOperator);
+        dispersive: optional (elements, a, g, 1/eps_inf) of the Debye slots."""
+        return cls(ctx, None, form, desc=desc, mass=mass, det=det, dispersive=dispersive)
 
     @classmethod
     def new(cls, ctx: GpuContext, op: MaxwellOperator, form: str = "fused_f64") -> "GpuOperator":
@@ -125,6 +136,11 @@ class GpuOperator:
 
     def n_dof(self) -> int:
         return self._n
+
+    @property
+    def n_dispersive(self) -> int:
+        """rhs.rs:1021-1024: number of Debye elements (P-block slots)."""
+        return int(L.lib.rfhip_op_n_dispersive(self._h))
 
     def _vec(self, a, dt):
         a = np.ascontiguousarray(a, dt)

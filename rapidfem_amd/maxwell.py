@@ -121,6 +121,26 @@ def flux_alpha(flux: str) -> float:
     raise ValueError(f"flux must be one of ['central', 'upwind'], got {flux!r}")
 
 
+class DebyeMaterial:
+    """dispersive.rs:28-54 DebyeMaterial: eps(w) = eps_inf + (eps_s - eps_inf)
+    / (1 + i w tau); the polarisation obeys tau Pdot + P = (eps_s - eps_inf) E."""
+
+    def __init__(self, eps_inf: float, eps_static: float, tau: float):
+        self.eps_inf = float(eps_inf)
+        self.eps_static = float(eps_static)
+        self.tau = float(tau)
+
+    def permittivity(self, omega: float) -> tuple[float, float]:
+        """dispersive.rs:41-47: (Re, Im) of the relative permittivity at omega."""
+        d = self.eps_static - self.eps_inf
+        den = 1.0 + (omega * self.tau) ** 2
+        return self.eps_inf + d / den, -d * omega * self.tau / den
+
+    def relaxation_coeffs(self) -> tuple[float, float]:
+        """dispersive.rs:50-53: (a, g) of Pdot = a P + g E."""
+        return -1.0 / self.tau, (self.eps_static - self.eps_inf) / self.tau
+
+
 class MaxwellOperator:
     """rhs.rs:502-527 MaxwellOperator, the data the GPU backend consumes.
 
@@ -128,9 +148,13 @@ class MaxwellOperator:
     of arrays {'eps': (K,3), 'mu': (K,3), 'sigma': (K,), 'sigma_m': (K,)}.
     face_port: optional (K, 4) int array, port id per boundary face or -1
     (a port tag on an interior face is ignored, as rhs.rs:867-872 drops it).
+    dispersive: optional list of (element, DebyeMaterial) (rhs.rs:782-788);
+    each listed element gets a P-block slot in list order and its material
+    eps should be eps_inf (rhs.rs:469-487, 963-985).
     """
 
-    def __init__(self, mesh: Mesh, order: int, flux_alpha: float = 1.0, materials=None, face_port=None):
+    def __init__(self, mesh: Mesh, order: int, flux_alpha: float = 1.0, materials=None, face_port=None,
+                 dispersive=None):
         self.mesh = mesh
         self.re = ReferenceElement.new(order)
         self.order = order
@@ -186,7 +210,17 @@ class MaxwellOperator:
             raise RuntimeError("non-manifold mesh: a face is shared by more than two tets")
         if rc != 0:
             raise RuntimeError(f"operator build failed ({rc})")
-        self.n_dof = 6 * np_ * K
+        # rhs.rs:963-985 Debye slots: element, a, g, 1/eps_inf per slot
+        disp = list(dispersive or [])
+        self.disp_elem = np.array([int(e) for e, _ in disp], np.int64)
+        if len(set(self.disp_elem.tolist())) != len(disp) or np.any(self.disp_elem < 0) or np.any(
+                self.disp_elem >= K):
+            raise ValueError("dispersive element out of range or listed twice")
+        self.disp_a = np.array([m.relaxation_coeffs()[0] for _, m in disp], np.float64)
+        self.disp_g = np.array([m.relaxation_coeffs()[1] for _, m in disp], np.float64)
+        self.disp_inv_eps_inf = np.array([1.0 / m.eps_inf for _, m in disp], np.float64)
+        self.n_dispersive = len(disp)
+        self.n_dof = 6 * np_ * K + 3 * np_ * self.n_dispersive
 
     def node_coords(self) -> np.ndarray:
         """(K, Np, 3) physical coordinates of every DG node."""

@@ -259,6 +259,34 @@ def test_product_builder_operator_matches_oracle(ctx, oracle):
         assert err < (1e-7 if form == "fused_f64c" else 1e-11)
 
 
+def test_product_builder_debye_operator_matches_oracle(ctx, oracle):
+    """The package's MaxwellOperator with Debye elements (DebyeMaterial,
+    eps = eps_inf on them) through TdSession's GPU operator: the reference
+    refuses this on the GPU (session.rs:592-593); here the fused and plain
+    FP64 forms step it and match the oracle over 40 LSERK4 steps."""
+    import rapidfem_amd as R
+
+    mesh = R.structured_box_jittered(4, 3, 3, 1.0, 1.0, 1.0, 0.1, 5)
+    K = mesh.n_tets
+    deb = R.DebyeMaterial(2.5, 7.0, 0.3)
+    elems = list(range(1, K, 3))
+    mats = [R.ElemMaterial.isotropic(2.5 if e in set(elems) else 1.0) for e in range(K)]
+    op = R.MaxwellOperator(mesh, 2, 1.0, mats, dispersive=[(e, deb) for e in elems])
+    eps = np.array([[m.eps] * 3 if np.isscalar(m.eps) else m.eps for m in mats], np.float64).reshape(K, 3)
+    oop = oracle.Operator(mesh.nodes, mesh.tets, 2, 1.0, eps=eps,
+                          dispersive=(np.array(elems), 2.5, 7.0, 0.3))
+    assert op.n_dof == oop.n_dof
+    y0 = np.cos(0.2 + 0.013 * np.arange(op.n_dof))
+    want = oop.lserk(y0, 0.01, 40)
+    for form in FORMS64:
+        s = R.TdSession(op, form=form)
+        assert s.gop.n_dispersive == len(elems)
+        got = s.gop.transient(s.ctx, y0, 0.01, 40, 1, record=False, precision="f64")
+        err = rel_l2(got, want)
+        print(f"builder+debye {form}: rel_l2 {err:.3e}")
+        assert err < (1e-7 if form == "fused_f64c" else 1e-11)
+
+
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
 @pytest.mark.parametrize("form", ["fused_f64", "fused_f64c"])
 def test_every_kernel_variant_matches_oracle(ctx, oracle, order, form):
@@ -397,3 +425,139 @@ def test_port_signals_on_device_match_oracle(ctx, oracle, form):
             scale = max(np.abs(want).max(), 1e-3)
             assert np.abs(sig[k, p] - want).max() < tol * scale, (k, p, sig[k, p], want)
     assert rel_l2(yf, y) < (1e-3 if form == "plain_f32" else 1e-6)
+
+
+# ---- Debye dispersive block (rhs.rs:469-500, 963-985, 1295-1324, 1456-1472)
+
+DEBYE = dict(eps_inf=2.0, eps_s=6.0, tau=0.3)
+
+
+def _debye_pair(ctx, oracle, form, order=2, n=3, every=2, start=0):
+    """Oracle operator with every `every`-th element Debye (eps = eps_inf
+    there, 1 elsewhere) and the GPU operator on the same data."""
+    xyz, tets = _mesh(oracle, n)
+    K = tets.shape[0]
+    elems = np.arange(start, K, every)
+    eps = np.ones((K, 3))
+    eps[elems] = DEBYE["eps_inf"]
+    oop = oracle.Operator(xyz, tets, order, 1.0, eps=eps,
+                          dispersive=(elems, DEBYE["eps_inf"], DEBYE["eps_s"], DEBYE["tau"]))
+    from rapidfem_amd import DebyeMaterial, GpuOperator
+
+    a, gg = DebyeMaterial(DEBYE["eps_inf"], DEBYE["eps_s"], DEBYE["tau"]).relaxation_coeffs()
+    d, arrs, re_, ex = oracle_desc(oop)
+    g = GpuOperator.from_desc(ctx, d, form, mass=re_["mass"], det=ex["det"],
+                              dispersive=(elems, a, gg, 1.0 / DEBYE["eps_inf"]))
+    return oop, g, elems
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("form", ALL_FORMS)
+def test_debye_apply_matches_oracle(ctx, oracle, order, form):
+    """The augmented apply (E rows with -Pdot/eps_inf, appended P rows) on a
+    mesh where every second element is Debye."""
+    oop, g, elems = _debye_pair(ctx, oracle, form, order)
+    n = oop.n_dof
+    assert g.n_dof() == n == 6 * oop.K * g_np(order) + 3 * len(elems) * g_np(order)
+    assert g.n_dispersive == len(elems)
+    y = np.sin(0.3 + 0.017 * np.arange(n))
+    want = oop.apply(y)
+    if form == "plain_f32":
+        got, tol = g.apply(ctx, y.astype(np.float32)), 1e-3
+    else:
+        got, tol = g.apply_f64(y), 1e-12
+    err = rel_l2(got, want)
+    eh = 6 * oop.K * g_np(order)
+    err_p = rel_l2(got[eh:], want[eh:])
+    print(f"debye apply order={order} form={form} rel_l2={err:.3e} P block {err_p:.3e}")
+    assert err < tol and err_p < tol
+
+
+def g_np(order):
+    return (order + 1) * (order + 2) * (order + 3) // 6
+
+
+@pytest.mark.parametrize("form", ALL_FORMS)
+@pytest.mark.parametrize("drive", ["free", "vector", "point_in_p"])
+def test_debye_transient_matches_oracle(ctx, oracle, form, drive):
+    """LSERK4 transient of the augmented system (the fused forms complete
+    each stage with the Debye stage kernel) against the oracle, every kernel
+    variant of the fused forms; the drive reaches the P block too."""
+    from rapidfem_amd import Drive
+
+    oop, g, elems = _debye_pair(ctx, oracle, form, 2)
+    n = oop.n_dof
+    eh = 6 * oop.K * g_np(2)
+    y0 = np.cos(0.2 + 0.013 * np.arange(n))
+    steps, sub, dt = 2, 3, 0.06
+    h = dt / sub
+    wave = np.cos(0.7 * np.arange(steps * sub))
+    if drive == "free":
+        d, od, gg = Drive.free(), None, None
+    elif drive == "vector":
+        b = np.sin(np.arange(n) * 0.01)
+        d, od, gg = Drive.vector(b), ("vector", b), wave
+    else:
+        dof = eh + 7
+        d, od, gg = Drive.point(dof), ("point", dof), wave
+    want = [y0]
+    y = y0
+    for k in range(steps):
+        y = oop.lserk(y, h, sub, od, None if gg is None else gg[k * sub:(k + 1) * sub])
+        want.append(y)
+    tol = {"plain_f32": 1e-3, "plain_f64": 1e-12, "fused_f64": 1e-12, "fused_f64c": 1e-7}[form]
+    prec = "f32" if form == "plain_f32" else "f64"
+    for v in range(g.n_variants()):
+        g.set_variant(v)
+        traj = g.transient(ctx, y0, dt, steps, sub, d, gg, True, precision=prec)
+        for k in range(steps + 1):
+            err = rel_l2(traj[k], want[k])
+            err_p = rel_l2(traj[k][eh:], want[k][eh:])
+            assert err < tol and err_p < tol, (form, drive, v, k, err, err_p)
+        print(f"debye transient form={form} drive={drive} v={v} final rel_l2 {err:.3e} P {err_p:.3e}")
+
+
+@pytest.mark.parametrize("form", FORMS64)
+def test_debye_kcl_matches_oracle(ctx, oracle, form):
+    """KCL + PI controller on the augmented system: same accept/reject
+    counts and final state as the oracle."""
+    from rapidfem_amd import Controller
+
+    oop, g, _ = _debye_pair(ctx, oracle, form, 2)
+    y0 = np.cos(0.2 + 0.013 * np.arange(oop.n_dof))
+    ctl = Controller(atol=1e-8, rtol=1e-6)
+    traj, st = g.transient_kcl(ctx, y0, 0.05, 3, ctl=ctl, precision="f64")
+    octl = oracle.Controller.default()
+    octl.atol, octl.rtol = 1e-8, 1e-6
+    y, acc, rej, hmin, hmax = oop.adaptive(y0, 0.05, 3, octl)
+    err = rel_l2(traj[-1], y)
+    print(f"debye kcl form={form}: gpu {st.accepted}/{st.rejected} oracle {acc}/{rej} rel_l2 {err:.3e}")
+    assert (st.accepted, st.rejected) == (acc, rej)
+    assert err < (1e-7 if form == "fused_f64c" else 1e-11)
+
+
+@pytest.mark.parametrize("form", ["plain_f64", "fused_f64"])
+def test_debye_expmv_matches_oracle(ctx, oracle, form):
+    """Krylov exp(tA)v of the augmented operator (the P block is ordinary
+    state to the propagator, dispersive.rs:20-23)."""
+    oop, g, _ = _debye_pair(ctx, oracle, form, 2)
+    y0 = np.cos(0.2 + 0.013 * np.arange(oop.n_dof))
+    want, wdim = oop.expmv(y0, 0.3, 30, 1e-10)
+    got = g.expmv(ctx, y0, 0.3, 30, 1e-10)
+    err = rel_l2(got, want)
+    print(f"debye expmv form={form}: dim {g.last_dim}/{wdim} rel_l2 {err:.3e}")
+    assert g.last_dim == wdim
+    assert err < 1e-10
+
+
+def test_debye_rejects_bad_element_lists(ctx, oracle):
+    """rhs.rs:970-974: an element out of range or listed twice is an error."""
+    from rapidfem_amd import GpuOperator, RfhipError
+
+    xyz, tets = _mesh(oracle, 2)
+    oop = oracle.Operator(xyz, tets, 1, 1.0)
+    d, arrs, re_, ex = oracle_desc(oop)
+    for bad in ([0, oop.K], [3, 3], [-1]):
+        with pytest.raises(RfhipError) as ei:
+            GpuOperator.from_desc(ctx, d, "fused_f64", dispersive=(bad, -1.0, 1.0, 0.5))
+        assert ei.value.code == 1

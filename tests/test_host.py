@@ -223,3 +223,30 @@ def test_port_face_data_weights_and_projection(oracle):
     pe, ph = oracle.port_projections(y, dof, pr, w)
     assert abs(pe - 1.0) < 1e-14 and abs(ph - 1.0) < 1e-14
     assert pr.shape == (len(faces), nfp, 6)
+
+
+def test_debye_material_and_dispersive_operator_host_data():
+    """dispersive.rs:41-53 closed forms (the values of its own test material
+    eps_inf 2, eps_s 5, tau 0.3) and the host operator's Debye slots:
+    n_dof gains 3 Np per slot (rhs.rs:1003-1010), slots follow list order with
+    (a, g, 1/eps_inf), and an element out of range or listed twice is refused
+    (rhs.rs:970-974)."""
+    import rapidfem_amd as R
+
+    m = R.DebyeMaterial(2.0, 5.0, 0.3)
+    a, g = m.relaxation_coeffs()
+    assert a == -1.0 / 0.3 and g == 3.0 / 0.3
+    for w in (0.0, 0.5, 4.0, 1e6):
+        re, im = m.permittivity(w)
+        z = 2.0 + 3.0 / (1.0 + 1j * w * 0.3)
+        assert abs(re - z.real) < 1e-13 and abs(im - z.imag) < 1e-13
+    mesh = R.structured_box(1, 1, 1)
+    K = mesh.n_tets
+    mats = [R.ElemMaterial.isotropic(2.0) for _ in range(K)]
+    op = R.MaxwellOperator(mesh, 2, 1.0, mats, dispersive=[(4, m), (1, m)])
+    assert op.n_dispersive == 2 and op.n_dof == 6 * 10 * K + 3 * 10 * 2
+    assert op.disp_elem.tolist() == [4, 1]
+    assert np.allclose(op.disp_a, a) and np.allclose(op.disp_g, g) and np.allclose(op.disp_inv_eps_inf, 0.5)
+    for bad in ([(K, m)], [(2, m), (2, m)], [(-1, m)]):
+        with pytest.raises(ValueError):
+            R.MaxwellOperator(mesh, 2, 1.0, mats, dispersive=bad)

@@ -620,3 +620,142 @@ def test_expmv_on_mesh_matches_dense(oracle):
     want = (scipy_expm(h * aug) @ np.concatenate([v, [1.0]]))[:-1]
     got = op.etd_step(v, b, h, 30)
     assert np.linalg.norm(got - want) < 1e-10 * np.linalg.norm(want)
+
+
+# ---- Debye dispersive block (rhs.rs:469-500, 963-985, 1295-1324, 1456-1472;
+# dispersive.rs:50-53) -------------------------------------------------------
+
+def _debye_op(oracle, order, alpha, every=1, eps_inf=2.0, eps_s=6.0, tau=0.3, cells=(1, 1, 1)):
+    xyz, tets = oracle.box_mesh(*cells)
+    K = tets.shape[0]
+    elems = np.arange(0, K, every)
+    op = oracle.Operator(xyz, tets, order, alpha, eps=eps_inf,
+                         dispersive=(elems, eps_inf, eps_s, tau))
+    return op, elems
+
+
+def _dense(op):
+    n = op.n_dof
+    a = np.zeros((n, n))
+    for j in range(n):
+        ej = np.zeros(n)
+        ej[j] = 1.0
+        a[:, j] = op.apply(ej)
+    return a
+
+
+def test_debye_dof_count_and_validation(oracle):
+    """rhs.rs:3077-3121: no Debye material leaves n_dof = 6 Np K; a Debye
+    material on every element appends exactly 3 Np per element; the apply
+    stays finite. Out-of-range or repeated elements are refused
+    (rhs.rs:970-974)."""
+    xyz, tets = oracle.box_mesh(2, 2, 2)
+    K = tets.shape[0]
+    plain = oracle.Operator(xyz, tets, 2, 1.0)
+    assert plain.n_disp == 0 and plain.n_dof == 6 * 10 * K
+    op, _ = _debye_op(oracle, 2, 1.0, eps_inf=2.0, eps_s=5.0, tau=0.4)
+    assert op.n_disp == 6 and op.n_dof == 6 * 10 * 6 + 3 * 10 * 6
+    y = np.sin(0.3 + 0.017 * np.arange(op.n_dof))
+    assert np.isfinite(op.apply(y)).all()
+    xyz1, tets1 = oracle.box_mesh(1, 1, 1)
+    with pytest.raises(ValueError):
+        oracle.Operator(xyz1, tets1, 1, 1.0, dispersive=([0, 6], 2.0, 5.0, 0.4))
+    with pytest.raises(ValueError):
+        oracle.Operator(xyz1, tets1, 1, 1.0, dispersive=([1, 1], 2.0, 5.0, 0.4))
+
+
+def test_debye_polarisation_rows_and_analytic_permittivity(oracle):
+    """rhs.rs:3164-3250: every appended P row holds exactly a = -1/tau on its
+    diagonal and g = (eps_s - eps_inf)/tau on the E DOF of the same element,
+    node and component, and eps(w) = eps_inf + g/(iw - a) equals the Debye
+    permittivity eps_inf + (eps_s - eps_inf)/(1 + i w tau) (dispersive.rs:41-47)."""
+    eps_inf, eps_s, tau = 2.0, 6.0, 0.3
+    op, elems = _debye_op(oracle, 2, 0.0, eps_inf=eps_inf, eps_s=eps_s, tau=tau)
+    a = _dense(op)
+    np_, eh = 10, 6 * 10 * op.K
+    want_a, want_g = -1.0 / tau, (eps_s - eps_inf) / tau
+    for slot, e in enumerate(elems):
+        for node in range(np_):
+            for c in range(3):
+                row = eh + (slot * np_ + node) * 3 + c
+                nz = np.flatnonzero(a[row])
+                ecol = (e * np_ + node) * 6 + c
+                assert sorted(nz) == sorted([row, ecol])
+                assert abs(a[row, row] - want_a) < 1e-12
+                assert abs(a[row, ecol] - want_g) < 1e-12
+    for w in (0.2, 1.0, 3.3, 12.0, 20.0):
+        op_eps = eps_inf + want_g / (1j * w - want_a)
+        ref = eps_inf + (eps_s - eps_inf) / (1.0 + 1j * w * tau)
+        assert abs(op_eps - ref) < 1e-12 * abs(ref)
+
+
+def test_debye_e_rows_carry_the_polarisation_current(oracle):
+    """rhs.rs:1456-1472: on a Debye element the E rows differ from the
+    non-dispersive operator (same eps = eps_inf) only by -Pdot/eps_inf: the
+    own-E diagonal gains -g/eps_inf and the matching P column holds
+    -a/eps_inf; rows of non-dispersive elements are unchanged."""
+    eps_inf, eps_s, tau = 2.5, 7.0, 0.3
+    xyz, tets = oracle.box_mesh(2, 1, 1)
+    K = tets.shape[0]
+    elems = np.arange(0, K, 2)
+    op = oracle.Operator(xyz, tets, 1, 1.0, eps=eps_inf, dispersive=(elems, eps_inf, eps_s, tau))
+    plain = oracle.Operator(xyz, tets, 1, 1.0, eps=eps_inf)
+    a, a0 = _dense(op), _dense(plain)
+    np_, eh = 4, 6 * 4 * K
+    g, aa = (eps_s - eps_inf) / tau, -1.0 / tau
+    d = a[:eh, :eh] - a0
+    slot_of = {int(e): s for s, e in enumerate(elems)}
+    for e in range(K):
+        for node in range(np_):
+            for c in range(6):
+                row = (e * np_ + node) * 6 + c
+                if c < 3 and e in slot_of:
+                    want = np.zeros(eh)
+                    want[row] = -g / eps_inf
+                    assert np.abs(d[row] - want).max() < 1e-12
+                    pcol = eh + (slot_of[e] * np_ + node) * 3 + c
+                    prow = a[row, eh:].copy()
+                    assert abs(prow[pcol - eh] + aa / eps_inf) < 1e-12
+                    prow[pcol - eh] = 0.0
+                    assert np.abs(prow).max() == 0.0
+                else:
+                    assert np.abs(d[row]).max() < 1e-12
+                    assert np.abs(a[row, eh:]).max() == 0.0
+
+
+def test_debye_energy_dissipation_identity(oracle):
+    """With the central flux and PEC walls the curl part is skew in the
+    energy inner product, so for the Debye-augmented system the energy
+    0.5 eps_inf |E|_M^2 + 0.5 mu |H|_M^2 + 0.5 |P|_M^2 / (eps_s - eps_inf)
+    changes at exactly -(eps_s - eps_inf)/tau * |E - P/(eps_s - eps_inf)|_M^2
+    summed over the Debye elements (from tau Pdot + P = (eps_s - eps_inf) E and
+    eps_inf Edot = curl H - Pdot). A wrong sign or a missing 1/eps_inf on
+    either coupling breaks it."""
+    eps_inf, eps_s, tau = 2.0, 5.0, 0.4
+    de = eps_s - eps_inf
+    xyz, tets = oracle.box_mesh(2, 1, 1, 1.0, 1.0, 1.0, 0.1, 3, True)
+    K = tets.shape[0]
+    elems = np.arange(1, K, 3)
+    op = oracle.Operator(xyz, tets, 2, 0.0, eps=eps_inf, dispersive=(elems, eps_inf, eps_s, tau))
+    ex = op.export()
+    r = oracle.refelem(2)
+    np_, eh = r["np"], 6 * r["np"] * K
+    w = energy_mass(oracle, op)
+    for s, e in enumerate(elems):
+        ix = eh + (s * np_ + np.arange(np_)) * 3
+        for c in range(3):
+            w[np.ix_(ix + c, ix + c)] = abs(ex["det"][e]) * r["mass"] / de
+    rng = np.random.default_rng(7)
+    for _ in range(3):
+        y = rng.standard_normal(op.n_dof)
+        rate = y @ w @ op.apply(y)
+        want = 0.0
+        for s, e in enumerate(elems):
+            m = abs(ex["det"][e]) * r["mass"]
+            for c in range(3):
+                ev = y[(e * np_ + np.arange(np_)) * 6 + c]
+                pv = y[eh + (s * np_ + np.arange(np_)) * 3 + c]
+                d = ev - pv / de
+                want -= de / tau * d @ m @ d
+        assert abs(rate - want) < 1e-9 * abs(want)
+        assert want < 0.0
