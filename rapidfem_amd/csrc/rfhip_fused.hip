@@ -39,6 +39,12 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// Two floats to bf16 (round to nearest even, as bf16_rne for finite values),
+// a in the low half: one v_cvt_pk_bf16_f32 on gfx950.
+__device__ __forceinline__ uint32_t bf16_pair(float a, float b) {
+  const __bf16 x = (__bf16)a, y = (__bf16)b;
+  return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
+}
 
 template <bool CIN>
 __device__ __forceinline__ double load_state(const StageArgs &io, int64_t i) {
@@ -118,13 +124,13 @@ __device__ __forceinline__ void node_update(const StageArgs &io, int64_t base, c
   } else {
     float pn[6];
     float yh[6];
-    uint16_t yl[6];
+    float yr[6];  // the part of y below f32, stored as bf16 (round to nearest even)
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       pn[c] = (float)(io.sa * pold[c] + io.dt * kv[c]);
       const double yn = own[c] + io.sb * (double)pn[c];
       yh[c] = (float)yn;
-      yl[c] = bf16_rne((float)(yn - (double)yh[c]));
+      yr[c] = (float)(yn - (double)yh[c]);
     }
     float2 *pp = reinterpret_cast<float2 *>(static_cast<float *>(io.p) + base);
     float2 *yo = reinterpret_cast<float2 *>(static_cast<float *>(io.yout) + base);
@@ -133,7 +139,7 @@ __device__ __forceinline__ void node_update(const StageArgs &io, int64_t base, c
     for (int q = 0; q < 3; ++q) {
       pp[q] = make_float2(pn[2 * q], pn[2 * q + 1]);
       yo[q] = make_float2(yh[2 * q], yh[2 * q + 1]);
-      lo[q] = (uint32_t)yl[2 * q] | ((uint32_t)yl[2 * q + 1] << 16);
+      lo[q] = bf16_pair(yr[2 * q], yr[2 * q + 1]);
     }
   }
 }
@@ -548,12 +554,19 @@ __device__ __forceinline__ void face_geom(const double *ji, int f, double *n, do
     const double r0 = ji[x], r1 = ji[3 + x], r2 = ji[6 + x];
     g[x] = f == 0 ? -r2 : f == 1 ? -r0 : f == 2 ? -r1 : (r0 + r1 + r2);
   }
-  const double len = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-  const double inv = 1.0 / len;
-  n[0] = g[0] * inv;
-  n[1] = g[1] * inv;
-  n[2] = g[2] * inv;
-  cf = 0.5 * len;
+  // 1/|g| from the hardware reciprocal square root and two Newton steps
+  // (y <- y (3/2 - s y^2 / 2), each squaring the relative error): a few ulp,
+  // without the correctly rounded sqrt and divide sequences. |g| > 0 for a
+  // non-degenerate element.
+  const double s = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+  const double hs = 0.5 * s;
+  double y = __builtin_amdgcn_rsq(s);
+  y = y * fma(-hs * y, y, 1.5);
+  y = y * fma(-hs * y, y, 1.5);
+  n[0] = g[0] * y;
+  n[1] = g[1] * y;
+  n[2] = g[2] * y;
+  cf = hs * y;  // 0.5 |g|
 }
 
 // Order of element groups over the blocks. With xcd, blocks are taken to
