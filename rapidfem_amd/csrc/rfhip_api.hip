@@ -37,7 +37,7 @@ constexpr double kKclErrFloor = 1e-12;   // constants.rs KCL_ERR_FLOOR
 constexpr double kKclNonfinite = 10.0;   // constants.rs KCL_NONFINITE_ERR
 constexpr double kArnoldiBreakdown = 1e-12;  // constants.rs:62
 constexpr int kMaxKrylov = 64;
-constexpr int kFusedVariants = 22;  // rfhip_fused.hip launch_stage
+constexpr int kFusedVariants = 24;  // rfhip_fused.hip launch_stage
 
 template <class F>
 int guarded(F &&f) {

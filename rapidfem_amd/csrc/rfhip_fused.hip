@@ -2851,6 +2851,300 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
   }
 }
 
+// k_fused11: k_fused9 with whole elements per wave (EPB = 64/Np elements,
+// one lane per element node; the block's 4 waves work independently on
+// their own groups). Every LDS hand-off of a group (state, flux, materials,
+// the LIFT reads) stays inside one wave, so after the initial table load
+// there are no block-wide barriers: a wave stalled on memory or LDS does not
+// hold the block's other waves.
+template <int NP, int NFP, bool CIN, int EPI, int UNR>
+__global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3))) k_fused11(StageArgs io) {
+  constexpr int EPB = 64 / NP;  // elements per WAVE: a wave owns whole elements
+  constexpr int NWB = kFusedWG / 64;
+  constexpr int NF4 = 4 * NFP;
+  constexpr int FPT = (NF4 + NP - 1) / NP;
+  // D_k as [k][i][j] (row stride DS) and LIFT as [i][fm] (row stride LS):
+  // a 16-byte LDS read then returns two consecutive j (or fm) entries of a
+  // thread's row; the padded strides spread the rows over the banks.
+  constexpr int DS = NP + (NP & 1) + 2, LS = NF4 + 2;
+  constexpr int ND = 3 * NP * DS;
+  constexpr int SD = ND + NP * LS;
+  constexpr int SFS = NF4 * 6 + 2;
+  constexpr int PPT = (EPB * NP * 3 + 63) / 64;
+  __shared__ __align__(16) double su_b[NWB][EPB * NP * 6];
+  __shared__ __align__(16) double sf_b[NWB][EPB * SFS];
+  __shared__ __align__(16) double sd[SD];
+  __shared__ __align__(16) double sm_b[NWB][EPB * 12];
+  __shared__ int sfn[NF4];
+  for (int q = threadIdx.x; q < SD; q += kFusedWG) {
+    double v = 0.0;
+    if (q < ND) {
+      const int k = q / (NP * DS), r = q - k * NP * DS, i = r / DS, j = r - i * DS;
+      if (j < NP) v = io.dT[(k * NP + j) * NP + i];
+    } else {
+      const int i = (q - ND) / LS, fm = (q - ND) - i * LS;
+      if (fm < NF4) v = io.lift[fm * NP + i];
+    }
+    sd[q] = v;
+  }
+  for (int q = threadIdx.x; q < NF4; q += kFusedWG) sfn[q] = io.face_nodes[q];
+  const double *dT = sd;
+  const double *lift = sd + ND;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double *su = su_b[wave];
+  double *sf = sf_b[wave];
+  double *sm = sm_b[wave];
+  const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;  // groups of EPB elements, one wave each
+  const int64_t gw0 = (int64_t)blockIdx.x * NWB + wave, gws = (int64_t)gridDim.x * NWB;
+  // The only block-wide barrier: D, LIFT and the face-node table are in LDS.
+  __syncthreads();
+  // Every other hand-off goes through the wave's own LDS slice; a wave's LDS
+  // operations complete in order, so ordering the compiler's code is enough.
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  // Lane ids come from an opaque copy of threadIdx.x wherever the group loop
+  // uses them, so per-lane addresses are not hoisted out of it and held in
+  // registers.
+  auto lane_ids = [&](int &tid, int &le, int &i) {
+    int tv = threadIdx.x;
+    asm volatile("" : "+v"(tv));
+    tid = tv & 63;  // lane
+    le = tid / NP;  // element of the wave's group (EPB: spare lanes)
+    i = tid - le * NP;
+  };
+
+  using HT = typename std::conditional<CIN, float2, double2>::type;
+  HT sh[PPT];
+  uint32_t sl[PPT];
+  int nbn[FPT];
+  int mnext;
+  auto read_ahead = [&](int64_t g) {
+    int tid, le, i;
+    lane_ids(tid, le, i);
+    const int64_t rg = io.n_elem - g * EPB;
+    const int ng = rg < EPB ? (int)rg : EPB;
+    const int64_t gb = g * EPB * NP * 6;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int q0 = tid + k * 64;
+      const int q = q0 < ng * NP * 3 ? q0 : ng * NP * 3 - 1;
+      if constexpr (CIN) {
+        sh[k] = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + gb)[q];
+        sl[k] = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + gb)[q];
+      } else {
+        sh[k] = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + gb)[q];
+      }
+    }
+    const int64_t eg = g * EPB + (le < ng ? le : ng - 1);
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int fm = i + s * NP;
+      const int v = io.nbase[eg * NF4 + (fm < NF4 ? fm : NF4 - 1)];
+      nbn[s] = fm < NF4 ? v : kCodePec;
+    }
+    mnext = io.mat[eg];
+  };
+  auto write_state = [&](int ng) {
+    int tid, le, i;
+    lane_ids(tid, le, i);
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int q = tid + k * 64;
+      if (q < ng * NP * 3) {
+        double2 v;
+        if constexpr (CIN) {
+          v.x = (double)sh[k].x + (double)__uint_as_float(sl[k] << 16);
+          v.y = (double)sh[k].y + (double)__uint_as_float(sl[k] & 0xFFFF0000u);
+        } else {
+          v = sh[k];
+        }
+        reinterpret_cast<double2 *>(su)[q] = v;
+      }
+    }
+  };
+  // Waves without a group at all (the last block) just leave; the last
+  // group of a wave reads itself ahead again, which keeps every read-ahead
+  // load unconditional.
+  if (gw0 >= ngroups) return;
+  read_ahead(gw0);
+  {
+    const int64_t r0 = io.n_elem - gw0 * EPB;
+    write_state(r0 < EPB ? (int)r0 : EPB);
+  }
+
+#pragma unroll 1
+  for (int64_t grp = gw0; grp < ngroups; grp += gws) {
+    int tid, le, i;
+    lane_ids(tid, le, i);
+    const int64_t e0 = grp * EPB;
+    const int64_t rem = io.n_elem - e0;
+    const int nel = rem < EPB ? (int)rem : EPB;
+    const bool active = tid < nel * NP;  // tid is the lane
+    const int64_t e = e0 + (le < nel ? le : nel - 1);
+    const int64_t base = (e * NP + i) * 6;
+
+    double ji[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) ji[q] = io.jinv[e * 9 + q];
+    TraceRaw<CIN> nr[FPT];
+    int fcodes[FPT];
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      fcodes[s] = nbn[s] >= 0 ? 0 : nbn[s];
+      load_trace_raw<CIN>(io, nbn[s] >= 0 ? nbn[s] : 0, nr[s]);
+    }
+    // material row entries i, i + Np, ... (< 12) of element le
+    constexpr int MR = (12 + NP - 1) / NP;
+    double mrow[MR];
+#pragma unroll
+    for (int k = 0; k < MR; ++k) {
+      const int q = i + k * NP;
+      mrow[k] = io.mat_tab[12 * mnext + (q < 12 ? q : 0)];
+    }
+    read_ahead(grp + gws < ngroups ? grp + gws : grp);
+    wave_sync();  // the group's state is in the wave's LDS
+    // Needed only after the flux; issued here, behind the traces, to keep
+    // them out of the registers of the volume contraction.
+    using PT = typename std::conditional<EPI == 2, float, double>::type;
+    PT pold[6];
+    if constexpr (EPI == 1) {
+      const double2 *pp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.p) + base);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double2 v = pp[q];
+        pold[2 * q] = v.x;
+        pold[2 * q + 1] = v.y;
+      }
+    } else if constexpr (EPI == 2) {
+      const float2 *pp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.p) + base);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float2 v = pp[q];
+        pold[2 * q] = v.x;
+        pold[2 * q + 1] = v.y;
+      }
+    }
+
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const double *ub = su + le * NP * 6;
+    if (active) {
+      double r[3][6];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
+      const double *dr = dT + i * DS;
+      auto vol_step = [&](int j, double d0, double d1, double d2) {
+        double u[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) u[c] = ub[j * 6 + c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          r[0][c] = fma(d0, u[c], r[0][c]);
+          r[1][c] = fma(d1, u[c], r[1][c]);
+          r[2][c] = fma(d2, u[c], r[2][c]);
+        }
+      };
+#pragma unroll UNR
+      for (int j = 0; j < NP - 1; j += 2) {
+        const double2 a0 = *reinterpret_cast<const double2 *>(dr + j);
+        const double2 a1 = *reinterpret_cast<const double2 *>(dr + NP * DS + j);
+        const double2 a2 = *reinterpret_cast<const double2 *>(dr + 2 * NP * DS + j);
+        vol_step(j, a0.x, a1.x, a2.x);
+        vol_step(j + 1, a0.y, a1.y, a2.y);
+      }
+      if constexpr (NP & 1) vol_step(NP - 1, dr[NP - 1], dr[NP * DS + NP - 1], dr[2 * NP * DS + NP - 1]);
+      auto pd = [&](int p, int c) { return ji[p] * r[0][c] + ji[3 + p] * r[1][c] + ji[6 + p] * r[2][c]; };
+      acc[0] = pd(1, 5) - pd(2, 4);
+      acc[1] = pd(2, 3) - pd(0, 5);
+      acc[2] = pd(0, 4) - pd(1, 3);
+      acc[3] = -(pd(1, 2) - pd(2, 1));
+      acc[4] = -(pd(2, 0) - pd(0, 2));
+      acc[5] = -(pd(0, 1) - pd(1, 0));
+      // The curl is formed here, before the flux: left to itself the
+      // compiler sinks it past the flux and keeps the 18 reference
+      // derivatives live there.
+      asm volatile("" ::"v"(acc[0]), "v"(acc[1]), "v"(acc[2]), "v"(acc[3]), "v"(acc[4]), "v"(acc[5]) : "memory");
+
+#pragma unroll
+      for (int s = 0; s < FPT; ++s) {
+        const int fm = i + s * NP;
+        if (fm >= NF4) continue;
+        const int f = fm / NFP;
+        double n[3], cf;
+        face_geom(ji, f, n, cf);
+        double u[6], nv[6], fo[6];
+        const double *uv = ub + sfn[fm] * 6;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) u[c] = uv[c];
+        trace_f64<CIN>(nr[s], nv);
+        face_flux(fcodes[s], io.alpha, n, cf, u, nv, fo);
+        double2 *out = reinterpret_cast<double2 *>(sf + le * SFS + fm * 6);
+        out[0] = make_double2(fo[0], fo[1]);
+        out[1] = make_double2(fo[2], fo[3]);
+        out[2] = make_double2(fo[4], fo[5]);
+      }
+#pragma unroll
+      for (int k = 0; k < MR; ++k)
+        if (i + k * NP < 12) sm[le * 12 + i + k * NP] = mrow[k];
+    }
+    wave_sync();  // flux and materials complete
+    // Every register loaded for this group is taken here on all lanes: left
+    // to lanes that are active only, a lane-predicated branch would leave the
+    // loads pending on its skip path, and the next group start would wait for
+    // all outstanding loads before reusing those registers.
+#pragma unroll
+    for (int q = 0; q < 9; ++q) asm volatile("" ::"v"(ji[q]));
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) touch_trace<CIN>(nr[s]);
+    if constexpr (EPI != 0) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) asm volatile("" ::"v"(pold[c]));
+    }
+#pragma unroll
+    for (int k = 0; k < MR; ++k) asm volatile("" ::"v"(mrow[k]));
+    double own[6];
+    if (active) {
+      const double *fb = sf + le * SFS;
+      const double *lr = lift + i * LS;
+#pragma unroll UNR
+      for (int fm = 0; fm < NF4; fm += 2) {  // NF4 = 4 Nfp is even
+        const double2 w = *reinterpret_cast<const double2 *>(lr + fm);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] = fma(w.x, fb[fm * 6 + c], acc[c]);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] = fma(w.y, fb[fm * 6 + 6 + c], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) own[c] = ub[i * 6 + c];
+    }
+    wave_sync();  // su and sf are read; su takes the next group's state
+    if (grp + gws < ngroups) {
+      const int64_t rn = io.n_elem - (grp + gws) * EPB;
+      write_state(rn < EPB ? (int)rn : EPB);
+    }
+    // The remaining read-ahead registers are taken here, before this
+    // group's stores: a wait placed after the (lane-predicated) stores would
+    // be counted as if they had not been issued and would wait on them.
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) asm volatile("" : "+v"(nbn[s]));
+    asm volatile("" : "+v"(mnext));
+    if (active) {
+      double po[6], srcv[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
+      if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
+      }
+      node_update<EPI>(io, base, sm + le * 12, acc, own, po, srcv);
+    }
+  }
+}
+
 template <int NP>
 struct Nfp {
   static constexpr int v = NP == 4 ? 3 : NP == 10 ? 6 : NP == 20 ? 10 : 15;
@@ -2890,7 +3184,8 @@ inline int64_t resident_blocks(const void *kernel) {
 // 16 / 17 = k_fused9 (k_fused3 layout with the memory traffic pipelined;
 // loops unrolled by 2 / 4 nodes), 18 / 19 = k_fused9 with the LIFT on the
 // matrix cores, 20 / 21 = k_fused10 (sparse volume, LIFT unrolled by 2 / 4;
-// only when the operator's D_k fit the equispaced pattern). Chosen by the
+// only when the operator's D_k fit the equispaced pattern), 22 / 23 =
+// k_fused11 (whole elements per wave, no block barriers in the group loop). Chosen by the
 // operator (autotuned or set).
 int g_variant_override = -1;
 
@@ -2938,6 +3233,22 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
         hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
       else
         hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+    }
+  } else if ((variant == 22 || variant == 23) && a.nbase) {
+    const void *kf = variant == 22 ? reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 1>)
+                                   : reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 2>);
+    static int64_t resident[2] = {0, 0};
+    int64_t &res = resident[variant - 22];
+    if (res == 0) res = resident_blocks(kf);
+    const int64_t epw = 64 / NP;
+    const int64_t wgroups = (a.n_elem + epw - 1) / epw;
+    const int64_t groups = (wgroups + 3) / 4;  // blocks of 4 waves
+    const int64_t grid = groups < res ? groups : res;
+    if (grid > 0) {
+      if (variant == 22)
+        hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
     }
   } else if ((variant == 20 || variant == 21) && a.nbase && a.dsparse) {
     const void *kf = variant == 20 ? reinterpret_cast<const void *>(&k_fused10<NP, F, CIN, EPI, 1>)
