@@ -177,6 +177,16 @@ int rfhip_op_set_port(rfhip_op *op, int32_t port, int64_t n_faces,
                       const int32_t *elem, const int32_t *local_face,
                       const double *profiles, const double *weights);
 int rfhip_op_n_ports(const rfhip_op *op);
+/* MaxwellOperator::port_source (rhs.rs:1157-1199), assembled on the device
+ * from the faces and mode profiles given to rfhip_op_set_port: the n_dof
+ * drive pattern of the port's incident mode (the characteristic flux of the
+ * jumps [E] = -e_t, [H] = -h_t, lifted and scaled by the element's 1/eps,
+ * 1/mu). It is written to b (n_dof doubles) when b is not NULL, and it stays
+ * on the device as the operator's vector source: a following transient with
+ * RFHIP_DRIVE_VECTOR and src = NULL drives with it, without a host copy
+ * (the reference builds it on the host and uploads it per run). Passing a
+ * src pattern again replaces it. */
+int rfhip_op_port_source_f64(rfhip_op *op, int32_t port, double *b);
 /* Explicit transient recording (P_e, P_h) of every port per frame on the
  * device: signals receives (steps+1)*n_ports*2 doubles (row 0 from y0),
  * y_final (may be NULL) the final state. Arguments otherwise as

@@ -744,8 +744,11 @@ void orc_op_export(const orc_op *op, double *jinv, double *det, double *normal,
 
 /* rhs.rs:1341-1454 apply_element (non-dispersive block). scratch holds
  * ee,hh,de,dh (3np each), rd (3np), pd (9np). */
+/* ghost: NULL, or the incident field (E_inc, H_inc) per face node of every
+ * face, [(4e + f)*Nfp + m][6], used as the ghost state on port faces
+ * (rhs.rs:1150-1156: the absorbing operator is the zero-ghost case) */
 static void apply_element_(const orc_op *op, int64_t e, const double *y,
-                           double *out, double *s) {
+                           double *out, double *s, const double *ghost) {
   const int np = op->np, nfp = op->nfp, cols = 4 * nfp;
   double *ee = s, *hh = s + 3 * np, *de = s + 6 * np, *dh = s + 9 * np;
   double *rd = s + 12 * np, *pd = s + 15 * np;
@@ -771,9 +774,13 @@ static void apply_element_(const orc_op *op, int64_t e, const double *y,
       double hm[3] = {hh[vi * 3], hh[vi * 3 + 1], hh[vi * 3 + 2]};
       double je[3], jh[3];
       if (port >= 0) {
-        /* rhs.rs:1380-1386 port: characteristic flux, zero incident field */
-        memcpy(je, em, sizeof je);
-        memcpy(jh, hm, sizeof jh);
+        /* rhs.rs:1380-1386 port: characteristic flux against the incident
+         * field (zero unless a ghost is given) */
+        const double *gh = ghost ? &ghost[(q * nfp + m) * 6] : NULL;
+        for (int k = 0; k < 3; ++k) {
+          je[k] = em[k] - (gh ? gh[k] : 0.0);
+          jh[k] = hm[k] - (gh ? gh[3 + k] : 0.0);
+        }
       } else if (nb < 0) {
         /* rhs.rs:1387-1402 PEC ghost */
         double edn = dot3(nrm, em), hdn = dot3(nrm, hm);
@@ -829,7 +836,14 @@ static void apply_element_(const orc_op *op, int64_t e, const double *y,
 
 /* rhs.rs:1262-1294 apply_into: elements dealt out in contiguous runs of
  * K/(4*threads) (constants.rs:203 APPLY_TASKS_PER_THREAD = 4). */
-void orc_apply(const orc_op *op, const double *y, double *dy) {
+static void apply_all_(const orc_op *op, const double *y, double *dy, const double *ghost);
+void orc_apply(const orc_op *op, const double *y, double *dy) { apply_all_(op, y, dy, NULL); }
+/* The operator with an incident field on the port faces: A y + b for the
+ * port source b of that field (rhs.rs:1157-1199). */
+void orc_apply_ghost(const orc_op *op, const double *y, const double *ghost, double *dy) {
+  apply_all_(op, y, dy, ghost);
+}
+static void apply_all_(const orc_op *op, const double *y, double *dy, const double *ghost) {
   const int64_t K = op->K;
   const int np = op->np;
   const int nt = nthreads_();
@@ -843,7 +857,7 @@ void orc_apply(const orc_op *op, const double *y, double *dy) {
     for (int64_t r = 0; r < nruns; ++r) {
       int64_t e1 = (r + 1) * chunk;
       if (e1 > K) e1 = K;
-      for (int64_t e = r * chunk; e < e1; ++e) apply_element_(op, e, y, &dy[e * np * 6], s);
+      for (int64_t e = r * chunk; e < e1; ++e) apply_element_(op, e, y, &dy[e * np * 6], s, ghost);
     }
     free(s);
   }

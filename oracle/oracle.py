@@ -52,6 +52,7 @@ def _load() -> C.CDLL:
         "orc_op_n_dispersive": (i64, [vp]),
         "orc_op_set_threads": (None, [C.c_int]),
         "orc_apply": (None, [vp, _dp, _dp]),
+        "orc_apply_ghost": (None, [vp, _dp, _dp, _dp]),
         "orc_field_energy": (d, [vp, _dp]),
         "orc_lserk_run": (None, [vp, _dp, d, i64, C.c_int, i64, vp, vp]),
         "orc_lserk_run_dense": (None, [_dp, C.c_int, _dp, d, i64]),
@@ -207,6 +208,17 @@ class Operator:
         lib().orc_apply(self._h, y, dy)
         return dy
 
+    def apply_ghost(self, y: np.ndarray, ghost: np.ndarray) -> np.ndarray:
+        """A y with the incident field `ghost` ((K, 4, Nfp, 6): E_inc, H_inc at
+        every face node, read on port faces only) as the port faces' ghost
+        state; the absorbing apply is the zero-ghost case."""
+        y = np.ascontiguousarray(y, np.float64)
+        ghost = np.ascontiguousarray(ghost, np.float64)
+        assert y.shape == (self.n_dof,) and ghost.size == self.K * 4 * lib().orc_nfp(self.order) * 6
+        dy = np.empty_like(y)
+        lib().orc_apply_ghost(self._h, y, ghost.reshape(-1), dy)
+        return dy
+
     def field_energy(self, y) -> float:
         return float(lib().orc_field_energy(self._h, np.ascontiguousarray(y, np.float64)))
 
@@ -324,6 +336,34 @@ def err_norm(ctl, y_old, y_new, err):
 
 def factor(ctl, e, prev, reject):
     return float(lib().orc_factor(C.byref(ctl), float(e), float(prev), 1 if reject else 0))
+
+
+def port_source(op: "Operator", faces, profiles) -> np.ndarray:
+    """rhs.rs:1157-1199 MaxwellOperator::port_source: the drive pattern b of a
+    port's incident mode. faces: (n, 2) (element, local face); profiles:
+    (n, Nfp, 6) e_t then h_t at the face nodes (face-node order). With the
+    ghost state (E_inc, H_inc) = (e_t, h_t) the port flux jumps are [E] = -e_t,
+    [H] = -h_t; their characteristic (alpha = 1) flux, lifted and scaled by
+    the element's 1/eps, 1/mu, is b (numpy, for parity tests)."""
+    ex = op.export()
+    r = refelem(op.order)
+    n_p, nfp = r["np"], r["nfp"]
+    b = np.zeros(op.n_dof)
+    prof = np.asarray(profiles, np.float64).reshape(len(faces), nfp, 6)
+    for q, (e, f) in enumerate(faces):
+        nv = ex["normal"][e, f]
+        coef = 0.5 * ex["fscale"][e, f]
+        ie, im = ex["inv_eps"][e], ex["inv_mu"][e]
+        base = (e * n_p + np.arange(n_p)) * 6
+        for m in range(nfp):
+            et, ht = prof[q, m, :3], prof[q, m, 3:]
+            nxe, nxh = np.cross(nv, et), np.cross(nv, ht)
+            s_de = nxh - np.cross(nv, nxe)
+            s_dh = -nxe - np.cross(nv, nxh)
+            w = coef * r["lift"][:, f * nfp + m]
+            b[base[:, None] + np.arange(3)] += ie * w[:, None] * s_de
+            b[base[:, None] + 3 + np.arange(3)] += im * w[:, None] * s_dh
+    return b
 
 
 def port_projections(y, dof, prof, w):

@@ -81,6 +81,7 @@ SIGNATURES = {
                                                 _dp, _vp]),
     "rfhip_op_set_port": (C.c_int, [_vp, _i32, _i64, _ip, _ip, _dp, _dp]),
     "rfhip_op_n_ports": (C.c_int, [_vp]),
+    "rfhip_op_port_source_f64": (C.c_int, [_vp, C.c_int32, _vp]),
     "rfhip_op_transient_ports_f64": (C.c_int, [_vp, _dp, _d, _i64, _i64, C.c_int, _i64, _vp, _vp, _i64, _dp, _vp]),
     "rfhip_op_transient_kcl": (C.c_int, [_vp, _fp, C.c_float, _i64, C.c_int, _i64, _vp, _vp, _i64,
                                          C.POINTER(ControllerC), _fp, C.POINTER(KclStatsC)]),

@@ -101,7 +101,9 @@ void sync(rfhip_op *op) { RF_HIP(hipStreamSynchronize(op->stream)); }
 void check_drive(const rfhip_op *op, int kind, int64_t dof, const double *src, int64_t n_g, int64_t need, const void *g) {
   RF_REQUIRE(kind >= RFHIP_DRIVE_FREE && kind <= RFHIP_DRIVE_VECTOR, "unknown drive kind");
   if (kind == RFHIP_DRIVE_POINT) RF_REQUIRE(dof >= 0 && dof < op->n_dof, "source_dof out of range");
-  if (kind == RFHIP_DRIVE_VECTOR) RF_REQUIRE(src != nullptr, "a vector drive needs a source pattern");
+  if (kind == RFHIP_DRIVE_VECTOR)
+    RF_REQUIRE(src != nullptr || op->src_resident,
+               "a vector drive needs a source pattern (or a port source assembled on the device)");
   if (kind != RFHIP_DRIVE_FREE) {
     RF_REQUIRE(g != nullptr, "a driven run needs waveform samples");
     RF_REQUIRE(n_g == need, "wrong number of waveform samples");
@@ -507,8 +509,10 @@ const double *state_aos_f64(rfhip_op *op) { return state_layout_f64(op); }
 
 void load_source(rfhip_op *op, int kind, const double *src) {
   if (kind != RFHIP_DRIVE_VECTOR) return;
+  if (src == nullptr) return;  // the device-assembled port source (checked by check_drive)
   op->src.alloc(8 * nvec(op));
   h2d_layout_f64(op, src, op->src.as<double>());
+  op->src_resident = false;
   sync(op);
 }
 
@@ -1044,6 +1048,8 @@ int rfhip_op_set_port(rfhip_op *op, int32_t port, int64_t n_faces, const int32_t
     pe.dof.resize((size_t)n_faces * nfp);
     pe.prof.assign(profiles, profiles + (size_t)n_faces * nfp * 6);
     pe.w.assign(weights, weights + (size_t)n_faces * nfp);
+    pe.face_elem.assign(elem, elem + n_faces);
+    pe.face_lf.assign(local_face, local_face + n_faces);
     for (int64_t q = 0; q < n_faces; ++q) {
       RF_REQUIRE(elem[q] >= 0 && elem[q] < op->n_elem && local_face[q] >= 0 && local_face[q] < 4,
                  "port face out of range");
@@ -1062,6 +1068,47 @@ int rfhip_op_set_port(rfhip_op *op, int32_t port, int64_t n_faces, const int32_t
 }
 
 int rfhip_op_n_ports(const rfhip_op *op) { return op ? (int)op->ports.size() : -1; }
+
+int rfhip_op_port_source_f64(rfhip_op *op, int32_t port, double *b) {
+  return guarded([&] {
+    RF_REQUIRE(op, "null argument");
+    RF_REQUIRE(port >= 0 && port < (int)op->ports.size(), "port index out of range");
+    const auto &pe = op->ports[port];
+    const int nfp = op->nfp;
+    // port faces grouped by element, so one thread per (element, node) sums
+    // every port face of its element
+    std::vector<int64_t> order(pe.face_elem.size());
+    for (size_t q = 0; q < order.size(); ++q) order[q] = (int64_t)q;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t x, int64_t y) { return pe.face_elem[x] < pe.face_elem[y]; });
+    std::vector<int64_t> elems, offs(1, 0);
+    std::vector<int32_t> lf;
+    std::vector<double> prof;
+    for (int64_t q : order) {
+      if (elems.empty() || elems.back() != pe.face_elem[q]) {
+        if (!elems.empty()) offs.push_back((int64_t)lf.size());
+        elems.push_back(pe.face_elem[q]);
+      }
+      lf.push_back(pe.face_lf[q]);
+      prof.insert(prof.end(), pe.prof.begin() + q * nfp * 6, pe.prof.begin() + (q + 1) * nfp * 6);
+    }
+    offs.push_back((int64_t)lf.size());
+    hipStream_t s = op->stream;
+    op->src.alloc(8 * nvec(op));
+    launch_fill(op->src.p, 8 * nvec(op), s);
+    if (!elems.empty()) {
+      upload(op->ps_elem, elems.data(), elems.size(), s);
+      upload(op->ps_off, offs.data(), offs.size(), s);
+      upload(op->ps_lf, lf.data(), lf.size(), s);
+      upload(op->ps_prof, prof.data(), prof.size(), s);
+      launch_port_source(op, (int64_t)elems.size(), op->ps_elem.as<int64_t>(), op->ps_off.as<int64_t>(),
+                         op->ps_lf.as<int32_t>(), op->ps_prof.as<double>(), op->src.as<double>(), s);
+    }
+    op->src_resident = true;
+    if (b) d2h_layout_f64(op, op->src.as<double>(), b);
+    sync(op);  // the host staging vectors above live until here
+  });
+}
 
 int rfhip_op_transient_ports_f64(rfhip_op *op, const double *y0, double dt, int64_t steps, int64_t substeps, int kind,
                                  int64_t dof, const double *src, const double *g, int64_t n_g, double *signals,
@@ -1144,6 +1191,7 @@ int rfhip_op_etd_step(rfhip_op *op, const double *y, const double *b, double h, 
     RF_REQUIRE(op && y && b && out, "null argument");
     op->src.alloc(8 * nvec(op));
     h2d_layout_f64(op, b, op->src.as<double>());
+    op->src_resident = false;
     const int dim = krylov_impl(op, y, h, max_dim, tol, true, out);
     if (dim_used) *dim_used = dim;
   });

@@ -3345,6 +3345,85 @@ void fused_dispatch(int np, int variant, const StageArgs &a, hipStream_t s) {
 
 }  // namespace
 
+// Port source (rhs.rs:1157-1199): for port face (e, f) and face node m with
+// the mode's (e_t, h_t), the incident-field flux jumps [E] = -e_t,
+// [H] = -h_t give, with the characteristic flux,
+//   s_E = n x h_t - n x (n x e_t),  s_H = -n x e_t - n x (n x h_t),
+// lifted to every node i of e with 0.5*fscale*LIFT[i][f*Nfp + m] and scaled
+// by the element's 1/eps, 1/mu. One thread per (element with port faces,
+// node) sums over that element's port faces: no two threads write the same
+// entry. FUSED reads J^-1 (face geometry as in the stage kernels), the
+// material table and the column-major LIFT of the fused forms; otherwise the
+// plain forms' face normals, fscale, materials and row-major LIFT (type T).
+template <bool FUSED, typename T>
+__global__ void __launch_bounds__(256) k_port_source(StageArgs sa, DevOp dv, int64_t n_elems,
+                                                     const int64_t *__restrict__ elem,
+                                                     const int64_t *__restrict__ off,
+                                                     const int32_t *__restrict__ lf,
+                                                     const double *__restrict__ prof, double *__restrict__ b) {
+  const int np = dv.np, nfp = dv.nfp;
+  const int64_t n = n_elems * np;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    const int64_t u = t / np;
+    const int i = (int)(t - u * np);
+    const int64_t e = elem[u];
+    double ae[3] = {0, 0, 0}, ah[3] = {0, 0, 0};
+    for (int64_t q = off[u]; q < off[u + 1]; ++q) {
+      const int f = lf[q];
+      double nv[3], cf;
+      if constexpr (FUSED) {
+        face_geom(sa.jinv + 9 * e, f, nv, cf);
+      } else {
+        const T *nn = static_cast<const T *>(dv.normal) + (4 * e + f) * 3;
+        nv[0] = (double)nn[0];
+        nv[1] = (double)nn[1];
+        nv[2] = (double)nn[2];
+        cf = 0.5 * (double)static_cast<const T *>(dv.fscale)[4 * e + f];
+      }
+      for (int m = 0; m < nfp; ++m) {
+        const double *pr = prof + (q * nfp + m) * 6;
+        const double et[3] = {pr[0], pr[1], pr[2]}, ht[3] = {pr[3], pr[4], pr[5]};
+        double nxe[3], nxh[3], nne[3], nnh[3];
+        cross3(nv, et, nxe);
+        cross3(nv, ht, nxh);
+        cross3(nv, nxe, nne);
+        cross3(nv, nxh, nnh);
+        double w;
+        if constexpr (FUSED)
+          w = cf * sa.lift[(size_t)(f * nfp + m) * np + i];
+        else
+          w = cf * (double)static_cast<const T *>(dv.lift)[(size_t)i * 4 * nfp + f * nfp + m];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          ae[c] += w * (nxh[c] - nne[c]);
+          ah[c] += w * (-nxe[c] - nnh[c]);
+        }
+      }
+    }
+    double ie[3], im[3];
+    if constexpr (FUSED) {
+      const double *row = sa.mat_tab + 12 * sa.mat[e];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ie[c] = row[c];
+        im[c] = row[3 + c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ie[c] = (double)static_cast<const T *>(dv.inv_eps)[3 * e + c];
+        im[c] = (double)static_cast<const T *>(dv.inv_mu)[3 * e + c];
+      }
+    }
+    const int64_t base = (e * np + i) * 6;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      b[base + c] = ie[c] * ae[c];
+      b[base + 3 + c] = im[c] * ah[c];
+    }
+  }
+}
+
 template <int NP>
 static bool d_fits_pattern_np(const double *dT) {
   for (int k = 0; k < 3; ++k) {
@@ -3365,6 +3444,24 @@ bool d_fits_sparse_pattern(int order, const double *dT) {
     case 4: return d_fits_pattern_np<35>(dT);
     default: return false;
   }
+}
+
+void launch_port_source(const rfhip_op *op, int64_t n_elems, const int64_t *elem, const int64_t *off,
+                        const int32_t *lf, const double *prof, double *b, hipStream_t s) {
+  if (n_elems == 0) return;
+  int64_t blocks = (n_elems * op->np + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  const unsigned g = (unsigned)blocks;
+  if (op->form == RFHIP_FORM_FUSED_F64 || op->form == RFHIP_FORM_FUSED_F64C)
+    hipLaunchKernelGGL((k_port_source<true, double>), dim3(g), dim3(256), 0, s, op->sa, op->dev, n_elems, elem, off,
+                       lf, prof, b);
+  else if (op->form == RFHIP_FORM_PLAIN_F64)
+    hipLaunchKernelGGL((k_port_source<false, double>), dim3(g), dim3(256), 0, s, op->sa, op->dev, n_elems, elem,
+                       off, lf, prof, b);
+  else
+    hipLaunchKernelGGL((k_port_source<false, float>), dim3(g), dim3(256), 0, s, op->sa, op->dev, n_elems, elem,
+                       off, lf, prof, b);
+  RF_HIP(hipGetLastError());
 }
 
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s) {

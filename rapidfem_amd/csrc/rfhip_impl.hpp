@@ -159,11 +159,14 @@ struct rfhip_op {
   // port modal extraction: host-side per-port entries, device copies
   struct PortEntries {
     std::vector<int64_t> dof;
+    std::vector<int32_t> face_elem, face_lf;  // the port faces (element, local face)
     std::vector<double> prof, w;
     double e_norm = 0.0, h_norm = 0.0;
   };
   std::vector<PortEntries> ports;
   rfhip::DevBuf port_dof, port_prof, port_w, port_off, port_norm, port_rows;
+  rfhip::DevBuf ps_elem, ps_off, ps_lf, ps_prof;  // port-source assembly input
+  bool src_resident = false;  // src holds a device-assembled port source
   bool ports_dirty = true;
   int kb_cap = 0;
   rfhip::DevOp dev{};
@@ -188,6 +191,12 @@ void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t 
 void launch_debye_apply(const rfhip_op *op, const void *y, void *dy, bool f32, int kind, int64_t dof,
                         const double *vec, double g, hipStream_t s);
 void launch_debye_stage(const rfhip_op *op, const StageArgs &a, hipStream_t s);
+
+// Port source b (rhs.rs:1157-1199) into the zeroed f64 vector b for the port
+// faces grouped by element: element elem[u] has faces [off[u], off[u+1]) with
+// local faces lf[q] and profiles prof[q*Nfp*6 ...] (rfhip_fused.hip).
+void launch_port_source(const rfhip_op *op, int64_t n_elems, const int64_t *elem, const int64_t *off,
+                        const int32_t *lf, const double *prof, double *b, hipStream_t s);
 
 // True when every entry of dT (3*Np*Np, [k][j][i]) outside the equispaced
 // nodal sparsity pattern is zero to round-off (|x| <= 1e-13 * max |D_k|).

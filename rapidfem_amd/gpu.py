@@ -63,6 +63,12 @@ class Drive:
     def vector(cls, b):
         return cls(L.DRIVE_VECTOR, 0, np.ascontiguousarray(b, np.float64))
 
+    @classmethod
+    def port_source(cls):
+        """Vector drive with the port source last assembled on the device
+        (GpuOperator.port_source), without a host copy of the pattern."""
+        return cls(L.DRIVE_VECTOR, 0, None)
+
 
 class GpuContext:
     """gpu/mod.rs:33-65: the first HIP device."""
@@ -208,6 +214,15 @@ class GpuOperator:
     @property
     def n_ports(self) -> int:
         return int(L.lib.rfhip_op_n_ports(self._h))
+
+    def port_source(self, port: int, download: bool = True) -> np.ndarray | None:
+        """rhs.rs:1157-1199 MaxwellOperator::port_source, assembled on the
+        device from set_port's faces and profiles; it stays there as the
+        vector source for Drive.port_source(). Returns the n_dof pattern when
+        download is true."""
+        b = np.empty(self._n) if download else None
+        L.check(L.lib.rfhip_op_port_source_f64(self._h, int(port), L.ptr(b)))
+        return b
 
     def transient_ports(self, ctx, y0, dt, steps, substeps=1, drive: Drive | None = None, g=None):
         """Explicit transient recording (P_e, P_h) of every port per frame on

@@ -561,3 +561,66 @@ def test_debye_rejects_bad_element_lists(ctx, oracle):
         with pytest.raises(RfhipError) as ei:
             GpuOperator.from_desc(ctx, d, "fused_f64", dispersive=(bad, -1.0, 1.0, 0.5))
         assert ei.value.code == 1
+
+
+# ---- port source assembled on the device (rhs.rs:1157-1199) ----------------
+
+def _port_case(ctx, oracle, form):
+    """Port faces on the z = 0 wall (tagged as ports in the operator),
+    element-varying eps and mu, random mode profiles."""
+    xyz, tets = oracle.box_mesh(3, 3, 2, 1.0, 1.0, 1.0, 0.1, 4, True)
+    K = tets.shape[0]
+    ex0 = oracle.Operator(xyz, tets, 2, 1.0).export()
+    faces = [(e, f) for e in range(K) for f in range(4)
+             if ex0["nbr"][e, f] < 0 and ex0["normal"][e, f, 2] < -0.99]
+    port = np.full(4 * K, -1, np.int32)
+    for e, f in faces:
+        port[4 * e + f] = 0
+    eps = 1.0 + 0.5 * (np.arange(K) % 3)[:, None] * np.ones((1, 3))
+    mu = 1.0 + 0.25 * (np.arange(K) % 2)[:, None] * np.ones((1, 3))
+    oop = oracle.Operator(xyz, tets, 2, 1.0, eps=eps, mu=mu, face_port=port)
+    g = _gop(ctx, oop, form)
+    prof = np.random.default_rng(11).standard_normal((len(faces), 6, 6))
+    fa = np.array(faces)
+    g.set_port(0, fa[:, 0], fa[:, 1], prof, np.ones((len(faces), 6)))
+    return oop, g, faces, prof
+
+
+@pytest.mark.parametrize("form", ALL_FORMS)
+def test_port_source_on_device_matches_oracle(ctx, oracle, form):
+    """rfhip_op_port_source_f64 against the oracle's port_source (itself
+    pinned by the ghost-state apply): every entry, all forms."""
+    oop, g, faces, prof = _port_case(ctx, oracle, form)
+    want = oracle.port_source(oop, faces, prof)
+    got = g.port_source(0)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"port source form={form}: max rel err {err:.3e}")
+    assert err < (1e-6 if form == "plain_f32" else 1e-13)
+
+
+@pytest.mark.parametrize("form", FORMS64)
+def test_resident_port_source_drives_like_the_uploaded_pattern(ctx, oracle, form):
+    """A transient driven by the device-resident port source equals the same
+    run with the pattern passed from the host, and the oracle's run with the
+    oracle's port_source; passing a host pattern afterwards replaces it."""
+    from rapidfem_amd import Drive
+
+    oop, g, faces, prof = _port_case(ctx, oracle, form)
+    b = g.port_source(0)
+    y0 = np.cos(0.2 + 0.013 * np.arange(oop.n_dof))
+    steps, sub, dt = 3, 2, 0.04
+    wave = np.sin(0.9 * np.arange(steps * sub))
+    res = g.transient(ctx, y0, dt, steps, sub, Drive.port_source(), wave, True, precision="f64")
+    up = g.transient(ctx, y0, dt, steps, sub, Drive.vector(b), wave, True, precision="f64")
+    assert np.abs(res - up).max() == 0.0
+    want = y0
+    bo = oracle.port_source(oop, faces, prof)
+    for k in range(steps):
+        want = oop.lserk(want, dt / sub, sub, ("vector", bo), wave[k * sub:(k + 1) * sub])
+    err = rel_l2(res[-1], want)
+    print(f"resident port drive form={form}: rel_l2 {err:.3e}")
+    assert err < (1e-7 if form == "fused_f64c" else 1e-11)
+    # the host pattern given above replaced the resident one
+    from rapidfem_amd import RfhipError
+    with pytest.raises(RfhipError):
+        g.transient(ctx, y0, dt, 1, 1, Drive.port_source(), wave[:1], True, precision="f64")

@@ -759,3 +759,47 @@ def test_debye_energy_dissipation_identity(oracle):
                 want -= de / tau * d @ m @ d
         assert abs(rate - want) < 1e-9 * abs(want)
         assert want < 0.0
+
+
+def test_port_source_is_the_incident_field_flux(oracle):
+    """rhs.rs:1150-1199: the port flux uses the ghost state (E_inc, H_inc);
+    its own-trace part is the absorbing operator A and the incident part is
+    port_source. So for any state y the operator with the incident field as
+    the port faces' ghost equals A y + b. The ghost apply runs the C flux
+    code, independent of the numpy port_source restatement; a sign, cross
+    product or scaling slip in either breaks it. Materials differ per element
+    so the 1/eps, 1/mu scaling is exercised."""
+    xyz, tets = oracle.box_mesh(2, 2, 2, 1.0, 1.0, 1.0, 0.1, 4, True)
+    K = tets.shape[0]
+    order = 2
+    nfp = 6
+    port = np.full(4 * K, -1, np.int32)
+    # every boundary face on z = 0 is a port face
+    tmp = oracle.Operator(xyz, tets, order, 1.0)
+    ex0 = tmp.export()
+    faces = [(e, f) for e in range(K) for f in range(4)
+             if ex0["nbr"][e, f] < 0 and ex0["normal"][e, f, 2] < -0.99]
+    assert len(faces) == 8
+    for e, f in faces:
+        port[4 * e + f] = 0
+    eps = 1.0 + 0.5 * (np.arange(K) % 3)[:, None] * np.ones((1, 3))
+    mu = 1.0 + 0.25 * (np.arange(K) % 2)[:, None] * np.ones((1, 3))
+    op = oracle.Operator(xyz, tets, order, 0.7, eps=eps, mu=mu, face_port=port)
+    rng = np.random.default_rng(3)
+    prof = rng.standard_normal((len(faces), nfp, 6))
+    ghost = np.zeros((K, 4, nfp, 6))
+    for q, (e, f) in enumerate(faces):
+        ghost[e, f] = prof[q]
+    # a ghost on non-port faces is ignored
+    ghost[0, 0] += 5.0 if port[0] < 0 else 0.0
+    b = oracle.port_source(op, faces, prof)
+    assert np.abs(b).max() > 0.1
+    for y in (np.zeros(op.n_dof), rng.standard_normal(op.n_dof)):
+        lhs = op.apply_ghost(y, ghost) - op.apply(y)
+        assert np.abs(lhs - b).max() < 1e-12 * np.abs(b).max()
+    # linear in the profile; rows outside the port elements are zero
+    assert np.allclose(oracle.port_source(op, faces, 2.0 * prof), 2.0 * b, rtol=0, atol=1e-13)
+    touched = {e for e, _ in faces}
+    for e in range(K):
+        if e not in touched:
+            assert not np.any(b[e * 60:(e + 1) * 60])
