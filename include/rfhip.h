@@ -263,6 +263,12 @@ int rfhip_op_state_load(rfhip_op *op, const double *y0);
 int rfhip_op_state_store(rfhip_op *op, double *y);
 /* nsteps free LSERK4 steps of dt on the resident state. */
 int rfhip_op_state_lserk(rfhip_op *op, double dt, int64_t nsteps);
+/* The same steps replayed from a captured HIP graph of two steps (ten stage
+ * launches), cached per (dt, kernel variant) and per starting state buffer:
+ * for small meshes, whose stage kernels run for microseconds, launch
+ * overhead is what it removes. Results equal rfhip_op_state_lserk's bit for
+ * bit. (No reference counterpart: the reference has no graph capture.) */
+int rfhip_op_state_lserk_graph(rfhip_op *op, double dt, int64_t nsteps);
 /* As rfhip_op_state_lserk, timed with HIP events on the operator's stream:
  * total_ms for the whole run and stage_ms_avg, the mean duration of one
  * launch of the dominant kernel (the fused stage kernel, or the plain form's

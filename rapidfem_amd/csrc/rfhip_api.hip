@@ -919,6 +919,8 @@ void rfhip_op_destroy(rfhip_op *op) {
   if (!op) return;
   if (op->stream) {
     (void)hipStreamSynchronize(op->stream);
+    for (auto &g : op->step_graph)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
     (void)hipStreamDestroy(op->stream);
   }
   delete op;
@@ -1300,6 +1302,51 @@ int rfhip_op_state_lserk(rfhip_op *op, double dt, int64_t nsteps) {
     RF_REQUIRE(op, "null argument");
     zero_p(op);
     for (int64_t k = 0; k < nsteps; ++k) lserk_step(op, dt, RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
+    sync(op);
+  });
+}
+
+// Two free LSERK4 steps (ten stage launches, and the Debye kernels if any)
+// captured into a graph: an even number of stages returns the fused forms
+// to the state buffer they started from, so the graph for a starting buffer
+// can be replayed back to back. Nothing in it is a memset node.
+hipGraphExec_t step_graph(rfhip_op *op, double dt) {
+  auto &g = op->step_graph[op->cur];
+  if (g.exec && g.dt == dt && g.variant == op->sa.variant) return g.exec;
+  if (g.exec) {
+    RF_HIP(hipGraphExecDestroy(g.exec));
+    g.exec = nullptr;
+  }
+  hipStream_t s = op->stream;
+  const int cur0 = op->cur;
+  hipGraph_t graph = nullptr;
+  RF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    lserk_step(op, dt, RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
+    lserk_step(op, dt, RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    op->cur = cur0;
+    throw;
+  }
+  RF_HIP(hipStreamEndCapture(s, &graph));
+  const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  RF_HIP(e);
+  op->cur = cur0;  // capture did not run anything
+  g.dt = dt;
+  g.variant = op->sa.variant;
+  return g.exec;
+}
+
+int rfhip_op_state_lserk_graph(rfhip_op *op, double dt, int64_t nsteps) {
+  return guarded([&] {
+    RF_REQUIRE(op && nsteps >= 0, "bad arguments");
+    zero_p(op);
+    hipStream_t s = op->stream;
+    for (int64_t k = 0; k + 1 < nsteps; k += 2) RF_HIP(hipGraphLaunch(step_graph(op, dt), s));
+    if (nsteps % 2) lserk_step(op, dt, RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
     sync(op);
   });
 }

@@ -167,6 +167,14 @@ struct rfhip_op {
   rfhip::DevBuf port_dof, port_prof, port_w, port_off, port_norm, port_rows;
   rfhip::DevBuf ps_elem, ps_off, ps_lf, ps_prof;  // port-source assembly input
   bool src_resident = false;  // src holds a device-assembled port source
+  // captured two-step LSERK4 graphs of the resident stepper, one per starting
+  // state buffer, with the (dt, variant) they were captured for
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    double dt = 0.0;
+    int variant = -1;
+  };
+  StepGraph step_graph[2];
   bool ports_dirty = true;
   int kb_cap = 0;
   rfhip::DevOp dev{};

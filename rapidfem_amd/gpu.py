@@ -310,6 +310,10 @@ class GpuOperator:
     def state_lserk(self, dt, nsteps):
         L.check(L.lib.rfhip_op_state_lserk(self._h, float(dt), int(nsteps)))
 
+    def state_lserk_graph(self, dt, nsteps):
+        """state_lserk replayed from a captured HIP graph of two steps."""
+        L.check(L.lib.rfhip_op_state_lserk_graph(self._h, float(dt), int(nsteps)))
+
     def state_lserk_timed(self, dt, nsteps):
         tot, stage = C.c_double(), C.c_double()
         L.check(L.lib.rfhip_op_state_lserk_timed(self._h, float(dt), int(nsteps), C.byref(tot), C.byref(stage)))

@@ -624,3 +624,31 @@ def test_resident_port_source_drives_like_the_uploaded_pattern(ctx, oracle, form
     from rapidfem_amd import RfhipError
     with pytest.raises(RfhipError):
         g.transient(ctx, y0, dt, 1, 1, Drive.port_source(), wave[:1], True, precision="f64")
+
+
+# ---- graph-captured resident stepping ----------------------------------------
+
+@pytest.mark.parametrize("form", ALL_FORMS + ["fused_f64c_debye"])
+def test_graph_stepping_equals_eager_stepping(ctx, oracle, form):
+    """rfhip_op_state_lserk_graph replays a captured two-step graph; with
+    eager work, reads of the state and a variant change between replays,
+    every replay must equal the eager stepper bit for bit (odd step counts
+    exercise the eager tail and both starting buffers)."""
+    if form.endswith("_debye"):
+        oop, g, _ = _debye_pair(ctx, oracle, "fused_f64c", 2)
+        g2 = _debye_pair(ctx, oracle, "fused_f64c", 2)[1]
+    else:
+        oop, _ = _te101(oracle, 3, 2)
+        g, g2 = _gop(ctx, oop, form), _gop(ctx, oop, form)
+    y0 = np.cos(0.2 + 0.013 * np.arange(oop.n_dof))
+    g.state_load(y0)
+    g2.state_load(y0)
+    for rep, (steps, variant) in enumerate([(4, None), (3, None), (5, None), (2, 1), (3, 1)]):
+        if variant is not None and g.n_variants() > variant:
+            g.set_variant(variant)
+            g2.set_variant(variant)
+        g.state_lserk_graph(0.01, steps)
+        g2.state_lserk(0.01, steps)
+        a, b = g.state_store(), g2.state_store()
+        print(f"graph form={form} replay {rep}: max |diff| {np.abs(a - b).max():.3e}")
+        assert np.array_equal(a, b), rep
