@@ -279,6 +279,39 @@ int rfhip_op_state_lserk_timed(rfhip_op *op, double dt, int64_t nsteps,
 int rfhip_op_state_energy(rfhip_op *op, const double *mass_ref,
                           const double *det, double *energy);
 
+/* ---- sparse state-space export (rhs.rs:1575-1730) ------------------------ */
+
+/* CsrMatrix (rhs.rs:1575-1591), held in device memory. */
+typedef struct rfhip_csr rfhip_csr;
+
+/* MaxwellOperator::assemble_sparse (rhs.rs:1621-1730) on the device: the
+ * operator desc describes (with n_disp Debye slots as in
+ * rfhip_op_create_dispersive; 0 for none) as an explicit CSR matrix, row
+ * block e holding the nonzeros of e's stencil columns (e and its distinct
+ * face neighbours, rhs.rs:1597-1610, then e's own P columns), each row's
+ * columns ascending, then the appended P rows with exactly g and a. Values
+ * and pattern are those of the reference's unit-column probe, bit for bit.
+ * Orders 1..4. RFHIP_EINVAL for a malformed description or Debye list,
+ * RFHIP_ENOMEM when the matrix does not fit in device memory. */
+int rfhip_assemble_sparse(rfhip_ctx *ctx, const rfhip_op_desc *desc,
+                          int64_t n_disp, const int64_t *disp_elem,
+                          const double *disp_a, const double *disp_g,
+                          const double *disp_inv_eps_inf, rfhip_csr **out);
+void rfhip_csr_destroy(rfhip_csr *csr);
+/* CsrMatrix::n and CsrMatrix::nnz (rhs.rs:1588-1590). */
+int rfhip_csr_shape(const rfhip_csr *csr, int64_t *n, int64_t *nnz);
+/* Copies row_ptr (n+1), col_idx (nnz) and values (nnz) to host memory; any
+ * of the three may be NULL. */
+int rfhip_csr_download(const rfhip_csr *csr, int64_t *row_ptr,
+                       int64_t *col_idx, double *values);
+/* The device arrays themselves, owned by csr (valid until
+ * rfhip_csr_destroy), for consumers that stay on the GPU. */
+int rfhip_csr_device(const rfhip_csr *csr, const int64_t **row_ptr,
+                     const int64_t **col_idx, const double **values);
+/* Durations of the counting and the filling pass over the rows (HIP events
+ * on the assembly stream), in milliseconds. (No reference counterpart.) */
+int rfhip_csr_timing(const rfhip_csr *csr, double *count_ms, double *fill_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -874,6 +874,108 @@ static void apply_all_(const orc_op *op, const double *y, double *dy, const doub
   }
 }
 
+/* rhs.rs:1597-1610 element_stencil: the element and its distinct face
+ * neighbours, ascending. Returns the count (1..5). */
+static int element_stencil_(const orc_op *op, int64_t e, int64_t *s) {
+  int n = 1;
+  s[0] = e;
+  for (int f = 0; f < 4; ++f) {
+    const int64_t nb = op->nbr[4 * e + f];
+    int seen = 0;
+    for (int q = 0; q < n; ++q) seen |= s[q] == nb;
+    if (nb >= 0 && !seen) s[n++] = nb;
+  }
+  for (int a = 1; a < n; ++a)
+    for (int b = a; b > 0 && s[b - 1] > s[b]; --b) {
+      const int64_t t = s[b];
+      s[b] = s[b - 1];
+      s[b - 1] = t;
+    }
+  return n;
+}
+
+/* rhs.rs:1621-1730 assemble_sparse: row block e of A found by probing the
+ * columns of its stencil (the [E,H] columns of every stencil element, then
+ * the element's own P columns when it is dispersive) with unit vectors
+ * through apply_element (rhs.rs:409-421 probe_column keeps v != 0.0), each
+ * row's columns ascending; then the appended P rows, two entries each (g at
+ * the E column, a on the diagonal) whatever their values. Called with
+ * col_idx == NULL it fills row_ptr (n_dof + 1) and returns nnz; called again
+ * with col_idx/values of nnz entries it fills them. */
+int64_t orc_assemble_sparse(const orc_op *op, int64_t *row_ptr, int64_t *col_idx, double *values) {
+  const int np = op->np, stride = 6 * np, ps = 3 * np;
+  const int64_t K = op->K, n = orc_op_n_dof(op), eh = 6 * (int64_t)np * K;
+  const int fill = col_idx != NULL;
+  if (!fill) row_ptr[0] = 0;
+#pragma omp parallel num_threads(nthreads_())
+  {
+    double *probe = calloc((size_t)n, sizeof(double));
+    double *out = malloc(sizeof(double) * stride);
+    double *s = malloc(sizeof(double) * 24 * np);
+    int64_t *cnt = malloc(sizeof(int64_t) * stride);
+    /* (column, value) per local row, a row holding at most 5 stride + ps */
+    const int cap = 5 * stride + ps;
+    int64_t *rc = malloc(sizeof(int64_t) * (size_t)stride * cap);
+    double *rv = malloc(sizeof(double) * (size_t)stride * cap);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t e = 0; e < K; ++e) {
+      int64_t st[5];
+      const int ns = element_stencil_(op, e, st);
+      for (int il = 0; il < stride; ++il) cnt[il] = 0;
+      const int64_t slot = op->n_disp ? op->disp_slot[e] : -1;
+      const int ncol_blocks = ns + (slot >= 0);
+      /* probing in ascending column order leaves every row sorted */
+      for (int b = 0; b < ncol_blocks; ++b) {
+        const int64_t c0 = b < ns ? st[b] * stride : eh + slot * ps;
+        const int w = b < ns ? stride : ps;
+        for (int jl = 0; jl < w; ++jl) {
+          const int64_t j = c0 + jl;
+          probe[j] = 1.0;
+          for (int il = 0; il < stride; ++il) out[il] = 0.0;
+          apply_element_(op, e, probe, out, s, NULL);
+          probe[j] = 0.0;
+          for (int il = 0; il < stride; ++il)
+            if (out[il] != 0.0) {
+              rc[(size_t)il * cap + cnt[il]] = j;
+              rv[(size_t)il * cap + cnt[il]] = out[il];
+              ++cnt[il];
+            }
+        }
+      }
+      for (int il = 0; il < stride; ++il) {
+        const int64_t row = e * stride + il;
+        if (!fill) {
+          row_ptr[row + 1] = cnt[il];
+        } else {
+          const int64_t o = row_ptr[row];
+          for (int64_t q = 0; q < cnt[il]; ++q) {
+            col_idx[o + q] = rc[(size_t)il * cap + q];
+            values[o + q] = rv[(size_t)il * cap + q];
+          }
+        }
+      }
+    }
+    free(probe); free(out); free(s); free(cnt); free(rc); free(rv);
+  }
+  if (!fill) {
+    for (int64_t r = eh; r < n; ++r) row_ptr[r + 1] = 2;
+    for (int64_t r = 0; r < n; ++r) row_ptr[r + 1] += row_ptr[r];
+    return row_ptr[n];
+  }
+  for (int64_t q = 0; q < op->n_disp; ++q) {
+    const int64_t e = op->disp_elem[q];
+    for (int node = 0; node < np; ++node)
+      for (int c = 0; c < 3; ++c) {
+        const int64_t row = eh + q * ps + node * 3 + c, o = row_ptr[row];
+        col_idx[o] = e * stride + node * 6 + c;
+        values[o] = op->disp_g[q];
+        col_idx[o + 1] = row;
+        values[o + 1] = op->disp_a[q];
+      }
+  }
+  return row_ptr[n];
+}
+
 /* rhs.rs:1509-1544 field_energy */
 double orc_field_energy(const orc_op *op, const double *y) {
   const int np = op->np;

@@ -75,6 +75,10 @@ void orc_op_export(const orc_op *op, double *jinv, double *det,
                    double *sig_eps, double *sig_mu);
 void orc_op_set_threads(int nthreads);
 void orc_apply(const orc_op *op, const double *y, double *dy);
+/* MaxwellOperator::assemble_sparse: col_idx == NULL fills row_ptr
+ * (n_dof + 1) and returns nnz; a second call fills col_idx/values. */
+int64_t orc_assemble_sparse(const orc_op *op, int64_t *row_ptr,
+                            int64_t *col_idx, double *values);
 double orc_field_energy(const orc_op *op, const double *y);
 
 /* ---- LSERK4 (explicit.rs) ---------------------------------------------- */

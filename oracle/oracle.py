@@ -53,6 +53,7 @@ def _load() -> C.CDLL:
         "orc_op_set_threads": (None, [C.c_int]),
         "orc_apply": (None, [vp, _dp, _dp]),
         "orc_apply_ghost": (None, [vp, _dp, _dp, _dp]),
+        "orc_assemble_sparse": (i64, [vp, _lp, vp, vp]),
         "orc_field_energy": (d, [vp, _dp]),
         "orc_lserk_run": (None, [vp, _dp, d, i64, C.c_int, i64, vp, vp]),
         "orc_lserk_run_dense": (None, [_dp, C.c_int, _dp, d, i64]),
@@ -218,6 +219,16 @@ class Operator:
         dy = np.empty_like(y)
         lib().orc_apply_ghost(self._h, y, ghost.reshape(-1), dy)
         return dy
+
+    def assemble_sparse(self):
+        """The operator as CSR (row_ptr, col_idx, values), probed column by
+        column as MaxwellOperator::assemble_sparse does (rhs.rs:1621-1730)."""
+        rp = np.empty(self.n_dof + 1, np.int64)
+        nnz = lib().orc_assemble_sparse(self._h, rp, None, None)
+        ci = np.empty(nnz, np.int64)
+        va = np.empty(nnz, np.float64)
+        lib().orc_assemble_sparse(self._h, rp, ci.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p))
+        return rp, ci, va
 
     def field_energy(self, y) -> float:
         return float(lib().orc_field_energy(self._h, np.ascontiguousarray(y, np.float64)))

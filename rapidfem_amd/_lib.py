@@ -100,6 +100,12 @@ SIGNATURES = {
     "rfhip_op_state_lserk_graph": (C.c_int, [_vp, _d, _i64]),
     "rfhip_op_state_lserk_timed": (C.c_int, [_vp, _d, _i64, C.POINTER(_d), C.POINTER(_d)]),
     "rfhip_op_state_energy": (C.c_int, [_vp, _dp, _dp, C.POINTER(_d)]),
+    "rfhip_assemble_sparse": (C.c_int, [_vp, C.POINTER(OpDesc), _i64, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "rfhip_csr_destroy": (None, [_vp]),
+    "rfhip_csr_shape": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
+    "rfhip_csr_download": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "rfhip_csr_device": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "rfhip_csr_timing": (C.c_int, [_vp, C.POINTER(_d), C.POINTER(_d)]),
     # rfhost.h
     "rfhost_np": (_i32, [_i32]),
     "rfhost_nfp": (_i32, [_i32]),

@@ -18,7 +18,8 @@ CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "librfhip.so")
 
-HIP_SOURCES = ["rfhip_api.hip", "rfhip_plain.hip", "rfhip_fused.hip", "rfhip_vec.hip", "rfhip_debye.hip"]
+HIP_SOURCES = ["rfhip_api.hip", "rfhip_plain.hip", "rfhip_fused.hip", "rfhip_vec.hip", "rfhip_debye.hip",
+               "rfhip_sparse.hip"]
 CXX_SOURCES = ["host_builder.cpp"]
 
 ARCH = os.environ.get("RFHIP_ARCH", "gfx950")
@@ -54,7 +55,7 @@ def build(verbose: bool = False) -> str:
     for src in CXX_SOURCES:
         obj = os.path.join(OUT_DIR, src.replace(".cpp", ".o"))
         objs.append(obj)
-        jobs.append(["g++", *common, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
+        jobs.append(["g++", *common, "-ffp-contract=off", *inc, "-c", os.path.join(CSRC, src), "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for log in ex.map(_compile, jobs):
             if verbose and log.strip():

@@ -3,9 +3,8 @@
 // from a tet mesh. Mirrors rhs.rs:805-1000 (MaxwellOperator::new),
 // dg_basis.rs:61-385 (ReferenceElement::new), geom_factors.rs:33-88 and
 // mesh_gen.rs:16-139, with its own construction:
-//  * the reference element is assembled in long double (80-bit) and rounded
-//    once, so D/M/LIFT are correctly rounded where the reference's f64
-//    Gauss-Jordan leaves ~1e-15 residue;
+//  * the reference element follows the reference's f64 operation sequence,
+//    so D/M/LIFT equal the reference's tables bit for bit;
 //  * faces are paired through a hash of their sorted vertex triple;
 //  * the face-node permutation is found exactly from integer barycentric
 //    coordinates on the shared triangle, O(Nfp) per face, instead of the
@@ -21,8 +20,6 @@
 #include <vector>
 
 namespace {
-
-using ld = long double;
 
 // ---- reference element ------------------------------------------------
 
@@ -40,8 +37,9 @@ std::vector<Ijk> node_ijk(int p) {
   return v;
 }
 
-// Monomial exponents of total degree <= p (any complete basis works; the
-// nodal operators do not depend on the choice).
+// Monomial exponents (a, b, deg - a - b) for deg = 0..p, a ascending, then
+// b ascending (dg_basis.rs:125-135). The order is part of the operation
+// sequence that fixes the tables' round-off.
 std::vector<std::array<int, 3>> monos3(int p) {
   std::vector<std::array<int, 3>> v;
   for (int d = 0; d <= p; ++d)
@@ -50,43 +48,88 @@ std::vector<std::array<int, 3>> monos3(int p) {
   return v;
 }
 
-ld fact(int n) {
-  ld r = 1;
-  for (int k = 2; k <= n; ++k) r *= k;
+// n! as a running product 1*1*2*...*n (dg_basis.rs:145-148).
+double factorial(int n) {
+  double r = 1.0;
+  for (int k = 1; k <= n; ++k) r *= (double)k;
   return r;
 }
 
-ld ipow(ld x, int e) {
-  ld r = 1;
-  for (int q = 0; q < e; ++q) r *= x;
+// x^e the way f64::powi evaluates it for a run-time exponent (LLVM's powi:
+// square-and-multiply over the exponent's bits), which is not always the
+// same double as x*x*...*x.
+double powi(double x, int e) {
+  double r = 1.0;
+  for (;;) {
+    if (e & 1) r *= x;
+    e /= 2;
+    if (e == 0) break;
+    x *= x;
+  }
   return r;
 }
 
-// Solve A X = B in place (A n x n, B n x m, row-major) by LU with partial
-// pivoting in long double.
-void lu_solve(std::vector<ld> &a, std::vector<ld> &b, int n, int m) {
+// Gauss-Jordan inverse with partial pivoting (dg_basis.rs:171-214): the
+// first row of largest |a| below the diagonal is the pivot, the pivot row is
+// divided through, every other row with a nonzero factor is eliminated.
+std::vector<double> gj_inverse(std::vector<double> a, int n) {
+  std::vector<double> inv((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
   for (int c = 0; c < n; ++c) {
     int piv = c;
+    double best = std::fabs(a[(size_t)c * n + c]);
     for (int r = c + 1; r < n; ++r)
-      if (std::fabs(a[r * n + c]) > std::fabs(a[piv * n + c])) piv = r;
-    if (piv != c) {
-      for (int q = 0; q < n; ++q) std::swap(a[c * n + q], a[piv * n + q]);
-      for (int q = 0; q < m; ++q) std::swap(b[c * m + q], b[piv * m + q]);
+      if (std::fabs(a[(size_t)r * n + c]) > best) {
+        best = std::fabs(a[(size_t)r * n + c]);
+        piv = r;
+      }
+    if (piv != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(a[(size_t)c * n + k], a[(size_t)piv * n + k]);
+        std::swap(inv[(size_t)c * n + k], inv[(size_t)piv * n + k]);
+      }
+    const double d = a[(size_t)c * n + c];
+    for (int k = 0; k < n; ++k) {
+      a[(size_t)c * n + k] /= d;
+      inv[(size_t)c * n + k] /= d;
     }
-    for (int r = c + 1; r < n; ++r) {
-      ld f = a[r * n + c] / a[c * n + c];
-      if (f == 0) continue;
-      for (int q = c; q < n; ++q) a[r * n + q] -= f * a[c * n + q];
-      for (int q = 0; q < m; ++q) b[r * m + q] -= f * b[c * m + q];
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = a[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        a[(size_t)r * n + k] -= f * a[(size_t)c * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+      }
     }
   }
-  for (int c = n - 1; c >= 0; --c) {
-    for (int q = 0; q < m; ++q) {
-      ld s = b[c * m + q];
-      for (int r = c + 1; r < n; ++r) s -= a[c * n + r] * b[r * m + q];
-      b[c * m + q] = s / a[c * n + c];
+  return inv;
+}
+
+// out = A B, rows of A accumulated left to right, zero entries of A skipped
+// (dg_basis.rs:220-243).
+std::vector<double> product(const std::vector<double> &a, const std::vector<double> &b, int rows, int inner,
+                            int cols) {
+  std::vector<double> out((size_t)rows * cols, 0.0);
+  for (int i = 0; i < rows; ++i)
+    for (int k = 0; k < inner; ++k) {
+      const double aik = a[(size_t)i * inner + k];
+      if (aik == 0.0) continue;
+      for (int j = 0; j < cols; ++j) out[(size_t)i * cols + j] += aik * b[(size_t)k * cols + j];
     }
-  }
+  return out;
+}
+
+// C^T G C as (C^T G) first, then the product above (dg_basis.rs:253-266).
+std::vector<double> congruence(const std::vector<double> &c, const std::vector<double> &g, int n) {
+  std::vector<double> ctg((size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < n; ++k) {
+      double acc = 0.0;
+      for (int m = 0; m < n; ++m) acc += c[(size_t)m * n + i] * g[(size_t)m * n + k];
+      ctg[(size_t)i * n + k] = acc;
+    }
+  return product(ctg, c, n, n, n);
 }
 
 struct RefElem {
@@ -115,6 +158,10 @@ void face_uv(int f, const Ijk &q, int &u, int &v) {
   }
 }
 
+// ReferenceElement::new (dg_basis.rs:61-119) and build_lift (:288-385) in
+// the reference's own f64 operation sequence, so that the tables carry the
+// reference's round-off, residues of exact zeros included: which entries of
+// A are stored by the sparse state-space export depends on them.
 RefElem build_refelem(int p) {
   RefElem re;
   re.p = p;
@@ -124,61 +171,52 @@ RefElem build_refelem(int p) {
   re.np = n;
   re.nfp = (p + 1) * (p + 2) / 2;
   const int nfp = re.nfp;
-  // V[node][mono]; nodal basis phi_i = sum_m C[m][i] mono_m with C = V^-1.
-  std::vector<ld> V(n * n), Cinv(n * n, 0);
+  const double pp = (double)p;
+  re.nodes.resize((size_t)n * 3);
   for (int a = 0; a < n; ++a) {
-    ld r = (ld)ijk[a].i / p, s = (ld)ijk[a].j / p, t = (ld)ijk[a].k / p;
-    for (int m = 0; m < n; ++m) V[a * n + m] = ipow(r, mon[m][0]) * ipow(s, mon[m][1]) * ipow(t, mon[m][2]);
-    Cinv[a * n + a] = 1;
+    re.nodes[a * 3] = (double)ijk[a].i / pp;
+    re.nodes[a * 3 + 1] = (double)ijk[a].j / pp;
+    re.nodes[a * 3 + 2] = (double)ijk[a].k / pp;
   }
-  {
-    std::vector<ld> A = V;
-    lu_solve(A, Cinv, n, n);  // Cinv <- V^-1 = C
-  }
-  const std::vector<ld> &C = Cinv;
-  // Mass M = C^T G C with the exact monomial Gram matrix.
-  std::vector<ld> G(n * n), CtG(n * n, 0), M(n * n, 0);
+  auto mono = [&](const std::array<int, 3> &m, const double *x) {
+    return powi(x[0], m[0]) * powi(x[1], m[1]) * powi(x[2], m[2]);
+  };
+  // V[node][mono]; the nodal basis is phi_i = sum_m C[m][i] mono_m, C = V^-1.
+  std::vector<double> V((size_t)n * n);
+  for (int a = 0; a < n; ++a)
+    for (int m = 0; m < n; ++m) V[(size_t)a * n + m] = mono(mon[m], &re.nodes[a * 3]);
+  const std::vector<double> C = gj_inverse(V, n);
+  // M = C^T G C with the exact monomial integrals over the reference tet.
+  std::vector<double> G((size_t)n * n);
   for (int a = 0; a < n; ++a)
     for (int b = 0; b < n; ++b) {
-      int x = mon[a][0] + mon[b][0], y = mon[a][1] + mon[b][1], z = mon[a][2] + mon[b][2];
-      G[a * n + b] = fact(x) * fact(y) * fact(z) / fact(x + y + z + 3);
+      const int x = mon[a][0] + mon[b][0], y = mon[a][1] + mon[b][1], z = mon[a][2] + mon[b][2];
+      G[(size_t)a * n + b] = factorial(x) * factorial(y) * factorial(z) / factorial(x + y + z + 3);
     }
-  for (int i = 0; i < n; ++i)
-    for (int k = 0; k < n; ++k) {
-      ld s = 0;
-      for (int m = 0; m < n; ++m) s += C[m * n + i] * G[m * n + k];
-      CtG[i * n + k] = s;
-    }
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      ld s = 0;
-      for (int k = 0; k < n; ++k) s += CtG[i * n + k] * C[k * n + j];
-      M[i * n + j] = s;
-    }
-  // D_axis = V_axis C.
-  std::vector<ld> D[3];
+  re.mass = congruence(C, G, n);
+  const std::vector<double> Minv = gj_inverse(re.mass, n);
+  // D_axis = V_axis C, V_axis holding the monomials' derivatives.
+  std::vector<double> *D[3] = {&re.dr, &re.ds, &re.dt};
   for (int ax = 0; ax < 3; ++ax) {
-    D[ax].assign(n * n, 0);
-    for (int a = 0; a < n; ++a) {
-      ld x[3] = {(ld)ijk[a].i / p, (ld)ijk[a].j / p, (ld)ijk[a].k / p};
-      for (int j = 0; j < n; ++j) {
-        ld s = 0;
-        for (int m = 0; m < n; ++m) {
-          int e = mon[m][ax];
-          if (e == 0) continue;
-          ld d = e;
-          for (int q = 0; q < 3; ++q) d *= ipow(x[q], mon[m][q] - (q == ax ? 1 : 0));
-          s += d * C[m * n + j];
+    std::vector<double> Vd((size_t)n * n);
+    for (int a = 0; a < n; ++a)
+      for (int m = 0; m < n; ++m) {
+        const int e = mon[m][ax];
+        if (e == 0) {
+          Vd[(size_t)a * n + m] = 0.0;
+          continue;
         }
-        D[ax][a * n + j] = s;
+        std::array<int, 3> lower = mon[m];
+        lower[ax] -= 1;
+        Vd[(size_t)a * n + m] = (double)e * mono(lower, &re.nodes[a * 3]);
       }
-    }
+    *D[ax] = product(Vd, C, n, n, n);
   }
-  // Face nodes and the face mass matrices; LIFT = M^-1 Emat.
+  // Face nodes, face mass matrices scattered into E (Np x 4Nfp), LIFT = M^-1 E.
   re.face_nodes.assign(4 * nfp, 0);
   const int cols = 4 * nfp;
-  std::vector<ld> E(n * cols, 0);
-  re.fw.assign(4 * nfp, 0);
+  std::vector<double> E((size_t)n * cols, 0.0);
+  re.fw.assign(4 * nfp, 0.0);
   std::vector<std::array<int, 2>> mon2;
   for (int d = 0; d <= p; ++d)
     for (int a = 0; a <= d; ++a) mon2.push_back({a, d - a});
@@ -186,50 +224,31 @@ RefElem build_refelem(int p) {
     int c = 0;
     for (int a = 0; a < n; ++a)
       if (on_face(f, ijk[a], p)) re.face_nodes[f * nfp + c++] = a;
-    std::vector<ld> V2(nfp * nfp), C2(nfp * nfp, 0);
+    std::vector<double> V2((size_t)nfp * nfp);
     for (int a = 0; a < nfp; ++a) {
       int u, v;
       face_uv(f, ijk[re.face_nodes[f * nfp + a]], u, v);
-      for (int m = 0; m < nfp; ++m) V2[a * nfp + m] = ipow((ld)u / p, mon2[m][0]) * ipow((ld)v / p, mon2[m][1]);
-      C2[a * nfp + a] = 1;
+      const double x = (double)u / pp, y = (double)v / pp;
+      for (int m = 0; m < nfp; ++m) V2[(size_t)a * nfp + m] = powi(x, mon2[m][0]) * powi(y, mon2[m][1]);
     }
-    lu_solve(V2, C2, nfp, nfp);
-    std::vector<ld> G2(nfp * nfp), Mf(nfp * nfp, 0);
+    const std::vector<double> C2 = gj_inverse(V2, nfp);
+    std::vector<double> G2((size_t)nfp * nfp);
     for (int a = 0; a < nfp; ++a)
       for (int b = 0; b < nfp; ++b) {
-        int x = mon2[a][0] + mon2[b][0], y = mon2[a][1] + mon2[b][1];
-        G2[a * nfp + b] = fact(x) * fact(y) / fact(x + y + 2);
+        const int x = mon2[a][0] + mon2[b][0], y = mon2[a][1] + mon2[b][1];
+        G2[(size_t)a * nfp + b] = factorial(x) * factorial(y) / factorial(x + y + 2);
       }
-    for (int i = 0; i < nfp; ++i)
-      for (int j = 0; j < nfp; ++j) {
-        ld s = 0;
-        for (int a = 0; a < nfp; ++a)
-          for (int b = 0; b < nfp; ++b) s += C2[a * nfp + i] * G2[a * nfp + b] * C2[b * nfp + j];
-        Mf[i * nfp + j] = s;
-      }
+    const std::vector<double> Mf = congruence(C2, G2, nfp);
     for (int a = 0; a < nfp; ++a) {
-      ld rs = 0;
+      double rs = 0.0;
       for (int m = 0; m < nfp; ++m) {
-        E[re.face_nodes[f * nfp + a] * cols + f * nfp + m] = Mf[a * nfp + m];
-        rs += Mf[a * nfp + m];
+        E[(size_t)re.face_nodes[f * nfp + a] * cols + f * nfp + m] = Mf[(size_t)a * nfp + m];
+        rs += Mf[(size_t)a * nfp + m];
       }
-      re.fw[f * nfp + a] = (double)rs;
+      re.fw[f * nfp + a] = rs;
     }
   }
-  std::vector<ld> Mc = M, L = E;
-  lu_solve(Mc, L, n, cols);
-  re.nodes.resize(n * 3);
-  for (int a = 0; a < n; ++a) {
-    re.nodes[a * 3] = (double)ijk[a].i / p;
-    re.nodes[a * 3 + 1] = (double)ijk[a].j / p;
-    re.nodes[a * 3 + 2] = (double)ijk[a].k / p;
-  }
-  auto cvt = [](const std::vector<ld> &s) { return std::vector<double>(s.begin(), s.end()); };
-  re.mass = cvt(M);
-  re.dr = cvt(D[0]);
-  re.ds = cvt(D[1]);
-  re.dt = cvt(D[2]);
-  re.lift = cvt(L);
+  re.lift = product(Minv, E, n, n, cols);
   return re;
 }
 

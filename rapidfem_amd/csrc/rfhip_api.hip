@@ -1432,3 +1432,98 @@ int rfhip_op_state_energy(rfhip_op *op, const double *mass_ref, const double *de
 }
 
 }  // extern "C"
+
+// ---- sparse state-space export ----------------------------------------------
+
+int rfhip_assemble_sparse(rfhip_ctx *ctx, const rfhip_op_desc *d, int64_t n_disp, const int64_t *disp_elem,
+                          const double *disp_a, const double *disp_g, const double *disp_inv_eps_inf,
+                          rfhip_csr **out) {
+  return guarded([&] {
+    RF_REQUIRE(ctx && d && out, "null argument");
+    *out = nullptr;
+    RF_REQUIRE(n_disp >= 0, "negative number of dispersive elements");
+    RF_REQUIRE(n_disp == 0 || (disp_elem && disp_a && disp_g && disp_inv_eps_inf), "missing dispersive array");
+    RF_REQUIRE(d->n_elem >= 1 && d->n_elem < ((int64_t)1 << 31), "element count out of range");
+    const int p = order_of_np(d->np);
+    RF_REQUIRE(p >= 1 && d->nfp == (p + 1) * (p + 2) / 2, "inconsistent Np / Nfp");
+    if (p > 4) throw Error{RFHIP_EUNSUPPORTED, "orders 1..4 are compiled (Np 4, 10, 20, 35)"};
+    RF_REQUIRE(d->diff_r && d->diff_s && d->diff_t && d->lift && d->face_nodes && d->jinv && d->inv_eps && d->inv_mu &&
+                   d->sigma_eps && d->sigma_mu && d->face_normal && d->face_fscale && d->face_neighbor &&
+                   d->face_nbr_local && d->face_port && d->face_perm,
+               "missing operator array");
+    const int64_t K = d->n_elem;
+    const int np = d->np, nfp = d->nfp;
+    // every index the row kernel follows (rhs.rs:1403-1418 on interior faces)
+    for (int x = 0; x < 4 * nfp; ++x)
+      RF_REQUIRE(d->face_nodes[x] >= 0 && d->face_nodes[x] < np, "face node out of range");
+    for (int64_t q = 0; q < 4 * K; ++q) {
+      const int nb = d->face_neighbor[q];
+      RF_REQUIRE(nb >= -1 && nb < K, "face neighbour out of range");
+      if (nb >= 0 && d->face_port[q] < 0) {
+        RF_REQUIRE(d->face_nbr_local[q] >= 0 && d->face_nbr_local[q] < 4, "neighbour local face out of range");
+        for (int m = 0; m < nfp; ++m)
+          RF_REQUIRE(d->face_perm[q * nfp + m] >= 0 && d->face_perm[q * nfp + m] < nfp, "face perm out of range");
+      }
+    }
+    std::vector<double> coef(3 * (size_t)n_disp);
+    if (n_disp > 0) {  // rhs.rs:963-985
+      std::vector<char> seen((size_t)K, 0);
+      for (int64_t q = 0; q < n_disp; ++q) {
+        const int64_t e = disp_elem[q];
+        RF_REQUIRE(e >= 0 && e < K, "dispersive element index out of range");
+        RF_REQUIRE(!seen[(size_t)e], "element listed twice as dispersive");
+        seen[(size_t)e] = 1;
+        coef[3 * q] = disp_a[q];
+        coef[3 * q + 1] = disp_g[q];
+        coef[3 * q + 2] = disp_inv_eps_inf[q];
+      }
+    }
+    RF_HIP(hipSetDevice(ctx->device));
+    auto csr = std::make_unique<rfhip_csr>();
+    csr->ctx = ctx;
+    RF_HIP(hipStreamCreateWithFlags(&csr->stream, hipStreamNonBlocking));
+    assemble_sparse(csr.get(), *d, n_disp, disp_elem, coef.data());
+    *out = csr.release();
+  });
+}
+
+void rfhip_csr_destroy(rfhip_csr *csr) { delete csr; }
+
+int rfhip_csr_shape(const rfhip_csr *csr, int64_t *n, int64_t *nnz) {
+  return guarded([&] {
+    RF_REQUIRE(csr, "null argument");
+    if (n) *n = csr->n;
+    if (nnz) *nnz = csr->nnz;
+  });
+}
+
+int rfhip_csr_download(const rfhip_csr *csr, int64_t *row_ptr, int64_t *col_idx, double *values) {
+  return guarded([&] {
+    RF_REQUIRE(csr, "null argument");
+    hipStream_t s = csr->stream;
+    if (row_ptr)
+      RF_HIP(hipMemcpyAsync(row_ptr, csr->row_ptr.p, 8 * (size_t)(csr->n + 1), hipMemcpyDeviceToHost, s));
+    if (col_idx && csr->nnz)
+      RF_HIP(hipMemcpyAsync(col_idx, csr->col_idx.p, 8 * (size_t)csr->nnz, hipMemcpyDeviceToHost, s));
+    if (values && csr->nnz)
+      RF_HIP(hipMemcpyAsync(values, csr->values.p, 8 * (size_t)csr->nnz, hipMemcpyDeviceToHost, s));
+    RF_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int rfhip_csr_device(const rfhip_csr *csr, const int64_t **row_ptr, const int64_t **col_idx, const double **values) {
+  return guarded([&] {
+    RF_REQUIRE(csr, "null argument");
+    if (row_ptr) *row_ptr = csr->row_ptr.as<int64_t>();
+    if (col_idx) *col_idx = csr->col_idx.as<int64_t>();
+    if (values) *values = csr->values.as<double>();
+  });
+}
+
+int rfhip_csr_timing(const rfhip_csr *csr, double *count_ms, double *fill_ms) {
+  return guarded([&] {
+    RF_REQUIRE(csr, "null argument");
+    if (count_ms) *count_ms = csr->count_ms;
+    if (fill_ms) *fill_ms = csr->fill_ms;
+  });
+}

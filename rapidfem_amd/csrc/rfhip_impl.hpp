@@ -181,6 +181,19 @@ struct rfhip_op {
   rfhip::StageArgs sa{};  // fused forms: geometry part filled at creation
 };
 
+// CsrMatrix (rhs.rs:1575-1591) held in HBM: row_ptr n+1, col_idx and values
+// nnz, column indices as int64 (the reference's usize).
+struct rfhip_csr {
+  rfhip_ctx *ctx = nullptr;
+  hipStream_t stream = nullptr;
+  int64_t n = 0, nnz = 0;
+  float count_ms = 0.0f, fill_ms = 0.0f;  // the two row passes, HIP events
+  rfhip::DevBuf row_ptr, col_idx, values;
+  ~rfhip_csr() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
 namespace rfhip {
 
 // Kernel launchers (implemented in the .hip files).
@@ -205,6 +218,13 @@ void launch_debye_stage(const rfhip_op *op, const StageArgs &a, hipStream_t s);
 // local faces lf[q] and profiles prof[q*Nfp*6 ...] (rfhip_fused.hip).
 void launch_port_source(const rfhip_op *op, int64_t n_elems, const int64_t *elem, const int64_t *off,
                         const int32_t *lf, const double *prof, double *b, hipStream_t s);
+
+// MaxwellOperator::assemble_sparse (rhs.rs:1621-1730) of the operator desc
+// describes, with n_disp Debye slots (elements disp_elem, coefficients
+// disp_coef[3q..] = a, g, 1/eps_inf), into csr (rfhip_sparse.hip). The
+// description's indices must already be validated.
+void assemble_sparse(rfhip_csr *csr, const rfhip_op_desc &d, int64_t n_disp, const int64_t *disp_elem,
+                     const double *disp_coef);
 
 // True when every entry of dT (3*Np*Np, [k][j][i]) outside the equispaced
 // nodal sparsity pattern is zero to round-off (|x| <= 1e-13 * max |D_k|).

@@ -323,3 +323,60 @@ class GpuOperator:
         e = C.c_double()
         L.check(L.lib.rfhip_op_state_energy(self._h, self._mass, self._det, C.byref(e)))
         return e.value
+
+
+@dataclass
+class CsrMatrix:
+    """rhs.rs:1575-1591 CsrMatrix: dimension n, row_ptr (n+1), col_idx and
+    values (nnz), column indices int64 (the reference's usize). The arrays
+    are None when the matrix was assembled without download."""
+    n: int
+    row_ptr: np.ndarray | None
+    col_idx: np.ndarray | None
+    values: np.ndarray | None
+    n_stored: int
+    count_ms: float = 0.0  # device time of the counting pass over the rows
+    fill_ms: float = 0.0   # device time of the filling pass
+
+    def nnz(self) -> int:
+        return self.n_stored
+
+
+def assemble_sparse(ctx: GpuContext, op: MaxwellOperator | None = None, *, desc: L.OpDesc | None = None,
+                    dispersive=None, download: bool = True) -> CsrMatrix:
+    """MaxwellOperator::assemble_sparse (rhs.rs:1621-1730) on the device: the
+    state-space A as CSR, pattern and values those of the reference's
+    unit-column probe. From a MaxwellOperator (its Debye elements included)
+    or from raw rfhip_op_desc data with dispersive = (elements, a, g,
+    1/eps_inf) as for GpuOperator.from_desc. download=False keeps only the
+    shape and the pass timings (the device copy is released)."""
+    keep = None
+    if op is not None:
+        desc, keep = op.desc()
+        if getattr(op, "n_dispersive", 0):
+            dispersive = (op.disp_elem, op.disp_a, op.disp_g, op.disp_inv_eps_inf)
+    if desc is None:
+        raise ValueError("assemble_sparse needs a MaxwellOperator or a desc")
+    n_disp = 0
+    arrs = (None, None, None, None)
+    if dispersive is not None:
+        de = np.ascontiguousarray(dispersive[0], np.int64)
+        n_disp = de.shape[0]
+        arrs = (de, *(np.ascontiguousarray(np.broadcast_to(v, (n_disp,)), np.float64) for v in dispersive[1:]))
+    h = C.c_void_p()
+    L.check(L.lib.rfhip_assemble_sparse(ctx._h, C.byref(desc), n_disp, *(L.ptr(a) for a in arrs), C.byref(h)))
+    del keep
+    try:
+        n, nnz = C.c_int64(), C.c_int64()
+        L.check(L.lib.rfhip_csr_shape(h, C.byref(n), C.byref(nnz)))
+        rp = ci = va = None
+        if download:
+            rp = np.empty(n.value + 1, np.int64)
+            ci = np.empty(nnz.value, np.int64)
+            va = np.empty(nnz.value, np.float64)
+            L.check(L.lib.rfhip_csr_download(h, L.ptr(rp), L.ptr(ci), L.ptr(va)))
+        cm, fm = C.c_double(), C.c_double()
+        L.check(L.lib.rfhip_csr_timing(h, C.byref(cm), C.byref(fm)))
+    finally:
+        L.lib.rfhip_csr_destroy(h)
+    return CsrMatrix(int(n.value), rp, ci, va, int(nnz.value), cm.value, fm.value)

@@ -652,3 +652,143 @@ def test_graph_stepping_equals_eager_stepping(ctx, oracle, form):
         a, b = g.state_store(), g2.state_store()
         print(f"graph form={form} replay {rep}: max |diff| {np.abs(a - b).max():.3e}")
         assert np.array_equal(a, b), rep
+
+
+# ---- sparse state-space export (rhs.rs:1575-1730) ---------------------------
+
+def _sparse_oracle_case(oracle, order, n=2, jitter=True, alpha=0.6, debye_every=0, ports=True, materials=True):
+    """Oracle operator with PEC walls, port faces on x = 0, per-element
+    anisotropic materials with both conductivities, optional Debye elements;
+    returns it with the GPU-side dispersive tuple."""
+    xyz, tets = _mesh(oracle, n, jitter)
+    K = tets.shape[0]
+    port = np.full(4 * K, -1, np.int32)
+    if ports:
+        ex0 = oracle.Operator(xyz, tets, 1, 1.0).export()
+        for e in range(K):
+            for f in range(4):
+                if ex0["nbr"][e, f] < 0 and ex0["normal"][e, f, 0] < -0.99:
+                    port[4 * e + f] = 0
+    kw = {}
+    if materials:
+        rng = np.random.default_rng(5)
+        kw = dict(eps=1.0 + rng.random((K, 3)), mu=1.0 + rng.random((K, 3)),
+                  sigma=np.where(np.arange(K) % 3 == 0, 0.4, 0.0),
+                  sigma_m=np.where(np.arange(K) % 4 == 1, 0.3, 0.0))
+    disp = None
+    if debye_every:
+        elems = np.arange(1, K, debye_every)
+        ei, es, tau = DEBYE["eps_inf"], DEBYE["eps_s"], DEBYE["tau"]
+        kw["dispersive"] = (elems, ei, es, tau)
+        disp = (elems, -1.0 / tau, (es - ei) / tau, 1.0 / ei)
+    oop = oracle.Operator(xyz, tets, order, alpha, face_port=port, **kw)
+    return oop, disp
+
+
+def _assert_csr_equal(got, want):
+    rp, ci, va = want
+    assert got.n == len(rp) - 1
+    assert np.array_equal(got.row_ptr, rp)
+    assert np.array_equal(got.col_idx, ci)
+    assert np.array_equal(got.values, va), f"max |diff| {np.abs(got.values - va).max()}"
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("debye", [0, 3])
+def test_sparse_export_matches_oracle_bit_for_bit(ctx, oracle, order, debye):
+    """Every branch of the flux (PEC, port, interior), materials with both
+    conductivities, upwind blend 0.6, and Debye elements: row_ptr, col_idx
+    and values equal the oracle's unit-column probe exactly."""
+    from rapidfem_amd import assemble_sparse
+
+    oop, disp = _sparse_oracle_case(oracle, order, debye_every=debye)
+    d, arrs, _, _ = oracle_desc(oop)
+    print(f"sparse export order={order} debye={debye} K={oop.K}", flush=True)
+    got = assemble_sparse(ctx, desc=d, dispersive=disp)
+    _assert_csr_equal(got, oop.assemble_sparse())
+    if debye:
+        assert got.n == oop.n_dof and oop.n_disp > 0
+
+
+@pytest.mark.parametrize("order,alpha", [(2, 0.0), (3, 1.0), (4, 0.0)])
+def test_sparse_export_exact_zeros_on_an_unjittered_box(ctx, oracle, order, alpha):
+    """Axis-aligned faces and a central flux make many products exactly
+    zero; the stored pattern must drop exactly the entries the oracle drops."""
+    from rapidfem_amd import assemble_sparse
+
+    oop, _ = _sparse_oracle_case(oracle, order, n=2, jitter=False, alpha=alpha, ports=False, materials=False)
+    d, arrs, _, _ = oracle_desc(oop)
+    print(f"sparse export unjittered order={order} alpha={alpha}", flush=True)
+    got = assemble_sparse(ctx, desc=d)
+    _assert_csr_equal(got, oop.assemble_sparse())
+
+
+def test_sparse_export_at_scale_matches_oracle_and_apply(ctx, oracle):
+    """1296 tets at N=3 (n = 155,520, 43.6M nonzeros): equal to the oracle,
+    within the 5 * stride * n stencil bound (rhs.rs:2288-2330), and CSR x
+    equals the GPU operator's own FP64 apply (rhs.rs:2259-2286)."""
+    import scipy.sparse as sp
+
+    from rapidfem_amd import assemble_sparse
+
+    oop, disp = _sparse_oracle_case(oracle, 3, n=6, debye_every=5)
+    d, arrs, re_, ex = oracle_desc(oop)
+    print(f"sparse export at scale K={oop.K} n={oop.n_dof}", flush=True)
+    got = assemble_sparse(ctx, desc=d, dispersive=disp)
+    _assert_csr_equal(got, oop.assemble_sparse())
+    n = got.n
+    assert got.nnz() <= 5 * 6 * 20 * n
+    from rapidfem_amd import GpuOperator
+
+    g = GpuOperator.from_desc(ctx, d, "fused_f64", dispersive=disp)
+    v = np.cos(1.0 + np.arange(n) * 0.07)
+    a = sp.csr_matrix((got.values, got.col_idx, got.row_ptr), shape=(n, n))
+    assert rel_l2(a @ v, g.apply_f64(v)) < 1e-12
+    assert got.count_ms > 0.0 and got.fill_ms > 0.0
+
+
+def test_sparse_export_rejects_bad_input(ctx, oracle):
+    from rapidfem_amd import RfhipError, assemble_sparse
+
+    oop, _ = _sparse_oracle_case(oracle, 2)
+    d, arrs, _, _ = oracle_desc(oop)
+    with pytest.raises(RfhipError):
+        assemble_sparse(ctx, desc=d, dispersive=([1, 1], -1.0, 1.0, 0.5))
+    with pytest.raises(RfhipError):
+        assemble_sparse(ctx, desc=d, dispersive=([oop.K], -1.0, 1.0, 0.5))
+    perm = arrs["face_perm"].reshape(-1)
+    q = int(np.flatnonzero((arrs["face_neighbor"].reshape(-1) >= 0) & (arrs["face_port"].reshape(-1) < 0))[0])
+    nfp = perm.size // (4 * oop.K)
+    old = perm[q * nfp]
+    perm[q * nfp] = nfp  # out of range: refused on the host, before any launch
+    try:
+        with pytest.raises(RfhipError):
+            assemble_sparse(ctx, desc=d)
+    finally:
+        perm[q * nfp] = old
+
+
+def test_product_builder_sparse_export_matches_oracle(ctx, oracle):
+    """The package's own host builder (MaxwellOperator with materials, a
+    port and Debye elements) into the device export: the same CSR as the
+    oracle's probe of the oracle's operator, bit for bit."""
+    import rapidfem_amd as R
+
+    mesh = R.structured_box_jittered(3, 3, 2, 1.0, 1.0, 0.7, 0.1, 9)
+    K = mesh.n_tets
+    rng = np.random.default_rng(2)
+    eps = rng.uniform(1, 4, (K, 3))
+    mu = rng.uniform(1, 2, (K, 3))
+    sig = rng.uniform(0, 0.5, K)
+    port = np.full((K, 4), -1, np.int32)
+    port[:9, 1] = 0
+    elems = [1, 4, 10]
+    deb = R.DebyeMaterial(DEBYE["eps_inf"], DEBYE["eps_s"], DEBYE["tau"])
+    eps[elems] = DEBYE["eps_inf"]
+    op = R.MaxwellOperator(mesh, 3, 0.8, materials={"eps": eps, "mu": mu, "sigma": sig}, face_port=port,
+                           dispersive=[(e, deb) for e in elems])
+    oop = oracle.Operator(mesh.nodes, mesh.tets, 3, 0.8, eps=eps, mu=mu, sigma=sig, face_port=port.ravel(),
+                          dispersive=(np.array(elems), DEBYE["eps_inf"], DEBYE["eps_s"], DEBYE["tau"]))
+    print(f"product sparse export K={K} n={op.n_dof}", flush=True)
+    got = R.assemble_sparse(ctx, op)
+    _assert_csr_equal(got, oop.assemble_sparse())

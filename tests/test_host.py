@@ -74,8 +74,10 @@ def test_error_message_is_thread_local_and_truncates():
 
 @pytest.mark.parametrize("p", [1, 2, 3, 4])
 def test_host_reference_element_matches_exact(oracle, p):
-    """The builder's reference element (long-double elimination) against the
-    exact-rational tables, and node/face order against the oracle."""
+    """The builder's reference element follows the reference's f64 operation
+    sequence: its tables equal the oracle's (pinned by the reference goldens)
+    bit for bit, round-off residues included, and the exact-rational tables
+    to the reference's own accuracy (p = 4 LIFT leaves ~1e-10 relative)."""
     from rapidfem_amd.maxwell import ReferenceElement
 
     re_ = ReferenceElement.new(p)
@@ -85,9 +87,10 @@ def test_host_reference_element_matches_exact(oracle, p):
     assert (re_.face_nodes == orc["face_nodes"]).all()
     for key, arr in (("mass", re_.mass), ("dr", re_.diff_r), ("ds", re_.diff_s), ("dt", re_.diff_t),
                      ("lift", re_.lift)):
+        assert np.array_equal(arr.ravel(), np.asarray(orc[key]).ravel()), key
         want = np.array(ex[key])
-        assert np.abs(arr.ravel() - want).max() < 1e-12 * np.abs(want).max(), key
-    assert np.abs(re_.face_node_weights - orc["fweights"]).max() < 1e-12
+        assert np.abs(arr.ravel() - want).max() < 1e-9 * np.abs(want).max(), key
+    assert np.array_equal(re_.face_node_weights.ravel(), np.asarray(orc["fweights"]).ravel())
 
 
 @pytest.mark.parametrize("jit", [False, True])
@@ -105,7 +108,9 @@ def test_host_box_mesh_matches_oracle(oracle, jit):
 
 def test_host_operator_arrays_match_oracle(oracle):
     """rfhost_operator_arrays against the oracle's MaxwellOperator::new:
-    topology and perm exact, geometry to rounding, materials exact."""
+    topology, perm, geometry and materials equal bit for bit (the sparse
+    state-space export stores exactly the entries these values make
+    nonzero)."""
     from rapidfem_amd.maxwell import MaxwellOperator, structured_box_jittered
 
     mesh = structured_box_jittered(4, 3, 3, 1.0, 1.0, 1.0, 0.15, 5)
@@ -121,13 +126,10 @@ def test_host_operator_arrays_match_oracle(oracle):
     ex = oop.export()
     assert (op.nbr == ex["nbr"]).all() and (op.nbr_lf == ex["nbr_lf"]).all()
     assert (op.perm == ex["perm"]).all() and (op.port == ex["port"]).all()
-    assert np.abs(op.jinv - ex["jinv"]).max() < 1e-13 * np.abs(ex["jinv"]).max()
-    assert np.abs(op.det - ex["det"]).max() < 1e-15
-    assert np.abs(op.normal - ex["normal"]).max() < 1e-14
-    assert np.abs(op.fscale - ex["fscale"]).max() < 1e-12 * ex["fscale"].max()
-    for a, b in ((op.inv_eps, ex["inv_eps"]), (op.inv_mu, ex["inv_mu"]), (op.sigma_eps, ex["sig_eps"]),
-                 (op.sigma_mu, ex["sig_mu"])):
-        assert np.abs(a - b).max() < 1e-15 * max(1.0, np.abs(b).max())
+    for a, b in ((op.jinv, ex["jinv"]), (op.det, ex["det"]), (op.normal, ex["normal"]),
+                 (op.fscale, ex["fscale"]), (op.inv_eps, ex["inv_eps"]), (op.inv_mu, ex["inv_mu"]),
+                 (op.sigma_eps, ex["sig_eps"]), (op.sigma_mu, ex["sig_mu"])):
+        assert np.array_equal(np.asarray(a).ravel(), np.asarray(b).ravel())
     # a port tag survives only on boundary faces (rhs.rs:862-880)
     assert ((op.port.ravel() >= 0) <= (op.nbr.ravel() < 0)).all()
 

@@ -803,3 +803,140 @@ def test_port_source_is_the_incident_field_flux(oracle):
     for e in range(K):
         if e not in touched:
             assert not np.any(b[e * 60:(e + 1) * 60])
+
+
+# ---- sparse state-space export (rhs.rs:1575-1730) ---------------------------
+
+def _csr_dense(rp, ci, va, n):
+    a = np.zeros((n, n))
+    for r in range(n):
+        for q in range(rp[r], rp[r + 1]):
+            a[r, ci[q]] += va[q]
+    return a
+
+
+def _sparse_case_ops(oracle):
+    """Operators covering every flux branch: PEC boundary, port faces,
+    jittered interior faces, per-element anisotropic materials with both
+    conductivities, central and partly upwind flux, Debye elements."""
+    xyz, tets = oracle.box_mesh(2, 1, 1, 2.0, 1.0, 1.0, 0.1, 7, True)
+    K = tets.shape[0]
+    tmp = oracle.Operator(xyz, tets, 1, 1.0).export()
+    port = np.full(4 * K, -1, np.int32)
+    for e in range(K):
+        for f in range(4):
+            if tmp["nbr"][e, f] < 0 and tmp["normal"][e, f, 0] < -0.99:
+                port[4 * e + f] = 0
+    assert (port >= 0).sum() == 2
+    rng = np.random.default_rng(11)
+    eps = 1.0 + rng.random((K, 3))
+    mu = 1.0 + rng.random((K, 3))
+    sig = np.where(np.arange(K) % 3 == 0, 0.4, 0.0)
+    sigm = np.where(np.arange(K) % 4 == 1, 0.3, 0.0)
+    out = []
+    for order, alpha in ((1, 0.0), (2, 0.6)):
+        out.append(oracle.Operator(xyz, tets, order, alpha, eps=eps, mu=mu, sigma=sig,
+                                   sigma_m=sigm, face_port=port))
+    xyz1, tets1 = oracle.box_mesh(1, 1, 1)
+    out.append(oracle.Operator(xyz1, tets1, 2, 1.0, eps=2.0,
+                               dispersive=(np.arange(0, 6, 2), 2.0, 6.0, 0.3)))
+    return out
+
+
+def test_sparse_assembly_is_the_dense_operator_exactly(oracle):
+    """Brute force: the CSR expanded is the dense A of unit-vector applies
+    bit for bit, its pattern is exactly the nonzeros of A (probe_column keeps
+    v != 0.0), rows are sorted by column, and every P row has exactly its two
+    entries (rhs.rs:1704-1727). A wrong stencil, a dropped P column, an
+    unsorted row or a lost neighbour block fails one of these."""
+    for op in _sparse_case_ops(oracle):
+        n = op.n_dof
+        rp, ci, va = op.assemble_sparse()
+        assert rp[0] == 0 and rp[-1] == len(ci) == len(va) and np.all(np.diff(rp) >= 0)
+        for r in range(n):
+            row = ci[rp[r]:rp[r + 1]]
+            assert np.all(np.diff(row) > 0), f"row {r} not strictly ascending"
+        a = dense_a(op)
+        got = _csr_dense(rp, ci, va, n)
+        assert np.array_equal(got, a)
+        eh = n - 3 * op.n_disp * orc_np_of(op)
+        mask = np.zeros((n, n), bool)
+        for r in range(n):
+            mask[r, ci[rp[r]:rp[r + 1]]] = True
+        assert np.array_equal(mask[:eh], a[:eh] != 0.0)
+        assert np.all(np.diff(rp)[eh:] == 2)
+
+
+def orc_np_of(op):
+    from oracle import oracle as O
+    return O.lib().orc_np(op.order)
+
+
+def test_sparse_assembly_matches_matrix_free_apply(oracle):
+    """rhs.rs:2259-2286: structured_box(2,2,2), N=2, upwind: CSR x equals the
+    matrix-free apply to 1e-10 and the operator is genuinely sparse."""
+    xyz, tets = oracle.box_mesh(2, 2, 2)
+    op = oracle.Operator(xyz, tets, 2, 1.0)
+    n = op.n_dof
+    rp, ci, va = op.assemble_sparse()
+    v = np.cos(1.0 + np.arange(n) * 0.07)
+    import scipy.sparse as sp
+    a = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    mf = op.apply(v)
+    assert np.linalg.norm(a @ v - mf) < 1e-10 * np.linalg.norm(mf)
+    assert len(va) < n * n // 4
+
+
+def test_sparse_assembly_scales_without_densifying(oracle):
+    """rhs.rs:2288-2330: structured_box(4,4,4), N=2: nnz within the
+    5 * stride * n stencil bound and CSR x equals the apply."""
+    xyz, tets = oracle.box_mesh(4, 4, 4)
+    op = oracle.Operator(xyz, tets, 2, 1.0)
+    n = op.n_dof
+    rp, ci, va = op.assemble_sparse()
+    assert len(va) <= 5 * 60 * n
+    v = np.sin(0.3 + np.arange(n) * 0.013)
+    import scipy.sparse as sp
+    a = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    mf = op.apply(v)
+    assert np.linalg.norm(a @ v - mf) < 1e-10 * np.linalg.norm(mf)
+
+
+def test_sparse_assembly_debye_rows_give_the_analytic_permittivity(oracle):
+    """rhs.rs:3133-3165 (half the elements dispersive, CSR x = apply) and
+    rhs.rs:3167-3255: every P row carries exactly g at an E column and a on
+    its diagonal, and eps_inf + g/(i w - a) is the Debye permittivity
+    eps_inf + (eps_s - eps_inf)/(1 + i w tau)."""
+    import scipy.sparse as sp
+    xyz, tets = oracle.box_mesh(2, 2, 2)
+    K = tets.shape[0]
+    op = oracle.Operator(xyz, tets, 2, 1.0, eps=2.5,
+                         dispersive=(np.arange(0, K, 2), 2.5, 7.0, 0.3))
+    n = op.n_dof
+    rp, ci, va = op.assemble_sparse()
+    v = np.cos(1.0 + np.arange(n) * 0.07)
+    mf = op.apply(v)
+    a = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    assert np.linalg.norm(a @ v - mf) < 1e-10 * max(np.linalg.norm(mf), 1.0)
+
+    xyz1, tets1 = oracle.box_mesh(1, 1, 1)
+    eps_inf, eps_s, tau = 2.0, 6.0, 0.3
+    op1 = oracle.Operator(xyz1, tets1, 2, 0.0, eps=eps_inf,
+                          dispersive=(np.arange(6), eps_inf, eps_s, tau))
+    rp, ci, va = op1.assemble_sparse()
+    eh = 6 * 10 * 6
+    assert op1.n_dof == eh + 3 * 10 * 6
+    avals, gvals = [], []
+    for r in range(eh, op1.n_dof):
+        assert rp[r + 1] - rp[r] == 2
+        for q in range(rp[r], rp[r + 1]):
+            (avals if ci[q] == r else gvals).append(va[q])
+            if ci[q] != r:
+                assert ci[q] < eh
+    want_a, want_g = -1.0 / tau, (eps_s - eps_inf) / tau
+    assert np.allclose(avals, want_a, rtol=0, atol=1e-12)
+    assert np.allclose(gvals, want_g, rtol=0, atol=1e-12)
+    for w in (0.2, 1.0, 3.3, 12.0, 20.0):
+        op_eps = eps_inf + want_g / (1j * w - want_a)
+        exact = eps_inf + (eps_s - eps_inf) / (1 + 1j * w * tau)
+        assert abs(op_eps - exact) < 1e-12 * abs(exact)
