@@ -5,11 +5,15 @@
 // stencil (e and its distinct face neighbours, ascending, rhs.rs:1597-1610,
 // then e's own polarisation columns) with a unit vector through
 // apply_element and keeping the outputs that are not 0.0 (rhs.rs:409-421).
-// Here one thread owns one row (e, node i, component co) and walks the same
-// columns in ascending order, evaluating what apply_element computes for
-// each unit column with the same operations in the same order, so every
-// stored value is the reference's bit for bit and a value is stored exactly
-// when the reference stores it:
+// Here a block takes element e and lists, in LDS, the columns of its rows that
+// can be nonzero, ascending (all of e's own, and of each neighbour the nodes
+// behind the shared face). A wave takes one row (e, node i, component co) at
+// a time, each lane evaluating one candidate column: what apply_element
+// computes for that unit column, with the same operations in the same
+// order, so every stored value is the reference's bit for bit and a value is
+// stored exactly when the reference stores it. A ballot compacts the
+// nonzeros, so consecutive lanes write a row's entries to consecutive
+// addresses:
 //  * volume (rhs.rs:39-85): a unit in component cc at node n gives the
 //    reference derivatives rd_k = D_k[i][n] (one nonzero term of the sum),
 //    pd_p = J^-1[0][p] rd_0 + J^-1[1][p] rd_1 + J^-1[2][p] rd_2, and a curl
@@ -110,157 +114,213 @@ __device__ void jump_row(const SparseArgs &a, int64_t q, int side, int ci, doubl
   }
 }
 
+// Persistent: each block loads D and LIFT into LDS once and then takes
+// elements e = blockIdx.x, blockIdx.x + gridDim.x, ...
 template <int NP, int NFP, bool FILL>
-__global__ void __launch_bounds__(256) k_sparse_rows(SparseArgs a) {
-  constexpr int ST = 6 * NP, NF4 = 4 * NFP;
+__global__ void __launch_bounds__(256) k_sparse_rows(SparseArgs a, int64_t K) {
+  constexpr int ST = 6 * NP, NF4 = 4 * NFP, MAXC = ST + 4 * NFP * 6;
+  __shared__ double Dl[3][NP][NP];  // D_k[i][n]
+  __shared__ double Ll[NP][NF4];    // LIFT
   __shared__ double S[2][4][6][6];  // [side][face][co][ci]
   __shared__ double coef[4];
   __shared__ int64_t sten[5];
   __shared__ int nsten;
-  __shared__ int fnb[4];             // neighbour of an interior face, else -1
+  __shared__ int fnb[4];                // neighbour of an interior face, else -1
   __shared__ signed char own_m[NP][4];  // m with face_nodes[f][m] == n, else -1
   __shared__ signed char nbr_m[4][NP];  // m whose neighbour trace is node n, else -1
-  const int64_t e = blockIdx.x;
+  // candidate columns of every row of the element, ascending: all of the
+  // element's own, and of each neighbour the nodes behind a shared face
+  __shared__ int64_t cand_col[MAXC];
+  __shared__ unsigned cand_key[MAXC];  // own << 16 | n << 3 | ci
+  __shared__ int cand_fm[MAXC];        // m at face f in byte f (0xff: none)
+  __shared__ int cand_off[6];
   const int t = threadIdx.x;
+  for (int x = t; x < NP * NP; x += blockDim.x) {
+    Dl[0][x / NP][x % NP] = a.dr[x];
+    Dl[1][x / NP][x % NP] = a.ds[x];
+    Dl[2][x / NP][x % NP] = a.dt[x];
+  }
+  for (int x = t; x < NP * NF4; x += blockDim.x) Ll[x / NF4][x % NF4] = a.lift[x];
+  const int lane = t & 63, wave = t >> 6, nwave = blockDim.x >> 6;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
 
-  for (int x = t; x < 4 * NP; x += blockDim.x) {
-    own_m[x / 4][x % 4] = -1;
-    nbr_m[x / NP][x % NP] = -1;
-  }
-  if (t < 4) {
-    const int64_t q = 4 * e + t;
-    coef[t] = 0.5 * a.fscale[q];
-    fnb[t] = (a.port[q] < 0 && a.nbr[q] >= 0) ? a.nbr[q] : -1;
-  }
-  if (t == 0) {  // rhs.rs:1597-1610 element_stencil
-    int n = 1;
-    sten[0] = e;
-    for (int f = 0; f < 4; ++f) {
-      const int64_t nb = a.nbr[4 * e + f];
-      bool seen = false;
-      for (int k = 0; k < n; ++k) seen |= sten[k] == nb;
-      if (nb >= 0 && !seen) sten[n++] = nb;
+  for (int64_t e = blockIdx.x; e < K; e += gridDim.x) {
+    __syncthreads();  // the previous element's tables are no longer read
+    for (int x = t; x < 4 * NP; x += blockDim.x) {
+      own_m[x / 4][x % 4] = -1;
+      nbr_m[x / NP][x % NP] = -1;
     }
-    for (int x = 1; x < n; ++x)
-      for (int y = x; y > 0 && sten[y - 1] > sten[y]; --y) {
-        const int64_t tmp = sten[y];
-        sten[y] = sten[y - 1];
-        sten[y - 1] = tmp;
+    if (t < 4) {
+      const int64_t q = 4 * e + t;
+      coef[t] = 0.5 * a.fscale[q];
+      fnb[t] = (a.port[q] < 0 && a.nbr[q] >= 0) ? a.nbr[q] : -1;
+    }
+    if (t == 0) {  // rhs.rs:1597-1610 element_stencil
+      int n = 1;
+      sten[0] = e;
+      for (int f = 0; f < 4; ++f) {
+        const int64_t nb = a.nbr[4 * e + f];
+        bool seen = false;
+        for (int k = 0; k < n; ++k) seen |= sten[k] == nb;
+        if (nb >= 0 && !seen) sten[n++] = nb;
       }
-    nsten = n;
-  }
-  if (t < 2 * 4 * 6) {
-    const int side = t / 24, f = (t / 6) % 4, ci = t % 6;
-    double s6[6];
-    jump_row(a, 4 * e + f, side, ci, s6);
-    for (int co = 0; co < 6; ++co) S[side][f][co][ci] = s6[co];
-  }
-  __syncthreads();
-  for (int x = t; x < NF4; x += blockDim.x) {
-    const int f = x / NFP, m = x % NFP;
-    own_m[a.face_nodes[x]][f] = (signed char)m;
-    const int64_t q = 4 * e + f;
-    if (a.port[q] < 0 && a.nbr[q] >= 0) {
-      const int vj = a.face_nodes[a.nbr_lf[q] * NFP + a.perm[q * NFP + m]];
-      nbr_m[f][vj] = (signed char)m;
+      for (int x = 1; x < n; ++x)
+        for (int y = x; y > 0 && sten[y - 1] > sten[y]; --y) {
+          const int64_t tmp = sten[y];
+          sten[y] = sten[y - 1];
+          sten[y - 1] = tmp;
+        }
+      nsten = n;
     }
-  }
-  __syncthreads();
-  if (t >= ST) return;
-
-  const int i = t / 6, co = t % 6, oc = co % 3;
-  const int64_t row = e * ST + t;
-  const double J[3][3] = {{a.jinv[9 * e + 0], a.jinv[9 * e + 1], a.jinv[9 * e + 2]},
-                          {a.jinv[9 * e + 3], a.jinv[9 * e + 4], a.jinv[9 * e + 5]},
-                          {a.jinv[9 * e + 6], a.jinv[9 * e + 7], a.jinv[9 * e + 8]}};
-  const bool erow = co < 3;
-  const double mat = erow ? a.inv_eps[3 * e + oc] : a.inv_mu[3 * e + oc];
-  const double sig = erow ? a.sig_eps[3 * e + oc] : a.sig_mu[3 * e + oc];
-  const int64_t slot = a.disp_slot ? a.disp_slot[e] : -1;
-  const bool debye = erow && slot >= 0;
-  const double pa = debye ? a.disp_coef[3 * slot] : 0.0;
-  const double pg = debye ? a.disp_coef[3 * slot + 1] : 0.0;
-  const double iei = debye ? a.disp_coef[3 * slot + 2] : 0.0;
-  const double *lrow = a.lift + (size_t)i * NF4;
-
-  int64_t o = FILL ? a.row_ptr[row] : 0;
-  auto emit = [&](int64_t col, double v) {
-    if (v != 0.0) {
-      if constexpr (FILL) {
-        a.col_idx[o] = col;
-        a.values[o] = v;
+    if (t < 2 * 4 * 6) {
+      const int side = t / 24, f = (t / 6) % 4, ci = t % 6;
+      double s6[6];
+      jump_row(a, 4 * e + f, side, ci, s6);
+      for (int co = 0; co < 6; ++co) S[side][f][co][ci] = s6[co];
+    }
+    __syncthreads();
+    for (int x = t; x < NF4; x += blockDim.x) {
+      const int f = x / NFP, m = x % NFP;
+      own_m[a.face_nodes[x]][f] = (signed char)m;
+      const int64_t q = 4 * e + f;
+      if (a.port[q] < 0 && a.nbr[q] >= 0) {
+        const int vj = a.face_nodes[a.nbr_lf[q] * NFP + a.perm[q * NFP + m]];
+        nbr_m[f][vj] = (signed char)m;
       }
-      ++o;
     }
-  };
-  // rhs.rs:1443-1472 for the row's component, own value dg of the unit
-  auto finish = [&](double de, double dg, double pv) {
-    double v = mat * de - sig * dg;
-    if (debye) v = v - iei * (pa * pv + pg * dg);
-    return v;
-  };
-
-  const int ns = nsten;
-  for (int b = 0; b < ns; ++b) {
-    const int64_t c = sten[b], cbase = c * ST;
-    if (c == e) {
-      for (int n = 0; n < NP; ++n) {
-        const double r0 = a.dr[i * NP + n], r1 = a.ds[i * NP + n], r2 = a.dt[i * NP + n];
-        for (int ci = 0; ci < 6; ++ci) {
-          // volume: curl component oc of a unit in component cc
-          double de = 0.0;
-          if (erow != (ci < 3)) {
-            const int cc = ci % 3;
-            int ph = -1;
-            bool neg = false;
-            if (oc == 0) {
-              if (cc == 2) ph = 1;
-              else if (cc == 1) ph = 2, neg = true;
-            } else if (oc == 1) {
-              if (cc == 0) ph = 2;
-              else if (cc == 2) ph = 0, neg = true;
-            } else {
-              if (cc == 1) ph = 0;
-              else if (cc == 0) ph = 1, neg = true;
-            }
-            if (ph >= 0) {
-              const double pd = J[0][ph] * r0 + J[1][ph] * r1 + J[2][ph] * r2;
-              de = neg ? -pd : pd;
-              if (!erow) de = -de;
-            }
-          }
-          for (int f = 0; f < 4; ++f) {
-            const int m = own_m[n][f];
-            if (m >= 0) {
-              const double w = coef[f] * lrow[f * NFP + m];
-              de = de + w * S[0][f][co][ci];
-            }
-          }
-          emit(cbase + n * 6 + ci, finish(de, (n == i && ci == co) ? 1.0 : 0.0, 0.0));
+    __syncthreads();
+    // Of a neighbour c only the nodes behind a face shared with it can give
+    // a nonzero (rhs.rs:1403-1418): the face-node bytes of such a node, 0xff
+    // on the other faces.
+    auto nbr_bytes = [&](int64_t c, int n) {
+      unsigned v = 0;
+      for (int f = 0; f < 4; ++f) {
+        const int m = (fnb[f] == c) ? nbr_m[f][n] : -1;
+        v |= (unsigned)(m >= 0 ? m : 0xff) << (8 * f);
+      }
+      return v;
+    };
+    if (t == 0) {  // offsets of the stencil slots in the candidate list
+      int off = 0;
+      for (int b = 0; b < nsten; ++b) {
+        cand_off[b] = off;
+        if (sten[b] == e) {
+          off += ST;
+        } else {
+          for (int n = 0; n < NP; ++n) off += nbr_bytes(sten[b], n) != 0xffffffffu ? 6 : 0;
         }
       }
-    } else {
-      for (int n = 0; n < NP; ++n) {
-        bool touch = false;
-        for (int f = 0; f < 4; ++f) touch |= fnb[f] == c && nbr_m[f][n] >= 0;
-        if (!touch) continue;  // every value of this node is exactly zero
-        for (int ci = 0; ci < 6; ++ci) {
-          double de = 0.0;
-          for (int f = 0; f < 4; ++f) {
-            const int m = nbr_m[f][n];
-            if (fnb[f] == c && m >= 0) {
-              const double w = coef[f] * lrow[f * NFP + m];
-              de = de + w * S[1][f][co][ci];
-            }
+      cand_off[nsten] = off;
+    }
+    __syncthreads();
+    for (int b = 0; b < nsten; ++b) {
+      const int64_t c = sten[b];
+      if (c == e) {
+        for (int x = t; x < ST; x += blockDim.x) {
+          const int n = x / 6;
+          unsigned fm = 0;
+          for (int f = 0; f < 4; ++f) fm |= (unsigned)(own_m[n][f] >= 0 ? own_m[n][f] : 0xff) << (8 * f);
+          cand_col[cand_off[b] + x] = c * ST + x;
+          cand_key[cand_off[b] + x] = 1u << 16 | (unsigned)n << 3 | (unsigned)(x % 6);
+          cand_fm[cand_off[b] + x] = (int)fm;
+        }
+      } else if (t == 0) {
+        int k = cand_off[b];
+        for (int n = 0; n < NP; ++n) {
+          const unsigned fm = nbr_bytes(c, n);
+          if (fm == 0xffffffffu) continue;
+          for (int ci = 0; ci < 6; ++ci, ++k) {
+            cand_col[k] = c * ST + n * 6 + ci;
+            cand_key[k] = (unsigned)n << 3 | (unsigned)ci;
+            cand_fm[k] = (int)fm;
           }
-          emit(cbase + n * 6 + ci, finish(de, 0.0, 0.0));
         }
       }
     }
+    __syncthreads();
+    const int ncand = cand_off[nsten];
+
+    const int64_t slot = a.disp_slot ? a.disp_slot[e] : -1;
+    // One wave per row at a time: lane l evaluates candidate c0 + l, the
+    // nonzeros are compacted with a ballot, so a row's entries are written
+    // by consecutive lanes to consecutive addresses.
+    for (int r = wave; r < ST; r += nwave) {
+      const int i = r / 6, co = r % 6, oc = co % 3;
+      const int64_t row = e * ST + r;
+      const bool erow = co < 3;
+      // curl component oc (rhs.rs:80-84) of a unit in component cc is
+      // +d/dx_php for cc == phm and -d/dx_phm for cc == php, else zero
+      const int php = oc == 2 ? 0 : oc + 1, phm = oc == 0 ? 2 : oc - 1;
+      const double *jv = a.jinv + 9 * e;  // jacobian_inv[k][p] at 3k + p
+      const double jp0 = jv[php], jp1 = jv[3 + php], jp2 = jv[6 + php];
+      const double jm0 = jv[phm], jm1 = jv[3 + phm], jm2 = jv[6 + phm];
+      const double mat = erow ? a.inv_eps[3 * e + oc] : a.inv_mu[3 * e + oc];
+      const double sig = erow ? a.sig_eps[3 * e + oc] : a.sig_mu[3 * e + oc];
+      const bool debye = erow && slot >= 0;
+      const double pa = debye ? a.disp_coef[3 * slot] : 0.0;
+      const double pg = debye ? a.disp_coef[3 * slot + 1] : 0.0;
+      const double iei = debye ? a.disp_coef[3 * slot + 2] : 0.0;
+      // rhs.rs:1443-1472 for the row's component, own value dg of the unit
+      auto finish = [&](double de, double dg, double pv) {
+        double v = mat * de - sig * dg;
+        if (debye) v = v - iei * (pa * pv + pg * dg);
+        return v;
+      };
+      int64_t o = FILL ? a.row_ptr[row] : 0;
+      const int64_t end = FILL ? a.row_ptr[row + 1] : 0;
+      for (int c0 = 0; c0 < ncand; c0 += 64) {
+        const int k = c0 + lane, kk = k < ncand ? k : ncand - 1;
+        const int64_t col = cand_col[kk];
+        const unsigned key = cand_key[kk], fm = (unsigned)cand_fm[kk];
+        const bool own = key >> 16;
+        const int n = (key >> 3) & 511, ci = key & 7, cc = ci < 3 ? ci : ci - 3;
+        // volume (own columns only); pd = sum_k J^-1[k][ph] D_k[i][n]
+        const bool plus = cc == phm;
+        const double pd = (plus ? jp0 : jm0) * Dl[0][i][n] + (plus ? jp1 : jm1) * Dl[1][i][n] +
+                          (plus ? jp2 : jm2) * Dl[2][i][n];
+        double de = plus ? pd : -pd;
+        if (!erow) de = -de;
+        if (!(own && erow != (ci < 3) && (plus || cc == php))) de = 0.0;
+        // flux, face by face; a face the column does not reach adds w = 0
+        // times its jump term, exactly zero, as the reference's loop does
+        const int side = own ? 0 : 1;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          const unsigned m = (fm >> (8 * f)) & 0xffu;
+          const bool on = m != 0xffu;
+          const double w = coef[f] * Ll[i][f * NFP + (on ? m : 0u)];
+          de = de + (on ? w : 0.0) * S[side][f][co][ci];
+        }
+        const double v = finish(de, (own && n == i && ci == co) ? 1.0 : 0.0, 0.0);
+      const bool nz = k < ncand && v != 0.0;
+        const uint64_t mask = __ballot(nz);
+        if constexpr (FILL) {
+          const int64_t pos = o + __popcll(mask & below);
+          if (nz && pos < end) {
+            a.col_idx[pos] = col;
+            a.values[pos] = v;
+          }
+        }
+        o += __popcll(mask);
+      }
+      // the element's own polarisation columns: only P(i, co) reaches this row
+      if (debye) {
+        const double v = finish(0.0, 0.0, 1.0);
+        if (v != 0.0) {
+          if constexpr (FILL) {
+            if (lane == 0 && o < end) {
+              a.col_idx[o] = a.eh + slot * 3 * NP + i * 3 + co;
+              a.values[o] = v;
+            }
+          }
+          ++o;
+        }
+      }
+      if constexpr (!FILL) {
+        if (lane == 0) a.row_ptr[row + 1] = o;
+      }
+    }
   }
-  // the element's own polarisation columns: only P(i, co) reaches this row
-  if (debye) emit(a.eh + slot * 3 * NP + i * 3 + co, finish(0.0, 0.0, 1.0));
-  if constexpr (!FILL) a.row_ptr[row + 1] = o;
 }
 
 // rhs.rs:1704-1727: P row (slot, node, c) holds g at E(node, c) of its
@@ -337,11 +397,17 @@ void scan_inclusive(int64_t *x, int64_t n, hipStream_t s) {
 
 template <int NP, int NFP>
 void launch_rows(const SparseArgs &a, int64_t K, bool fill, hipStream_t s) {
-  const unsigned threads = (unsigned)((6 * NP + 63) / 64 * 64);
+  const void *kern = fill ? reinterpret_cast<const void *>(&k_sparse_rows<NP, NFP, true>)
+                          : reinterpret_cast<const void *>(&k_sparse_rows<NP, NFP, false>);
+  int dev = 0, cus = 0, per_cu = 0;
+  RF_HIP(hipGetDevice(&dev));
+  RF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  RF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
+  const int64_t grid = std::min<int64_t>(K, (int64_t)std::max(cus, 1) * std::max(per_cu, 1));
   if (fill)
-    hipLaunchKernelGGL((k_sparse_rows<NP, NFP, true>), dim3((unsigned)K), dim3(threads), 0, s, a);
+    hipLaunchKernelGGL((k_sparse_rows<NP, NFP, true>), dim3((unsigned)grid), dim3(256), 0, s, a, K);
   else
-    hipLaunchKernelGGL((k_sparse_rows<NP, NFP, false>), dim3((unsigned)K), dim3(threads), 0, s, a);
+    hipLaunchKernelGGL((k_sparse_rows<NP, NFP, false>), dim3((unsigned)grid), dim3(256), 0, s, a, K);
   RF_HIP(hipGetLastError());
 }
 
