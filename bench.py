@@ -58,9 +58,143 @@ def parse():
     ap.add_argument("--sparse-export", action="store_true",
                     help="time the sparse state-space export (assemble_sparse) on the device instead of the "
                          "stepper: --sparse-cells box at --order, one JSON line in nonzeros/s")
+    ap.add_argument("--krylov", action="store_true",
+                    help="time the Krylov exponential (expmv, FP64 Arnoldi with CGS2) on the --krylov-cells box "
+                         "instead of the stepper: one JSON line in Arnoldi DOF-steps/s")
+    ap.add_argument("--krylov-cells", type=int, nargs=3, default=[40, 30, 28])
+    ap.add_argument("--krylov-dim", type=int, default=40)
+    ap.add_argument("--kcl", action="store_true",
+                    help="time the KCL adaptive stepper (RK4(3)5 + PI controller) on the stepper workload: one "
+                         "JSON line in DOF-stage/s over accepted and rejected substeps")
     ap.add_argument("--sparse-cells", type=int, nargs=3, default=[16, 16, 16])
     ap.add_argument("--sparse-reps", type=int, default=3)
     return ap.parse_args()
+
+
+def apply_bytes_per_dof(order: int) -> float:
+    """Fused FP64 apply: y read + dy written (16 B) + per-element operator
+    data (92 B over 6 Np DOFs)."""
+    np_ = (order + 1) * (order + 2) * (order + 3) // 6
+    return 16.0 + 92.0 / (6 * np_)
+
+
+def krylov_bench(args) -> int:
+    """expmv (fixed Arnoldi dimension m, no a-posteriori stop) of the pulse
+    field on a uniform box of the Krylov workload's size (201,600 tets at
+    N=3; the graded slab mesh is not built). Unit: one Arnoldi DOF-step =
+    one basis vector entry of one Arnoldi step (apply + two CGS2 passes)."""
+    import numpy as np
+
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+
+    nx, ny, nz = args.krylov_cells
+    op, y0 = W.pulse_box(nx, ny, nz, args.order)
+    n, m = op.n_dof, args.krylov_dim
+    ctx = R.GpuContext()
+    g = R.GpuOperator(ctx, op, "fused_f64")
+    t = 2.0e-3  # about 3 / rho on this mesh (||tA|| ~ 3)
+    log(f"krylov: {op.n_elem} tets, N={args.order}, n={n}, m={m}, t={t}")
+    times = []
+    for r in range(4):
+        g.expmv(ctx, y0, t, m)
+        log(f"  rep {r}: dim {g.last_dim}, device part {g.last_call_ms:.2f} ms")
+        if r > 0:
+            times.append(g.last_call_ms)
+    ms = float(np.median(times))
+    dim = g.last_dim
+    # algorithmic HBM bytes of the whole call: per step j (cols = j + 1 basis
+    # vectors) the apply, two CGS2 passes (multi-dot reads cols + 1 vectors,
+    # the update reads cols + 1 and writes 1) and the normalisation (read w,
+    # write the next basis vector); plus the start (norm, scale) and the
+    # final combination (reads dim vectors, writes one)
+    b = 3 * 8.0 * n + (dim + 1) * 8.0 * n
+    for j in range(dim):
+        cols = j + 1
+        b += apply_bytes_per_dof(args.order) * n + 2 * (2 * cols + 3) * 8.0 * n
+        if j + 1 < dim:
+            b += 16.0 * n
+    gbs = b / (ms * 1e-3) / 1e9
+    out = {"metric": "Krylov exponential (expmv, CGS2 Arnoldi) DOF-steps/s", "value": n * dim / (ms * 1e-3),
+           "unit": "Arnoldi DOF-steps/s", "n_gpus": 1, "reps": len(times), "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic", "ms_per_call": ms, "ms_per_arnoldi_step": ms / dim,
+           "config": {"workload": f"uniform pulse box {nx}x{ny}x{nz} x 6 tets, N={args.order}, m={dim}, t={t}",
+                      "n_elem": op.n_elem, "n": n},
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                        "algorithmic_bytes_per_call": b,
+                        "kernel": "whole call (apply + k_multi_dot + k_axpy_basis_norm + scale + lincomb), "
+                                  "host round trip per CGS2 pass included"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        O.set_threads(threads)
+        sop, sy = W.pulse_box(12, 9, 8, args.order)
+        oop = O.Operator(sop.mesh.nodes, sop.mesh.tets, args.order, 1.0, eps=1.0 / sop.inv_eps)
+        t0 = time.perf_counter()
+        _, cdim = oop.expmv(sy, t, m)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": sop.n_dof * cdim / tc, "unit": "Arnoldi DOF-steps/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle expmv (propagator.rs:186-326 restated, OpenMP apply) on the "
+                                         f"12x9x8 box ({sop.n_elem} tets, n={sop.n_dof}), m={cdim}, {tc:.1f} s"}
+    print(json.dumps(out))
+    return 0
+
+
+def kcl_bench(args) -> int:
+    """KCL RK4(3)5[2R+]C with the PI controller on the stepper workload: two
+    frames of 8 CFL steps each, the device part of the call timed (frame
+    downloads excluded). Unit: DOF-stage over all substeps, rejected ones
+    included (5 stages each)."""
+    import numpy as np
+
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+
+    nx, ny, nz = args.cells
+    op, y0 = W.pulse_box(nx, ny, nz, args.order)
+    n = op.n_dof
+    ctx = R.GpuContext()
+    sess = R.TdSession(op, 1.0, "fused_f64", ctx)
+    dt = sess.cfl_dt()
+    frame = 8.0 * dt
+    g = sess.gop
+    log(f"kcl: {op.n_elem} tets, N={args.order}, n={n}, cfl dt {dt:.4e}, frame {frame:.4e}")
+    runs = []
+    for r in range(3):
+        _, st = g.transient_kcl(ctx, y0, frame, 2, precision="f64")
+        ms = g.last_call_ms
+        sub = st.accepted + st.rejected
+        log(f"  rep {r}: {st.accepted} accepted + {st.rejected} rejected substeps, device part {ms:.2f} ms")
+        if r > 0:
+            runs.append((ms, sub, st.accepted, st.rejected))
+    ms = float(np.median([x[0] for x in runs]))
+    sub, acc, rej = runs[-1][1], runs[-1][2], runs[-1][3]
+    value = n * 5 * sub / (ms * 1e-3)
+    out = {"metric": "KCL adaptive stepper DOF-stage/s (RK4(3)5[2R+]C + PI controller)", "value": value,
+           "unit": "DOF-stage/s", "n_gpus": 1, "reps": len(runs), "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic", "ms_per_call": ms, "substeps": sub, "accepted": acc, "rejected": rej,
+           "ms_per_substep": ms / max(sub, 1),
+           "config": {"workload": f"pulse box {nx}x{ny}x{nz} x 6 tets, N={args.order}, 2 frames of 8 CFL steps",
+                      "n_elem": op.n_elem, "n": n, "form": "fused_f64"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        O.set_threads(threads)
+        oop = O.Operator(op.mesh.nodes, op.mesh.tets, args.order, op.flux_alpha, eps=1.0 / op.inv_eps)
+        t0 = time.perf_counter()
+        _, oacc, orej, _, _ = oop.adaptive(y0, frame, 1)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * 5 * (oacc + orej) / tc, "unit": "DOF-stage/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle adaptive KCL (explicit_adaptive.rs:84-152 + session.rs:781-827 "
+                                         f"restated, OpenMP) on the same mesh, 1 frame ({oacc} + {orej} substeps), "
+                                         f"{tc:.1f} s"}
+    print(json.dumps(out))
+    return 0
 
 
 def sparse_export(args) -> int:
@@ -174,6 +308,10 @@ def main() -> int:
         return traffic_probe(args)
     if args.sparse_export:
         return sparse_export(args)
+    if args.krylov:
+        return krylov_bench(args)
+    if args.kcl:
+        return kcl_bench(args)
     import numpy as np
 
     import rapidfem_amd as R

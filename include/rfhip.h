@@ -239,6 +239,12 @@ int rfhip_op_expmv(rfhip_op *op, const double *v, double t, int32_t max_dim,
 int rfhip_op_etd_step(rfhip_op *op, const double *y, const double *b,
                       double h, int32_t max_dim, double tol, double *out,
                       int32_t *dim_used);
+/* Wall time, in milliseconds, of the device part of the operator's last
+ * rfhip_op_expmv, rfhip_op_etd_step or rfhip_op_transient_kcl(_f64) call:
+ * from its input resident on the device to its output ready, the host
+ * transfers at the boundary left out. (No reference counterpart: the
+ * benchmark's Krylov and KCL measurements read it.) */
+int rfhip_op_last_call_ms(const rfhip_op *op, double *ms);
 
 /* ---- kernel variants (no reference counterpart) --------------------------
  * The fused forms have several kernel variants per order (work per thread,

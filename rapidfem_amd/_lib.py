@@ -89,6 +89,7 @@ SIGNATURES = {
                                              C.POINTER(ControllerC), _dp, C.POINTER(KclStatsC)]),
     "rfhip_op_expmv": (C.c_int, [_vp, _dp, _d, _i32, _d, _dp, C.POINTER(_i32)]),
     "rfhip_op_etd_step": (C.c_int, [_vp, _dp, _dp, _d, _i32, _d, _dp, C.POINTER(_i32)]),
+    "rfhip_op_last_call_ms": (C.c_int, [_vp, C.POINTER(_d)]),
     "rfhip_op_n_variants": (C.c_int, [_vp]),
     "rfhip_op_autotune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(_d)]),
     "rfhip_op_set_variant": (C.c_int, [_vp, C.c_int]),

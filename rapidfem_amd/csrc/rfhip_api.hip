@@ -4,6 +4,7 @@
 // and rfhip_vec.hip.
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <functional>
 #include <map>
@@ -565,7 +566,21 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
   double h = dt, prev = 0.0, hmin = INFINITY, hmax = 0.0;
   int64_t acc = 0, rej = 0;
   const double h_floor = ctl.min_step_factor * dt;
-  emit(0);
+  // wall time of the substeps, the frame downloads left out
+  double emit_ms = 0.0;
+  auto timed_emit = [&](int64_t r) {
+    const auto a = std::chrono::steady_clock::now();
+    emit(r);
+    emit_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+  };
+  sync(op);
+  const auto t_begin = std::chrono::steady_clock::now();
+  auto finish_timing = [&] {
+    sync(op);
+    op->last_call_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_begin).count() - emit_ms;
+  };
+  timed_emit(0);
   for (int64_t fr = 0; fr < steps; ++fr) {
     double t_rel = 0.0;
     const double gv = kind == RFHIP_DRIVE_FREE ? 0.0 : (double)g[fr];
@@ -599,13 +614,15 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
       }
       if (h < h_floor) {
         st = {acc, rej, hmin, hmax};
+        finish_timing();
         throw Error{RFHIP_ECOLLAPSE, "GPU KCL: step size collapsed below " + std::to_string(ctl.min_step_factor) +
                                          "*dt after " + std::to_string(acc) + " accepted, " + std::to_string(rej) +
                                          " rejected substeps"};
       }
     }
-    emit(fr + 1);
+    timed_emit(fr + 1);
   }
+  finish_timing();
   st = {acc, rej, hmin, hmax};
 }
 
@@ -618,14 +635,6 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
   const int64_t n = nvec(op);
   RF_REQUIRE(max_dim >= 1 && max_dim <= kMaxKrylov, "Krylov dimension must be in [1, 64]");
   hipStream_t s = op->stream;
-  double nrm2 = 0.0;
-  for (int64_t i = 0; i < op->n_dof; ++i) nrm2 += v_host[i] * v_host[i];
-  if (aug) nrm2 += 1.0;
-  const double beta = std::sqrt(nrm2);
-  if (beta == 0.0) {
-    std::fill(out, out + op->n_dof, 0.0);
-    return 0;
-  }
   if (op->kb_cap < max_dim) {
     op->kb_basis.alloc(8 * (size_t)(max_dim + 1) * n);
     op->kb_cap = max_dim;
@@ -646,6 +655,20 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
     op->k.alloc(4 * n);
   }
   h2d_layout_f64(op, v_host, op->kb_x.as<double>());
+  sync(op);
+  const auto t_begin = std::chrono::steady_clock::now();
+  // beta = |v| (with the augmented unit component), reduced on the device
+  double nrm2 = 0.0;
+  launch_multi_dot(op->kb_x.as<double>(), n, 1, op->kb_x.as<double>(), part, nblk, proj, s);
+  RF_HIP(hipMemcpyAsync(&nrm2, proj, 8, hipMemcpyDeviceToHost, s));
+  sync(op);
+  if (aug) nrm2 += 1.0;
+  const double beta = std::sqrt(nrm2);
+  if (beta == 0.0) {
+    std::fill(out, out + op->n_dof, 0.0);
+    op->last_call_ms = 0.0;
+    return 0;
+  }
   launch_scale(op->kb_x.as<double>(), basis, n, 1.0 / beta, s);
   std::vector<double> H((size_t)max_dim * max_dim, 0.0), sc(max_dim + 1, 0.0), pj(kMaxKrylov);
   sc[0] = aug ? 1.0 / beta : 0.0;
@@ -712,6 +735,8 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
   for (int i = 0; i < dim; ++i) pj[i] = beta * eth[(size_t)i * dim];
   RF_HIP(hipMemcpyAsync(proj, pj.data(), 8 * dim, hipMemcpyHostToDevice, s));
   launch_lincomb(basis, n, dim, proj, op->kb_out.as<double>(), s);
+  sync(op);
+  op->last_call_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_begin).count();
   d2h_layout_f64(op, op->kb_out.as<double>(), out);
   return dim;
 }
@@ -1196,6 +1221,13 @@ int rfhip_op_etd_step(rfhip_op *op, const double *y, const double *b, double h, 
     op->src_resident = false;
     const int dim = krylov_impl(op, y, h, max_dim, tol, true, out);
     if (dim_used) *dim_used = dim;
+  });
+}
+
+int rfhip_op_last_call_ms(const rfhip_op *op, double *ms) {
+  return guarded([&] {
+    RF_REQUIRE(op && ms, "null argument");
+    *ms = op->last_call_ms;
   });
 }
 

@@ -167,6 +167,9 @@ struct rfhip_op {
   rfhip::DevBuf port_dof, port_prof, port_w, port_off, port_norm, port_rows;
   rfhip::DevBuf ps_elem, ps_off, ps_lf, ps_prof;  // port-source assembly input
   bool src_resident = false;  // src holds a device-assembled port source
+  // wall time of the device part of the last Krylov or KCL call (input
+  // resident to output ready, host transfers excluded), milliseconds
+  double last_call_ms = 0.0;
   // captured two-step LSERK4 graphs of the resident stepper, one per starting
   // state buffer, with the (dt, variant) they were captured for
   struct StepGraph {

@@ -298,6 +298,14 @@ class GpuOperator:
         it did not write then fails its parity check)."""
         L.check(L.lib.rfhip_op_poison_lds(self._h))
 
+    @property
+    def last_call_ms(self) -> float:
+        """Device-part wall time of the last expmv / etd_step / transient_kcl
+        call, host transfers excluded (no reference counterpart)."""
+        ms = C.c_double()
+        L.check(L.lib.rfhip_op_last_call_ms(self._h, C.byref(ms)))
+        return ms.value
+
     # ---- device-resident state (no reference counterpart) ----
     def state_load(self, y0):
         L.check(L.lib.rfhip_op_state_load(self._h, self._vec(y0, np.float64)))
