@@ -161,7 +161,9 @@ def kcl_bench(args) -> int:
     dt = sess.cfl_dt()
     frame = 8.0 * dt
     g = sess.gop
-    log(f"kcl: {op.n_elem} tets, N={args.order}, n={n}, cfl dt {dt:.4e}, frame {frame:.4e}")
+    variant, tune_ms = g.autotune()
+    log(f"kcl: {op.n_elem} tets, N={args.order}, n={n}, cfl dt {dt:.4e}, frame {frame:.4e}, "
+        f"stage kernel variant {variant}")
     runs = []
     for r in range(3):
         _, st = g.transient_kcl(ctx, y0, frame, 2, precision="f64")
@@ -178,7 +180,7 @@ def kcl_bench(args) -> int:
            "data": "synthetic", "ms_per_call": ms, "substeps": sub, "accepted": acc, "rejected": rej,
            "ms_per_substep": ms / max(sub, 1),
            "config": {"workload": f"pulse box {nx}x{ny}x{nz} x 6 tets, N={args.order}, 2 frames of 8 CFL steps",
-                      "n_elem": op.n_elem, "n": n, "form": "fused_f64"}}
+                      "n_elem": op.n_elem, "n": n, "form": "fused_f64", "variant": variant}}
     if not args.no_cpu:
         from oracle import oracle as O
 

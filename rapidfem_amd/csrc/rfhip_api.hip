@@ -555,10 +555,9 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
   const size_t w = f64 ? 8 : 4;
   hipStream_t s = op->stream;
   op->kst.alloc(w * n);
-  op->kdtf.alloc(w * n);
   op->kerr.alloc(w * n);
   op->kbak.alloc(w * n);
-  for (DevBuf *b : {&op->kst, &op->kdtf, &op->kerr, &op->kbak}) launch_fill(b->p, (int64_t)b->bytes, s);
+  for (DevBuf *b : {&op->kst, &op->kerr, &op->kbak}) launch_fill(b->p, (int64_t)b->bytes, s);
   const int nblk = weighted_err_blocks(n);
   op->red.alloc(8 * (nblk + 1));
   double *part = op->red.as<double>();
@@ -587,13 +586,14 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
     while (t_rel < dt) {
       const double ht = std::min(h, dt - t_rel);
       launch_copy(op->kbak.p, ys, (int64_t)(w * n), s);
-      // explicit_adaptive.rs:107-149
-      rhs_into(op, ys, op->k.p, kind, dof, gv);
-      launch_kcl_stage0(f64, ys, op->kdtf.p, op->kerr.p, op->k.p, ht, kKclB[0], kKclBhat[0] - kKclB[0], n, s);
-      for (int sg = 1; sg < 5; ++sg) {
-        launch_kcl_build(f64, ys, op->kdtf.p, op->kst.p, kKclA[sg - 1] - kKclB[sg - 1], n, s);
-        rhs_into(op, op->kst.p, op->k.p, kind, dof, gv);
-        launch_kcl_accum(f64, ys, op->kdtf.p, op->kerr.p, op->k.p, ht, kKclB[sg], kKclBhat[sg] - kKclB[sg], n, s);
+      // explicit_adaptive.rs:107-149. After each stage's apply one pass
+      // does f = h k, y += b f, err = (or +=) (bhat - b) f and, but for the
+      // last stage, builds the next stage's input y + (a - b) f.
+      for (int sg = 0; sg < 5; ++sg) {
+        rhs_into(op, sg == 0 ? ys : op->kst.p, op->k.p, kind, dof, gv);
+        const double amb = sg < 4 ? kKclA[sg] - kKclB[sg] : 0.0;
+        launch_kcl_update(f64, ys, op->kerr.p, op->kst.p, op->k.p, ht, kKclB[sg], kKclBhat[sg] - kKclB[sg], amb,
+                          sg == 0, sg < 4, n, s);
       }
       launch_weighted_err(f64, op->kbak.p, ys, op->kerr.p, ctl.atol, ctl.rtol, n, part, nblk, res, s);
       double ssum = 0.0;

@@ -153,7 +153,7 @@ struct rfhip_op {
   rfhip::DevBuf y[2], ylo[2], p, k, src;
   int cur = 0;  // index of the current state buffer for fused forms
   // KCL / Krylov / scratch buffers
-  rfhip::DevBuf kst, kdtf, kerr, kbak, red, aos;
+  rfhip::DevBuf kst, kerr, kbak, red, aos;
   rfhip::DevBuf kb_basis, kb_w, kb_proj, kb_part, kb_out, kb_x;
   rfhip::DevBuf probe_idx, probe_rows;  // device-side probe recording
   // port modal extraction: host-side per-port entries, device copies
@@ -249,11 +249,8 @@ void launch_gather(const void *y, const uint16_t *lo, int kind, const int64_t *i
                    hipStream_t s);
 
 // KCL vector kernels on T = float or double (is_f64 selects).
-void launch_kcl_stage0(bool f64, void *y, void *dtf, void *err, const void *k, double dt, double b0, double e0,
-                       int64_t n, hipStream_t s);
-void launch_kcl_build(bool f64, const void *y, const void *dtf, void *stage, double amb, int64_t n, hipStream_t s);
-void launch_kcl_accum(bool f64, void *y, void *dtf, void *err, const void *k, double dt, double b, double ew,
-                      int64_t n, hipStream_t s);
+void launch_kcl_update(bool f64, void *y, void *err, void *st, const void *k, double dt, double b, double ew,
+                       double amb, bool first, bool next, int64_t n, hipStream_t s);
 void launch_copy(void *dst, const void *src, int64_t nbytes, hipStream_t s);
 // sum((err/(atol + rtol*max|yo|,|yn|))^2) into *out (device double), with a
 // non-finite y_new entry flagged as +inf.

@@ -117,30 +117,19 @@ __global__ void __launch_bounds__(kBS) k_copy(uint32_t *__restrict__ d, const ui
 
 // ---- KCL RK4(3)5[2R+]C (explicit_adaptive.rs:84-152) -------------------
 
-template <typename T>
-__global__ void __launch_bounds__(kBS) k_kcl0(T *__restrict__ y, T *__restrict__ dtf, T *__restrict__ err,
-                                              const T *__restrict__ k, T dt, T b0, T e0, int64_t n) {
+// After stage s of a KCL substep: f = h k, y += b f, err = e f on the first
+// stage and err += e f after it, and, when NEXT, the next stage's input
+// st = y + (a - b) f with the updated y (the 2R update and the next stage's
+// build in one pass; f itself is not stored).
+template <typename T, bool FIRST, bool NEXT>
+__global__ void __launch_bounds__(kBS) k_kcl_update(T *__restrict__ y, T *__restrict__ err, T *__restrict__ st,
+                                                    const T *__restrict__ k, T dt, T b, T ew, T amb, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
     const T f = dt * k[i];
-    dtf[i] = f;
-    y[i] += b0 * f;
-    err[i] = e0 * f;
-  }
-}
-template <typename T>
-__global__ void __launch_bounds__(kBS) k_kcl_build(const T *__restrict__ y, const T *__restrict__ dtf,
-                                                   T *__restrict__ st, T amb, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS)
-    st[i] = y[i] + amb * dtf[i];
-}
-template <typename T>
-__global__ void __launch_bounds__(kBS) k_kcl_accum(T *__restrict__ y, T *__restrict__ dtf, T *__restrict__ err,
-                                                   const T *__restrict__ k, T dt, T b, T ew, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    const T f = dt * k[i];
-    dtf[i] = f;
-    y[i] += b * f;
-    err[i] += ew * f;
+    const T yn = y[i] + b * f;
+    y[i] = yn;
+    err[i] = FIRST ? ew * f : err[i] + ew * f;
+    if (NEXT) st[i] = yn + amb * f;
   }
 }
 
@@ -358,34 +347,29 @@ void launch_copy(void *dst, const void *src, int64_t nbytes, hipStream_t s) {
   RF_HIP(hipGetLastError());
 }
 
-void launch_kcl_stage0(bool f64, void *y, void *dtf, void *err, const void *k, double dt, double b0, double e0,
-                       int64_t n, hipStream_t s) {
-  if (f64)
-    hipLaunchKernelGGL(k_kcl0<double>, dim3(grid_for(n)), dim3(kBS), 0, s, (double *)y, (double *)dtf,
-                       (double *)err, (const double *)k, dt, b0, e0, n);
+template <typename T>
+void kcl_update_t(void *y, void *err, void *st, const void *k, double dt, double b, double ew, double amb, bool first,
+                  bool next, int64_t n, hipStream_t s) {
+  T *yy = static_cast<T *>(y), *ee = static_cast<T *>(err), *ss = static_cast<T *>(st);
+  const T *kk = static_cast<const T *>(k);
+  const dim3 g(grid_for(n)), bs(kBS);
+  if (first && next)
+    hipLaunchKernelGGL((k_kcl_update<T, true, true>), g, bs, 0, s, yy, ee, ss, kk, (T)dt, (T)b, (T)ew, (T)amb, n);
+  else if (next)
+    hipLaunchKernelGGL((k_kcl_update<T, false, true>), g, bs, 0, s, yy, ee, ss, kk, (T)dt, (T)b, (T)ew, (T)amb, n);
+  else if (first)
+    hipLaunchKernelGGL((k_kcl_update<T, true, false>), g, bs, 0, s, yy, ee, ss, kk, (T)dt, (T)b, (T)ew, (T)amb, n);
   else
-    hipLaunchKernelGGL(k_kcl0<float>, dim3(grid_for(n)), dim3(kBS), 0, s, (float *)y, (float *)dtf, (float *)err,
-                       (const float *)k, (float)dt, (float)b0, (float)e0, n);
+    hipLaunchKernelGGL((k_kcl_update<T, false, false>), g, bs, 0, s, yy, ee, ss, kk, (T)dt, (T)b, (T)ew, (T)amb, n);
   RF_HIP(hipGetLastError());
 }
-void launch_kcl_build(bool f64, const void *y, const void *dtf, void *stage, double amb, int64_t n, hipStream_t s) {
+
+void launch_kcl_update(bool f64, void *y, void *err, void *st, const void *k, double dt, double b, double ew,
+                       double amb, bool first, bool next, int64_t n, hipStream_t s) {
   if (f64)
-    hipLaunchKernelGGL(k_kcl_build<double>, dim3(grid_for(n)), dim3(kBS), 0, s, (const double *)y,
-                       (const double *)dtf, (double *)stage, amb, n);
+    kcl_update_t<double>(y, err, st, k, dt, b, ew, amb, first, next, n, s);
   else
-    hipLaunchKernelGGL(k_kcl_build<float>, dim3(grid_for(n)), dim3(kBS), 0, s, (const float *)y, (const float *)dtf,
-                       (float *)stage, (float)amb, n);
-  RF_HIP(hipGetLastError());
-}
-void launch_kcl_accum(bool f64, void *y, void *dtf, void *err, const void *k, double dt, double b, double ew,
-                      int64_t n, hipStream_t s) {
-  if (f64)
-    hipLaunchKernelGGL(k_kcl_accum<double>, dim3(grid_for(n)), dim3(kBS), 0, s, (double *)y, (double *)dtf,
-                       (double *)err, (const double *)k, dt, b, ew, n);
-  else
-    hipLaunchKernelGGL(k_kcl_accum<float>, dim3(grid_for(n)), dim3(kBS), 0, s, (float *)y, (float *)dtf,
-                       (float *)err, (const float *)k, (float)dt, (float)b, (float)ew, n);
-  RF_HIP(hipGetLastError());
+    kcl_update_t<float>(y, err, st, k, dt, b, ew, amb, first, next, n, s);
 }
 
 int weighted_err_blocks(int64_t n) { return (int)grid_for(n, 2048); }

@@ -127,6 +127,40 @@ def test_kcl_adaptive_matches_oracle(ctx, oracle, form):
     assert abs(st.h_min - hmin) <= 1e-9 * hmin and abs(st.h_max - hmax) <= 1e-9 * hmax
 
 
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [17, 23])
+@pytest.mark.parametrize("drive", ["free", "vector"])
+def test_kcl_on_stage_kernel_variants_matches_oracle(ctx, oracle, order, variant, drive):
+    """KCL with its applies on the block-wide stage kernel (variant 17) and
+    the whole-elements-per-wave one (variant 23), each stage followed by the
+    merged register update and next-stage build, free and vector-driven:
+    accept/reject counts and states equal the oracle's."""
+    from rapidfem_amd import Controller, Drive
+
+    oop, y0 = _te101(oracle, 3, order)
+    n = oop.n_dof
+    g = _gop(ctx, oop, "fused_f64")
+    g.set_variant(variant)
+    g.poison_lds()
+    dt, steps = 0.05, 3
+    kw, okw = {}, {}
+    if drive == "vector":
+        b = np.cos(0.37 * np.arange(n))
+        wave = np.array([0.4, -0.3, 0.2])
+        kw = dict(drive=Drive.vector(b), g=wave)
+        okw = dict(drive=("vector", b), g=wave)
+    ctl = Controller(atol=1e-8, rtol=1e-6)
+    traj, st = g.transient_kcl(ctx, y0, dt, steps, ctl=ctl, precision="f64", **kw)
+    octl = oracle.Controller.default()
+    octl.atol, octl.rtol = 1e-8, 1e-6
+    y, acc, rej, _, _ = oop.adaptive(y0, dt, steps, octl, **okw)
+    err = rel_l2(traj[-1], y)
+    print(f"kcl order={order} variant={variant} drive={drive}: gpu {st.accepted}/{st.rejected}, "
+          f"oracle {acc}/{rej}, rel_l2 {err:.3e}", flush=True)
+    assert (st.accepted, st.rejected) == (acc, rej)
+    assert err < 1e-11
+
+
 def test_kcl_collapse_is_an_error(ctx, oracle):
     """A step collapse is reported (session.rs:817-824), not a hang."""
     from rapidfem_amd import Controller, RfhipError
