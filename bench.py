@@ -93,6 +93,7 @@ def krylov_bench(args) -> int:
     n, m = op.n_dof, args.krylov_dim
     ctx = R.GpuContext()
     g = R.GpuOperator(ctx, op, "fused_f64")
+    variant, _ = g.autotune()
     t = 2.0e-3  # about 3 / rho on this mesh (||tA|| ~ 3)
     log(f"krylov: {op.n_elem} tets, N={args.order}, n={n}, m={m}, t={t}")
     times = []
@@ -119,7 +120,7 @@ def krylov_bench(args) -> int:
            "unit": "Arnoldi DOF-steps/s", "n_gpus": 1, "reps": len(times), "higher_is_better": True,
            "dtype": "f64", "data": "synthetic", "ms_per_call": ms, "ms_per_arnoldi_step": ms / dim,
            "config": {"workload": f"uniform pulse box {nx}x{ny}x{nz} x 6 tets, N={args.order}, m={dim}, t={t}",
-                      "n_elem": op.n_elem, "n": n},
+                      "n_elem": op.n_elem, "n": n, "variant": variant},
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                         "algorithmic_bytes_per_call": b,
