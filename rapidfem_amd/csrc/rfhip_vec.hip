@@ -183,21 +183,37 @@ constexpr int kMaxCols = 64;
 // partials[b*cols + i] = sum over block b's slice of basis_i . w. Columns
 // are taken eight at a time so w is read ceil(cols/8) times, the basis once.
 constexpr int kDotTile = 8;
+// Per-block partial dot products of w with basis columns 0..cols-1, kDotTile
+// columns per sweep over the block's contiguous range. With VEC (n even, so
+// every column starts 16-byte aligned) each lane reads two consecutive
+// entries per 16-byte load.
+template <bool VEC>
 __global__ void __launch_bounds__(kBS) k_multi_dot(const double *__restrict__ basis, int64_t n, int cols,
                                                    const double *__restrict__ w, double *__restrict__ part) {
   __shared__ double sh[kBS / 64];
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t m = VEC ? n / 2 : n;  // items: pairs or single entries
+  const int64_t chunk = (m + gridDim.x - 1) / gridDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * chunk;
-  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+  const int64_t i1 = i0 + chunk < m ? i0 + chunk : m;
   for (int c0 = 0; c0 < cols; c0 += kDotTile) {
     double acc[kDotTile];
 #pragma unroll
     for (int t = 0; t < kDotTile; ++t) acc[t] = 0.0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kBS) {
-      const double wi = w[i];
+      if constexpr (VEC) {
+        const double2 wi = reinterpret_cast<const double2 *>(w)[i];
 #pragma unroll
-      for (int t = 0; t < kDotTile; ++t)
-        if (c0 + t < cols) acc[t] += basis[(int64_t)(c0 + t) * n + i] * wi;
+        for (int t = 0; t < kDotTile; ++t)
+          if (c0 + t < cols) {
+            const double2 b = reinterpret_cast<const double2 *>(basis + (int64_t)(c0 + t) * n)[i];
+            acc[t] += b.x * wi.x + b.y * wi.y;
+          }
+      } else {
+        const double wi = w[i];
+#pragma unroll
+        for (int t = 0; t < kDotTile; ++t)
+          if (c0 + t < cols) acc[t] += basis[(int64_t)(c0 + t) * n + i] * wi;
+      }
     }
 #pragma unroll
     for (int t = 0; t < kDotTile; ++t) {
@@ -216,7 +232,9 @@ __global__ void __launch_bounds__(kBS) k_dot_finish(const double *__restrict__ p
   const double t = block_sum(acc, sh);
   if (threadIdx.x == 0) proj[c] = t;
 }
-// w -= sum_i proj[i]*basis_i, then per-block partial ||w||^2.
+// w -= sum_i proj[i]*basis_i, then per-block partial ||w||^2; VEC as in
+// k_multi_dot.
+template <bool VEC>
 __global__ void __launch_bounds__(kBS) k_axpy_basis_norm(double *__restrict__ w, const double *__restrict__ basis,
                                                          int64_t n, int cols, const double *__restrict__ proj,
                                                          double *__restrict__ part) {
@@ -225,11 +243,23 @@ __global__ void __launch_bounds__(kBS) k_axpy_basis_norm(double *__restrict__ w,
   for (int c = threadIdx.x; c < cols; c += kBS) sp[c] = proj[c];
   __syncthreads();
   double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    double wi = w[i];
-    for (int c = 0; c < cols; ++c) wi -= sp[c] * basis[(int64_t)c * n + i];
-    w[i] = wi;
-    acc += wi * wi;
+  const int64_t m = VEC ? n / 2 : n;
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < m; i += (int64_t)gridDim.x * kBS) {
+    if constexpr (VEC) {
+      double2 wi = reinterpret_cast<const double2 *>(w)[i];
+      for (int c = 0; c < cols; ++c) {
+        const double2 b = reinterpret_cast<const double2 *>(basis + (int64_t)c * n)[i];
+        wi.x -= sp[c] * b.x;
+        wi.y -= sp[c] * b.y;
+      }
+      reinterpret_cast<double2 *>(w)[i] = wi;
+      acc += wi.x * wi.x + wi.y * wi.y;
+    } else {
+      double wi = w[i];
+      for (int c = 0; c < cols; ++c) wi -= sp[c] * basis[(int64_t)c * n + i];
+      w[i] = wi;
+      acc += wi * wi;
+    }
   }
   const double t = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -386,19 +416,25 @@ void launch_weighted_err(bool f64, const void *yo, const void *yn, const void *e
   RF_HIP(hipGetLastError());
 }
 
-int krylov_dot_blocks(int64_t n) { return (int)grid_for(n, 1024); }
+int krylov_dot_blocks(int64_t n) { return (int)grid_for(n, 2048); }
 
 void launch_multi_dot(const double *basis, int64_t n, int cols, const double *w, double *partials, int nblk,
                       double *proj, hipStream_t s) {
   RF_REQUIRE(cols <= kMaxCols, "Krylov dimension above 64");
-  hipLaunchKernelGGL(k_multi_dot, dim3(nblk), dim3(kBS), 0, s, basis, n, cols, w, partials);
+  if (n % 2 == 0)
+    hipLaunchKernelGGL(k_multi_dot<true>, dim3(nblk), dim3(kBS), 0, s, basis, n, cols, w, partials);
+  else
+    hipLaunchKernelGGL(k_multi_dot<false>, dim3(nblk), dim3(kBS), 0, s, basis, n, cols, w, partials);
   hipLaunchKernelGGL(k_dot_finish, dim3(cols), dim3(kBS), 0, s, partials, nblk, cols, proj);
   RF_HIP(hipGetLastError());
 }
 void launch_axpy_basis_norm(double *w, const double *basis, int64_t n, int cols, const double *proj,
                             double *partials, int nblk, double *norm2, hipStream_t s) {
   RF_REQUIRE(cols <= kMaxCols, "Krylov dimension above 64");
-  hipLaunchKernelGGL(k_axpy_basis_norm, dim3(nblk), dim3(kBS), 0, s, w, basis, n, cols, proj, partials);
+  if (n % 2 == 0)
+    hipLaunchKernelGGL(k_axpy_basis_norm<true>, dim3(nblk), dim3(kBS), 0, s, w, basis, n, cols, proj, partials);
+  else
+    hipLaunchKernelGGL(k_axpy_basis_norm<false>, dim3(nblk), dim3(kBS), 0, s, w, basis, n, cols, proj, partials);
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, partials, nblk, norm2);
   RF_HIP(hipGetLastError());
 }
