@@ -4,6 +4,8 @@
 // CGS2 sweep instead of one work-group per basis row) and the field energy
 // (rhs.rs:1509-1544). All HBM-bound; grid-stride loops over 256-thread
 // blocks.
+#include <type_traits>
+
 #include "rfhip_impl.hpp"
 
 namespace rfhip {
@@ -377,9 +379,45 @@ void launch_copy(void *dst, const void *src, int64_t nbytes, hipStream_t s) {
   RF_HIP(hipGetLastError());
 }
 
+// k_kcl_update on pairs of doubles (16-byte accesses; n even).
+template <bool FIRST, bool NEXT>
+__global__ void __launch_bounds__(kBS) k_kcl_update2(double2 *__restrict__ y, double2 *__restrict__ err,
+                                                     double2 *__restrict__ st, const double2 *__restrict__ k,
+                                                     double dt, double b, double ew, double amb, int64_t m) {
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < m; i += (int64_t)gridDim.x * kBS) {
+    const double2 kk = k[i], yo = y[i];
+    const double f0 = dt * kk.x, f1 = dt * kk.y;
+    const double2 yn = make_double2(yo.x + b * f0, yo.y + b * f1);
+    y[i] = yn;
+    if (FIRST) {
+      err[i] = make_double2(ew * f0, ew * f1);
+    } else {
+      const double2 e = err[i];
+      err[i] = make_double2(e.x + ew * f0, e.y + ew * f1);
+    }
+    if (NEXT) st[i] = make_double2(yn.x + amb * f0, yn.y + amb * f1);
+  }
+}
+
 template <typename T>
 void kcl_update_t(void *y, void *err, void *st, const void *k, double dt, double b, double ew, double amb, bool first,
                   bool next, int64_t n, hipStream_t s) {
+  if (std::is_same<T, double>::value && n % 2 == 0) {
+    double2 *yy = static_cast<double2 *>(y), *ee = static_cast<double2 *>(err), *ss = static_cast<double2 *>(st);
+    const double2 *kk = static_cast<const double2 *>(k);
+    const int64_t m = n / 2;
+    const dim3 g(grid_for(m)), bs(kBS);
+    if (first && next)
+      hipLaunchKernelGGL((k_kcl_update2<true, true>), g, bs, 0, s, yy, ee, ss, kk, dt, b, ew, amb, m);
+    else if (next)
+      hipLaunchKernelGGL((k_kcl_update2<false, true>), g, bs, 0, s, yy, ee, ss, kk, dt, b, ew, amb, m);
+    else if (first)
+      hipLaunchKernelGGL((k_kcl_update2<true, false>), g, bs, 0, s, yy, ee, ss, kk, dt, b, ew, amb, m);
+    else
+      hipLaunchKernelGGL((k_kcl_update2<false, false>), g, bs, 0, s, yy, ee, ss, kk, dt, b, ew, amb, m);
+    RF_HIP(hipGetLastError());
+    return;
+  }
   T *yy = static_cast<T *>(y), *ee = static_cast<T *>(err), *ss = static_cast<T *>(st);
   const T *kk = static_cast<const T *>(k);
   const dim3 g(grid_for(n)), bs(kBS);
