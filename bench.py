@@ -331,6 +331,8 @@ def main() -> int:
     dt = sess.cfl_dt()
     log(f"cfl_dt {dt:.6e} ({time.perf_counter() - t0:.1f}s, rho={sess.rho:.4e})")
     ref = sess.gop
+    ref_variant, _ = ref.autotune()
+    log(f"fused_f64 autotune: variant {ref_variant}")
     n = op.n_dof
     K, Wm = args.steps, args.warmup
 
@@ -410,7 +412,7 @@ def main() -> int:
                    "variant": variant,
                    "registers": "y f32+bf16, p f32" if args.form == "fused_f64c" else "f64"},
         "plain": {"form": "plain_f64", "value": pvalue, "ms_per_step": ptot / args.plain_steps},
-        "fused_f64": {"value": fvalue},
+        "fused_f64": {"value": fvalue, "variant": ref_variant},
         "speedup_vs_plain": value / pvalue,
         "rel_l2_vs_f64": err, "rel_l2_steps": Wm + K,
         "roofline": roofline, "cpu_baseline": cpu,

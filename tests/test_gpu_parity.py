@@ -826,3 +826,20 @@ def test_product_builder_sparse_export_matches_oracle(ctx, oracle):
     print(f"product sparse export K={K} n={op.n_dof}", flush=True)
     got = R.assemble_sparse(ctx, op)
     _assert_csr_equal(got, oop.assemble_sparse())
+
+
+def test_cfl_step_matches_oracle_on_the_cavity(ctx, oracle):
+    """TdSession::cfl_dt (session.rs:514-569) over the GPU operator against the
+    oracle's restatement on the 1,296-tet N=2 jittered cavity: the same
+    power iteration from the same seeded vector and the same bracket and
+    bisection give the same step (the GPU and CPU applies differ only by
+    round-off, which moves rho in the last digits)."""
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+
+    op, _ = W.cavity_te101(6, 2)
+    oop = oracle.Operator(op.mesh.nodes, op.mesh.tets, 2, 1.0)
+    want = oop.cfl_dt()
+    got = R.TdSession(op, 1.0, "fused_f64", ctx).cfl_dt()
+    print(f"cfl_dt gpu {got:.15e} oracle {want:.15e}", flush=True)
+    assert abs(got - want) <= 1e-10 * want

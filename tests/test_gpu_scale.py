@@ -154,4 +154,5 @@ def test_krylov_exponential_inverse_and_semigroup_at_scale(ctx):
     print(f"krylov at scale: t {t:.3e}, inverse {e_inv:.3e}, semigroup {e_semi:.3e}", flush=True)
     assert e_inv < 1e-9
     assert e_semi < 1e-9
+    assert g.last_call_ms > 0.0
     assert rel_l2(w, y0) > 1e-3  # the field did move
