@@ -3187,7 +3187,6 @@ inline int64_t resident_blocks(const void *kernel) {
 // only when the operator's D_k fit the equispaced pattern), 22 / 23 =
 // k_fused11 (whole elements per wave, no block barriers in the group loop). Chosen by the
 // operator (autotuned or set).
-int g_variant_override = -1;
 
 template <int NP, bool CIN, int EPI>
 void fused_np(int variant, const StageArgs &a, hipStream_t s) {
@@ -3466,12 +3465,7 @@ void launch_port_source(const rfhip_op *op, int64_t n_elems, const int64_t *elem
 
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s) {
   const int np = (order + 1) * (order + 2) * (order + 3) / 6;
-  int variant = a.variant;
-  if (g_variant_override < 0) {
-    const char *env = getenv("RFHIP_FUSED_VARIANT");
-    g_variant_override = env ? atoi(env) : 99;
-  }
-  if (g_variant_override != 99) variant = g_variant_override;
+  const int variant = a.variant;
   if (epi == 0)
     fused_dispatch<false, 0>(np, variant, a, s);
   else if (form == RFHIP_FORM_FUSED_F64C)
