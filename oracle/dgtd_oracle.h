@@ -23,8 +23,11 @@
  *                        a closed-form linear ODE.
  *   orc_expm/expmv/etd   propagator_expm_test.rs (sympy 30-digit values),
  *                        closed forms, scipy.linalg.expm, ETD analytic.
- *   orc_cfl_dt           parity unpinned for the exact value (no stored
- *                        reference dt); checked for stability bracketing.
+ *   orc_cfl_dt           the LSERK4 stability boundary along the dominant
+ *                        eigenvalue (dense eigensolve) within the
+ *                        bisection's resolution; session.rs:1352-1364.
+ *   orc_assemble_sparse  dense A of unit-vector applies bit for bit;
+ *                        rhs.rs:2259-2330, 3133-3255.
  */
 #ifndef DGTD_ORACLE_H
 #define DGTD_ORACLE_H

@@ -524,10 +524,45 @@ def test_normal_vector_statistics(oracle):
     assert abs(a.mean()) < 0.01 and abs(a.std() - 1.0) < 0.01
 
 
+@pytest.mark.parametrize("cells,order,alpha", [((1, 1, 1), 2, 0.0), ((1, 1, 1), 3, 0.0), ((2, 1, 1), 2, 0.0),
+                                                ((1, 1, 1), 2, 1.0), ((1, 1, 1), 3, 1.0)])
+def test_cfl_step_is_the_lserk4_stability_boundary(oracle, cells, order, alpha):
+    """The value of cfl_dt, pinned by the mathematics it approximates.
+    session.rs:514-569 brackets and bisects the largest z for which 64
+    LSERK4 steps of h = z / rho grow the dominant mode by at most 1e-3 per
+    step, and returns 0.8 z / rho. That z is the boundary of
+    |R(w)| <= 1 + 1e-3 for the LSERK4 amplification factor R along the
+    direction of the dominant eigenvalue: the imaginary axis for the central
+    flux (M~-skew operator), the negative real axis for the upwind one. R is
+    taken from the oracle's dense LSERK4 on the 2x2 rotation generator (or
+    the 1x1 decay), the spectral radius from a dense eigensolve, and the
+    bisection leaves z at most its resolution (12/128) below the boundary.
+    A wrong safety factor, bracket or growth test moves h off that window."""
+    xyz, tets = oracle.box_mesh(*cells)
+    op = oracle.Operator(xyz, tets, order, alpha)
+    lam = np.linalg.eigvals(dense_a(op))
+    rho = np.abs(lam).max()
+    dom = lam[np.argmax(np.abs(lam))]
+    if alpha == 0.0:
+        assert abs(dom.real) < 1e-8 * rho  # central flux: imaginary spectrum
+        gen = lambda y: np.array([[0.0, -y], [y, 0.0]])  # noqa: E731
+        start = np.array([1.0, 0.0])
+    else:
+        assert abs(dom.imag) < 1e-8 * rho and dom.real < 0  # upwind: the top mode decays
+        gen = lambda y: np.array([[-y]])  # noqa: E731
+        start = np.array([1.0])
+    lo, hi = 0.5, 8.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        grow = np.linalg.norm(oracle.lserk_dense(gen(mid), start, 1.0, 1))
+        lo, hi = (mid, hi) if grow <= 1.0 + 1e-3 else (lo, mid)
+    z = op.cfl_dt() * rho / 0.8
+    assert lo - 12.0 / 128 <= z <= lo + 1e-3, (z, lo)
+
+
 def test_cfl_step_is_stable(oracle):
     """session.rs:1352-1364: the CFL step is positive, finite and stable over
-    500 steps (norm growth <= 1%); twice the step is not. The exact value is
-    parity-unpinned (no stored reference dt)."""
+    500 steps (norm growth <= 1%); four times the step is not."""
     xyz, tets = oracle.box_mesh(2, 2, 2)
     op = oracle.Operator(xyz, tets, 2)
     h = op.cfl_dt()
