@@ -28,6 +28,10 @@
  *                        bisection's resolution; session.rs:1352-1364.
  *   orc_assemble_sparse  dense A of unit-vector applies bit for bit;
  *                        rhs.rs:2259-2330, 3133-3255.
+ *   orc_op_set_dispersive  n_dof and P rows (rhs.rs:3077-3121), analytic
+ *                        Debye permittivity, exact energy-dissipation
+ *                        identity of the central-flux Debye system.
+ *   orc_apply_ghost      A y + b for the numpy port source b, any y.
  */
 #ifndef DGTD_ORACLE_H
 #define DGTD_ORACLE_H
