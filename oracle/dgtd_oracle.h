@@ -81,7 +81,15 @@ void orc_op_export(const orc_op *op, double *jinv, double *det,
                    int *port, int *perm, double *inv_eps, double *inv_mu,
                    double *sig_eps, double *sig_mu);
 void orc_op_set_threads(int nthreads);
+/* Debye slots in list order (rhs.rs:963-985); -1 for a bad element list. */
+int orc_op_set_dispersive(orc_op *op, int64_t n, const int64_t *elem,
+                          const double *eps_inf, const double *eps_static,
+                          const double *tau);
+int64_t orc_op_n_dispersive(const orc_op *op);
 void orc_apply(const orc_op *op, const double *y, double *dy);
+/* A y with ghost[(4e + f)*Nfp + m][6] as the port faces' incident field. */
+void orc_apply_ghost(const orc_op *op, const double *y, const double *ghost,
+                     double *dy);
 /* MaxwellOperator::assemble_sparse: col_idx == NULL fills row_ptr
  * (n_dof + 1) and returns nnz; a second call fills col_idx/values. */
 int64_t orc_assemble_sparse(const orc_op *op, int64_t *row_ptr,
