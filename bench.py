@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--kcl", action="store_true",
                     help="time the KCL adaptive stepper (RK4(3)5 + PI controller) on the stepper workload: one "
                          "JSON line in DOF-stage/s over accepted and rejected substeps")
+    ap.add_argument("--record", action="store_true",
+                    help="time trajectory recording on the stepper workload: every frame's full state "
+                         "downloaded (the reference GPU path) against probe DOFs recorded on the device")
     ap.add_argument("--sparse-cells", type=int, nargs=3, default=[16, 16, 16])
     ap.add_argument("--sparse-reps", type=int, default=3)
     return ap.parse_args()
@@ -200,6 +203,56 @@ def kcl_bench(args) -> int:
     return 0
 
 
+def record_bench(args) -> int:
+    """20 frames of one LSERK4 step each at dt = 0.002 (below the CFL step
+    2.32e-3) on the stepper workload, three ways: every frame's full state
+    downloaded (Record::States through the reference GPU path's per-frame
+    download, session.rs:914-921), 64 probe DOFs recorded on the device and
+    downloaded once (Record::Dofs, rfhip_op_transient_probes_f64), and the
+    final state only. Wall time of each call; value = recorded frames/s of
+    the probe path."""
+    import numpy as np
+
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+
+    nx, ny, nz = args.cells
+    op, y0 = W.pulse_box(nx, ny, nz, args.order)
+    n = op.n_dof
+    ctx = R.GpuContext()
+    g = R.GpuOperator(ctx, op, args.form)
+    variant, _ = g.autotune()
+    frames, dt = 20, 0.002
+    probes = np.linspace(0, n - 1, 64).astype(np.int64)
+    g.transient(ctx, y0, dt, 2, 1, record=False, precision="f64")  # warm-up
+    res = {}
+    for name in ("full", "probes", "final"):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            if name == "full":
+                out = g.transient(ctx, y0, dt, frames, 1, record=True, precision="f64")
+            elif name == "probes":
+                out, _ = g.transient_probes(ctx, y0, dt, frames, probes)
+            else:
+                out = g.transient(ctx, y0, dt, frames, 1, record=False, precision="f64")
+            ts.append(time.perf_counter() - t0)
+            del out
+        res[name] = float(np.median(ts))
+        log(f"record {name}: {res[name] * 1e3:.1f} ms for {frames} frames")
+    value = frames / res["probes"]
+    out = {"metric": "recorded frames/s, probe DOFs on the device vs full-state download per frame",
+           "value": value, "unit": "frames/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic", "frames": frames,
+           "ms_per_frame": {k: v * 1e3 / frames for k, v in res.items()},
+           "full_state_bytes_per_frame": 8 * n,
+           "speedup_probes_vs_full": res["full"] / res["probes"],
+           "config": {"workload": f"pulse box {nx}x{ny}x{nz} x 6 tets, N={args.order}, dt 0.002, 1 step per frame",
+                      "n": n, "form": args.form, "variant": variant, "probes": int(probes.shape[0])}}
+    print(json.dumps(out))
+    return 0
+
+
 def sparse_export(args) -> int:
     """Device assemble_sparse on the jittered box (PEC walls, upwind flux):
     value = stored nonzeros per second of the two row passes (count + fill,
@@ -313,6 +366,8 @@ def main() -> int:
         return sparse_export(args)
     if args.krylov:
         return krylov_bench(args)
+    if args.record:
+        return record_bench(args)
     if args.kcl:
         return kcl_bench(args)
     import numpy as np
