@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--record", action="store_true",
                     help="time trajectory recording on the stepper workload: every frame's full state "
                          "downloaded (the reference GPU path) against probe DOFs recorded on the device")
+    ap.add_argument("--batch", action="store_true",
+                    help="time the batching form: 8 seeded boxes of 20k-300k tets at orders 2/3/4, 4 right-hand "
+                         "sides each (32 operators), stepped one after another and then together from one graph")
     ap.add_argument("--sparse-cells", type=int, nargs=3, default=[16, 16, 16])
     ap.add_argument("--sparse-reps", type=int, default=3)
     return ap.parse_args()
@@ -199,6 +202,64 @@ def kcl_bench(args) -> int:
                                "sample": f"oracle adaptive KCL (explicit_adaptive.rs:84-152 + session.rs:781-827 "
                                          f"restated, OpenMP) on the same mesh, 1 frame ({oacc} + {orej} substeps), "
                                          f"{tc:.1f} s"}
+    print(json.dumps(out))
+    return 0
+
+
+def batch_bench(args) -> int:
+    """The batching form on one GPU: 8 boxes with seeded sizes (20k-300k
+    tets, log-uniform) at orders 2, 3, 4 in turn, each with 4 right-hand
+    sides (one operator each, 32 operators), f64c registers, free LSERK4 at a
+    fixed small step (throughput only; no port sources). One autotune per
+    mesh, shared by its right-hand sides. Timed: every operator's 20 steps
+    one after another, then all of them from one graph (rfhip_batch_lserk).
+    Unit: DOF-stage over all operators."""
+    import numpy as np
+
+    import rapidfem_amd as R
+
+    rng = np.random.default_rng(2024)
+    ctx = R.GpuContext()
+    ops, dofs, meshes = [], 0, []
+    for b in range(8):
+        target = float(np.exp(rng.uniform(np.log(20_000), np.log(300_000))))
+        c = (target / 6.0) ** (1.0 / 3.0)
+        asp = rng.uniform(0.8, 1.25, 2)
+        nx = max(2, int(round(c * asp[0])))
+        ny = max(2, int(round(c / asp[0])))
+        nz = max(2, int(round(target / (6.0 * nx * ny))))
+        order = (2, 3, 4)[b % 3]
+        mesh = R.structured_box_jittered(nx, ny, nz, 1.0, 1.0, 1.0, 0.1, 11 + b)
+        op = R.MaxwellOperator(mesh, order, 1.0)
+        first = R.GpuOperator(ctx, op, "fused_f64c")
+        variant, _ = first.autotune()
+        group = [first] + [R.GpuOperator(ctx, op, "fused_f64c") for _ in range(3)]
+        for q, g in enumerate(group):
+            g.set_variant(variant)
+            g.state_load(np.random.default_rng(100 * b + q).standard_normal(op.n_dof) * 0.1)
+        ops += group
+        dofs += 4 * op.n_dof
+        meshes.append(f"{nx}x{ny}x{nz} N={order} ({op.n_elem} tets, v{variant})")
+        log(f"mesh {b}: {meshes[-1]}")
+    dt, steps = 1.0e-4, 20
+    for g in ops:
+        g.state_lserk(dt, 2)  # warm-up
+    t0 = time.perf_counter()
+    for g in ops:
+        g.state_lserk(dt, steps)
+    t_seq = time.perf_counter() - t0
+    R.batch_lserk(ops, dt, 2)  # warm-up
+    t0 = time.perf_counter()
+    R.batch_lserk(ops, dt, steps)
+    t_batch = time.perf_counter() - t0
+    log(f"sequential {t_seq * 1e3:.1f} ms, batched {t_batch * 1e3:.1f} ms for {steps} steps of {len(ops)} operators")
+    out = {"metric": "batching form DOF-stage/s (8 meshes x 4 right-hand sides, one graph)",
+           "value": dofs * 5 * steps / t_batch, "unit": "DOF-stage/s", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic", "steps": steps, "ms_batched": t_batch * 1e3,
+           "ms_sequential": t_seq * 1e3, "sequential_value": dofs * 5 * steps / t_seq,
+           "config": {"workload": "8 seeded jittered boxes (20k-300k tets), orders 2/3/4, 4 RHS each, "
+                                  "f64c registers, free LSERK4 at dt 1e-4", "meshes": meshes,
+                      "operators": len(ops), "total_dofs": dofs}}
     print(json.dumps(out))
     return 0
 
@@ -368,6 +429,8 @@ def main() -> int:
         return krylov_bench(args)
     if args.record:
         return record_bench(args)
+    if args.batch:
+        return batch_bench(args)
     if args.kcl:
         return kcl_bench(args)
     import numpy as np

@@ -275,6 +275,15 @@ int rfhip_op_state_lserk(rfhip_op *op, double dt, int64_t nsteps);
  * overhead is what it removes. Results equal rfhip_op_state_lserk's bit for
  * bit. (No reference counterpart: the reference has no graph capture.) */
 int rfhip_op_state_lserk_graph(rfhip_op *op, double dt, int64_t nsteps);
+/* Several operators' resident states stepped together (the batching form;
+ * no reference counterpart): nsteps free LSERK4 steps of dt[i] on ops[i],
+ * the stage launches of two steps of every operator captured into one graph
+ * that forks from ops[0]'s stream to every operator's stream and joins back,
+ * so independent meshes (or several right-hand sides of one mesh, one
+ * operator each) run concurrently. Each state ends as rfhip_op_state_lserk
+ * would leave it. The operators must share a context and be distinct. */
+int rfhip_batch_lserk(rfhip_op *const *ops, int32_t n_ops, const double *dt,
+                      int64_t nsteps);
 /* As rfhip_op_state_lserk, timed with HIP events on the operator's stream:
  * total_ms for the whole run and stage_ms_avg, the mean duration of one
  * launch of the dominant kernel (the fused stage kernel, or the plain form's

@@ -1383,6 +1383,74 @@ int rfhip_op_state_lserk_graph(rfhip_op *op, double dt, int64_t nsteps) {
   });
 }
 
+int rfhip_batch_lserk(rfhip_op *const *ops, int32_t n_ops, const double *dt, int64_t nsteps) {
+  return guarded([&] {
+    RF_REQUIRE(ops && dt && n_ops >= 1 && nsteps >= 0, "bad arguments");
+    for (int32_t i = 0; i < n_ops; ++i) {
+      RF_REQUIRE(ops[i], "null operator in the batch");
+      RF_REQUIRE(ops[i]->ctx == ops[0]->ctx, "batched operators must share one context");
+      RF_REQUIRE(std::isfinite(dt[i]), "non-finite step");
+      for (int32_t j = 0; j < i; ++j) RF_REQUIRE(ops[j] != ops[i], "operator listed twice in the batch");
+    }
+    for (int32_t i = 0; i < n_ops; ++i) {
+      zero_p(ops[i]);
+      sync(ops[i]);
+    }
+    hipStream_t s0 = ops[0]->stream;
+    std::vector<hipEvent_t> ev((size_t)n_ops + 1, nullptr);
+    hipGraphExec_t exec = nullptr;
+    auto cleanup = [&] {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    };
+    try {
+      for (auto &e : ev) RF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      if (nsteps >= 2) {
+        // Two steps of every operator in one graph: fork from the first
+        // operator's stream to every other one, join back at the end. The
+        // capture runs nothing, so each operator's buffer parity is restored.
+        std::vector<int> cur0((size_t)n_ops);
+        for (int32_t i = 0; i < n_ops; ++i) cur0[(size_t)i] = ops[i]->cur;
+        hipGraph_t graph = nullptr;
+        RF_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+        try {
+          RF_HIP(hipEventRecord(ev[0], s0));
+          for (int32_t i = 1; i < n_ops; ++i) RF_HIP(hipStreamWaitEvent(ops[i]->stream, ev[0], 0));
+          for (int32_t i = 0; i < n_ops; ++i) {
+            lserk_step(ops[i], dt[i], RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
+            lserk_step(ops[i], dt[i], RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
+          }
+          for (int32_t i = 1; i < n_ops; ++i) {
+            RF_HIP(hipEventRecord(ev[(size_t)i], ops[i]->stream));
+            RF_HIP(hipStreamWaitEvent(s0, ev[(size_t)i], 0));
+          }
+        } catch (...) {
+          (void)hipStreamEndCapture(s0, &graph);
+          if (graph) (void)hipGraphDestroy(graph);
+          for (int32_t i = 0; i < n_ops; ++i) ops[i]->cur = cur0[(size_t)i];
+          throw;
+        }
+        RF_HIP(hipStreamEndCapture(s0, &graph));
+        for (int32_t i = 0; i < n_ops; ++i) ops[i]->cur = cur0[(size_t)i];
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        RF_HIP(e);
+        for (int64_t k = 0; k + 1 < nsteps; k += 2) RF_HIP(hipGraphLaunch(exec, s0));
+        RF_HIP(hipStreamSynchronize(s0));
+      }
+      if (nsteps % 2)
+        for (int32_t i = 0; i < n_ops; ++i) lserk_step(ops[i], dt[i], RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
+      for (int32_t i = 0; i < n_ops; ++i) sync(ops[i]);
+    } catch (...) {
+      for (int32_t i = 0; i < n_ops; ++i) (void)hipStreamSynchronize(ops[i]->stream);
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
 int rfhip_op_state_lserk_timed(rfhip_op *op, double dt, int64_t nsteps, double *total_ms, double *stage_ms_avg) {
   return guarded([&] {
     RF_REQUIRE(op && nsteps >= 1, "bad arguments");

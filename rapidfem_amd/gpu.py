@@ -333,6 +333,16 @@ class GpuOperator:
         return e.value
 
 
+def batch_lserk(ops: list[GpuOperator], dts, nsteps: int) -> None:
+    """nsteps free LSERK4 steps of every operator's resident state, dts[i]
+    for ops[i], replayed from one graph in which the operators' streams run
+    concurrently (the batching form; no reference counterpart)."""
+    n = len(ops)
+    handles = (C.c_void_p * n)(*[g._h.value for g in ops])
+    d = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, np.float64), (n,)))
+    L.check(L.lib.rfhip_batch_lserk(handles, n, d, int(nsteps)))
+
+
 @dataclass
 class CsrMatrix:
     """rhs.rs:1575-1591 CsrMatrix: dimension n, row_ptr (n+1), col_idx and

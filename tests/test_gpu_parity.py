@@ -843,3 +843,41 @@ def test_cfl_step_matches_oracle_on_the_cavity(ctx, oracle):
     got = R.TdSession(op, 1.0, "fused_f64", ctx).cfl_dt()
     print(f"cfl_dt gpu {got:.15e} oracle {want:.15e}", flush=True)
     assert abs(got - want) <= 1e-10 * want
+
+
+def test_batch_lserk_equals_individual_runs(ctx, oracle):
+    """Operators of different meshes, orders and forms (two of them on one
+    mesh, as two right-hand sides; one with Debye elements) stepped together
+    from one graph end exactly where each operator's own run ends; an odd
+    step count exercises the step after the graph replays."""
+    import rapidfem_amd as R
+
+    def make(order, n, form, seed, debye=False):
+        xyz, tets = _mesh(oracle, n)
+        K = tets.shape[0]
+        kw, disp = {}, None
+        if debye:
+            elems = np.arange(0, K, 3)
+            kw["dispersive"] = (elems, DEBYE["eps_inf"], DEBYE["eps_s"], DEBYE["tau"])
+            a, gg = R.DebyeMaterial(DEBYE["eps_inf"], DEBYE["eps_s"], DEBYE["tau"]).relaxation_coeffs()
+            disp = (elems, a, gg, 1.0 / DEBYE["eps_inf"])
+        oop = oracle.Operator(xyz, tets, order, 1.0, **kw)
+        d, arrs, re_, ex = oracle_desc(oop)
+        pair = [R.GpuOperator.from_desc(ctx, d, form, dispersive=disp) for _ in range(2)]
+        y0 = np.random.default_rng(seed).standard_normal(oop.n_dof) * 0.1
+        return pair, y0
+
+    cases = [make(2, 3, "fused_f64", 1), make(3, 2, "fused_f64c", 2), make(2, 3, "plain_f64", 3),
+             make(2, 3, "fused_f64", 4), make(1, 3, "fused_f64", 5, debye=True)]
+    dts = [0.01, 0.008, 0.01, 0.012, 0.01]
+    batch = [pair[0] for pair, _ in cases]
+    for (pair, y0) in cases:
+        for g in pair:
+            g.state_load(y0)
+    print("batch of 5 operators, 7 steps", flush=True)
+    R.batch_lserk(batch, dts, 7)
+    for (pair, _), dt in zip(cases, dts):
+        pair[1].state_lserk(dt, 7)
+        assert np.array_equal(pair[0].state_store(), pair[1].state_store())
+    with pytest.raises(R.RfhipError):
+        R.batch_lserk([batch[0], batch[0]], [0.01, 0.01], 2)
