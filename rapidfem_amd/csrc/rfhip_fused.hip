@@ -24,6 +24,9 @@
 // lanes.
 #include <cmath>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "rfhip_impl.hpp"
@@ -3151,24 +3154,31 @@ struct Nfp {
 };
 
 // Blocks of `kernel` (launched with `nt` threads) resident on the whole
-// device at once.
+// device at once, computed once per (kernel, device, block size) and kept
+// (the lookups run on every launch, from any host thread).
 inline int64_t resident_blocks_nt(const void *kernel, int nt) {
-  int dev = 0, cus = 0, per_cu = 0;
+  int dev = 0;
   RF_HIP(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::map<std::tuple<const void *, int, int>, int64_t> cache;
+  const auto key = std::make_tuple(kernel, dev, nt);
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  int cus = 0, per_cu = 0;
   RF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   RF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, 0));
-  return (int64_t)(cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+  const int64_t r = (int64_t)(cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+  std::lock_guard<std::mutex> lock(mu);
+  cache.emplace(key, r);
+  return r;
 }
 
 // Blocks of `kernel` resident on the whole device at once (grid-stride
 // launches use this many).
-inline int64_t resident_blocks(const void *kernel) {
-  int dev = 0, cus = 0, per_cu = 0;
-  RF_HIP(hipGetDevice(&dev));
-  RF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  RF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kFusedWG, 0));
-  return (int64_t)(cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
-}
+inline int64_t resident_blocks(const void *kernel) { return resident_blocks_nt(kernel, kFusedWG); }
 
 // Kernel variant per launch: 0 = k_fused (one node per thread), 1/2 =
 // k_fused2 with 2/1 nodes per thread, 3 = k_fused3 (early neighbour
@@ -3223,9 +3233,7 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
   } else if ((variant == 16 || variant == 17) && a.nbase) {
     const void *kf = variant == 16 ? reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 1>)
                                    : reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 2>);
-    static int64_t resident[2] = {0, 0};
-    int64_t &res = resident[variant - 16];
-    if (res == 0) res = resident_blocks(kf);
+    const int64_t res = resident_blocks(kf);
     const int64_t grid = groups1 < res ? groups1 : res;
     if (grid > 0) {
       if (variant == 16)
@@ -3236,9 +3244,7 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
   } else if ((variant == 22 || variant == 23) && a.nbase) {
     const void *kf = variant == 22 ? reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 1>)
                                    : reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 2>);
-    static int64_t resident[2] = {0, 0};
-    int64_t &res = resident[variant - 22];
-    if (res == 0) res = resident_blocks(kf);
+    const int64_t res = resident_blocks(kf);
     const int64_t epw = 64 / NP;
     const int64_t wgroups = (a.n_elem + epw - 1) / epw;
     const int64_t groups = (wgroups + 3) / 4;  // blocks of 4 waves
@@ -3252,9 +3258,7 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
   } else if ((variant == 20 || variant == 21) && a.nbase && a.dsparse) {
     const void *kf = variant == 20 ? reinterpret_cast<const void *>(&k_fused10<NP, F, CIN, EPI, 1>)
                                    : reinterpret_cast<const void *>(&k_fused10<NP, F, CIN, EPI, 2>);
-    static int64_t resident[2] = {0, 0};
-    int64_t &res = resident[variant - 20];
-    if (res == 0) res = resident_blocks(kf);
+    const int64_t res = resident_blocks(kf);
     const int64_t epb = 64 / NodeGroups<NP>::RPW;
     const int64_t groups = (a.n_elem + epb - 1) / epb;
     const int64_t grid = groups < res ? groups : res;
@@ -3267,9 +3271,7 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
   } else if ((variant == 18 || variant == 19) && a.nbase) {
     const void *kf = variant == 18 ? reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 1, true>)
                                    : reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 2, true>);
-    static int64_t resident[2] = {0, 0};
-    int64_t &res = resident[variant - 18];
-    if (res == 0) res = resident_blocks(kf);
+    const int64_t res = resident_blocks(kf);
     const int64_t grid = groups1 < res ? groups1 : res;
     if (grid > 0) {
       if (variant == 18)
@@ -3278,8 +3280,7 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
         hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2, true>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
     }
   } else if (variant == 15 && a.nbase && NP <= 20) {
-    static int64_t res = 0;
-    if (res == 0) res = resident_blocks_nt(reinterpret_cast<const void *>(&k_fused8<NP, F, CIN, EPI>), Mfma7Cfg<NP>::NT);
+    const int64_t res = resident_blocks_nt(reinterpret_cast<const void *>(&k_fused8<NP, F, CIN, EPI>), Mfma7Cfg<NP>::NT);
     const int64_t groups = (a.n_elem + 15) / 16;
     const int64_t grid = groups < res ? groups : res;
     if (grid > 0)
@@ -3289,9 +3290,7 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
     hipLaunchKernelGGL((k_fused7<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))),
                        dim3(Mfma7Cfg<NP>::NT), 0, s, a);
   } else if (variant == 14) {
-    static int64_t res = 0;
-    if (res == 0)
-      res = resident_blocks(
+    const int64_t res = resident_blocks(
           reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 2, true>));
     const int64_t grid = groups1 < res ? groups1 : res;
     hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 2, true>), dim3((unsigned)grid),
@@ -3302,9 +3301,7 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
     const void *kf = variant == 12
                          ? reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 4>)
                          : reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 2>);
-    static int64_t resident[2] = {0, 0};
-    int64_t &res = resident[variant - 12];
-    if (res == 0) res = resident_blocks(kf);
+    const int64_t res = resident_blocks(kf);
     const int64_t grid = groups1 < res ? groups1 : res;
     if (variant == 12)
       hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 4>), dim3((unsigned)grid),
@@ -3313,18 +3310,14 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
       hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 2>), dim3((unsigned)grid),
                          dim3(kFusedWG), 0, s, a);
   } else if (variant == 9) {
-    static int64_t resident = 0;
-    if (resident == 0)
-      resident = resident_blocks(reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, true>));
+    const int64_t resident = resident_blocks(reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, true>));
     const int64_t cap = 8 * ((groups1 + 7) / 8);
     int64_t grid = (resident / 8) * 8;
     if (grid < 8) grid = 8;
     if (grid > cap) grid = cap;
     hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, true>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
   } else {
-    static int64_t resident = 0;
-    if (resident == 0)
-      resident = resident_blocks(reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false>));
+    const int64_t resident = resident_blocks(reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false>));
     const int64_t grid = groups1 < resident ? groups1 : resident;
     hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
   }
