@@ -9,6 +9,12 @@ benchmark and the parity tests run on.
                 s=0.05, c=(0.4,0.5,0.5), PEC, upwind.
   modes_box     n^3 jittered unit box (55^3*6 = 998,250 tets at full size),
                 N=4, sum of 16 PEC cavity modes with N(0,1) amplitudes.
+  slab_box      unit box graded along x toward a thin slab at a seeded
+                position (cell widths 20:1 between far field and slab),
+                nx x ny x nz cells x 6 tets (40x30x28 = 201,600 tets at full
+                size), N=3, eps_r=10 and sigma=50 in the slab, the pulse
+                E_x of pulse_box centred off the slab: the stiff operator of
+                the Krylov workload.
 """
 from __future__ import annotations
 
@@ -16,7 +22,7 @@ import math
 
 import numpy as np
 
-from .maxwell import MaxwellOperator, structured_box, structured_box_jittered
+from .maxwell import Mesh, MaxwellOperator, structured_box, structured_box_jittered
 
 
 def te101_field(op: MaxwellOperator) -> np.ndarray:
@@ -40,6 +46,35 @@ def pulse_box(nx=60, ny=30, nz=30, order=3, eps_r=4.0, flux_alpha=1.0) -> tuple[
     op = MaxwellOperator(mesh, order, flux_alpha, materials={"eps": eps})
     x = op.node_coords()
     r2 = (x[..., 0] - 0.4) ** 2 + (x[..., 1] - 0.5) ** 2 + (x[..., 2] - 0.5) ** 2
+    y = np.zeros((op.n_elem, op.re.n_nodes, 6))
+    y[..., 0] = np.exp(-r2 / (2 * 0.05 ** 2)) * np.sin(2 * math.pi * 5 * x[..., 0])
+    return op, y.reshape(-1)
+
+
+def slab_box(nx=40, ny=30, nz=28, order=3, seed=5, ratio=20.0, width=0.02, flux_alpha=1.0
+             ) -> tuple[MaxwellOperator, np.ndarray]:
+    rng = np.random.default_rng(seed)
+    x0 = float(rng.uniform(0.35, 0.65))
+    # cell widths along x: 1 far from the slab, 1/ratio at it (Gaussian
+    # profile three slab widths wide), scaled to the unit length
+    xc = (np.arange(nx) + 0.5) / nx
+    w = 1.0 / (1.0 + (ratio - 1.0) * np.exp(-(((xc - x0) / (3.0 * width)) ** 2)))
+    xs = np.concatenate([[0.0], np.cumsum(w)])
+    xs /= xs[-1]
+    base = structured_box(nx, ny, nz, 1.0, 1.0, 1.0)
+    nodes = base.nodes.copy()
+    i = np.rint(nodes[:, 0] * nx).astype(np.int64)
+    nodes[:, 0] = xs[i]
+    mesh = Mesh(nodes, base.tets)
+    cx = nodes[mesh.tets].mean(axis=1)[:, 0]
+    slab = np.abs(cx - x0) <= 0.5 * width + 0.5 * np.diff(xs).min()
+    eps = np.ones((mesh.n_tets, 3))
+    eps[slab] = 10.0
+    sigma = np.where(slab, 50.0, 0.0)
+    op = MaxwellOperator(mesh, order, flux_alpha, materials={"eps": eps, "sigma": sigma})
+    x = op.node_coords()
+    xc0 = x0 - 0.2 if x0 > 0.5 else x0 + 0.2
+    r2 = (x[..., 0] - xc0) ** 2 + (x[..., 1] - 0.5) ** 2 + (x[..., 2] - 0.5) ** 2
     y = np.zeros((op.n_elem, op.re.n_nodes, 6))
     y[..., 0] = np.exp(-r2 / (2 * 0.05 ** 2)) * np.sin(2 * math.pi * 5 * x[..., 0])
     return op, y.reshape(-1)

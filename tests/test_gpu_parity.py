@@ -187,6 +187,34 @@ def test_expmv_matches_oracle(ctx, oracle, form, tol):
     assert err < (1e-3 if form == "plain_f32" else 1e-10)
 
 
+@pytest.mark.parametrize("form", ["plain_f64", "fused_f64", "fused_f64c"])
+def test_expmv_on_graded_slab_matches_oracle(ctx, oracle, form):
+    """The stiff Krylov workload at oracle size: the 5,184-tet N=3 box graded
+    20:1 toward an eps_r 10, sigma 50 slab, built by the package's host
+    builder; t = 3 / rho, m = 40 (the benchmark's step), FP64 Arnoldi on
+    every form."""
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+
+    op, y0 = W.slab_box(12, 9, 8, 3)
+    oop = oracle.Operator(op.mesh.nodes, op.mesh.tets, 3, 1.0, eps=1.0 / op.inv_eps,
+                          sigma=op.sigma_eps[:, 0] / op.inv_eps[:, 0])
+    v = y0 / np.linalg.norm(y0)
+    for _ in range(20):
+        av = oop.apply(v)
+        rho = float(np.linalg.norm(av))
+        v = av / rho
+    t = 3.0 / rho
+    want, wdim = oop.expmv(y0, t, 40)
+    g = R.GpuOperator(ctx, op, form)
+    got = g.expmv(ctx, y0, t, 40)
+    err = rel_l2(got, want)
+    print(f"slab expmv form={form}: t {t:.3e}, dim {g.last_dim}/{wdim}, rel_l2 {err:.3e}", flush=True)
+    assert g.last_dim == wdim
+    assert err < 1e-10
+    assert rel_l2(got, y0) > 1e-3
+
+
 @pytest.mark.parametrize("form", ["plain_f64", "fused_f64"])
 def test_etd_matches_oracle(ctx, oracle, form):
     """GpuOperator::etd_step vs propagator.rs etd_step_into (augmented Arnoldi)."""

@@ -193,6 +193,29 @@ def test_workload_shapes_and_unit_costs():
     assert [round(W.flops_per_dof_stage(p)) for p in (2, 3, 4)] == [166, 253, 379]
 
 
+def test_slab_box_grading_and_materials():
+    """The stiff Krylov workload: cell widths graded 20:1 toward a seeded slab
+    of eps_r 10, sigma 50; the remap keeps the box's extent and conformity."""
+    from rapidfem_amd import workloads as W
+
+    op, y0 = W.slab_box(12, 9, 8, 3)
+    assert op.n_elem == 12 * 9 * 8 * 6 and op.n_dof == y0.size
+    xs = np.unique(np.round(op.mesh.nodes[:, 0], 14))
+    assert xs.size == 13 and xs[0] == 0.0 and abs(xs[-1] - 1.0) < 1e-15
+    w = np.diff(xs)
+    assert 10.0 < w.max() / w.min() <= 20.0 + 1e-9
+    eps = 1.0 / op.inv_eps[:, 0]
+    slab = eps > 1.0
+    assert set(np.round(eps, 12)) == {1.0, 10.0}
+    sigma = op.sigma_eps[:, 0] / op.inv_eps[:, 0]
+    assert np.all(sigma[slab] == 50.0) and np.all(sigma[~slab] == 0.0)
+    # the slab sits where the cells are finest
+    cx = op.mesh.nodes[op.mesh.tets, 0].mean(axis=1)
+    xc = 0.5 * (xs[1:] + xs[:-1])
+    assert abs(cx[slab].mean() - xc[np.argmin(w)]) < 2 * w.max()
+    assert np.linalg.norm(y0) > 0.0
+
+
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
