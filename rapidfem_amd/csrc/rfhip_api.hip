@@ -643,6 +643,7 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
   op->kb_x.alloc(8 * n);
   op->kb_out.alloc(8 * n);
   op->kb_proj.alloc(8 * (kMaxKrylov + 8));
+  op->kb_host.alloc(8 * (2 * kMaxKrylov + 8));
   launch_fill(op->kb_w.p, 8 * n, s);
   const int nblk = krylov_dot_blocks(n);
   op->kb_part.alloc(8 * (size_t)nblk * (kMaxKrylov + 1));
@@ -688,19 +689,40 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
     double ws = 0.0;  // augmented scalar of w (zero row of the augmented matrix)
     const int cols = j + 1;
     double norm2 = 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-      launch_multi_dot(basis, n, cols, w, part, nblk, proj, s);
+    // Classical Gram-Schmidt, two passes. The first pass's update and the
+    // second pass's projections share one sweep over the basis.
+    launch_multi_dot(basis, n, cols, w, part, nblk, proj, s);
+    if (!aug) {
+      // The host only accumulates H here: both passes' projections and the
+      // norm come back asynchronously, one wait per step.
+      double *hp = op->kb_host.as<double>();
+      RF_HIP(hipMemcpyAsync(hp, proj, 8 * cols, hipMemcpyDeviceToHost, s));
+      launch_axpy_dot(w, basis, n, cols, proj, part, nblk, proj, s);
+      RF_HIP(hipMemcpyAsync(hp + kMaxKrylov, proj, 8 * cols, hipMemcpyDeviceToHost, s));
+      launch_axpy_basis_norm(w, basis, n, cols, proj, part, nblk, norm_dev, s);
+      RF_HIP(hipMemcpyAsync(hp + 2 * kMaxKrylov, norm_dev, 8, hipMemcpyDeviceToHost, s));
+      sync(op);
+      for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < cols; ++i) H[(size_t)i * max_dim + j] += hp[pass * kMaxKrylov + i];
+      norm2 = hp[2 * kMaxKrylov];
+    }
+    // Augmented: each pass's projections take the augmented scalar's share
+    // on the host before the update uses them.
+    for (int pass = 0; aug && pass < 2; ++pass) {
       RF_HIP(hipMemcpyAsync(pj.data(), proj, 8 * cols, hipMemcpyDeviceToHost, s));
       sync(op);
       double dscal = 0.0;
       for (int i = 0; i < cols; ++i) {
-        if (aug) pj[i] += sc[i] * ws;
+        pj[i] += sc[i] * ws;
         H[(size_t)i * max_dim + j] += pj[i];
         dscal += pj[i] * sc[i];
       }
-      if (aug) RF_HIP(hipMemcpyAsync(proj, pj.data(), 8 * cols, hipMemcpyHostToDevice, s));
-      launch_axpy_basis_norm(w, basis, n, cols, proj, part, nblk, norm_dev, s);
-      if (aug) ws -= dscal;
+      RF_HIP(hipMemcpyAsync(proj, pj.data(), 8 * cols, hipMemcpyHostToDevice, s));
+      if (pass == 0)
+        launch_axpy_dot(w, basis, n, cols, proj, part, nblk, proj, s);
+      else
+        launch_axpy_basis_norm(w, basis, n, cols, proj, part, nblk, norm_dev, s);
+      ws -= dscal;
       if (pass == 1) {
         RF_HIP(hipMemcpyAsync(&norm2, norm_dev, 8, hipMemcpyDeviceToHost, s));
         sync(op);

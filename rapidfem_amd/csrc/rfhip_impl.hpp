@@ -65,6 +65,36 @@ struct DevBuf {
   }
 };
 
+// Page-locked host memory (hipHostMalloc), for device-to-host copies that
+// must stay asynchronous.
+struct HostPinned {
+  void *p = nullptr;
+  size_t bytes = 0;
+  HostPinned() = default;
+  HostPinned(const HostPinned &) = delete;
+  HostPinned &operator=(const HostPinned &) = delete;
+  ~HostPinned() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t nbytes) {
+    if (nbytes <= bytes && p) return;
+    release();
+    hipError_t e = hipHostMalloc(&p, nbytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error{RFHIP_ENOMEM, "hipHostMalloc of " + std::to_string(nbytes) + " bytes failed: " + hipGetErrorString(e)};
+    }
+    bytes = nbytes;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
 template <class T>
 void upload(DevBuf &b, const T *host, size_t count, hipStream_t s) {
   b.alloc(sizeof(T) * count);
@@ -155,6 +185,7 @@ struct rfhip_op {
   // KCL / Krylov / scratch buffers
   rfhip::DevBuf kst, kerr, kbak, red, aos;
   rfhip::DevBuf kb_basis, kb_w, kb_proj, kb_part, kb_out, kb_x;
+  rfhip::HostPinned kb_host;  // Krylov projections and norm, copied without a sync per pass
   rfhip::DevBuf probe_idx, probe_rows;  // device-side probe recording
   // port modal extraction: host-side per-port entries, device copies
   struct PortEntries {
@@ -264,6 +295,11 @@ void launch_multi_dot(const double *basis, int64_t n, int cols, const double *w,
                       double *proj, hipStream_t s);
 void launch_axpy_basis_norm(double *w, const double *basis, int64_t n, int cols, const double *proj,
                             double *partials, int nblk, double *norm2, hipStream_t s);
+// w -= basis[:, :cols] proj, then proj_out = basis[:, :cols]^T w (the updated
+// w): a CGS2 first-pass update fused with the second pass's projections.
+// proj_out may be proj: it is written after every block has read proj.
+void launch_axpy_dot(double *w, const double *basis, int64_t n, int cols, const double *proj, double *partials,
+                     int nblk, double *proj_out, hipStream_t s);
 void launch_scale(const double *w, double *dst, int64_t n, double inv, hipStream_t s);
 void launch_lincomb(const double *basis, int64_t n, int dim, const double *coef, double *out, hipStream_t s);
 void launch_axpy(double *w, const double *x, double a, int64_t n, hipStream_t s);

@@ -183,9 +183,14 @@ __global__ void __launch_bounds__(kBS) k_finish_sum(const double *__restrict__ p
 constexpr int kMaxCols = 64;
 
 // partials[b*cols + i] = sum over block b's slice of basis_i . w. Columns
-// are taken eight at a time so w is read ceil(cols/8) times, the basis once.
+// are taken sixteen at a time so w is read ceil(cols/16) times, the basis
+// once. The streaming updates take their columns eight at a time.
 constexpr int kDotTile = 8;
-// Per-block partial dot products of w with basis columns 0..cols-1, kDotTile
+constexpr int kDotTileM = 16;
+// Largest column count k_axpy_dot holds in registers; above it the update
+// and the products run as two sweeps.
+constexpr int kAxpyDotMaxCols = 48;
+// Per-block partial dot products of w with basis columns 0..cols-1, kDotTileM
 // columns per sweep over the block's contiguous range. With VEC (n even, so
 // every column starts 16-byte aligned) each lane reads two consecutive
 // entries per 16-byte load.
@@ -197,15 +202,15 @@ __global__ void __launch_bounds__(kBS) k_multi_dot(const double *__restrict__ ba
   const int64_t chunk = (m + gridDim.x - 1) / gridDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * chunk;
   const int64_t i1 = i0 + chunk < m ? i0 + chunk : m;
-  for (int c0 = 0; c0 < cols; c0 += kDotTile) {
-    double acc[kDotTile];
+  for (int c0 = 0; c0 < cols; c0 += kDotTileM) {
+    double acc[kDotTileM];
 #pragma unroll
-    for (int t = 0; t < kDotTile; ++t) acc[t] = 0.0;
+    for (int t = 0; t < kDotTileM; ++t) acc[t] = 0.0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kBS) {
       if constexpr (VEC) {
         const double2 wi = reinterpret_cast<const double2 *>(w)[i];
 #pragma unroll
-        for (int t = 0; t < kDotTile; ++t)
+        for (int t = 0; t < kDotTileM; ++t)
           if (c0 + t < cols) {
             const double2 b = reinterpret_cast<const double2 *>(basis + (int64_t)(c0 + t) * n)[i];
             acc[t] += b.x * wi.x + b.y * wi.y;
@@ -213,12 +218,12 @@ __global__ void __launch_bounds__(kBS) k_multi_dot(const double *__restrict__ ba
       } else {
         const double wi = w[i];
 #pragma unroll
-        for (int t = 0; t < kDotTile; ++t)
+        for (int t = 0; t < kDotTileM; ++t)
           if (c0 + t < cols) acc[t] += basis[(int64_t)(c0 + t) * n + i] * wi;
       }
     }
 #pragma unroll
-    for (int t = 0; t < kDotTile; ++t) {
+    for (int t = 0; t < kDotTileM; ++t) {
       if (c0 + t >= cols) break;
       const double s = block_sum(acc[t], sh);
       if (threadIdx.x == 0) part[(int64_t)blockIdx.x * cols + c0 + t] = s;
@@ -247,24 +252,86 @@ __global__ void __launch_bounds__(kBS) k_axpy_basis_norm(double *__restrict__ w,
   double acc = 0.0;
   const int64_t m = VEC ? n / 2 : n;
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < m; i += (int64_t)gridDim.x * kBS) {
+    // Columns in tiles of kDotTile: the tile's loads issue together, then
+    // the subtractions run in column order (the same order as one at a time).
     if constexpr (VEC) {
       double2 wi = reinterpret_cast<const double2 *>(w)[i];
-      for (int c = 0; c < cols; ++c) {
-        const double2 b = reinterpret_cast<const double2 *>(basis + (int64_t)c * n)[i];
-        wi.x -= sp[c] * b.x;
-        wi.y -= sp[c] * b.y;
+      for (int c0 = 0; c0 < cols; c0 += kDotTile) {
+        double2 b[kDotTile];
+#pragma unroll
+        for (int t = 0; t < kDotTile; ++t)
+          if (c0 + t < cols) b[t] = reinterpret_cast<const double2 *>(basis + (int64_t)(c0 + t) * n)[i];
+#pragma unroll
+        for (int t = 0; t < kDotTile; ++t)
+          if (c0 + t < cols) {
+            wi.x -= sp[c0 + t] * b[t].x;
+            wi.y -= sp[c0 + t] * b[t].y;
+          }
       }
       reinterpret_cast<double2 *>(w)[i] = wi;
       acc += wi.x * wi.x + wi.y * wi.y;
     } else {
       double wi = w[i];
-      for (int c = 0; c < cols; ++c) wi -= sp[c] * basis[(int64_t)c * n + i];
+      for (int c0 = 0; c0 < cols; c0 += kDotTile) {
+        double b[kDotTile];
+#pragma unroll
+        for (int t = 0; t < kDotTile; ++t)
+          if (c0 + t < cols) b[t] = basis[(int64_t)(c0 + t) * n + i];
+#pragma unroll
+        for (int t = 0; t < kDotTile; ++t)
+          if (c0 + t < cols) wi -= sp[c0 + t] * b[t];
+      }
       w[i] = wi;
       acc += wi * wi;
     }
   }
   const double t = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+// The first pass's update and the second pass's projections of classical
+// Gram-Schmidt with reorthogonalisation in one sweep over the basis:
+// w -= sum_c proj[c] basis_c, then the per-block partials of basis_c . w
+// (the updated w) over contiguous per-block slices as in k_multi_dot. Each
+// lane keeps its entry of every column in registers between the update and
+// the products, so the basis is read once instead of twice.
+// MAXC >= cols is a multiple of 4 and the body has no branch per column:
+// columns cols..MAXC-1 reload column cols-1 (a cache hit) with coefficient
+// 0, so every load issues before the first use and the waits count down.
+template <int MAXC>
+__global__ void __launch_bounds__(kBS) k_axpy_dot(double *__restrict__ w, const double *__restrict__ basis,
+                                                  int64_t n, int cols, const double *__restrict__ proj,
+                                                  double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  __shared__ double sp[MAXC];
+  for (int c = threadIdx.x; c < MAXC; c += kBS) sp[c] = c < cols ? proj[c] : 0.0;
+  __syncthreads();
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk;
+  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+  double acc[MAXC];
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c) acc[c] = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBS) {
+    double b[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) b[c] = basis[(int64_t)(c < cols ? c : cols - 1) * n + i];
+    double wi = w[i];
+    // z is 0 (i < n < 2^62) but depends on i, so the coefficients are read
+    // from LDS (a broadcast) at each use instead of being held in two VGPRs
+    // per column across the loop.
+    const int z = (int)(i >> 62);
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) wi -= sp[c + z] * b[c];
+    w[i] = wi;
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) acc[c] += b[c] * wi;
+  }
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c) {
+    if (c >= cols) break;
+    const double t = block_sum(acc[c], sh);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * cols + c] = t;
+  }
 }
 __global__ void __launch_bounds__(kBS) k_scale(const double *__restrict__ w, double *__restrict__ d, int64_t n,
                                                double inv) {
@@ -277,7 +344,15 @@ __global__ void __launch_bounds__(kBS) k_lincomb(const double *__restrict__ basi
   __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
     double acc = 0.0;
-    for (int c = 0; c < dim; ++c) acc += sc[c] * basis[(int64_t)c * n + i];
+    for (int c0 = 0; c0 < dim; c0 += kDotTile) {
+      double b[kDotTile];
+#pragma unroll
+      for (int t = 0; t < kDotTile; ++t)
+        if (c0 + t < dim) b[t] = basis[(int64_t)(c0 + t) * n + i];
+#pragma unroll
+      for (int t = 0; t < kDotTile; ++t)
+        if (c0 + t < dim) acc += sc[c0 + t] * b[t];
+    }
     out[i] = acc;
   }
 }
@@ -454,7 +529,9 @@ void launch_weighted_err(bool f64, const void *yo, const void *yn, const void *e
   RF_HIP(hipGetLastError());
 }
 
-int krylov_dot_blocks(int64_t n) { return (int)grid_for(n, 2048); }
+// 1024 blocks: k_multi_dot streams 5.7 TB/s over 1024 blocks and 5.3 over
+// 2048 on a 24.2M-entry basis; the other sweeps are flat between the two.
+int krylov_dot_blocks(int64_t n) { return (int)grid_for(n, 1024); }
 
 void launch_multi_dot(const double *basis, int64_t n, int cols, const double *w, double *partials, int nblk,
                       double *proj, hipStream_t s) {
@@ -474,6 +551,27 @@ void launch_axpy_basis_norm(double *w, const double *basis, int64_t n, int cols,
   else
     hipLaunchKernelGGL(k_axpy_basis_norm<false>, dim3(nblk), dim3(kBS), 0, s, w, basis, n, cols, proj, partials);
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, partials, nblk, norm2);
+  RF_HIP(hipGetLastError());
+}
+void launch_axpy_dot(double *w, const double *basis, int64_t n, int cols, const double *proj, double *partials,
+                     int nblk, double *proj_out, hipStream_t s) {
+  RF_REQUIRE(cols >= 1 && cols <= kMaxCols, "Krylov dimension above 64");
+  if (cols > kAxpyDotMaxCols) {
+    launch_axpy_basis_norm(w, basis, n, cols, proj, partials, nblk, proj_out, s);
+    launch_multi_dot(basis, n, cols, w, partials, nblk, proj_out, s);
+    return;
+  }
+  const dim3 g(nblk), b(kBS);
+  switch ((cols + 3) / 4) {
+#define RF_AXPY_DOT(q)                                                                          \
+  case q:                                                                                       \
+    hipLaunchKernelGGL(k_axpy_dot<4 * q>, g, b, 0, s, w, basis, n, cols, proj, partials);       \
+    break;
+    RF_AXPY_DOT(1) RF_AXPY_DOT(2) RF_AXPY_DOT(3) RF_AXPY_DOT(4) RF_AXPY_DOT(5) RF_AXPY_DOT(6)
+    RF_AXPY_DOT(7) RF_AXPY_DOT(8) RF_AXPY_DOT(9) RF_AXPY_DOT(10) RF_AXPY_DOT(11) RF_AXPY_DOT(12)
+#undef RF_AXPY_DOT
+  }
+  hipLaunchKernelGGL(k_dot_finish, dim3(cols), dim3(kBS), 0, s, partials, nblk, cols, proj_out);
   RF_HIP(hipGetLastError());
 }
 void launch_scale(const double *w, double *dst, int64_t n, double inv, hipStream_t s) {

@@ -187,6 +187,21 @@ def test_expmv_matches_oracle(ctx, oracle, form, tol):
     assert err < (1e-3 if form == "plain_f32" else 1e-10)
 
 
+@pytest.mark.parametrize("m", [3, 9, 48, 56])
+def test_expmv_every_gram_schmidt_width_matches_oracle(ctx, oracle, m):
+    """Arnoldi dimensions that cross every column bucket of the fused
+    Gram-Schmidt sweep (4-column steps up to 48) and the two-sweep form
+    above 48; no early stop, so the basis reaches m columns."""
+    oop, y0 = _te101(oracle, 3, 2)
+    g = _gop(ctx, oop, "fused_f64")
+    want, wdim = oop.expmv(y0, 0.3, m, 0.0)
+    got = g.expmv(ctx, y0, 0.3, m, 0.0)
+    err = rel_l2(got, want)
+    print(f"expmv m={m}: dim gpu {g.last_dim} oracle {wdim} rel_l2 {err:.3e}", flush=True)
+    assert g.last_dim == wdim == m
+    assert err < 1e-10
+
+
 @pytest.mark.parametrize("form", ["plain_f64", "fused_f64", "fused_f64c"])
 def test_expmv_on_graded_slab_matches_oracle(ctx, oracle, form):
     """The stiff Krylov workload at oracle size: the 5,184-tet N=3 box graded
