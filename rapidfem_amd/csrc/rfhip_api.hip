@@ -585,17 +585,33 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
     const double gv = kind == RFHIP_DRIVE_FREE ? 0.0 : (double)g[fr];
     while (t_rel < dt) {
       const double ht = std::min(h, dt - t_rel);
-      launch_copy(op->kbak.p, ys, (int64_t)(w * n), s);
       // explicit_adaptive.rs:107-149. After each stage's apply one pass
       // does f = h k, y += b f, err = (or +=) (bhat - b) f and, but for the
-      // last stage, builds the next stage's input y + (a - b) f.
+      // last stage, builds the next stage's input y + (a - b) f. The first
+      // pass also saves the starting y (kbak); the last one reduces the
+      // error norm (session.rs:160-171) against it.
       for (int sg = 0; sg < 5; ++sg) {
         rhs_into(op, sg == 0 ? ys : op->kst.p, op->k.p, kind, dof, gv);
-        const double amb = sg < 4 ? kKclA[sg] - kKclB[sg] : 0.0;
-        launch_kcl_update(f64, ys, op->kerr.p, op->kst.p, op->k.p, ht, kKclB[sg], kKclBhat[sg] - kKclB[sg], amb,
-                          sg == 0, sg < 4, n, s);
+        KclStage a{};
+        a.y = ys;
+        a.err = op->kerr.p;
+        a.st = op->kst.p;
+        a.bak = op->kbak.p;
+        a.k = op->k.p;
+        a.dt = ht;
+        a.b = kKclB[sg];
+        a.ew = kKclBhat[sg] - kKclB[sg];
+        a.amb = sg < 4 ? kKclA[sg] - kKclB[sg] : 0.0;
+        a.first = sg == 0;
+        a.next = sg < 4;
+        a.n = n;
+        a.atol = ctl.atol;
+        a.rtol = ctl.rtol;
+        a.partials = part;
+        a.nblk = nblk;
+        a.err_sum = res;
+        launch_kcl_update(f64, a, s);
       }
-      launch_weighted_err(f64, op->kbak.p, ys, op->kerr.p, ctl.atol, ctl.rtol, n, part, nblk, res, s);
       double ssum = 0.0;
       RF_HIP(hipMemcpyAsync(&ssum, res, 8, hipMemcpyDeviceToHost, s));
       sync(op);

@@ -279,14 +279,24 @@ void launch_port_proj(const void *y, const uint16_t *lo, int kind, const int64_t
 void launch_gather(const void *y, const uint16_t *lo, int kind, const int64_t *idx, int64_t n, double *out,
                    hipStream_t s);
 
-// KCL vector kernels on T = float or double (is_f64 selects).
-void launch_kcl_update(bool f64, void *y, void *err, void *st, const void *k, double dt, double b, double ew,
-                       double amb, bool first, bool next, int64_t n, hipStream_t s);
+// One KCL stage's register update on T = float or double (f64 selects):
+// f = dt k, y += b f, err = ew f (first) or err += ew f, st = y + amb f
+// (next). The first stage saves the starting y into bak; the last (no next)
+// reduces sum((err/(atol + rtol*max(|bak|,|y|)))^2) into *err_sum (device
+// double, +inf for a non-finite y) through nblk partials.
+struct KclStage {
+  void *y, *err, *st, *bak;
+  const void *k;
+  double dt, b, ew, amb;
+  bool first, next;
+  int64_t n;
+  double atol, rtol;
+  double *partials;
+  int nblk;
+  double *err_sum;
+};
+void launch_kcl_update(bool f64, const KclStage &a, hipStream_t s);
 void launch_copy(void *dst, const void *src, int64_t nbytes, hipStream_t s);
-// sum((err/(atol + rtol*max|yo|,|yn|))^2) into *out (device double), with a
-// non-finite y_new entry flagged as +inf.
-void launch_weighted_err(bool f64, const void *yo, const void *yn, const void *err, double atol, double rtol,
-                         int64_t n, double *partials, int nblk, double *out, hipStream_t s);
 int weighted_err_blocks(int64_t n);
 
 // Krylov (FP64).

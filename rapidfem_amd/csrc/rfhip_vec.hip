@@ -119,19 +119,46 @@ __global__ void __launch_bounds__(kBS) k_copy(uint32_t *__restrict__ d, const ui
 
 // ---- KCL RK4(3)5[2R+]C (explicit_adaptive.rs:84-152) -------------------
 
+__device__ __forceinline__ double block_sum(double v, double *sh);
+
+// One entry of the controller's error norm (session.rs:160-171):
+// (err / (atol + rtol max(|y_old|, |y_new|)))^2, +inf for a non-finite y_new.
+__device__ __forceinline__ double werr_term(double yo, double yn, double e, double atol, double rtol) {
+  if (!isfinite(yn)) return INFINITY;
+  const double r = e / (atol + rtol * fmax(fabs(yo), fabs(yn)));
+  return r * r;
+}
+
 // After stage s of a KCL substep: f = h k, y += b f, err = e f on the first
 // stage and err += e f after it, and, when NEXT, the next stage's input
 // st = y + (a - b) f with the updated y (the 2R update and the next stage's
-// build in one pass; f itself is not stored).
+// build in one pass; f itself is not stored). The first stage also saves
+// the substep's starting y into bak (restored on a rejection); the last
+// (no NEXT) also reads bak and leaves the block's partial sum of the error
+// norm's entries in part[blockIdx.x].
 template <typename T, bool FIRST, bool NEXT>
 __global__ void __launch_bounds__(kBS) k_kcl_update(T *__restrict__ y, T *__restrict__ err, T *__restrict__ st,
-                                                    const T *__restrict__ k, T dt, T b, T ew, T amb, int64_t n) {
+                                                    const T *__restrict__ k, T *__restrict__ bak, T dt, T b, T ew,
+                                                    T amb, double atol, double rtol, int64_t n,
+                                                    double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    const T yo = y[i];
     const T f = dt * k[i];
-    const T yn = y[i] + b * f;
+    const T yn = yo + b * f;
     y[i] = yn;
-    err[i] = FIRST ? ew * f : err[i] + ew * f;
-    if (NEXT) st[i] = yn + amb * f;
+    if (FIRST) bak[i] = yo;
+    const T en = FIRST ? ew * f : err[i] + ew * f;
+    err[i] = en;
+    if (NEXT)
+      st[i] = yn + amb * f;
+    else
+      acc += werr_term((double)bak[i], (double)yn, (double)en, atol, rtol);
+  }
+  if (!NEXT) {
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
   }
 }
 
@@ -146,27 +173,6 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
     for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += sh[q];
   __syncthreads();
   return t;  // valid on thread 0
-}
-
-// Per-block sum of (err / (atol + rtol*max(|yo|, |yn|)))^2; a non-finite y_new
-// makes the block report +inf (session.rs:160-171 returns INFINITY).
-template <typename T>
-__global__ void __launch_bounds__(kBS) k_werr(const T *__restrict__ yo, const T *__restrict__ yn,
-                                              const T *__restrict__ er, double atol, double rtol, int64_t n,
-                                              double *__restrict__ part) {
-  __shared__ double sh[kBS / 64];
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
-    const double b = (double)yn[i];
-    if (!isfinite(b)) {
-      acc = INFINITY;
-      continue;
-    }
-    const double r = (double)er[i] / (atol + rtol * fmax(fabs((double)yo[i]), fabs(b)));
-    acc += r * r;
-  }
-  const double t = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 // Sums nblk partials into out[0] with one block (fixed order: deterministic).
@@ -458,76 +464,95 @@ void launch_copy(void *dst, const void *src, int64_t nbytes, hipStream_t s) {
 template <bool FIRST, bool NEXT>
 __global__ void __launch_bounds__(kBS) k_kcl_update2(double2 *__restrict__ y, double2 *__restrict__ err,
                                                      double2 *__restrict__ st, const double2 *__restrict__ k,
-                                                     double dt, double b, double ew, double amb, int64_t m) {
+                                                     double2 *__restrict__ bak, double dt, double b, double ew,
+                                                     double amb, double atol, double rtol, int64_t m,
+                                                     double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < m; i += (int64_t)gridDim.x * kBS) {
     const double2 kk = k[i], yo = y[i];
     const double f0 = dt * kk.x, f1 = dt * kk.y;
     const double2 yn = make_double2(yo.x + b * f0, yo.y + b * f1);
     y[i] = yn;
+    if (FIRST) bak[i] = yo;
+    double2 en;
     if (FIRST) {
-      err[i] = make_double2(ew * f0, ew * f1);
+      en = make_double2(ew * f0, ew * f1);
     } else {
       const double2 e = err[i];
-      err[i] = make_double2(e.x + ew * f0, e.y + ew * f1);
+      en = make_double2(e.x + ew * f0, e.y + ew * f1);
     }
-    if (NEXT) st[i] = make_double2(yn.x + amb * f0, yn.y + amb * f1);
+    err[i] = en;
+    if (NEXT) {
+      st[i] = make_double2(yn.x + amb * f0, yn.y + amb * f1);
+    } else {
+      const double2 bo = bak[i];
+      acc += werr_term(bo.x, yn.x, en.x, atol, rtol);
+      acc += werr_term(bo.y, yn.y, en.y, atol, rtol);
+    }
+  }
+  if (!NEXT) {
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
   }
 }
 
 template <typename T>
-void kcl_update_t(void *y, void *err, void *st, const void *k, double dt, double b, double ew, double amb, bool first,
-                  bool next, int64_t n, hipStream_t s) {
-  if (std::is_same<T, double>::value && n % 2 == 0) {
-    double2 *yy = static_cast<double2 *>(y), *ee = static_cast<double2 *>(err), *ss = static_cast<double2 *>(st);
-    const double2 *kk = static_cast<const double2 *>(k);
-    const int64_t m = n / 2;
-    const dim3 g(grid_for(m)), bs(kBS);
-    if (first && next)
-      hipLaunchKernelGGL((k_kcl_update2<true, true>), g, bs, 0, s, yy, ee, ss, kk, dt, b, ew, amb, m);
-    else if (next)
-      hipLaunchKernelGGL((k_kcl_update2<false, true>), g, bs, 0, s, yy, ee, ss, kk, dt, b, ew, amb, m);
-    else if (first)
-      hipLaunchKernelGGL((k_kcl_update2<true, false>), g, bs, 0, s, yy, ee, ss, kk, dt, b, ew, amb, m);
+void kcl_update_t(const KclStage &a, hipStream_t s) {
+  // the last stage writes one error partial per block: its grid is nblk
+  auto grid = [&](int64_t items) { return a.next ? dim3(grid_for(items)) : dim3((unsigned)a.nblk); };
+  const dim3 bs(kBS);
+  if (std::is_same<T, double>::value && a.n % 2 == 0) {
+    double2 *yy = static_cast<double2 *>(a.y), *ee = static_cast<double2 *>(a.err);
+    double2 *ss = static_cast<double2 *>(a.st), *bb = static_cast<double2 *>(a.bak);
+    const double2 *kk = static_cast<const double2 *>(a.k);
+    const int64_t m = a.n / 2;
+    const dim3 g = grid(m);
+    if (a.first && a.next)
+      hipLaunchKernelGGL((k_kcl_update2<true, true>), g, bs, 0, s, yy, ee, ss, kk, bb, a.dt, a.b, a.ew, a.amb,
+                         a.atol, a.rtol, m, a.partials);
+    else if (a.next)
+      hipLaunchKernelGGL((k_kcl_update2<false, true>), g, bs, 0, s, yy, ee, ss, kk, bb, a.dt, a.b, a.ew, a.amb,
+                         a.atol, a.rtol, m, a.partials);
+    else if (a.first)
+      hipLaunchKernelGGL((k_kcl_update2<true, false>), g, bs, 0, s, yy, ee, ss, kk, bb, a.dt, a.b, a.ew, a.amb,
+                         a.atol, a.rtol, m, a.partials);
     else
-      hipLaunchKernelGGL((k_kcl_update2<false, false>), g, bs, 0, s, yy, ee, ss, kk, dt, b, ew, amb, m);
-    RF_HIP(hipGetLastError());
-    return;
+      hipLaunchKernelGGL((k_kcl_update2<false, false>), g, bs, 0, s, yy, ee, ss, kk, bb, a.dt, a.b, a.ew, a.amb,
+                         a.atol, a.rtol, m, a.partials);
+  } else {
+    T *yy = static_cast<T *>(a.y), *ee = static_cast<T *>(a.err), *ss = static_cast<T *>(a.st);
+    T *bb = static_cast<T *>(a.bak);
+    const T *kk = static_cast<const T *>(a.k);
+    const dim3 g = grid(a.n);
+    const T dt = (T)a.dt, b = (T)a.b, ew = (T)a.ew, amb = (T)a.amb;
+    if (a.first && a.next)
+      hipLaunchKernelGGL((k_kcl_update<T, true, true>), g, bs, 0, s, yy, ee, ss, kk, bb, dt, b, ew, amb, a.atol,
+                         a.rtol, a.n, a.partials);
+    else if (a.next)
+      hipLaunchKernelGGL((k_kcl_update<T, false, true>), g, bs, 0, s, yy, ee, ss, kk, bb, dt, b, ew, amb, a.atol,
+                         a.rtol, a.n, a.partials);
+    else if (a.first)
+      hipLaunchKernelGGL((k_kcl_update<T, true, false>), g, bs, 0, s, yy, ee, ss, kk, bb, dt, b, ew, amb, a.atol,
+                         a.rtol, a.n, a.partials);
+    else
+      hipLaunchKernelGGL((k_kcl_update<T, false, false>), g, bs, 0, s, yy, ee, ss, kk, bb, dt, b, ew, amb, a.atol,
+                         a.rtol, a.n, a.partials);
   }
-  T *yy = static_cast<T *>(y), *ee = static_cast<T *>(err), *ss = static_cast<T *>(st);
-  const T *kk = static_cast<const T *>(k);
-  const dim3 g(grid_for(n)), bs(kBS);
-  if (first && next)
-    hipLaunchKernelGGL((k_kcl_update<T, true, true>), g, bs, 0, s, yy, ee, ss, kk, (T)dt, (T)b, (T)ew, (T)amb, n);
-  else if (next)
-    hipLaunchKernelGGL((k_kcl_update<T, false, true>), g, bs, 0, s, yy, ee, ss, kk, (T)dt, (T)b, (T)ew, (T)amb, n);
-  else if (first)
-    hipLaunchKernelGGL((k_kcl_update<T, true, false>), g, bs, 0, s, yy, ee, ss, kk, (T)dt, (T)b, (T)ew, (T)amb, n);
-  else
-    hipLaunchKernelGGL((k_kcl_update<T, false, false>), g, bs, 0, s, yy, ee, ss, kk, (T)dt, (T)b, (T)ew, (T)amb, n);
+  if (!a.next) hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, a.partials, a.nblk, a.err_sum);
   RF_HIP(hipGetLastError());
 }
 
-void launch_kcl_update(bool f64, void *y, void *err, void *st, const void *k, double dt, double b, double ew,
-                       double amb, bool first, bool next, int64_t n, hipStream_t s) {
+void launch_kcl_update(bool f64, const KclStage &a, hipStream_t s) {
+  RF_REQUIRE(a.next || (a.nblk >= 1 && a.partials && a.err_sum), "KCL last stage needs the error partials");
   if (f64)
-    kcl_update_t<double>(y, err, st, k, dt, b, ew, amb, first, next, n, s);
+    kcl_update_t<double>(a, s);
   else
-    kcl_update_t<float>(y, err, st, k, dt, b, ew, amb, first, next, n, s);
+    kcl_update_t<float>(a, s);
 }
 
 int weighted_err_blocks(int64_t n) { return (int)grid_for(n, 2048); }
 
-void launch_weighted_err(bool f64, const void *yo, const void *yn, const void *err, double atol, double rtol,
-                         int64_t n, double *partials, int nblk, double *out, hipStream_t s) {
-  if (f64)
-    hipLaunchKernelGGL(k_werr<double>, dim3(nblk), dim3(kBS), 0, s, (const double *)yo, (const double *)yn,
-                       (const double *)err, atol, rtol, n, partials);
-  else
-    hipLaunchKernelGGL(k_werr<float>, dim3(nblk), dim3(kBS), 0, s, (const float *)yo, (const float *)yn,
-                       (const float *)err, atol, rtol, n, partials);
-  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, partials, nblk, out);
-  RF_HIP(hipGetLastError());
-}
 
 // 1024 blocks: k_multi_dot streams 5.7 TB/s over 1024 blocks and 5.3 over
 // 2048 on a 24.2M-entry basis; the other sweeps are flat between the two.
