@@ -110,13 +110,17 @@ def krylov_bench(args) -> int:
         v = av / rho
     t = 3.0 / rho
     log(f"krylov: {op.n_elem} tets, N={args.order}, n={n}, m={m}, rho {rho:.4e}, t={t:.4e}")
-    times = []
+    times, walls = [], []
     for r in range(4):
+        t0 = time.perf_counter()
         g.expmv(ctx, y0, t, m)
-        log(f"  rep {r}: dim {g.last_dim}, device part {g.last_call_ms:.2f} ms")
+        wall = (time.perf_counter() - t0) * 1e3
+        log(f"  rep {r}: dim {g.last_dim}, device part {g.last_call_ms:.2f} ms, call {wall:.1f} ms")
         if r > 0:
             times.append(g.last_call_ms)
+            walls.append(wall)
     ms = float(np.median(times))
+    wall = float(np.median(walls))
     dim = g.last_dim
     # algorithmic HBM bytes of the whole call: per step j (cols = j + 1 basis
     # vectors) the apply, three sweeps of the two-pass Gram-Schmidt (the
@@ -135,6 +139,9 @@ def krylov_bench(args) -> int:
     out = {"metric": "Krylov exponential (expmv, CGS2 Arnoldi) DOF-steps/s", "value": n * dim / (ms * 1e-3),
            "unit": "Arnoldi DOF-steps/s", "n_gpus": 1, "reps": len(times), "higher_is_better": True,
            "dtype": "f64", "data": "synthetic", "ms_per_call": ms, "ms_per_arnoldi_step": ms / dim,
+           "host_inclusive": {"ms_per_call": wall, "value": n * dim / (wall * 1e-3),
+                              "note": "the whole call: v uploaded and the result downloaded as f64 "
+                                      "(194 MB each way) with their layout conversions"},
            "config": {"workload": f"slab box {nx}x{ny}x{nz} x 6 tets graded 20:1 to an eps_r 10, sigma 50 slab, "
                                   f"N={args.order}, m={dim}, t=3/rho", "n_elem": op.n_elem, "n": n, "t": t,
                       "variant": variant},
@@ -185,19 +192,26 @@ def kcl_bench(args) -> int:
         f"stage kernel variant {variant}")
     runs = []
     for r in range(3):
+        t0 = time.perf_counter()
         _, st = g.transient_kcl(ctx, y0, frame, 2, precision="f64")
+        wall = (time.perf_counter() - t0) * 1e3
         ms = g.last_call_ms
         sub = st.accepted + st.rejected
-        log(f"  rep {r}: {st.accepted} accepted + {st.rejected} rejected substeps, device part {ms:.2f} ms")
+        log(f"  rep {r}: {st.accepted} accepted + {st.rejected} rejected substeps, device part {ms:.2f} ms, "
+            f"call {wall:.1f} ms")
         if r > 0:
-            runs.append((ms, sub, st.accepted, st.rejected))
+            runs.append((ms, sub, st.accepted, st.rejected, wall))
     ms = float(np.median([x[0] for x in runs]))
+    wall = float(np.median([x[4] for x in runs]))
     sub, acc, rej = runs[-1][1], runs[-1][2], runs[-1][3]
     value = n * 5 * sub / (ms * 1e-3)
     out = {"metric": "KCL adaptive stepper DOF-stage/s (RK4(3)5[2R+]C + PI controller)", "value": value,
            "unit": "DOF-stage/s", "n_gpus": 1, "reps": len(runs), "higher_is_better": True, "dtype": "f64",
            "data": "synthetic", "ms_per_call": ms, "substeps": sub, "accepted": acc, "rejected": rej,
            "ms_per_substep": ms / max(sub, 1),
+           "host_inclusive": {"ms_per_call": wall, "value": n * 5 * sub / (wall * 1e-3),
+                              "note": "the whole call: state upload, layout conversions and the 3 frames "
+                                      "downloaded as f64 (311 MB each)"},
            "config": {"workload": f"pulse box {nx}x{ny}x{nz} x 6 tets, N={args.order}, 2 frames of 8 CFL steps",
                       "n_elem": op.n_elem, "n": n, "form": "fused_f64", "variant": variant}}
     if not args.no_cpu:
@@ -350,10 +364,20 @@ def sparse_export(args) -> int:
             runs.append((c.count_ms + c.fill_ms, c.count_ms, c.fill_ms, wall))
     ms, cms, fms, wall = (float(np.median([x[k] for x in runs])) for k in range(4))
     nnz, n = c.nnz(), c.n
+    del c
+    t0 = time.perf_counter()
+    ch = R.assemble_sparse(ctx, op, download=True)
+    wall_dl = time.perf_counter() - t0
+    host_bytes = ch.row_ptr.nbytes + ch.col_idx.nbytes + ch.values.nbytes
+    log(f"  with the CSR downloaded: {wall_dl * 1e3:.1f} ms, {host_bytes / 1e9:.2f} GB to the host")
+    del ch
     fill_bytes = 16.0 * nnz + 8.0 * n  # (col_idx, value) per nonzero + row_ptr read
     out = {"metric": "sparse state-space export (assemble_sparse) stored nonzeros/s", "value": nnz / (ms * 1e-3),
            "unit": "nnz/s", "n_gpus": 1, "reps": args.sparse_reps, "higher_is_better": True, "dtype": "f64",
            "data": "synthetic", "ms_two_passes": ms, "ms_count": cms, "ms_fill": fms, "ms_call": wall * 1e3,
+           "host_inclusive": {"ms_per_call": wall_dl * 1e3, "value": nnz / wall_dl, "host_bytes": host_bytes,
+                              "note": "one call with the CSR (row_ptr, col_idx, values) downloaded to host "
+                                      "arrays, as MaxwellOperator::assemble_sparse returns it"},
            "config": {"workload": f"jittered box {nx}x{ny}x{nz} x 6 tets, N={args.order}, PEC, upwind",
                       "n_elem": op.n_elem, "n": n, "nnz": nnz},
            "roofline": {"bound": "hbm", "achieved": fill_bytes / (fms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
