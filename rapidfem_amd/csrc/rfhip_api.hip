@@ -376,11 +376,10 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
 int64_t nvec(const rfhip_op *op) { return op->n_vec; }
 
 void h2d_layout_f64(rfhip_op *op, const double *host, double *dev) {
-  RF_HIP(hipMemcpyAsync(dev, host, 8 * op->n_dof, hipMemcpyHostToDevice, op->stream));
+  copy_h2d(dev, host, 8 * op->n_dof, op->stream);
 }
 void d2h_layout_f64(rfhip_op *op, const double *dev, double *host) {
-  RF_HIP(hipMemcpyAsync(host, dev, 8 * op->n_dof, hipMemcpyDeviceToHost, op->stream));
-  sync(op);
+  copy_d2h(host, dev, 8 * op->n_dof, op->stream);
 }
 void h2d_reg_f64(rfhip_op *op, const double *host, void *dev) {
   if (op->form != RFHIP_FORM_PLAIN_F32) {
@@ -388,7 +387,7 @@ void h2d_reg_f64(rfhip_op *op, const double *host, void *dev) {
     return;
   }
   op->aos.alloc(8 * op->n_dof);
-  RF_HIP(hipMemcpyAsync(op->aos.p, host, 8 * op->n_dof, hipMemcpyHostToDevice, op->stream));
+  copy_h2d(op->aos.p, host, 8 * op->n_dof, op->stream);
   launch_cast_d2f(op->aos.as<double>(), static_cast<float *>(dev), op->n_dof, op->stream);
 }
 void d2h_reg_f64(rfhip_op *op, const void *dev, double *host) {
@@ -398,8 +397,7 @@ void d2h_reg_f64(rfhip_op *op, const void *dev, double *host) {
   }
   op->aos.alloc(8 * op->n_dof);
   launch_cast_f2d(static_cast<const float *>(dev), op->aos.as<double>(), op->n_dof, op->stream);
-  RF_HIP(hipMemcpyAsync(host, op->aos.p, 8 * op->n_dof, hipMemcpyDeviceToHost, op->stream));
-  sync(op);
+  copy_d2h(host, op->aos.p, 8 * op->n_dof, op->stream);
 }
 std::vector<double> widen(const float *x, int64_t n) { return std::vector<double>(x, x + n); }
 void narrow(const std::vector<double> &x, float *out) {
@@ -1639,12 +1637,9 @@ int rfhip_csr_download(const rfhip_csr *csr, int64_t *row_ptr, int64_t *col_idx,
   return guarded([&] {
     RF_REQUIRE(csr, "null argument");
     hipStream_t s = csr->stream;
-    if (row_ptr)
-      RF_HIP(hipMemcpyAsync(row_ptr, csr->row_ptr.p, 8 * (size_t)(csr->n + 1), hipMemcpyDeviceToHost, s));
-    if (col_idx && csr->nnz)
-      RF_HIP(hipMemcpyAsync(col_idx, csr->col_idx.p, 8 * (size_t)csr->nnz, hipMemcpyDeviceToHost, s));
-    if (values && csr->nnz)
-      RF_HIP(hipMemcpyAsync(values, csr->values.p, 8 * (size_t)csr->nnz, hipMemcpyDeviceToHost, s));
+    if (row_ptr) copy_d2h(row_ptr, csr->row_ptr.p, 8 * (size_t)(csr->n + 1), s);
+    if (col_idx && csr->nnz) copy_d2h(col_idx, csr->col_idx.p, 8 * (size_t)csr->nnz, s);
+    if (values && csr->nnz) copy_d2h(values, csr->values.p, 8 * (size_t)csr->nnz, s);
     RF_HIP(hipStreamSynchronize(s));
   });
 }

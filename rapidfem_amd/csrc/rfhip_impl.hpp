@@ -95,6 +95,12 @@ struct HostPinned {
   }
 };
 
+// Copies of large pageable host buffers through a pinned staging ring
+// (rfhip_xfer.hip). copy_d2h returns with the data in dst; copy_h2d returns
+// once src may be reused, the device copy ordered on s.
+void copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s);
+void copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 template <class T>
 void upload(DevBuf &b, const T *host, size_t count, hipStream_t s) {
   b.alloc(sizeof(T) * count);
