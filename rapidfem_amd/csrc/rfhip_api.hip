@@ -720,27 +720,30 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
         for (int i = 0; i < cols; ++i) H[(size_t)i * max_dim + j] += hp[pass * kMaxKrylov + i];
       norm2 = hp[2 * kMaxKrylov];
     }
-    // Augmented: each pass's projections take the augmented scalar's share
-    // on the host before the update uses them.
-    for (int pass = 0; aug && pass < 2; ++pass) {
-      RF_HIP(hipMemcpyAsync(pj.data(), proj, 8 * cols, hipMemcpyDeviceToHost, s));
+    if (aug) {
+      // Augmented: each pass's projections take the augmented scalar's
+      // share, sc[i] * ws, before the update uses them. ws is 0 in the first
+      // pass, so its projections go to the update unchanged and only the
+      // second pass waits for the host.
+      double *hp = op->kb_host.as<double>();
+      RF_HIP(hipMemcpyAsync(hp, proj, 8 * cols, hipMemcpyDeviceToHost, s));
+      launch_axpy_dot(w, basis, n, cols, proj, part, nblk, proj, s);
+      RF_HIP(hipMemcpyAsync(hp + kMaxKrylov, proj, 8 * cols, hipMemcpyDeviceToHost, s));
       sync(op);
-      double dscal = 0.0;
-      for (int i = 0; i < cols; ++i) {
-        pj[i] += sc[i] * ws;
-        H[(size_t)i * max_dim + j] += pj[i];
-        dscal += pj[i] * sc[i];
+      for (int pass = 0; pass < 2; ++pass) {
+        double *p = hp + pass * kMaxKrylov;
+        double dscal = 0.0;
+        for (int i = 0; i < cols; ++i) {
+          p[i] += sc[i] * ws;
+          H[(size_t)i * max_dim + j] += p[i];
+          dscal += p[i] * sc[i];
+        }
+        ws -= dscal;
       }
-      RF_HIP(hipMemcpyAsync(proj, pj.data(), 8 * cols, hipMemcpyHostToDevice, s));
-      if (pass == 0)
-        launch_axpy_dot(w, basis, n, cols, proj, part, nblk, proj, s);
-      else
-        launch_axpy_basis_norm(w, basis, n, cols, proj, part, nblk, norm_dev, s);
-      ws -= dscal;
-      if (pass == 1) {
-        RF_HIP(hipMemcpyAsync(&norm2, norm_dev, 8, hipMemcpyDeviceToHost, s));
-        sync(op);
-      }
+      RF_HIP(hipMemcpyAsync(proj, hp + kMaxKrylov, 8 * cols, hipMemcpyHostToDevice, s));
+      launch_axpy_basis_norm(w, basis, n, cols, proj, part, nblk, norm_dev, s);
+      RF_HIP(hipMemcpyAsync(&norm2, norm_dev, 8, hipMemcpyDeviceToHost, s));
+      sync(op);
     }
     const double hnext = std::sqrt(norm2 + ws * ws);
     const int d = j + 1;

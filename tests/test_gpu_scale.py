@@ -58,6 +58,23 @@ def test_pulse_box_fused_forms_equal_plain_f64_and_are_linear(ctx, pulse_box):
     assert rel_l2(comp, want) < 1e-7
 
 
+def test_state_round_trip_through_the_staging_ring(ctx, pulse_box):
+    """38.88M DOFs of f64 (311 MB: four full 64 MB pinned slots and a partial
+    one) uploaded and downloaded bit for bit, twice in a row so the second
+    transfer reuses slots the first one left."""
+    import rapidfem_amd as R
+
+    op, y0 = pulse_box
+    rng = np.random.default_rng(8)
+    g = R.GpuOperator(ctx, op, "fused_f64")
+    for _ in range(2):
+        y = rng.standard_normal(op.n_dof)
+        g.state_load(y)
+        back = g.state_store()
+        assert back.dtype == np.float64 and back.shape == y.shape
+        assert np.array_equal(back.view(np.uint64), y.view(np.uint64))
+
+
 def test_pulse_box_energy_never_grows_and_upwind_dissipates_more(ctx, pulse_box):
     """LSERK4 below the CFL step on M~-skew (central) and M~-dissipative
     (upwind) operators: the energy 0.5 y^T M~ y is non-increasing step after
