@@ -3,6 +3,9 @@
 // while host threads copy the other to or from the caller's buffer (which
 // also spreads the first-touch page faults of a fresh destination over the
 // threads). Copies below kStageMin go through hipMemcpyAsync directly.
+// Measured on a 311 MB download into fresh memory: pageable hipMemcpy
+// 4.1 GB/s, the ring 20 GB/s with 8 threads and 27.5 GB/s with 16 (the
+// host copy into fresh pages is the limit; pinned DMA alone does 57 GB/s).
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -41,10 +44,10 @@ Ring &ring_for_current_device() {
   return *r;
 }
 
-// memcpy with up to 8 host threads, slices of at least kThreadSlice bytes.
+// memcpy with up to 16 host threads, slices of at least kThreadSlice bytes.
 void host_copy(void *dst, const void *src, size_t bytes) {
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>({8, hw, std::max<size_t>(1, bytes / kThreadSlice)});
+  const size_t nt = std::min<size_t>({16, hw, std::max<size_t>(1, bytes / kThreadSlice)});
   if (nt <= 1) {
     std::memcpy(dst, src, bytes);
     return;
