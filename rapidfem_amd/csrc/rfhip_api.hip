@@ -164,6 +164,20 @@ void build_plain(rfhip_op *op, const rfhip_op_desc &d) {
   dv.port = op->port.as<int>();
   dv.perm = op->perm.as<int>();
 }
+// Stage-kernel variant an operator starts with: the fastest per order and
+// register form in a sweep of every variant over LSERK4 steps on MI355X
+// (pulse boxes of 1.3M / 393k / 324k / 96k tets at N = 1..4). autotune
+// re-picks on the operator's own mesh.
+int default_variant(int order, int form) {
+  const bool c = form == RFHIP_FORM_FUSED_F64C;
+  switch (order) {
+    case 1: return c ? 23 : 17;
+    case 2: return c ? 16 : 15;
+    case 3: return c ? 23 : 16;
+    case 4: return c ? 12 : 14;
+    default: return 0;
+  }
+}
 
 void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   const int64_t K = d.n_elem;
@@ -365,6 +379,7 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   a.face_nodes = op->face_nodes.as<int>();
   a.nbr_node = op->nbr_node.as<int>();
   a.nbase = nbase.empty() ? nullptr : op->nbase.as<int>();
+  a.variant = default_variant(op->order, op->form);
 }
 
 // ---- vectors at the boundary ---------------------------------------------
