@@ -250,8 +250,9 @@ int rfhip_op_last_call_ms(const rfhip_op *op, double *ms);
  * The fused forms have several kernel variants per order (work per thread,
  * block shape). rfhip_op_autotune times each on the operator's own mesh with
  * scratch registers (the state is not modified) and keeps the fastest;
- * chosen/ms_best may be NULL. rfhip_op_set_variant pins one. Plain forms have
- * a single variant (0). */
+ * chosen/ms_best may be NULL. rfhip_op_set_variant pins one. A fused
+ * operator starts on the variant measured fastest for its order and register
+ * form on MI355X. Plain forms have a single variant (0). */
 int rfhip_op_n_variants(const rfhip_op *op);
 int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best);
 int rfhip_op_set_variant(rfhip_op *op, int variant);
