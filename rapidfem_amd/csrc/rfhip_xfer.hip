@@ -53,14 +53,21 @@ void host_copy(void *dst, const void *src, size_t bytes) {
     return;
   }
   const size_t per = (bytes + nt - 1) / nt;
+  auto slice = [=](size_t a, size_t b) {
+    std::memcpy(static_cast<char *>(dst) + a, static_cast<const char *>(src) + a, b - a);
+  };
   std::vector<std::thread> th;
   th.reserve(nt - 1);
   for (size_t t = 1; t < nt; ++t) {
     const size_t a = t * per, b = std::min(bytes, a + per);
-    if (a < b)
-      th.emplace_back([=] { std::memcpy(static_cast<char *>(dst) + a, static_cast<const char *>(src) + a, b - a); });
+    if (a >= b) continue;
+    try {
+      th.emplace_back(slice, a, b);
+    } catch (...) {
+      slice(a, b);  // no thread available: this slice on the calling thread
+    }
   }
-  std::memcpy(dst, src, std::min(bytes, per));
+  slice(0, std::min(bytes, per));
   for (auto &x : th) x.join();
 }
 
