@@ -332,11 +332,15 @@ __global__ void __launch_bounds__(kBS) k_axpy_dot(double *__restrict__ w, const 
 #pragma unroll
     for (int c = 0; c < MAXC; ++c) acc[c] += b[c] * wi;
   }
+  // A guard, not a break: with a constant trip count the loop unrolls fully
+  // and acc stays in registers (an early exit left it in scratch memory
+  // above 16 columns).
 #pragma unroll
   for (int c = 0; c < MAXC; ++c) {
-    if (c >= cols) break;
-    const double t = block_sum(acc[c], sh);
-    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * cols + c] = t;
+    if (c < cols) {
+      const double t = block_sum(acc[c], sh);
+      if (threadIdx.x == 0) part[(int64_t)blockIdx.x * cols + c] = t;
+    }
   }
 }
 __global__ void __launch_bounds__(kBS) k_scale(const double *__restrict__ w, double *__restrict__ d, int64_t n,
