@@ -8,9 +8,12 @@ step (5 stages) over the whole mesh with the state resident in HBM.
 Reported: the chosen form (fused_f64c: FP64 arithmetic, f32+bf16 state and
 f32 residual registers) and the plain FP64 form (the reference kernel
 structure) on the same state; the chosen form's relative L2 deviation from
-the FP64 fused form after the timed steps; the dominant kernel's roofline
-from HIP-event timing; and the CPU oracle (C restatement of the reference
-CPU path) timed on a bounded sample.
+the FP64 fused form after the timed steps and from the CPU oracle's
+trajectory over the CPU sample's steps; the dominant kernel's roofline from
+HIP-event timing, bound by whichever roof its arithmetic intensity reaches;
+the CPU oracle (C restatement of the reference CPU path) timed on a bounded
+sample on all host threads and on one; and the largest single-GPU stepper
+workload (998,250 tets at N=4) beside the headline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -48,6 +51,8 @@ def parse():
     ap.add_argument("--form", default="fused_f64c")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--c3-steps", type=int, default=10,
+                    help="steps of the 998,250-tet N=4 line reported beside the headline (0 = skip)")
     ap.add_argument("--autotune", type=int, default=1, help="pick the fastest kernel variant on this mesh")
     ap.add_argument("--variant", type=int, default=-1,
                     help="pin this stage-kernel variant instead of autotuning (fused forms)")
@@ -430,10 +435,22 @@ def traffic_probe(args) -> int:
     return 0
 
 
-def cpu_baseline(op, y0, dt, steps):
-    """Oracle LSERK4 on the full mesh for `steps` steps, all host threads."""
-    import numpy as np
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(op, y0, dt, steps):
+    """Oracle LSERK4 on the full mesh: `steps` steps on all host threads,
+    then one step on one thread. Returns the baseline record and the
+    oracle's state after the `steps` steps (the parity anchor of the
+    benchmark's GPU forms)."""
     from oracle import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -443,12 +460,99 @@ def cpu_baseline(op, y0, dt, steps):
                      eps=1.0 / op.inv_eps)
     t_build = time.perf_counter() - t0
     t0 = time.perf_counter()
-    oop.lserk(y0, dt, steps)
+    y_or = oop.lserk(y0, dt, steps)
     t = time.perf_counter() - t0
     rate = oop.n_dof * 5 * steps / t
-    return {"value": rate, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C restatement of the reference CPU path, OpenMP) LSERK4, {steps} steps on the "
-                      f"full {op.n_elem}-tet mesh, {t:.1f} s (operator build {t_build:.1f} s excluded)"}
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    oop.lserk(y0, dt, 1)
+    t1 = time.perf_counter() - t0
+    O.set_threads(threads)
+    rate1 = oop.n_dof * 5 / t1
+    rec = {"value": rate, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "sample": f"oracle (C restatement of the reference CPU path, OpenMP) LSERK4, {steps} steps on the "
+                     f"full {op.n_elem}-tet mesh, {t:.1f} s (operator build {t_build:.1f} s excluded)",
+           "one_core": {"value": rate1, "cores": 1,
+                        "sample": f"the same oracle, 1 LSERK4 step on the full mesh on one thread, {t1:.1f} s"}}
+    return rec, y_or
+
+
+def roofline_record(order, form, n, stage_ms, traffic=None):
+    """Roofline of one fused stage launch over n DOFs: algorithmic bytes and
+    flops per DOF-stage over the launch's mean duration. The binding roof is
+    the one whose ridge the kernel's intensity passes: FP64 compute (vector
+    and matrix FP64 share one 78.6 TFLOP/s peak on MI355X) when flops/bytes
+    exceeds 78.6e12 / 8e12 = 9.8 flop/B, else HBM."""
+    from rapidfem_amd import workloads as W
+
+    bpd = W.bytes_per_dof_stage(order, form)
+    fpd = W.flops_per_dof_stage(order)
+    gbs = bpd * n / (stage_ms * 1e-3) / 1e9
+    tfl = fpd * n / (stage_ms * 1e-3) / 1e12
+    intensity = fpd / bpd
+    ridge = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+    hbm = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+           "bytes_per_dof_stage": round(bpd, 3)}
+    fp64 = {"achieved": round(tfl, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tfl / FP64_PEAK_TFLOPS, 4), "flops_per_dof_stage": round(fpd, 1)}
+    top = dict(fp64 if intensity > ridge else hbm)
+    top.update({"bound": "mfma" if intensity > ridge else "hbm", "traffic": traffic,
+                "intensity_flop_per_byte": round(intensity, 2), "ridge_flop_per_byte": round(ridge, 2),
+                "algorithmic_bytes_per_launch": bpd * n, "algorithmic_flops_per_launch": fpd * n,
+                "compute_pipe": "FP64 (VALU and MFMA share the 78.6 TFLOP/s peak)", "hbm": hbm, "fp64": fp64})
+    return top
+
+
+def c3_line(ctx, args):
+    """The largest single-GPU stepper workload beside the headline: the
+    998,250-tet jittered N=4 box with 16 cavity modes (209.6M DOFs). The
+    chosen form and the plain FP64 form step the same state at a fixed
+    stable dt (throughput does not depend on dt; the CFL-step precision run
+    of this workload is a GPU test); rel L2 of the chosen form against the
+    FP64 fused form after the timed steps."""
+    import numpy as np
+
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+
+    t0 = time.perf_counter()
+    op, y0 = W.modes_box(55, 4)
+    n = op.n_dof
+    log(f"c3: {op.n_elem} tets N=4, n={n}, built in {time.perf_counter() - t0:.1f}s")
+    dt, steps = 2.0e-4, args.c3_steps
+    ch = R.GpuOperator(ctx, op, args.form)
+    variant, _ = ch.autotune()
+    ch.state_load(y0)
+    ch.state_lserk(dt, 2)
+    tot, stage = ch.state_lserk_timed(dt, steps)
+    y_ch = ch.state_store()
+    del ch
+    ref = R.GpuOperator(ctx, op, "fused_f64")
+    rvar, _ = ref.autotune()
+    ref.state_load(y0)
+    ref.state_lserk(dt, 2)
+    rtot, _ = ref.state_lserk_timed(dt, steps)
+    err = float(np.linalg.norm(ref.state_store() - y_ch) / np.linalg.norm(y_ch))
+    del ref
+    pl = R.GpuOperator(ctx, op, "plain_f64")
+    pl.state_load(y0)
+    pl.state_lserk(dt, 1)
+    ptot, papply = pl.state_lserk_timed(dt, max(4, steps // 2))
+    pms = ptot / max(4, steps // 2)
+    del pl
+    value = n * 5 * steps / (tot * 1e-3)
+    log(f"c3: {args.form} v{variant} {tot / steps:.3f} ms/step ({value:.4e} DOF-stage/s), stage {stage:.4f} ms; "
+        f"fused_f64 v{rvar} {rtot / steps:.3f} ms/step; plain_f64 {pms:.3f} ms/step; rel L2 {err:.3e}")
+    return {"workload": "modes_box 55^3 cells x 6 tets (jittered unit box), N=4, 16 PEC cavity modes, upwind, "
+                        f"LSERK4 at dt {dt:g}",
+            "n_elem": op.n_elem, "n_dof": n, "order": 4, "form": args.form, "variant": variant, "steps": steps,
+            "value": value, "ms_per_step": tot / steps,
+            "fused_f64": {"variant": rvar, "ms_per_step": rtot / steps},
+            "plain": {"form": "plain_f64", "ms_per_step": pms, "value": n * 5 / (pms * 1e-3),
+                      "apply_kernel_ms": papply},
+            "speedup_vs_plain": pms / (tot / steps), "rel_l2_vs_f64": err, "rel_l2_steps": steps + 2,
+            "roofline": roofline_record(4, args.form, n, stage)}
 
 
 def main() -> int:
@@ -532,10 +636,6 @@ def main() -> int:
     log(f"fused_f64: {fvalue:.4e} DOF-stage/s, stage kernel {fstage:.4f} ms")
 
     # Roofline of the dominant kernel (the fused stage kernel).
-    bpd = W.bytes_per_dof_stage(args.order, args.form)
-    fpd = W.flops_per_dof_stage(args.order)
-    gbs = bpd * n / (stage_ms * 1e-3) / 1e9
-    tfl = fpd * n / (stage_ms * 1e-3) / 1e12
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:
@@ -544,18 +644,28 @@ def main() -> int:
                 traffic = tr.get("variants", {}).get(str(variant), {}).get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
-    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes_per_launch": bpd * n,
-                "fp64": {"achieved_tflops": round(tfl, 2), "peak_tflops": FP64_PEAK_TFLOPS,
-                         "frac": round(tfl / FP64_PEAK_TFLOPS, 4), "flops_per_dof_stage": round(fpd, 1)}}
+    roofline = roofline_record(args.order, args.form, n, stage_ms, traffic)
     cpu = None
+    vs_oracle = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(op, y0, dt, args.cpu_steps)
-            log(f"cpu baseline {cpu['value']:.4e} DOF-stage/s on {cpu['cores']} threads")
+            cpu, y_or = cpu_baseline(op, y0, dt, args.cpu_steps)
+            log(f"cpu baseline {cpu['value']:.4e} DOF-stage/s on {cpu['cores']} threads, "
+                f"{cpu['one_core']['value']:.4e} on one ({cpu['cpu_model']})")
+            vs_oracle = {"steps": args.cpu_steps}
+            for name, g in (("chosen", chosen), ("fused_f64", ref)):
+                g.state_load(y0)
+                g.state_lserk(dt, args.cpu_steps)
+                vs_oracle[name] = float(np.linalg.norm(g.state_store() - y_or) / np.linalg.norm(y_or))
+            log(f"rel L2 vs the oracle after {args.cpu_steps} steps: {args.form} {vs_oracle['chosen']:.3e}, "
+                f"fused_f64 {vs_oracle['fused_f64']:.3e}")
+            del y_or
         except Exception as e:  # the oracle is optional on a box without gcc
             log(f"cpu baseline failed: {e}")
+    c3 = None
+    if args.c3_steps > 0:
+        del chosen, ref, sess
+        c3 = c3_line(ctx, args)
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-stage/s", "n_gpus": 1, "steps": K, "warmup": Wm,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -568,8 +678,8 @@ def main() -> int:
         "plain": {"form": "plain_f64", "value": pvalue, "ms_per_step": ptot / args.plain_steps},
         "fused_f64": {"value": fvalue, "variant": ref_variant},
         "speedup_vs_plain": value / pvalue,
-        "rel_l2_vs_f64": err, "rel_l2_steps": Wm + K,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "rel_l2_vs_f64": err, "rel_l2_steps": Wm + K, "rel_l2_vs_oracle": vs_oracle,
+        "roofline": roofline, "cpu_baseline": cpu, "c3": c3,
     }
     print(json.dumps(out))
     return 0

@@ -1015,8 +1015,10 @@ int rfhip_op_apply(rfhip_op *op, const float *y, float *dy) {
     const int64_t n = op->n_dof;
     hipStream_t s = op->stream;
     if (op->form == RFHIP_FORM_PLAIN_F32) {
-      RF_HIP(hipMemcpyAsync(op->y[0].p, y, 4 * n, hipMemcpyHostToDevice, s));
-      plain_apply_full(op, op->y[0].p, op->k.p, s);
+      // staged in the scratch register: the resident state in y[0] stays
+      op->kst.alloc(4 * nvec(op));
+      RF_HIP(hipMemcpyAsync(op->kst.p, y, 4 * n, hipMemcpyHostToDevice, s));
+      plain_apply_full(op, op->kst.p, op->k.p, s);
       RF_HIP(hipMemcpyAsync(dy, op->k.p, 4 * n, hipMemcpyDeviceToHost, s));
       sync(op);
       return;
@@ -1293,7 +1295,11 @@ int rfhip_op_set_variant(rfhip_op *op, int variant) {
   return guarded([&] {
     RF_REQUIRE(op, "null argument");
     RF_REQUIRE(variant >= 0 && variant < rfhip_op_n_variants(op), "variant out of range");
-    if (is_fused(op->form)) op->sa.variant = variant;
+    if (!is_fused(op->form)) return;
+    if (!variant_available(op->order, variant, op->sa))
+      throw Error{RFHIP_EUNSUPPORTED, "stage-kernel variant " + std::to_string(variant) +
+                                          " cannot run on this operator (order or neighbour table)"};
+    op->sa.variant = variant;
   });
 }
 
@@ -1332,6 +1338,7 @@ int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best) {
     try {
       for (int pass = 0; pass < 3; ++pass)
       for (int v = 0; v < kFusedVariants; ++v) {
+        if (!variant_available(op->order, v, op->sa)) continue;
         StageArgs a = op->sa;
         a.variant = v;
         a.yin = op->y[op->cur].p;
@@ -1359,8 +1366,8 @@ int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    int best = 0;
-    for (int v = 1; v < kFusedVariants; ++v)
+    int best = op->sa.variant;
+    for (int v = 0; v < kFusedVariants; ++v)
       if (v_ms[v] < v_ms[best]) best = v;
     op->sa.variant = best;
     if (chosen) *chosen = best;

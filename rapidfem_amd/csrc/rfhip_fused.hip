@@ -3456,6 +3456,20 @@ void launch_port_source(const rfhip_op *op, int64_t n_elems, const int64_t *elem
   RF_HIP(hipGetLastError());
 }
 
+bool variant_available(int order, int variant, const StageArgs &a) {
+  const int np = (order + 1) * (order + 2) * (order + 3) / 6;
+  switch (variant) {
+    case 8: case 10: case 16: case 17: case 18: case 19: case 22: case 23:
+      return a.nbase != nullptr;
+    case 20: case 21:
+      return a.nbase != nullptr && a.dsparse != 0;
+    case 11: case 15:
+      return a.nbase != nullptr && np <= 20;
+    default:
+      return variant >= 0 && variant < 24;
+  }
+}
+
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s) {
   const int np = (order + 1) * (order + 2) * (order + 3) / 6;
   const int variant = a.variant;

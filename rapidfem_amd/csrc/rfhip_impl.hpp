@@ -246,6 +246,10 @@ void launch_add_vec(const rfhip_op *op, void *k, const void *src, double g, hipS
 // Fused forms: epi 0 = apply (dy), 1 = LSERK stage (FP64 or compressed
 // registers by form).
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s);
+// Whether stage-kernel variant `variant` runs as itself on this operator's
+// data (some need the neighbour-trace table, the sparse D_k pattern or
+// Np <= 20; launch_stage runs the base variant for them otherwise).
+bool variant_available(int order, int variant, const StageArgs &a);
 // Debye block after an applied derivative (dy = A y; f32: plain FP32 form),
 // with the drive of the P rows, and after a fused LSERK4 stage
 // (rfhip_debye.hip). No-ops without Debye elements.

@@ -46,22 +46,33 @@ __global__ void __launch_bounds__(kPlainWG) k_plain_apply(const T *__restrict__ 
 #pragma unroll
     for (int q = 0; q < 9; ++q) ji[q] = jinv[e * 9 + q];
     // Volume term: dE = curl H, dH = -curl E (rhs.rs:1359-1368, element_curl :39-85).
+    // The reference derivatives of all six components are accumulated in
+    // one pass over the node's D rows (each entry read once, each sum in
+    // the reference's j order), then mapped through J^-1 into the curls.
+    T r[3][6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) r[0][c] = r[1][c] = r[2][c] = 0;
+#pragma unroll 5
+    for (int j = 0; j < NP; ++j) {
+      const T dr = Dr[node * NP + j], ds = Ds[node * NP + j], dt = Dt[node * NP + j];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const T u = su[le][j * 6 + c];
+        r[0][c] += dr * u;
+        r[1][c] += ds * u;
+        r[2][c] += dt * u;
+      }
+    }
 #pragma unroll
     for (int fld = 0; fld < 2; ++fld) {
       const int off = fld == 0 ? 3 : 0;
       T pd[3][3];
 #pragma unroll
-      for (int comp = 0; comp < 3; ++comp) {
-        T r0 = 0, r1 = 0, r2 = 0;
-        for (int j = 0; j < NP; ++j) {
-          const T u = su[le][j * 6 + off + comp];
-          r0 += Dr[node * NP + j] * u;
-          r1 += Ds[node * NP + j] * u;
-          r2 += Dt[node * NP + j] * u;
-        }
+      for (int comp = 0; comp < 3; ++comp)
 #pragma unroll
-        for (int ph = 0; ph < 3; ++ph) pd[ph][comp] = ji[0 * 3 + ph] * r0 + ji[1 * 3 + ph] * r1 + ji[2 * 3 + ph] * r2;
-      }
+        for (int ph = 0; ph < 3; ++ph)
+          pd[ph][comp] = ji[0 * 3 + ph] * r[0][off + comp] + ji[1 * 3 + ph] * r[1][off + comp] +
+                         ji[2 * 3 + ph] * r[2][off + comp];
       const T cx = pd[1][2] - pd[2][1], cy = pd[2][0] - pd[0][2], cz = pd[0][1] - pd[1][0];
       if (fld == 0) {
         acc[0] = cx;

@@ -136,6 +136,10 @@ class TdSession:
         y0 = np.zeros(n) if y0 is None else np.ascontiguousarray(y0, np.float64)
         if not (math.isfinite(dt) and dt > 0):
             raise ValueError(f"dt must be positive, got {dt}")
+        if drive.kind == 1 and not 0 <= drive.dof < n:
+            raise ValueError(f"source DOF {drive.dof} out of range (n_dof = {n})")
+        if drive.kind == 2 and drive.vec is not None and drive.vec.shape != (n,):
+            raise ValueError(f"the source vector has length {drive.vec.shape[0]}, expected {n}")
         if drive.kind != 0 and waveform is None:
             raise ValueError("a driven run needs a waveform")
         if method == "explicit":
@@ -152,6 +156,9 @@ class TdSession:
                                               precision="f64")
             return traj, KclStats(st.accepted, st.rejected, st.h_min / self.c, st.h_max / self.c)
         if method == "exponential":
+            if drive.kind == 2 and drive.vec is None:
+                raise ValueError("the exponential route needs the source pattern on the host: pass "
+                                 "Drive.vector(gop.port_source(port)) instead of Drive.port_source()")
             frames = [y0]
             y = y0
             for k in range(steps):
