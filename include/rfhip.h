@@ -295,6 +295,21 @@ int rfhip_op_state_lserk_timed(rfhip_op *op, double dt, int64_t nsteps,
 int rfhip_op_state_energy(rfhip_op *op, const double *mass_ref,
                           const double *det, double *energy);
 
+/* Allocation-free variant of rfhip_op_state_energy. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Every pointer is device memory the caller owns: y the f64 AoS
+ * state (6*Np*K values of [E,H]; an appended Debye block is not read),
+ * mass_ref (Np*Np), det (K), inv_eps and inv_mu (3K each, per element and
+ * component); *energy receives 0.5 y^T M~ y and is ready when the call
+ * returns. The reduction's partial sums use a fixed device array of 4096
+ * doubles per GPU, so K is limited to 4096 * 256 = 1,048,576 elements
+ * (RFHIP_EINVAL beyond). Calls that overlap on one GPU race on that array. */
+int rfhip_op_energy_noalloc(rfhip_op *op, const double *y,
+                            const double *mass_ref, const double *det,
+                            const double *inv_eps, const double *inv_mu,
+                            double *energy);
+
 /* ---- sparse state-space export (rhs.rs:1575-1730) ------------------------ */
 
 /* CsrMatrix (rhs.rs:1575-1591), held in device memory. */

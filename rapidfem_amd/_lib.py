@@ -102,6 +102,7 @@ SIGNATURES = {
     "rfhip_batch_lserk": (C.c_int, [C.POINTER(_vp), _i32, _dp, _i64]),
     "rfhip_op_state_lserk_timed": (C.c_int, [_vp, _d, _i64, C.POINTER(_d), C.POINTER(_d)]),
     "rfhip_op_state_energy": (C.c_int, [_vp, _dp, _dp, C.POINTER(_d)]),
+    "rfhip_op_energy_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rfhip_assemble_sparse": (C.c_int, [_vp, C.POINTER(OpDesc), _i64, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
     "rfhip_csr_destroy": (None, [_vp]),
     "rfhip_csr_shape": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),

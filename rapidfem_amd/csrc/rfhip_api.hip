@@ -903,6 +903,7 @@ int rfhip_ctx_create(rfhip_ctx **out) {
     auto ctx = std::make_unique<rfhip_ctx>();
     ctx->device = 0;
     RF_HIP(hipSetDevice(0));
+    noalloc_init(0);
     hipDeviceProp_t prop;
     RF_HIP(hipGetDeviceProperties(&prop, 0));
     ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";

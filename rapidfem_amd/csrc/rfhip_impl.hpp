@@ -327,6 +327,17 @@ void launch_energy(const double *y, const double *mass, const double *det, const
                    const double *inv_mu, int np, int64_t K, double *partials, int nblk, double *out,
                    hipStream_t s);
 
+// The same energy with a fixed grid of nblk blocks (each of
+// energy_elems_per_block() elements covers at most one element per thread
+// when nblk * that >= K), written as scale * sum to out on the device.
+int energy_elems_per_block();
+void launch_energy_scaled(const double *y, const double *mass, const double *det, const double *inv_eps,
+                          const double *inv_mu, int np, int64_t K, double *partials, int nblk, double scale,
+                          double *out, hipStream_t s);
+// The allocation-free entry points' scratch array (rfhip_noalloc.hip): its
+// device address on `device`, looked up once with that device current.
+void noalloc_init(int device);
+
 // Host-side dense exponential (propagator.rs:69-114).
 void expm_host(const double *a, int n, double *out);
 

@@ -184,6 +184,16 @@ __global__ void __launch_bounds__(kBS) k_finish_sum(const double *__restrict__ p
   if (threadIdx.x == 0) out[0] = t;
 }
 
+// out[0] = scale * sum of part[0..nblk).
+__global__ void __launch_bounds__(kBS) k_finish_sum_scaled(const double *__restrict__ part, int nblk, double scale,
+                                                           double *out) {
+  __shared__ double sh[kBS / 64];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kBS) acc += part[i];
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[0] = scale * t;
+}
+
 // ---- Krylov CGS2 pieces (propagator.rs:231-271) -------------------------
 
 constexpr int kMaxCols = 64;
@@ -620,6 +630,15 @@ void launch_energy(const double *y, const double *mass, const double *det, const
                    int64_t K, double *partials, int nblk, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_energy, dim3(nblk), dim3(kBS), 0, s, y, mass, det, ie, im, np, K, partials);
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, partials, nblk, out);
+  RF_HIP(hipGetLastError());
+}
+
+int energy_elems_per_block() { return kBS; }
+
+void launch_energy_scaled(const double *y, const double *mass, const double *det, const double *ie, const double *im,
+                          int np, int64_t K, double *partials, int nblk, double scale, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy, dim3(nblk), dim3(kBS), 0, s, y, mass, det, ie, im, np, K, partials);
+  hipLaunchKernelGGL(k_finish_sum_scaled, dim3(1), dim3(kBS), 0, s, partials, nblk, scale, out);
   RF_HIP(hipGetLastError());
 }
 
