@@ -36,6 +36,17 @@ FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_stage_pulse_box_traffic.json")
 
 
+# The arithmetic each form computes in, and how its state registers are stored.
+DTYPE_LABEL = {
+    "fused_f64": "f64",
+    "fused_f64c": "f64 arithmetic; y stored f32+bf16, p f32",
+    "fused_bf16x3": "f64 arithmetic, volume contraction bf16x3 MFMA with f32 accumulation; y f32+bf16, p f32",
+    "fused_bf16x2": "f64 arithmetic, volume contraction bf16 hi+lo MFMA with f32 accumulation; y f32+bf16, p f32",
+    "plain_f64": "f64",
+    "plain_f32": "f32",
+}
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -68,6 +79,14 @@ def parse():
                          "instead of the stepper: one JSON line in Arnoldi DOF-steps/s")
     ap.add_argument("--krylov-cells", type=int, nargs=3, default=[40, 30, 28])
     ap.add_argument("--krylov-dim", type=int, default=40)
+    ap.add_argument("--proposer", action="store_true",
+                    help="the learned-proposal form: plain vs learned (h, m) adaptive Krylov exponential on the "
+                         "--krylov-cells slab box, final error against a fine-step run, one JSON line")
+    ap.add_argument("--cost-only", action="store_true",
+                    help="with --proposer: only fit and print the per-step cost model used for training labels")
+    ap.add_argument("--proposer-cfl", type=float, default=200.0, help="horizon in CFL steps")
+    ap.add_argument("--proposer-tol", type=float, default=1e-6)
+    ap.add_argument("--proposer-weights", default="")
     ap.add_argument("--kcl", action="store_true",
                     help="time the KCL adaptive stepper (RK4(3)5 + PI controller) on the stepper workload: one "
                          "JSON line in DOF-stage/s over accepted and rejected substeps")
@@ -77,6 +96,7 @@ def parse():
     ap.add_argument("--batch", action="store_true",
                     help="time the batching form: 8 seeded boxes of 20k-300k tets at orders 2/3/4, 4 right-hand "
                          "sides each (32 operators), stepped one after another and then together from one graph")
+    ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--sparse-cells", type=int, nargs=3, default=[16, 16, 16])
     ap.add_argument("--sparse-reps", type=int, default=3)
     return ap.parse_args()
@@ -174,6 +194,94 @@ def krylov_bench(args) -> int:
     return 0
 
 
+def slab_c5(args):
+    """The stiff workload of the learned-proposal form: the 201,600-tet N=3
+    slab box graded 20:1 toward an eps_r 10, sigma 50 slab, pulse field."""
+    from rapidfem_amd import workloads as W
+
+    nx, ny, nz = args.krylov_cells
+    return W.slab_box(nx, ny, nz, args.order)
+
+
+def krylov_cost_model(g, ctx, y0, cfl, dims=(10, 20, 30, 40, 50, 60)):
+    """Per-step wall time of the adaptive Krylov step at fixed dimensions on
+    this operator, fitted as c0 + c1 m + c2 m (m + 1) / 2 (ms): per step,
+    per operator apply, per basis column swept."""
+    import numpy as np
+
+    from rapidfem_amd.proposer import ExpController
+
+    rows, ms = [], []
+    for m in dims:
+        ctl = ExpController(tol=1e-6, m=m, h0=cfl)
+        _, steps, _ = g.expmv_adaptive(ctx, y0, 3 * cfl, ctl, log_cap=64)
+        for s in steps[1:] if len(steps) > 1 else steps:
+            rows.append([1.0, s["m"], 0.5 * s["m"] * (s["m"] + 1)])
+            ms.append(s["ms"])
+    coef, *_ = np.linalg.lstsq(np.array(rows), np.array(ms), rcond=None)
+    return [float(c) for c in coef], len(ms)
+
+
+def proposer_bench(args) -> int:
+    """The learned-proposal form: exp(t_end A) v on the stiff slab workload
+    with the plain error-controlled controller (m = 40, the session's
+    krylov_dim) and with the trained (h, m) proposer, at the same tolerance;
+    both final states against a fine-step FP64 propagation (the plain
+    controller at a thousand times tighter tolerance). value = plain wall
+    time / learned wall time."""
+    import numpy as np
+
+    import rapidfem_amd as R
+    from rapidfem_amd.proposer import WEIGHTS, ExpController, Proposer
+
+    op, y0 = slab_c5(args)
+    ctx = R.GpuContext()
+    g = R.GpuOperator(ctx, op, "fused_f64")
+    variant, _ = g.autotune()
+    cfl = R.TdSession(op, 1.0, "fused_f64", ctx).cfl_dt()
+    t_end = args.proposer_cfl * cfl
+    tol = args.proposer_tol
+    log(f"proposer: {op.n_elem} tets, n={op.n_dof}, cfl {cfl:.4e}, t_end {t_end:.4e} ({args.proposer_cfl} CFL steps), "
+        f"tol {tol:g}")
+    cost, nfit = krylov_cost_model(g, ctx, y0, cfl)
+    log(f"  cost model (ms): c0 {cost[0]:.3f} + c1 {cost[1]:.4f} m + c2 {cost[2]:.5f} m(m+1)/2 from {nfit} steps")
+    if args.cost_only:
+        print(json.dumps({"metric": "adaptive Krylov step cost model", "cost_model_ms": cost, "steps_fitted": nfit,
+                          "config": {"workload": f"slab box {op.n_elem} tets N={args.order}"}}))
+        return 0
+    fine_ctl = ExpController(tol=tol * 1e-3, m=40)
+    yf, sf, _ = g.expmv_adaptive(ctx, y0, t_end, fine_ctl)
+    log(f"  fine run: {len(sf)} steps (m 40, tol {tol * 1e-3:g}), {sf['ms'].sum():.0f} ms")
+    res = {}
+    prop = Proposer(ctx)
+    prop.load(args.proposer_weights or WEIGHTS)
+    for name, p in (("plain", None), ("learned", prop)):
+        ctl = ExpController(tol=tol, m=40, m_min=10, m_max=60)
+        g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=p, log_cap=4096)  # warm-up
+        t0 = time.perf_counter()
+        y, st, _ = g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=p, log_cap=4096)
+        wall = (time.perf_counter() - t0) * 1e3
+        err = float(np.linalg.norm(y - yf) / np.linalg.norm(yf))
+        res[name] = {"device_ms": g.last_call_ms, "call_ms": wall, "steps": int(len(st)),
+                     "applies": int(st["m"].sum()), "first_proposals_rejected": int((st["n_judged"] > 1).sum()),
+                     "m_mean": float(st["m"].mean()), "rel_l2_vs_fine": err,
+                     "h_over_cfl": [round(float(h / cfl), 2) for h in st["h"][:40]]}
+        log(f"  {name}: {len(st)} steps, {int(st['m'].sum())} applies, device {g.last_call_ms:.0f} ms, "
+            f"rel L2 vs fine {err:.2e}")
+    gain = res["plain"]["device_ms"] / res["learned"]["device_ms"]
+    out = {"metric": "Krylov-exp wall time, learned (h, m) proposer vs plain controller at equal tolerance",
+           "value": round(gain, 3), "unit": "x (plain ms / learned ms)", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f64 (Arnoldi, CGS2, Pade-13), f32 network", "data": "synthetic", "tol": tol,
+           "plain": res["plain"], "learned": res["learned"], "fine": {"steps": int(len(sf)), "tol": tol * 1e-3},
+           "cost_model_ms": cost,
+           "config": {"workload": f"slab box {args.krylov_cells[0]}x{args.krylov_cells[1]}x{args.krylov_cells[2]} x 6 "
+                                  f"tets graded 20:1 to an eps_r 10, sigma 50 slab, N={args.order}, pulse field, "
+                                  f"t_end {args.proposer_cfl} CFL steps",
+                      "n_elem": op.n_elem, "n": op.n_dof, "variant": variant}}
+    print(json.dumps(out))
+    return 0
+
+
 def kcl_bench(args) -> int:
     """KCL RK4(3)5[2R+]C with the PI controller on the stepper workload: two
     frames of 8 CFL steps each, the device part of the call timed (frame
@@ -238,59 +346,71 @@ def kcl_bench(args) -> int:
 
 
 def batch_bench(args) -> int:
-    """The batching form on one GPU: 8 boxes with seeded sizes (20k-300k
-    tets, log-uniform) at orders 2, 3, 4 in turn, each with 4 right-hand
-    sides (one operator each, 32 operators), f64c registers, free LSERK4 at a
-    fixed small step (throughput only; no port sources). One autotune per
-    mesh, shared by its right-hand sides. Timed: every operator's 20 steps
-    one after another, then all of them from one graph (rfhip_batch_lserk).
-    Unit: DOF-stage over all operators."""
+    """The batching form on one GPU: 8 jittered boxes with seeded sizes
+    (20k-300k tets, log-uniform) and a seeded dielectric block, orders 2, 3, 4
+    in turn, four port walls each with a TE10-shaped incident profile; each
+    mesh carries 4 right-hand sides (one operator each, 32 operators), the
+    r-th driven by port r's source (rhs.rs:1157-1199) under a Gaussian-
+    modulated pulse with a centre frequency seeded in [3, 8]
+    (session.rs:242-261), f64c registers, LSERK4 at each mesh's CFL step. One
+    autotune per mesh, shared by its right-hand sides. Timed: --batch-steps
+    steps of all operators from one graph (rfhip_batch_lserk_ports), and a
+    shorter run of each operator alone for comparison. Unit: DOF-stage over
+    all operators."""
     import numpy as np
 
     import rapidfem_amd as R
+    from rapidfem_amd import GaussianPulse
+    from rapidfem_amd import workloads as W
 
-    rng = np.random.default_rng(2024)
     ctx = R.GpuContext()
-    ops, dofs, meshes = [], 0, []
-    for b in range(8):
-        target = float(np.exp(rng.uniform(np.log(20_000), np.log(300_000))))
-        c = (target / 6.0) ** (1.0 / 3.0)
-        asp = rng.uniform(0.8, 1.25, 2)
-        nx = max(2, int(round(c * asp[0])))
-        ny = max(2, int(round(c / asp[0])))
-        nz = max(2, int(round(target / (6.0 * nx * ny))))
-        order = (2, 3, 4)[b % 3]
-        mesh = R.structured_box_jittered(nx, ny, nz, 1.0, 1.0, 1.0, 0.1, 11 + b)
-        op = R.MaxwellOperator(mesh, order, 1.0)
-        first = R.GpuOperator(ctx, op, "fused_f64c")
-        variant, _ = first.autotune()
-        group = [first] + [R.GpuOperator(ctx, op, "fused_f64c") for _ in range(3)]
-        for q, g in enumerate(group):
+    ops, dts, waves, dofs, meshes = [], [], [], 0, []
+    steps = args.batch_steps
+    for b, (nx, ny, nz, order) in enumerate(W.port_batch_specs()):
+        op, ports = W.port_box(nx, ny, nz, order, 11 + b)
+        cfl = R.TdSession(op, 1.0, "fused_f64c", ctx).cfl_dt()
+        pulses = W.port_waveforms(100 + b, 4)
+        variant = None
+        for r in range(4):
+            g = R.GpuOperator(ctx, op, "fused_f64c")
+            for q, (el, lf, prof, w) in enumerate(ports):
+                g.set_port(q, el, lf, prof, w)
+            g.port_source(r, download=False)
+            if variant is None:
+                variant, _ = g.autotune()
             g.set_variant(variant)
-            g.state_load(np.random.default_rng(100 * b + q).standard_normal(op.n_dof) * 0.1)
-        ops += group
+            g.state_load(np.zeros(op.n_dof))
+            pulse = GaussianPulse(*pulses[r])
+            ops.append(g)
+            dts.append(cfl)
+            waves.append(np.array([pulse(k * cfl) for k in range(steps)]))
         dofs += 4 * op.n_dof
-        meshes.append(f"{nx}x{ny}x{nz} N={order} ({op.n_elem} tets, v{variant})")
+        meshes.append(f"{nx}x{ny}x{nz} N={order} ({op.n_elem} tets, v{variant}, f0 "
+                      f"{', '.join(f'{p[2]:.2f}' for p in pulses)})")
         log(f"mesh {b}: {meshes[-1]}")
-    dt, steps = 1.0e-4, 20
+    R.batch_lserk_ports(ops, dts, 2, [w[:2] for w in waves])  # warm-up (graph capture, kernels)
     for g in ops:
-        g.state_lserk(dt, 2)  # warm-up
+        g.state_load(np.zeros(g.n_dof()))
     t0 = time.perf_counter()
-    for g in ops:
-        g.state_lserk(dt, steps)
-    t_seq = time.perf_counter() - t0
-    R.batch_lserk(ops, dt, 2)  # warm-up
-    t0 = time.perf_counter()
-    R.batch_lserk(ops, dt, steps)
+    R.batch_lserk_ports(ops, dts, steps, waves)
     t_batch = time.perf_counter() - t0
-    log(f"sequential {t_seq * 1e3:.1f} ms, batched {t_batch * 1e3:.1f} ms for {steps} steps of {len(ops)} operators")
-    out = {"metric": "batching form DOF-stage/s (8 meshes x 4 right-hand sides, one graph)",
+    norms = [float(np.linalg.norm(g.state_store())) for g in ops[::4]]
+    seq_steps = min(steps, 100)
+    t0 = time.perf_counter()
+    for g, dt, w in zip(ops, dts, waves):
+        R.batch_lserk_ports([g], [dt], seq_steps, [w[:seq_steps]])
+    t_seq = time.perf_counter() - t0
+    log(f"batched {t_batch * 1e3:.1f} ms for {steps} steps of {len(ops)} operators; one after another "
+        f"{t_seq * 1e3:.1f} ms for {seq_steps} steps")
+    out = {"metric": "batching form DOF-stage/s (8 meshes x 4 port excitations, one graph)",
            "value": dofs * 5 * steps / t_batch, "unit": "DOF-stage/s", "n_gpus": 1, "higher_is_better": True,
-           "dtype": "f64", "data": "synthetic", "steps": steps, "ms_batched": t_batch * 1e3,
-           "ms_sequential": t_seq * 1e3, "sequential_value": dofs * 5 * steps / t_seq,
-           "config": {"workload": "8 seeded jittered boxes (20k-300k tets), orders 2/3/4, 4 RHS each, "
-                                  "f64c registers, free LSERK4 at dt 1e-4", "meshes": meshes,
-                      "operators": len(ops), "total_dofs": dofs}}
+           "dtype": "f64 arithmetic, y f32+bf16 / p f32 registers", "data": "synthetic", "steps": steps,
+           "ms_batched": t_batch * 1e3, "ms_per_step": t_batch * 1e3 / steps,
+           "sequential": {"steps": seq_steps, "ms": t_seq * 1e3, "value": dofs * 5 * seq_steps / t_seq},
+           "final_norms_first_rhs": norms,
+           "config": {"workload": "8 seeded jittered port boxes (20k-300k tets), orders 2/3/4, 4 TE10 port "
+                                  "excitations each (Gaussian pulses, f0 in [3, 8]), f64c registers, LSERK4 at the "
+                                  "CFL step", "meshes": meshes, "operators": len(ops), "total_dofs": dofs}}
     print(json.dumps(out))
     return 0
 
@@ -565,6 +685,8 @@ def main() -> int:
         return traffic_probe(args)
     if args.sparse_export:
         return sparse_export(args)
+    if args.proposer:
+        return proposer_bench(args)
     if args.krylov:
         return krylov_bench(args)
     if args.record:
@@ -669,12 +791,12 @@ def main() -> int:
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-stage/s", "n_gpus": 1, "steps": K, "warmup": Wm,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": DTYPE_LABEL.get(args.form, "f64"), "data": "synthetic",
         "config": {"workload": f"pulse_box {nx}x{ny}x{nz} cells x 6 tets (2x1x1 box), N={args.order}, eps_r=4 "
                                f"block, Gaussian E_x, PEC, upwind, LSERK4 at cfl_dt",
                    "n_elem": op.n_elem, "n_dof": n, "order": args.order, "dt": dt, "form": args.form,
                    "variant": variant,
-                   "registers": "y f32+bf16, p f32" if args.form == "fused_f64c" else "f64"},
+                   "registers": "f64" if args.form == "fused_f64" else "y f32+bf16, p f32"},
         "plain": {"form": "plain_f64", "value": pvalue, "ms_per_step": ptot / args.plain_steps},
         "fused_f64": {"value": fvalue, "variant": ref_variant},
         "speedup_vs_plain": value / pvalue,

@@ -46,11 +46,18 @@ extern "C" {
  *  FUSED_F64  one fused kernel per RK stage (volume curl + upwind flux + LIFT
  *             + materials + stage update), FP64 state and arithmetic.
  *  FUSED_F64C the fused kernel with FP64 arithmetic and compressed
- *             registers: y stored as f32 hi + bf16 lo, p as f32. */
+ *             registers: y stored as f32 hi + bf16 lo, p as f32.
+ *  FUSED_BF16X3, FUSED_BF16X2  the precision forms: FUSED_F64C's registers,
+ *             with the volume contraction D_k u on the matrix cores in BF16
+ *             split into 3 (or 2: hi + lo) pieces per operand, products
+ *             accumulated in FP32; J^-1, curl, flux, LIFT and the update stay
+ *             FP64. Orders 2..4. */
 #define RFHIP_FORM_PLAIN_F32 0
 #define RFHIP_FORM_PLAIN_F64 1
 #define RFHIP_FORM_FUSED_F64 2
 #define RFHIP_FORM_FUSED_F64C 3
+#define RFHIP_FORM_FUSED_BF16X3 4
+#define RFHIP_FORM_FUSED_BF16X2 5
 
 /* Drive of dy/dt = A*y + b*g(t) (session.rs:107-114 Drive). */
 #define RFHIP_DRIVE_FREE 0
@@ -239,6 +246,113 @@ int rfhip_op_expmv(rfhip_op *op, const double *v, double t, int32_t max_dim,
 int rfhip_op_etd_step(rfhip_op *op, const double *y, const double *b,
                       double h, int32_t max_dim, double tol, double *out,
                       int32_t *dim_used);
+/* ---- adaptive Krylov exponential and its learned (h, m) proposer ---------
+ * No reference counterpart: the reference's exponential route takes the
+ * frame step with a fixed (or KRYLOV_TOL-stopped) dimension
+ * (session.rs:700-726, propagator.rs:186-326). These calls integrate
+ * y' = A y over [0, t_end] with a step h and Krylov dimension m chosen per
+ * step. Every step is judged by the a-posteriori estimate of
+ * propagator.rs:278-294, scaled by the propagated norm beta:
+ *   est(h, m) = beta * H[m][m-1] * |[exp(h H_m)]_{m-1,0}|,
+ * accepted when est <= tol * |v| * h / t_end (so the estimates of a run add
+ * up to at most tol * |v|). Hessenberg exponentials: degree-13 Padé with
+ * scaling and squaring, FP64; Arnoldi: CGS2 on the device, FP64. */
+typedef struct rfhip_mlp rfhip_mlp;
+
+typedef struct {
+  double tol;              /* relative error target of the whole run */
+  int32_t m;               /* the plain controller's Krylov dimension (2..64) */
+  int32_t m_min, m_max;    /* range of the proposer's dimensions (2..64) */
+  double h0;               /* first step (operator time); <= 0: the largest
+                              admissible step on the first basis */
+  double safety;           /* step growth factor safety * (bound/est)^(1/m) */
+  double growth_limit;     /* clamps of that factor */
+  double shrink_limit;
+  int64_t max_steps;       /* accepted steps before RFHIP_EINVAL */
+} rfhip_expctl;
+
+/* One accepted step of an adaptive run (the accept/reject log). */
+typedef struct {
+  double t;                /* operator time at the step's start */
+  double h;                /* accepted step */
+  double h_first;          /* first proposal (== h when it was accepted) */
+  double est_first, bound_first;  /* its estimate and bound */
+  double est, bound;       /* estimate and bound of the accepted step */
+  double beta;             /* norm of the propagated vector */
+  double t_left;           /* t_end - t */
+  double ms;               /* wall time of the step: Arnoldi, exponentials */
+  int32_t m;               /* Krylov dimension built */
+  int32_t proposer;        /* 0 plain controller, 1 network */
+  int32_t n_judged;        /* proposals judged (1: the first was accepted) */
+  int32_t breakdown;       /* 1: the Krylov subspace was invariant (exact) */
+  float feat[12];          /* the proposer's features for this step */
+} rfhip_exp_step;
+
+/* out = exp(t_end A) v. proposer NULL: the plain controller (dimension
+ * ctl->m every step, the next step grown from the last estimate, a rejected
+ * proposal shrunk on the same basis, which costs no operator applies).
+ * Otherwise, after two Arnoldi columns of each step the network proposes
+ * (h, m), m clamped to [m_min, m_max]; a rejected proposal falls back to the
+ * largest admissible step <= h on the built basis. steps (capacity
+ * steps_cap, may be NULL) receives each step's record; hess (may be NULL)
+ * receives each logged step's Hessenberg matrix as a zero-padded
+ * (hess_dim+1) x hess_dim row-major block. n_steps (may be NULL) gets the
+ * number of steps taken. */
+int rfhip_op_expmv_adaptive(rfhip_op *op, const double *v, double t_end,
+                            const rfhip_expctl *ctl, rfhip_mlp *proposer,
+                            rfhip_exp_step *steps, int64_t steps_cap,
+                            double *hess, int32_t hess_dim, int64_t *n_steps,
+                            double *out);
+
+/* Hindsight labels for the proposer (host computation, nthreads threads):
+ * for each of count logged steps (Hessenberg blocks as written by
+ * rfhip_op_expmv_adaptive with hess_dim = m_log, dim_built[q] columns
+ * valid), the largest admissible step h_j <= h_cap[q] at each grid
+ * dimension j (estimate as above with beta[q] and bound rate[q] * h, found
+ * by bracketing from h_ref[q] and 12 bisections of log h), and the j of the
+ * highest h_j / cost(j), cost(j) = c0 + c1 j + c2 j (j+1) / 2
+ * (cost_model = {c0, c1, c2}: per step, per operator apply, per basis
+ * column swept). eff_best may be NULL. */
+int rfhip_krylov_labels(const double *hess, int32_t m_log, int64_t count,
+                        const int32_t *dim_built, const double *beta,
+                        const double *rate, const double *h_ref,
+                        const double *h_cap, const int32_t *m_grid,
+                        int32_t n_grid, const double *cost_model,
+                        int32_t nthreads, double *h_best, int32_t *m_best,
+                        double *eff_best);
+/* exp(a) for a row-major n x n matrix (n <= 4096), degree-13 Padé with
+ * scaling and squaring (Higham 2005), FP64, on the host: the Hessenberg
+ * exponential of the adaptive path. (The fixed-step rfhip_op_expmv keeps
+ * the reference's Taylor-18, propagator.rs:69-114.) */
+int rfhip_expm_pade(const double *a, int32_t n, double *out);
+
+/* The proposer: a 12 -> 64 -> 64 -> 2 perceptron, tanh hidden layers,
+ * FP32, run and trained by the library's own HIP kernels. Parameters
+ * (5122 floats): W1[64][12] b1[64] W2[64][64] b2[64] W3[2][64] b3[2],
+ * row-major; normalisation (28 floats): in_mean[12] in_inv_scale[12]
+ * out_mean[2] out_scale[2]. Outputs are (log(h * |H[1][0]|), m). */
+int rfhip_mlp_create(rfhip_ctx *ctx, uint64_t seed, rfhip_mlp **out);
+void rfhip_mlp_destroy(rfhip_mlp *mlp);
+int rfhip_mlp_n_params(void);
+/* Either pointer may be NULL. */
+int rfhip_mlp_get_params(rfhip_mlp *mlp, float *params, float *norm);
+/* Also resets the optimiser's moments. */
+int rfhip_mlp_set_params(rfhip_mlp *mlp, const float *params,
+                         const float *norm);
+/* y[s*2 + o] for the n feature rows x[s*12 + i]. */
+int rfhip_mlp_forward(rfhip_mlp *mlp, const float *x, int64_t n, float *y);
+/* Loss sum_s w_s |yn_s - tn_s|^2 / sum_s w_s in normalised outputs (w NULL:
+ * equal weights) and its gradient with respect to the parameters (grad may
+ * be NULL): the backward kernel's check. */
+int rfhip_mlp_loss_grad(rfhip_mlp *mlp, const float *x, const float *t,
+                        const float *w, int64_t n, double *loss, float *grad);
+/* Adam training (beta1 0.9, beta2 0.999, eps 1e-8) over epochs shuffled
+ * passes (seeded) in minibatches; epoch_loss (may be NULL) gets each
+ * epoch's mean batch loss. */
+int rfhip_mlp_train(rfhip_mlp *mlp, const float *x, const float *t,
+                    const float *w, int64_t n, int32_t epochs, int32_t batch,
+                    float lr, uint64_t seed, double *epoch_loss);
+
 /* Wall time, in milliseconds, of the device part of the operator's last
  * rfhip_op_expmv, rfhip_op_etd_step or rfhip_op_transient_kcl(_f64) call:
  * from its input resident on the device to its output ready, the host
@@ -257,6 +371,11 @@ int rfhip_op_n_variants(const rfhip_op *op);
 int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best);
 int rfhip_op_set_variant(rfhip_op *op, int variant);
 int rfhip_op_variant(const rfhip_op *op);
+/* 1 when the variant runs as itself on this operator (some variants need the
+ * neighbour-trace table of a conforming mesh, one needs Np <= 20);
+ * rfhip_op_set_variant refuses the others with RFHIP_EUNSUPPORTED and the
+ * autotuner skips them. */
+int rfhip_op_variant_available(const rfhip_op *op, int variant);
 /* Test aid (no reference counterpart): fills the shared memory (LDS) of
  * every CU with NaN on the operator's stream and waits, so a kernel variant
  * that reads LDS it did not write fails its parity check instead of
@@ -285,6 +404,18 @@ int rfhip_op_state_lserk_graph(rfhip_op *op, double dt, int64_t nsteps);
  * would leave it. The operators must share a context and be distinct. */
 int rfhip_batch_lserk(rfhip_op *const *ops, int32_t n_ops, const double *dt,
                       int64_t nsteps);
+/* The batching form with port excitations (no reference counterpart): as
+ * rfhip_batch_lserk, with ops[i] driven, when g[i] is not NULL, by its port
+ * source as last assembled on the device (rfhip_op_port_source_f64) times
+ * the waveform g[i][k] held over step k (nsteps samples; the reference
+ * samples the waveform once per substep, session.rs:673-678, 885-927).
+ * g[i] == NULL steps ops[i] free. The samples are uploaded once and indexed
+ * on the device, so one captured graph of two steps of every operator serves
+ * the whole run. RFHIP_EINVAL when a driven operator has no resident port
+ * source. */
+int rfhip_batch_lserk_ports(rfhip_op *const *ops, int32_t n_ops,
+                            const double *dt, int64_t nsteps,
+                            const double *const *g);
 /* As rfhip_op_state_lserk, timed with HIP events on the operator's stream:
  * total_ms for the whole run and stage_ms_avg, the mean duration of one
  * launch of the dominant kernel (the fused stage kernel, or the plain form's

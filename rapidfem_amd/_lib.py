@@ -29,7 +29,9 @@ FORM_PLAIN_F32 = 0
 FORM_PLAIN_F64 = 1
 FORM_FUSED_F64 = 2
 FORM_FUSED_F64C = 3
-FORMS = {"plain_f32": 0, "plain_f64": 1, "fused_f64": 2, "fused_f64c": 3}
+FORM_FUSED_BF16X3 = 4
+FORM_FUSED_BF16X2 = 5
+FORMS = {"plain_f32": 0, "plain_f64": 1, "fused_f64": 2, "fused_f64c": 3, "fused_bf16x3": 4, "fused_bf16x2": 5}
 
 DRIVE_FREE, DRIVE_POINT, DRIVE_VECTOR = 0, 1, 2
 
@@ -53,6 +55,20 @@ class ControllerC(C.Structure):
 
 class KclStatsC(C.Structure):
     _fields_ = [("accepted", _i64), ("rejected", _i64), ("h_min", _d), ("h_max", _d)]
+
+
+class ExpCtlC(C.Structure):
+    """rfhip_expctl: the adaptive Krylov exponential's controller."""
+
+    _fields_ = [("tol", _d), ("m", _i32), ("m_min", _i32), ("m_max", _i32), ("h0", _d), ("safety", _d),
+                ("growth_limit", _d), ("shrink_limit", _d), ("max_steps", _i64)]
+
+
+# rfhip_exp_step as a numpy record (same layout as the C struct)
+EXP_STEP_DTYPE = np.dtype([("t", "f8"), ("h", "f8"), ("h_first", "f8"), ("est_first", "f8"), ("bound_first", "f8"),
+                           ("est", "f8"), ("bound", "f8"), ("beta", "f8"), ("t_left", "f8"), ("ms", "f8"),
+                           ("m", "i4"), ("proposer", "i4"), ("n_judged", "i4"), ("breakdown", "i4"),
+                           ("feat", "f4", (12,))])
 
 
 # Exported symbols and signatures, in header order. tests/ check every one
@@ -89,17 +105,32 @@ SIGNATURES = {
                                              C.POINTER(ControllerC), _dp, C.POINTER(KclStatsC)]),
     "rfhip_op_expmv": (C.c_int, [_vp, _dp, _d, _i32, _d, _dp, C.POINTER(_i32)]),
     "rfhip_op_etd_step": (C.c_int, [_vp, _dp, _dp, _d, _i32, _d, _dp, C.POINTER(_i32)]),
+    "rfhip_op_expmv_adaptive": (C.c_int, [_vp, _dp, _d, C.POINTER(ExpCtlC), _vp, _vp, _i64, _vp, _i32,
+                                          C.POINTER(_i64), _dp]),
+    "rfhip_krylov_labels": (C.c_int, [_dp, _i32, _i64, _ip, _dp, _dp, _dp, _dp, _ip, _i32, _dp, _i32, _dp, _ip,
+                                      _vp]),
+    "rfhip_expm_pade": (C.c_int, [_dp, _i32, _dp]),
+    "rfhip_mlp_create": (C.c_int, [_vp, C.c_uint64, C.POINTER(_vp)]),
+    "rfhip_mlp_destroy": (None, [_vp]),
+    "rfhip_mlp_n_params": (C.c_int, []),
+    "rfhip_mlp_get_params": (C.c_int, [_vp, _vp, _vp]),
+    "rfhip_mlp_set_params": (C.c_int, [_vp, _vp, _vp]),
+    "rfhip_mlp_forward": (C.c_int, [_vp, _fp, _i64, _fp]),
+    "rfhip_mlp_loss_grad": (C.c_int, [_vp, _fp, _fp, _vp, _i64, C.POINTER(_d), _vp]),
+    "rfhip_mlp_train": (C.c_int, [_vp, _fp, _fp, _vp, _i64, _i32, _i32, C.c_float, C.c_uint64, _vp]),
     "rfhip_op_last_call_ms": (C.c_int, [_vp, C.POINTER(_d)]),
     "rfhip_op_n_variants": (C.c_int, [_vp]),
     "rfhip_op_autotune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(_d)]),
     "rfhip_op_set_variant": (C.c_int, [_vp, C.c_int]),
     "rfhip_op_poison_lds": (C.c_int, [_vp]),
     "rfhip_op_variant": (C.c_int, [_vp]),
+    "rfhip_op_variant_available": (C.c_int, [_vp, C.c_int]),
     "rfhip_op_state_load": (C.c_int, [_vp, _dp]),
     "rfhip_op_state_store": (C.c_int, [_vp, _dp]),
     "rfhip_op_state_lserk": (C.c_int, [_vp, _d, _i64]),
     "rfhip_op_state_lserk_graph": (C.c_int, [_vp, _d, _i64]),
     "rfhip_batch_lserk": (C.c_int, [C.POINTER(_vp), _i32, _dp, _i64]),
+    "rfhip_batch_lserk_ports": (C.c_int, [C.POINTER(_vp), _i32, _dp, _i64, C.POINTER(_vp)]),
     "rfhip_op_state_lserk_timed": (C.c_int, [_vp, _d, _i64, C.POINTER(_d), C.POINTER(_d)]),
     "rfhip_op_state_energy": (C.c_int, [_vp, _dp, _dp, C.POINTER(_d)]),
     "rfhip_op_energy_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -19,8 +19,8 @@ OUT_DIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "librfhip.so")
 
 HIP_SOURCES = ["rfhip_api.hip", "rfhip_plain.hip", "rfhip_fused.hip", "rfhip_vec.hip", "rfhip_debye.hip",
-               "rfhip_sparse.hip", "rfhip_xfer.hip", "rfhip_noalloc.hip"]
-CXX_SOURCES = ["host_builder.cpp"]
+               "rfhip_sparse.hip", "rfhip_xfer.hip", "rfhip_mlp.hip", "rfhip_noalloc.hip"]
+CXX_SOURCES = ["host_builder.cpp", "krylov_host.cpp"]
 
 ARCH = os.environ.get("RFHIP_ARCH", "gfx950")
 

@@ -12,6 +12,7 @@
 #include <type_traits>
 
 #include "rfhip_impl.hpp"
+#include "rfhip_krylov.hpp"
 
 namespace rfhip {
 
@@ -38,7 +39,6 @@ constexpr double kKclErrFloor = 1e-12;   // constants.rs KCL_ERR_FLOOR
 constexpr double kKclNonfinite = 10.0;   // constants.rs KCL_NONFINITE_ERR
 constexpr double kArnoldiBreakdown = 1e-12;  // constants.rs:62
 constexpr int kMaxKrylov = 64;
-constexpr int kFusedVariants = 24;  // rfhip_fused.hip launch_stage
 
 template <class F>
 int guarded(F &&f) {
@@ -57,7 +57,8 @@ int guarded(F &&f) {
   }
 }
 
-bool is_fused(int form) { return form == RFHIP_FORM_FUSED_F64 || form == RFHIP_FORM_FUSED_F64C; }
+bool is_fused(int form) { return form_fused(form); }
+int n_stage_variants(int form) { return form_vmf(form) ? kStageVariantsBf16 : kStageVariants; }
 bool f64_regs(int form) { return form != RFHIP_FORM_PLAIN_F32; }
 
 // ---- reference-element conventions used to check and encode faces -------
@@ -169,12 +170,13 @@ void build_plain(rfhip_op *op, const rfhip_op_desc &d) {
 // (pulse boxes of 1.3M / 393k / 324k / 96k tets at N = 1..4). autotune
 // re-picks on the operator's own mesh.
 int default_variant(int order, int form) {
+  if (form_vmf(form)) return 1;
   const bool c = form == RFHIP_FORM_FUSED_F64C;
   switch (order) {
-    case 1: return c ? 23 : 17;
-    case 2: return c ? 16 : 15;
-    case 3: return c ? 23 : 16;
-    case 4: return c ? 12 : 14;
+    case 1: return c ? 7 : 5;
+    case 2: return c ? 4 : 3;
+    case 3: return c ? 7 : 4;
+    case 4: return c ? 1 : 2;
     default: return 0;
   }
 }
@@ -303,20 +305,10 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
       for (int i = 0; i < np; ++i) dT[(k * np + j) * np + i] = D[k][i * np + j];
   for (int fm = 0; fm < nf4; ++fm)
     for (int i = 0; i < np; ++i) liftc[(size_t)fm * np + i] = d.lift[(size_t)i * nf4 + fm];
-  // A-operand fragments of the stacked [Dr; Ds; Dt] (3Np x Np) for the FP64
-  // MFMA 16x16x4 volume kernel: afrag[(mt*KS + ks)*64 + lane] holds row
-  // s = 16 mt + lane%16 (= k*Np + i), column j = 4 ks + lane/16, zero padded.
-  const int mtiles = (3 * np + 15) / 16, ksteps = (np + 3) / 4;
-  std::vector<double> afrag((size_t)mtiles * ksteps * 64, 0.0);
-  for (int mt = 0; mt < mtiles; ++mt)
-    for (int ks = 0; ks < ksteps; ++ks)
-      for (int l = 0; l < 64; ++l) {
-        const int srow = 16 * mt + l % 16, j = 4 * ks + l / 16;
-        if (srow < 3 * np && j < np) afrag[((size_t)mt * ksteps + ks) * 64 + l] = D[srow / np][(srow % np) * np + j];
-      }
   upload(op->dT, dT.data(), dT.size(), s);
   upload(op->liftc, liftc.data(), liftc.size(), s);
-  // ... and of LIFT (Np x 4Nfp): lfrag[(mt*KSL + ks)*64 + lane] holds
+  // A-operand fragments of LIFT (Np x 4Nfp) for the FP64 MFMA 16x16x4 lift
+  // of k_fused8: lfrag[(mt*KSL + ks)*64 + lane] holds
   // LIFT[16 mt + lane%16][4 ks + lane/16], zero padded.
   const int ltiles = (np + 15) / 16, lsteps = (nf4 + 3) / 4;
   std::vector<double> lfrag((size_t)ltiles * lsteps * 64, 0.0);
@@ -326,11 +318,11 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
         const int i = 16 * mt + l % 16, fm = 4 * ks + l / 16;
         if (i < np && fm < nf4) lfrag[((size_t)mt * lsteps + ks) * 64 + l] = d.lift[(size_t)i * nf4 + fm];
       }
-  // ... and node-interleaved for k_fused7: wave w owns nodes 4w..4w+3 and
+  // ... and the node-interleaved [Dr; Ds; Dt] of its volume: wave w owns nodes 4w..4w+3 and
   // its A rows are (k, node) = (row / 4, 4w + row % 4), row 4k + q, k < 3
   // (row tile 16 = 3 derivatives x 4 nodes + a zero row per node), so one
   // lane receives all three reference derivatives of its node.
-  const int nwaves7 = (np + 3) / 4;
+  const int nwaves7 = (np + 3) / 4, ksteps = (np + 3) / 4;
   std::vector<double> afrag7((size_t)nwaves7 * ksteps * 64, 0.0);
   for (int w = 0; w < nwaves7; ++w)
     for (int ks = 0; ks < ksteps; ++ks)
@@ -338,9 +330,44 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
         const int row = l % 16, k = row / 4, node = 4 * w + row % 4, j = 4 * ks + l / 16;
         if (k < 3 && node < np && j < np) afrag7[((size_t)w * ksteps + ks) * 64 + l] = D[k][node * np + j];
       }
-  upload(op->afrag, afrag.data(), afrag.size(), s);
   upload(op->lfrag, lfrag.data(), lfrag.size(), s);
   upload(op->afrag7, afrag7.data(), afrag7.size(), s);
+  // Split-BF16 A fragments of [Dr; Ds; Dt] for the precision forms' volume on
+  // v_mfma_f32_16x16x32_bf16: afragb[((p*MT + mt)*KS + ks)*64 + lane] holds
+  // 8 bf16, piece p of D_k[i][j] for row 16 mt + lane%16 = k*Np + i and
+  // j = 32 ks + 8 (lane/16) + t, zero outside; with t_0 = D_k[i][j], piece
+  // q = bf16(t_q) rounded to nearest even and t_{q+1} = t_q - piece q in FP64.
+  std::vector<uint16_t> fragb;
+  if (const int vmf = form_vmf(op->form)) {
+    const int mtv = (3 * np + 15) / 16, ksv = (np + 31) / 32;
+    fragb.assign((size_t)vmf * mtv * ksv * 64 * 8, 0);
+    auto bf16_bits = [](float x) {
+      uint32_t u;
+      std::memcpy(&u, &x, 4);
+      u += 0x7FFFu + ((u >> 16) & 1u);
+      return (uint16_t)(u >> 16);
+    };
+    auto bf16_value = [](uint16_t b) {
+      const uint32_t u = (uint32_t)b << 16;
+      float f;
+      std::memcpy(&f, &u, 4);
+      return f;
+    };
+    for (int mt = 0; mt < mtv; ++mt)
+      for (int ks = 0; ks < ksv; ++ks)
+        for (int l = 0; l < 64; ++l)
+          for (int t = 0; t < 8; ++t) {
+            const int row = 16 * mt + l % 16, j = 32 * ks + 8 * (l / 16) + t;
+            if (row >= 3 * np || j >= np) continue;
+            double v = D[row / np][(row % np) * np + j];
+            for (int p = 0; p < vmf; ++p) {
+              const uint16_t b = bf16_bits((float)v);
+              fragb[(((size_t)(p * mtv + mt) * ksv + ks) * 64 + l) * 8 + t] = b;
+              v -= (double)bf16_value(b);
+            }
+          }
+    upload(op->afragb, fragb.data(), fragb.size(), s);
+  }
   upload(op->jinv64, d.jinv, 9 * (size_t)K, s);
   upload(op->fcode, fcode.data(), fcode.size(), s);
   upload(op->mat, mat.data(), mat.size(), s);
@@ -367,10 +394,9 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   a.n_elem = K;
   a.alpha = d.flux_alpha;
   a.dT = op->dT.as<double>();
-  a.dsparse = d_fits_sparse_pattern(op->order, dT.data()) ? 1 : 0;
   a.lift = op->liftc.as<double>();
-  a.afrag = op->afrag.as<double>();
   a.lfrag = op->lfrag.as<double>();
+  a.afragb = form_vmf(op->form) ? op->afragb.p : nullptr;
   a.afrag7 = op->afrag7.as<double>();
   a.jinv = op->jinv64.as<double>();
   a.fcode = op->fcode.as<int>();
@@ -456,9 +482,13 @@ void rhs_into(rfhip_op *op, const void *x, void *k, int kind, int64_t dof, doubl
 }
 
 // One LSERK4 step on the resident state; stage_ev (optional) gets an event
-// pair around each dominant-kernel launch.
-void lserk_step(rfhip_op *op, double h, int kind, int64_t dof, double g, hipEvent_t *stage_ev) {
+// pair around each dominant-kernel launch. dev_wave: the drive is the
+// device-assembled port source held at the waveform sample
+// op->gseq[*op->gctr] (kind/dof/g unused), and the step advances *op->gctr.
+void lserk_step(rfhip_op *op, double h, int kind, int64_t dof, double g, hipEvent_t *stage_ev,
+                bool dev_wave = false) {
   hipStream_t s = op->stream;
+  if (dev_wave) kind = RFHIP_DRIVE_FREE;
   for (int st = 0; st < 5; ++st) {
     if (stage_ev) RF_HIP(hipEventRecord(stage_ev[2 * st], s));
     if (is_fused(op->form)) {
@@ -475,6 +505,7 @@ void lserk_step(rfhip_op *op, double h, int kind, int64_t dof, double g, hipEven
       launch_stage(op->order, op->form, 1, a, s);
       if (stage_ev) RF_HIP(hipEventRecord(stage_ev[2 * st + 1], s));
       launch_debye_stage(op, a, s);
+      if (dev_wave) launch_port_stage(op, false, a, nullptr, h, kLsB[st], s);
       op->cur = 1 - op->cur;
     } else {
       plain_apply_full(op, op->y[0].p, op->k.p, s);
@@ -483,20 +514,22 @@ void lserk_step(rfhip_op *op, double h, int kind, int64_t dof, double g, hipEven
         launch_add_point(op, op->k.p, dof, g, s);
       else if (kind == RFHIP_DRIVE_VECTOR)
         launch_add_vec(op, op->k.p, op->src.p, g, s);
+      if (dev_wave) launch_port_stage(op, true, op->sa, op->k.p, h, kLsB[st], s);
       launch_plain_stage(op, op->p.p, op->k.p, op->y[0].p, kLsA[st], kLsB[st], h, s);
     }
   }
+  if (dev_wave) launch_step_advance(op, s);
 }
 
 void zero_p(rfhip_op *op) {
-  const size_t w = (op->form == RFHIP_FORM_PLAIN_F32 || op->form == RFHIP_FORM_FUSED_F64C) ? 4 : 8;
+  const size_t w = (op->form == RFHIP_FORM_PLAIN_F32 || form_compressed(op->form)) ? 4 : 8;
   launch_fill(op->p.p, (int64_t)(w * nvec(op)), op->stream);
 }
 
 // Load an f64 host state into the form's state registers.
 void state_from_host_f64(rfhip_op *op, const double *y0) {
   op->cur = 0;
-  if (op->form == RFHIP_FORM_FUSED_F64C) {
+  if (form_compressed(op->form)) {
     h2d_layout_f64(op, y0, op->k.as<double>());
     launch_f64_to_c(op->k.as<double>(), op->y[0].as<float>(), op->ylo[0].as<uint16_t>(), nvec(op), op->stream);
   } else {
@@ -505,7 +538,7 @@ void state_from_host_f64(rfhip_op *op, const double *y0) {
 }
 // Current state as an f64 layout vector on the device.
 const double *state_layout_f64(rfhip_op *op) {
-  if (op->form == RFHIP_FORM_FUSED_F64C) {
+  if (form_compressed(op->form)) {
     launch_c_to_f64(op->y[op->cur].as<float>(), op->ylo[op->cur].as<uint16_t>(), op->k.as<double>(), nvec(op),
                     op->stream);
     return op->k.as<double>();
@@ -795,6 +828,316 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
   return dim;
 }
 
+// ---- adaptive Krylov exponential ------------------------------------------
+//
+// Arnoldi with CGS2 (the device sweeps of krylov_impl's plain path), built
+// one column at a time so a step's dimension can be chosen after its first
+// columns. The Hessenberg matrix is (cap+1) x cap, row stride cap; a column
+// j+1 of the basis is scaled from w when column j+1 is extended, so a step
+// that stops at dimension d never writes basis column d.
+struct Arnoldi {
+  rfhip_op *op = nullptr;
+  int64_t n = 0;
+  int cap = 0, cols = 0, nblk = 0;
+  bool broke = false;
+  double beta = 0.0;
+  std::vector<double> H;
+
+  Arnoldi(rfhip_op *o, int capacity) : op(o), n(nvec(o)), cap(capacity) {
+    if (op->kb_cap < cap) {
+      op->kb_basis.alloc(8 * (size_t)(cap + 1) * n);
+      op->kb_cap = cap;
+    }
+    op->kb_w.alloc(8 * n);
+    op->kb_x.alloc(8 * n);
+    op->kb_out.alloc(8 * n);
+    op->kb_proj.alloc(8 * (kMaxKrylov + 8));
+    op->kb_host.alloc(8 * (2 * kMaxKrylov + 8));
+    nblk = krylov_dot_blocks(n);
+    op->kb_part.alloc(8 * (size_t)nblk * (kMaxKrylov + 1));
+    if (op->form == RFHIP_FORM_PLAIN_F32) {
+      op->kst.alloc(4 * n);
+      op->k.alloc(4 * n);
+    }
+    H.assign((size_t)(cap + 1) * cap, 0.0);
+  }
+  double *basis() const { return op->kb_basis.as<double>(); }
+  // v_0 = x / |x| for the f64 layout vector x on the device.
+  void begin(const double *x) {
+    hipStream_t s = op->stream;
+    std::fill(H.begin(), H.end(), 0.0);
+    cols = 0;
+    broke = false;
+    double *proj = op->kb_proj.as<double>();
+    launch_multi_dot(x, n, 1, x, op->kb_part.as<double>(), nblk, proj, s);
+    double nrm2 = 0.0;
+    RF_HIP(hipMemcpyAsync(&nrm2, proj, 8, hipMemcpyDeviceToHost, s));
+    sync(op);
+    beta = std::sqrt(nrm2);
+    if (beta == 0.0) {
+      broke = true;
+      return;
+    }
+    launch_scale(x, basis(), n, 1.0 / beta, s);
+  }
+  // Column `cols` of H: w = A v_j, two classical Gram-Schmidt passes, the
+  // norm; sets broke (and leaves H[j+1][j] = 0) below the breakdown
+  // threshold of propagator.rs.
+  void extend() {
+    hipStream_t s = op->stream;
+    const int j = cols;
+    double *w = op->kb_w.as<double>(), *proj = op->kb_proj.as<double>(), *part = op->kb_part.as<double>();
+    if (j > 0) launch_scale(w, basis() + (int64_t)j * n, n, 1.0 / H[(size_t)j * cap + (j - 1)], s);
+    const double *bj = basis() + (int64_t)j * n;
+    if (op->form == RFHIP_FORM_PLAIN_F32) {
+      launch_cast_d2f(bj, op->kst.as<float>(), n, s);
+      plain_apply_full(op, op->kst.p, op->k.p, s);
+      launch_cast_f2d(op->k.as<float>(), w, n, s);
+    } else {
+      rhs_into(op, bj, w, RFHIP_DRIVE_FREE, 0, 0.0);
+    }
+    const int c = j + 1;
+    double *hp = op->kb_host.as<double>();
+    launch_multi_dot(basis(), n, c, w, part, nblk, proj, s);
+    RF_HIP(hipMemcpyAsync(hp, proj, 8 * c, hipMemcpyDeviceToHost, s));
+    launch_axpy_dot(w, basis(), n, c, proj, part, nblk, proj, s);
+    RF_HIP(hipMemcpyAsync(hp + kMaxKrylov, proj, 8 * c, hipMemcpyDeviceToHost, s));
+    launch_axpy_basis_norm(w, basis(), n, c, proj, part, nblk, proj + kMaxKrylov, s);
+    RF_HIP(hipMemcpyAsync(hp + 2 * kMaxKrylov, proj + kMaxKrylov, 8, hipMemcpyDeviceToHost, s));
+    sync(op);
+    for (int pass = 0; pass < 2; ++pass)
+      for (int i = 0; i < c; ++i) H[(size_t)i * cap + j] += hp[pass * kMaxKrylov + i];
+    const double hnext = std::sqrt(hp[2 * kMaxKrylov]);
+    cols = c;
+    if (!(hnext >= kArnoldiBreakdown)) {
+      broke = true;
+      return;
+    }
+    H[(size_t)c * cap + j] = hnext;
+  }
+  // out = beta V_d exp(h H_d) e_1 (f64 layout vector on the device).
+  void combine(int d, double h, double *out, KrylovScratch &ws) {
+    ws.m.assign((size_t)d * d, 0.0);
+    ws.e.assign((size_t)d * d, 0.0);
+    for (int a = 0; a < d; ++a)
+      for (int b = 0; b < d; ++b) ws.m[(size_t)a * d + b] = h * H[(size_t)a * cap + b];
+    expm_pade13(ws.m.data(), d, ws.e.data(), ws.work);
+    std::vector<double> coef(d);
+    for (int i = 0; i < d; ++i) coef[i] = beta * ws.e[(size_t)i * d];
+    double *proj = op->kb_proj.as<double>();
+    RF_HIP(hipMemcpyAsync(proj, coef.data(), 8 * d, hipMemcpyHostToDevice, op->stream));
+    launch_lincomb(basis(), n, d, proj, out, op->stream);
+    sync(op);  // coef lives on this stack frame
+  }
+};
+
+// exp(t_end A) v with a step size and Krylov dimension chosen per step: the
+// plain error-controlled controller, or the network's proposals judged by the
+// same a-posteriori estimate with the plain controller's largest admissible
+// step as the fallback.
+void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip_expctl &ctl, rfhip_mlp *mlp,
+                   rfhip_exp_step *log, int64_t log_cap, double *hess, int hess_dim, int64_t *n_steps, double *out) {
+  const int cap = std::max(ctl.m, ctl.m_max);
+  Arnoldi ar(op, cap);
+  const int64_t n = ar.n;
+  double *x = op->kb_x.as<double>(), *y = op->kb_out.as<double>();
+  h2d_layout_f64(op, v_host, x);
+  sync(op);
+  const auto t_begin = std::chrono::steady_clock::now();
+  KrylovScratch ws;
+  ProposalHistory hist;
+  double t = 0.0, h_next = ctl.h0 > 0.0 ? ctl.h0 : 0.0, rate = 0.0;
+  int64_t nst = 0;
+  bool zero = false;
+  while (t_end - t > 1e-14 * t_end) {
+    RF_REQUIRE(nst < ctl.max_steps, "adaptive exponential: more than max_steps steps");
+    const auto t_step = std::chrono::steady_clock::now();
+    ar.begin(x);
+    if (ar.beta == 0.0) {
+      zero = true;
+      break;
+    }
+    if (nst == 0) rate = ctl.tol * ar.beta / t_end;
+    const double t_left = t_end - t;
+    ar.extend();
+    if (!ar.broke) ar.extend();
+    rfhip_exp_step rec{};
+    rec.t = t;
+    rec.beta = ar.beta;
+    rec.t_left = t_left;
+    const bool early = ar.broke;  // an invariant subspace of dimension 1 or 2
+    if (!early) proposal_features(ar.H.data(), cap, hist, ar.beta, ctl.tol, t_left, rec.feat);
+    double h_prop;
+    int m_prop;
+    if (early) {
+      h_prop = t_left;
+      m_prop = ar.cols;
+    } else if (mlp) {
+      float yo[2];
+      int rc = rfhip_mlp_forward(mlp, rec.feat, 1, yo);
+      if (rc != RFHIP_OK) throw Error{rc, "proposer forward pass failed"};
+      const double s = std::max(std::fabs(ar.H[(size_t)cap]), 1e-300);
+      const double hp = std::exp(std::clamp((double)yo[0], -700.0, 700.0)) / s;
+      h_prop = std::isfinite(hp) ? std::clamp(hp, 1e-9 * t_end, t_left) : t_left;
+      m_prop = std::clamp((int)std::lround(std::isfinite(yo[1]) ? yo[1] : (float)ctl.m), std::max(ctl.m_min, 2), ctl.m_max);
+      rec.proposer = 1;
+    } else {
+      h_prop = h_next > 0.0 ? std::min(h_next, t_left) : 0.0;
+      m_prop = ctl.m;
+    }
+    while (!ar.broke && ar.cols < m_prop) ar.extend();
+    const int d = ar.cols;
+    rec.m = d;
+    rec.breakdown = ar.broke ? 1 : 0;
+    if (ar.broke) h_prop = t_left;  // exact in an invariant subspace
+    if (h_prop <= 0.0) h_prop = max_admissible_h(ar.H.data(), cap, d, ar.beta, rate, t_left * 1e-3, t_left, 12, ws);
+    RF_REQUIRE(h_prop > 0.0, "adaptive exponential: no admissible step on the first basis");
+    double est = ar.broke ? 0.0 : krylov_estimate(ar.H.data(), cap, d, ar.beta, h_prop, ws);
+    double h = h_prop;
+    rec.h_first = h_prop;
+    rec.est_first = est;
+    rec.bound_first = rate * h_prop;
+    rec.n_judged = 1;
+    if (!(est <= rate * h)) {
+      if (mlp) {  // fallback: the plain controller's largest admissible step on this basis
+        h = max_admissible_h(ar.H.data(), cap, d, ar.beta, rate, h_prop, h_prop, 12, ws);
+        est = krylov_estimate(ar.H.data(), cap, d, ar.beta, h, ws);
+        ++rec.n_judged;
+      } else {  // shrink on the same basis until the estimate accepts
+        for (;;) {
+          const double r = est / (rate * h);
+          h *= std::max(ctl.shrink_limit, std::min(ctl.safety * std::pow(r, -1.0 / d), 1.0 - 1e-3));
+          est = krylov_estimate(ar.H.data(), cap, d, ar.beta, h, ws);
+          ++rec.n_judged;
+          if (est <= rate * h) break;
+          RF_REQUIRE(h > 1e-12 * t_end && rec.n_judged < 200,
+                     "adaptive exponential: step size collapsed (tolerance too tight for this dimension)");
+        }
+      }
+      RF_REQUIRE(h > 0.0, "adaptive exponential: step size collapsed");
+    }
+    rec.h = h;
+    rec.est = est;
+    rec.bound = rate * h;
+    ar.combine(d, h, y, ws);
+    if (hess && hess_dim > 0 && nst < log_cap) {
+      double *blk = hess + (size_t)nst * (hess_dim + 1) * hess_dim;
+      std::fill(blk, blk + (size_t)(hess_dim + 1) * hess_dim, 0.0);
+      for (int a = 0; a <= std::min(d, hess_dim); ++a)
+        for (int b = 0; b < std::min(d, hess_dim); ++b) blk[(size_t)a * hess_dim + b] = ar.H[(size_t)a * cap + b];
+    }
+    rec.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_step).count();
+    if (log && nst < log_cap) log[nst] = rec;
+    ++nst;
+    // next step: the controller's growth from this step's estimate
+    const double r = est / (rate * h);
+    const double grow = r > 0.0 ? ctl.safety * std::pow(r, -1.0 / std::max(d, 1)) : ctl.growth_limit;
+    h_next = h * std::clamp(grow, ctl.shrink_limit, ctl.growth_limit);
+    hist.h_prev = h;
+    hist.s_prev = std::fabs(ar.H[(size_t)cap]);
+    hist.beta_prev = ar.beta;
+    hist.est_prev = est;
+    hist.bound_prev = rate * h;
+    hist.m_prev = d;
+    hist.accepted_prev = rec.n_judged == 1;
+    t += h;
+    std::swap(x, y);
+  }
+  if (zero) launch_fill(x, 8 * n, op->stream);
+  sync(op);
+  op->last_call_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_begin).count();
+  if (n_steps) *n_steps = nst;
+  d2h_layout_f64(op, x, out);
+}
+
+
+// nsteps free (or, with waves, port-driven) LSERK4 steps of every operator's
+// resident state, two steps of all of them captured into one graph that
+// forks from ops[0]'s stream to every operator's stream and joins back.
+// waves[i] (when waves is given and waves[i] is not null): nsteps waveform
+// samples of ops[i]'s device-assembled port source, held per step; they are
+// uploaded once and read on the device through a step index, so the same
+// graph replays every pair of steps.
+void batch_lserk_impl(rfhip_op *const *ops, int32_t n_ops, const double *dt, int64_t nsteps,
+                      const double *const *waves) {
+  RF_REQUIRE(ops && dt && n_ops >= 1 && nsteps >= 0, "bad arguments");
+  std::vector<char> driven((size_t)n_ops, 0);
+  for (int32_t i = 0; i < n_ops; ++i) {
+    RF_REQUIRE(ops[i], "null operator in the batch");
+    RF_REQUIRE(ops[i]->ctx == ops[0]->ctx, "batched operators must share one context");
+    RF_REQUIRE(std::isfinite(dt[i]), "non-finite step");
+    for (int32_t j = 0; j < i; ++j) RF_REQUIRE(ops[j] != ops[i], "operator listed twice in the batch");
+    if (waves && waves[i]) {
+      RF_REQUIRE(ops[i]->src_resident, "a port-driven operator needs its port source assembled on the device "
+                                       "(rfhip_op_port_source_f64) and not replaced since");
+      for (int64_t k = 0; k < nsteps; ++k) RF_REQUIRE(std::isfinite(waves[i][k]), "non-finite waveform sample");
+      driven[(size_t)i] = 1;
+    }
+  }
+  for (int32_t i = 0; i < n_ops; ++i) {
+    rfhip_op *op = ops[i];
+    zero_p(op);
+    if (driven[(size_t)i]) {
+      op->gseq.alloc(8 * (size_t)std::max<int64_t>(nsteps, 1));
+      op->gctr.alloc(8);
+      if (nsteps > 0)
+        RF_HIP(hipMemcpyAsync(op->gseq.p, waves[i], 8 * (size_t)nsteps, hipMemcpyHostToDevice, op->stream));
+      launch_fill(op->gctr.p, 8, op->stream);
+    }
+    sync(op);
+  }
+  hipStream_t s0 = ops[0]->stream;
+  std::vector<hipEvent_t> ev((size_t)n_ops + 1, nullptr);
+  hipGraphExec_t exec = nullptr;
+  auto cleanup = [&] {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  auto step_all = [&] {
+    for (int32_t i = 0; i < n_ops; ++i)
+      lserk_step(ops[i], dt[i], RFHIP_DRIVE_FREE, 0, 0.0, nullptr, driven[(size_t)i] != 0);
+  };
+  try {
+    for (auto &e : ev) RF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (nsteps >= 2) {
+      // The capture runs nothing, so each operator's buffer parity is restored.
+      std::vector<int> cur0((size_t)n_ops);
+      for (int32_t i = 0; i < n_ops; ++i) cur0[(size_t)i] = ops[i]->cur;
+      hipGraph_t graph = nullptr;
+      RF_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+      try {
+        RF_HIP(hipEventRecord(ev[0], s0));
+        for (int32_t i = 1; i < n_ops; ++i) RF_HIP(hipStreamWaitEvent(ops[i]->stream, ev[0], 0));
+        step_all();
+        step_all();
+        for (int32_t i = 1; i < n_ops; ++i) {
+          RF_HIP(hipEventRecord(ev[(size_t)i], ops[i]->stream));
+          RF_HIP(hipStreamWaitEvent(s0, ev[(size_t)i], 0));
+        }
+      } catch (...) {
+        (void)hipStreamEndCapture(s0, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (int32_t i = 0; i < n_ops; ++i) ops[i]->cur = cur0[(size_t)i];
+        throw;
+      }
+      RF_HIP(hipStreamEndCapture(s0, &graph));
+      for (int32_t i = 0; i < n_ops; ++i) ops[i]->cur = cur0[(size_t)i];
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      RF_HIP(e);
+      for (int64_t k = 0; k + 1 < nsteps; k += 2) RF_HIP(hipGraphLaunch(exec, s0));
+      RF_HIP(hipStreamSynchronize(s0));
+    }
+    if (nsteps % 2) step_all();
+    for (int32_t i = 0; i < n_ops; ++i) sync(ops[i]);
+  } catch (...) {
+    for (int32_t i = 0; i < n_ops; ++i) (void)hipStreamSynchronize(ops[i]->stream);
+    cleanup();
+    throw;
+  }
+  cleanup();
+}
 }  // namespace
 
 // propagator.rs:69-114: infinity-norm scaling to <= 0.5, Taylor 18, squaring.
@@ -935,7 +1278,7 @@ int rfhip_op_create_dispersive(rfhip_ctx *ctx, const rfhip_op_desc *d, int form,
     RF_REQUIRE(ctx && d && out, "null argument");
     RF_REQUIRE(n_disp >= 0, "negative number of dispersive elements");
     RF_REQUIRE(n_disp == 0 || (disp_elem && disp_a && disp_g && disp_inv_eps_inf), "missing dispersive array");
-    RF_REQUIRE(form >= RFHIP_FORM_PLAIN_F32 && form <= RFHIP_FORM_FUSED_F64C, "unknown form");
+    RF_REQUIRE(form >= RFHIP_FORM_PLAIN_F32 && form <= RFHIP_FORM_FUSED_BF16X2, "unknown form");
     RF_REQUIRE(d->n_elem >= 1, "empty operator");
     const int p = order_of_np(d->np);
     RF_REQUIRE(p >= 1 && d->nfp == (p + 1) * (p + 2) / 2, "inconsistent Np / Nfp");
@@ -1109,7 +1452,7 @@ int rfhip_op_transient_probes_f64(rfhip_op *op, const double *y0, double dt, int
     transient_impl<double>(op, steps, substeps, dt, kind, dof, g, true, [&](int64_t r) {
       if (r <= 0 || n_probes == 0) return;
       const int cur = is_fused(op->form) ? op->cur : 0;
-      const int kind_y = op->form == RFHIP_FORM_FUSED_F64C ? 2 : op->form == RFHIP_FORM_PLAIN_F32 ? 1 : 0;
+      const int kind_y = form_compressed(op->form) ? 2 : op->form == RFHIP_FORM_PLAIN_F32 ? 1 : 0;
       launch_gather(op->y[cur].p, op->ylo[cur].as<uint16_t>(), kind_y, op->probe_idx.as<int64_t>(), n_probes,
                     op->probe_rows.as<double>() + (r - 1) * n_probes, s);
     });
@@ -1189,6 +1532,7 @@ int rfhip_op_port_source_f64(rfhip_op *op, int32_t port, double *b) {
                          op->ps_lf.as<int32_t>(), op->ps_prof.as<double>(), op->src.as<double>(), s);
     }
     op->src_resident = true;
+    op->ps_count = (int64_t)elems.size();
     if (b) d2h_layout_f64(op, op->src.as<double>(), b);
     sync(op);  // the host staging vectors above live until here
   });
@@ -1236,7 +1580,7 @@ int rfhip_op_transient_ports_f64(rfhip_op *op, const double *y0, double dt, int6
     transient_impl<double>(op, steps, substeps, dt, kind, dof, g, true, [&](int64_t r) {
       if (r < 0 || np_ == 0) return;
       const int cur = is_fused(op->form) ? op->cur : 0;
-      const int kind_y = op->form == RFHIP_FORM_FUSED_F64C ? 2 : op->form == RFHIP_FORM_PLAIN_F32 ? 1 : 0;
+      const int kind_y = form_compressed(op->form) ? 2 : op->form == RFHIP_FORM_PLAIN_F32 ? 1 : 0;
       launch_port_proj(op->y[cur].p, op->ylo[cur].as<uint16_t>(), kind_y, op->port_dof.as<int64_t>(),
                        op->port_prof.as<double>(), op->port_w.as<double>(), op->port_off.as<int64_t>(),
                        op->port_norm.as<double>(), np_, op->port_rows.as<double>() + r * row, s);
@@ -1281,6 +1625,26 @@ int rfhip_op_etd_step(rfhip_op *op, const double *y, const double *b, double h, 
   });
 }
 
+int rfhip_op_expmv_adaptive(rfhip_op *op, const double *v, double t_end, const rfhip_expctl *ctl, rfhip_mlp *proposer,
+                            rfhip_exp_step *steps, int64_t steps_cap, double *hess, int32_t hess_dim, int64_t *n_steps,
+                            double *out) {
+  return guarded([&] {
+    RF_REQUIRE(op && v && ctl && out, "null argument");
+    RF_REQUIRE(std::isfinite(t_end) && t_end > 0.0, "t_end must be positive and finite");
+    RF_REQUIRE(std::isfinite(ctl->tol) && ctl->tol > 0.0, "tol must be positive");
+    RF_REQUIRE(ctl->m >= 2 && ctl->m <= kMaxKrylov, "m must be in [2, 64]");
+    RF_REQUIRE(ctl->m_min >= 2 && ctl->m_min <= ctl->m_max && ctl->m_max <= kMaxKrylov,
+               "need 2 <= m_min <= m_max <= 64");
+    RF_REQUIRE(ctl->safety > 0.0 && ctl->safety <= 1.0 && ctl->shrink_limit > 0.0 && ctl->shrink_limit < 1.0 &&
+                   ctl->growth_limit > 1.0 && ctl->max_steps >= 1,
+               "controller needs 0 < safety <= 1, 0 < shrink_limit < 1 < growth_limit, max_steps >= 1");
+    RF_REQUIRE(steps_cap >= 0 && (steps || !hess), "a Hessenberg log needs the step log");
+    if (hess) RF_REQUIRE(hess_dim >= std::max(ctl->m, proposer ? ctl->m_max : ctl->m),
+                         "hess_dim must cover every Krylov dimension the run can build");
+    adaptive_impl(op, v, t_end, *ctl, proposer, steps, steps ? steps_cap : 0, hess, hess ? hess_dim : 0, n_steps, out);
+  });
+}
+
 int rfhip_op_last_call_ms(const rfhip_op *op, double *ms) {
   return guarded([&] {
     RF_REQUIRE(op && ms, "null argument");
@@ -1288,7 +1652,13 @@ int rfhip_op_last_call_ms(const rfhip_op *op, double *ms) {
   });
 }
 
-int rfhip_op_n_variants(const rfhip_op *op) { return op && is_fused(op->form) ? kFusedVariants : 1; }
+int rfhip_op_n_variants(const rfhip_op *op) { return op && is_fused(op->form) ? n_stage_variants(op->form) : 1; }
+
+int rfhip_op_variant_available(const rfhip_op *op, int variant) {
+  if (!op) return 0;
+  if (!is_fused(op->form)) return variant == 0 ? 1 : 0;
+  return variant_available(op->order, op->form, variant, op->sa) ? 1 : 0;
+}
 
 int rfhip_op_variant(const rfhip_op *op) { return op && is_fused(op->form) ? op->sa.variant : 0; }
 
@@ -1297,7 +1667,7 @@ int rfhip_op_set_variant(rfhip_op *op, int variant) {
     RF_REQUIRE(op, "null argument");
     RF_REQUIRE(variant >= 0 && variant < rfhip_op_n_variants(op), "variant out of range");
     if (!is_fused(op->form)) return;
-    if (!variant_available(op->order, variant, op->sa))
+    if (!variant_available(op->order, op->form, variant, op->sa))
       throw Error{RFHIP_EUNSUPPORTED, "stage-kernel variant " + std::to_string(variant) +
                                           " cannot run on this operator (order or neighbour table)"};
     op->sa.variant = variant;
@@ -1322,7 +1692,7 @@ int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best) {
     }
     hipStream_t s = op->stream;
     const int64_t n = nvec(op);
-    const bool c = op->form == RFHIP_FORM_FUSED_F64C;
+    const bool c = form_compressed(op->form);
     DevBuf yo, ylo, pp;
     yo.alloc((c ? 4 : 8) * n);
     ylo.alloc(2 * n);
@@ -1334,12 +1704,13 @@ int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best) {
     // Three round-robin passes over the variants, 8 timed launches each,
     // keeping each variant's best pass: variants that differ by a few
     // percent are then not ordered by timing noise or clock drift.
-    double v_ms[kFusedVariants];
-    for (int v = 0; v < kFusedVariants; ++v) v_ms[v] = 1e300;
+    double v_ms[kStageVariants];
+    const int nv = n_stage_variants(op->form);
+    for (int v = 0; v < kStageVariants; ++v) v_ms[v] = 1e300;
     try {
       for (int pass = 0; pass < 3; ++pass)
-      for (int v = 0; v < kFusedVariants; ++v) {
-        if (!variant_available(op->order, v, op->sa)) continue;
+      for (int v = 0; v < nv; ++v) {
+        if (!variant_available(op->order, op->form, v, op->sa)) continue;
         StageArgs a = op->sa;
         a.variant = v;
         a.yin = op->y[op->cur].p;
@@ -1368,7 +1739,7 @@ int rfhip_op_autotune(rfhip_op *op, int *chosen, double *ms_best) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     int best = op->sa.variant;
-    for (int v = 0; v < kFusedVariants; ++v)
+    for (int v = 0; v < nv; ++v)
       if (v_ms[v] < v_ms[best]) best = v;
     op->sa.variant = best;
     if (chosen) *chosen = best;
@@ -1446,70 +1817,14 @@ int rfhip_op_state_lserk_graph(rfhip_op *op, double dt, int64_t nsteps) {
 }
 
 int rfhip_batch_lserk(rfhip_op *const *ops, int32_t n_ops, const double *dt, int64_t nsteps) {
+  return guarded([&] { batch_lserk_impl(ops, n_ops, dt, nsteps, nullptr); });
+}
+
+int rfhip_batch_lserk_ports(rfhip_op *const *ops, int32_t n_ops, const double *dt, int64_t nsteps,
+                            const double *const *g) {
   return guarded([&] {
-    RF_REQUIRE(ops && dt && n_ops >= 1 && nsteps >= 0, "bad arguments");
-    for (int32_t i = 0; i < n_ops; ++i) {
-      RF_REQUIRE(ops[i], "null operator in the batch");
-      RF_REQUIRE(ops[i]->ctx == ops[0]->ctx, "batched operators must share one context");
-      RF_REQUIRE(std::isfinite(dt[i]), "non-finite step");
-      for (int32_t j = 0; j < i; ++j) RF_REQUIRE(ops[j] != ops[i], "operator listed twice in the batch");
-    }
-    for (int32_t i = 0; i < n_ops; ++i) {
-      zero_p(ops[i]);
-      sync(ops[i]);
-    }
-    hipStream_t s0 = ops[0]->stream;
-    std::vector<hipEvent_t> ev((size_t)n_ops + 1, nullptr);
-    hipGraphExec_t exec = nullptr;
-    auto cleanup = [&] {
-      if (exec) (void)hipGraphExecDestroy(exec);
-      for (auto e : ev)
-        if (e) (void)hipEventDestroy(e);
-    };
-    try {
-      for (auto &e : ev) RF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-      if (nsteps >= 2) {
-        // Two steps of every operator in one graph: fork from the first
-        // operator's stream to every other one, join back at the end. The
-        // capture runs nothing, so each operator's buffer parity is restored.
-        std::vector<int> cur0((size_t)n_ops);
-        for (int32_t i = 0; i < n_ops; ++i) cur0[(size_t)i] = ops[i]->cur;
-        hipGraph_t graph = nullptr;
-        RF_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
-        try {
-          RF_HIP(hipEventRecord(ev[0], s0));
-          for (int32_t i = 1; i < n_ops; ++i) RF_HIP(hipStreamWaitEvent(ops[i]->stream, ev[0], 0));
-          for (int32_t i = 0; i < n_ops; ++i) {
-            lserk_step(ops[i], dt[i], RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
-            lserk_step(ops[i], dt[i], RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
-          }
-          for (int32_t i = 1; i < n_ops; ++i) {
-            RF_HIP(hipEventRecord(ev[(size_t)i], ops[i]->stream));
-            RF_HIP(hipStreamWaitEvent(s0, ev[(size_t)i], 0));
-          }
-        } catch (...) {
-          (void)hipStreamEndCapture(s0, &graph);
-          if (graph) (void)hipGraphDestroy(graph);
-          for (int32_t i = 0; i < n_ops; ++i) ops[i]->cur = cur0[(size_t)i];
-          throw;
-        }
-        RF_HIP(hipStreamEndCapture(s0, &graph));
-        for (int32_t i = 0; i < n_ops; ++i) ops[i]->cur = cur0[(size_t)i];
-        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        RF_HIP(e);
-        for (int64_t k = 0; k + 1 < nsteps; k += 2) RF_HIP(hipGraphLaunch(exec, s0));
-        RF_HIP(hipStreamSynchronize(s0));
-      }
-      if (nsteps % 2)
-        for (int32_t i = 0; i < n_ops; ++i) lserk_step(ops[i], dt[i], RFHIP_DRIVE_FREE, 0, 0.0, nullptr);
-      for (int32_t i = 0; i < n_ops; ++i) sync(ops[i]);
-    } catch (...) {
-      for (int32_t i = 0; i < n_ops; ++i) (void)hipStreamSynchronize(ops[i]->stream);
-      cleanup();
-      throw;
-    }
-    cleanup();
+    RF_REQUIRE(g, "null waveform list");
+    batch_lserk_impl(ops, n_ops, dt, nsteps, g);
   });
 }
 

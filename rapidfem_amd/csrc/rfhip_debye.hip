@@ -146,7 +146,7 @@ void launch_debye_stage(const rfhip_op *op, const StageArgs &a, hipStream_t s) {
   if (op->n_disp == 0) return;
   const DebyeArgs d = debye_args(op, a.src_kind, a.src_dof, a.src_vec, a.g);
   const unsigned grid = grid_of(d.n_disp * d.np * 3);
-  if (op->form == RFHIP_FORM_FUSED_F64C)
+  if (form_compressed(op->form))
     hipLaunchKernelGGL((k_debye_stage<2, 1>), dim3(grid), dim3(kDB), 0, s, a.yin,
                        static_cast<const uint16_t *>(a.yin_lo), a.yout, static_cast<uint16_t *>(a.yout_lo), a.p,
                        a.sa, a.sb, a.dt, d);

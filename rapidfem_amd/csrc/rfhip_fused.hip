@@ -89,6 +89,9 @@ __device__ __forceinline__ void load_trace(const StageArgs &io, int64_t nb, doub
   }
 }
 
+// EPI: 0 = apply (dy = A y, FP64), 1 = LSERK stage with FP64 registers,
+// 2 = LSERK stage with compressed registers (y f32+bf16, p f32).
+//
 // Stage update with the old p already in registers (prefetched by the
 // kernels that use it): materials, held source, p and y.
 template <int EPI>
@@ -147,407 +150,6 @@ __device__ __forceinline__ void node_update(const StageArgs &io, int64_t base, c
   }
 }
 
-// EPI: 0 = apply (dy = A y, FP64), 1 = LSERK stage with FP64 registers,
-// 2 = LSERK stage with compressed registers (y f32+bf16, p f32).
-template <int NP, int NFP, bool CIN, int EPI>
-__global__ void __launch_bounds__(kFusedWG) k_fused(StageArgs io) {
-  const StageArgs &d = io;
-  const double sa = io.sa, sb = io.sb, dt = io.dt;
-  constexpr int EPB = kFusedWG / NP;
-  constexpr int NF4 = 4 * NFP;
-  __shared__ double su[EPB * NP * 6];
-  __shared__ double sf[EPB * NF4 * 6];
-  const int tid = threadIdx.x;
-  const int64_t e0 = (int64_t)blockIdx.x * EPB;
-  const int64_t rem = d.n_elem - e0;
-  const int nel = rem < EPB ? (int)rem : EPB;
-  {
-    const int64_t g0 = e0 * NP * 6;
-    const int cnt = nel * NP * 6;
-    for (int q = tid; q < cnt; q += kFusedWG) su[q] = load_state<CIN>(io, g0 + q);
-  }
-  __syncthreads();
-  const int le = tid / NP, i = tid - (tid / NP) * NP;
-  const bool active = tid < nel * NP;
-  const int64_t e = e0 + le;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  double ji[9];
-  if (active) {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) ji[q] = d.jinv[e * 9 + q];
-    // Volume: r[k][c] = sum_j D_k[i][j] u[j][c] for all six components.
-    double r[3][6];
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
-    const double *ub = su + le * NP * 6;
-#pragma unroll 4
-    for (int j = 0; j < NP; ++j) {
-      double u[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) u[c] = ub[j * 6 + c];
-      const double d0 = d.dT[(0 * NP + j) * NP + i];
-      const double d1 = d.dT[(1 * NP + j) * NP + i];
-      const double d2 = d.dT[(2 * NP + j) * NP + i];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        r[0][c] = fma(d0, u[c], r[0][c]);
-        r[1][c] = fma(d1, u[c], r[1][c]);
-        r[2][c] = fma(d2, u[c], r[2][c]);
-      }
-    }
-    // Physical derivative d(f_c)/dx_p = sum_k J^-1[k][p] r[k][c]; curl.
-    auto pd = [&](int p, int c) { return ji[0 * 3 + p] * r[0][c] + ji[1 * 3 + p] * r[1][c] + ji[2 * 3 + p] * r[2][c]; };
-    acc[0] = pd(1, 5) - pd(2, 4);
-    acc[1] = pd(2, 3) - pd(0, 5);
-    acc[2] = pd(0, 4) - pd(1, 3);
-    acc[3] = -(pd(1, 2) - pd(2, 1));
-    acc[4] = -(pd(2, 0) - pd(0, 2));
-    acc[5] = -(pd(0, 1) - pd(1, 0));
-
-    // Surface flux per face node, scaled by 0.5*fscale (folded into g).
-    for (int fm = i; fm < NF4; fm += NP) {
-      const int f = fm / NFP, m = fm - f * NFP;
-      double g[3];
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        const double r0 = ji[0 * 3 + x], r1 = ji[1 * 3 + x], r2 = ji[2 * 3 + x];
-        g[x] = f == 0 ? -r2 : f == 1 ? -r0 : f == 2 ? -r1 : (r0 + r1 + r2);
-      }
-      const double g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
-      const double inv_g = 1.0 / sqrt(g2);
-      const int code = d.fcode[e * 4 + f];
-      const int vi = d.face_nodes[f * NFP + m];
-      double em[3], hm[3], je[3], jh[3];
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        em[x] = ub[vi * 6 + x];
-        hm[x] = ub[vi * 6 + 3 + x];
-      }
-      double a = d.alpha;
-      if (code >= 0) {
-        const int64_t nb = code >> 5;
-        const int nlf = (code >> 3) & 3, o = code & 7;
-        const int vj = d.nbr_node[((f * 4 + nlf) * 6 + o) * NFP + m];
-        const int64_t nbb = (nb * NP + vj) * 6;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          je[x] = em[x] - load_state<CIN>(io, nbb + x);
-          jh[x] = hm[x] - load_state<CIN>(io, nbb + 3 + x);
-        }
-      } else if (code == kCodePec) {
-        // PEC ghost: [E] = 2 E_t, [H] = 2 (n.H) n (rhs.rs:1387-1402).
-        const double ig2 = inv_g * inv_g;
-        const double edg = (g[0] * em[0] + g[1] * em[1] + g[2] * em[2]) * ig2;
-        const double hdg = (g[0] * hm[0] + g[1] * hm[1] + g[2] * hm[2]) * ig2;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          je[x] = 2.0 * (em[x] - edg * g[x]);
-          jh[x] = 2.0 * hdg * g[x];
-        }
-      } else {
-        // Port: characteristic flux against a zero incident field (rhs.rs:1380-1386).
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          je[x] = em[x];
-          jh[x] = hm[x];
-        }
-        a = 1.0;
-      }
-      // coef n x [.] = 0.5 g x [.];  coef n x (n x [.]) = 0.5 g x (g x [.]) / |g|.
-      double gxh[3], gxe[3], gge[3], ggh[3];
-      cross3(g, jh, gxh);
-      cross3(g, je, gxe);
-      cross3(g, gxe, gge);
-      cross3(g, gxh, ggh);
-      const double ha = 0.5 * a * inv_g;
-      double *out = sf + (le * NF4 + fm) * 6;
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        out[x] = -0.5 * gxh[x] + ha * gge[x];
-        out[3 + x] = 0.5 * gxe[x] + ha * ggh[x];
-      }
-    }
-  }
-  __syncthreads();
-  if (!active) return;
-  {
-    const double *fb = sf + le * NF4 * 6;
-#pragma unroll 4
-    for (int fm = 0; fm < NF4; ++fm) {
-      const double w = d.lift[fm * NP + i];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) acc[c] = fma(w, fb[fm * 6 + c], acc[c]);
-    }
-  }
-  const double *mt = d.mat_tab + 12 * d.mat[e];
-  const double *ub = su + (le * NP + i) * 6;
-  double kv[6];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    kv[c] = mt[c] * acc[c] - mt[6 + c] * ub[c];
-    kv[3 + c] = mt[3 + c] * acc[3 + c] - mt[9 + c] * ub[3 + c];
-  }
-  const int64_t base = (e * NP + i) * 6;
-  if (io.src_kind == RFHIP_DRIVE_POINT) {
-    const int64_t off = io.src_dof - base;
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-      if (off == c) kv[c] += io.g;
-  } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) kv[c] += io.src_vec[base + c] * io.g;
-  }
-  if constexpr (EPI == 0) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) io.dy[base + c] = kv[c];
-  } else if constexpr (EPI == 1) {
-    double *pp = static_cast<double *>(io.p);
-    double *yo = static_cast<double *>(io.yout);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      const double pn = sa * pp[base + c] + dt * kv[c];
-      pp[base + c] = pn;
-      yo[base + c] = ub[c] + sb * pn;
-    }
-  } else {
-    float *pp = static_cast<float *>(io.p);
-    float *yh = static_cast<float *>(io.yout);
-    uint16_t *yl = static_cast<uint16_t *>(io.yout_lo);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      const float pn = (float)(sa * (double)pp[base + c] + dt * kv[c]);
-      pp[base + c] = pn;
-      const double yn = ub[c] + sb * (double)pn;
-      const float h = (float)yn;
-      yh[base + c] = h;
-      yl[base + c] = bf16_rne((float)(yn - (double)h));
-    }
-  }
-}
-
-// Register-blocked variant: each thread owns NB nodes of one element, so
-// every state value and flux value read from LDS feeds NB rows of D and
-// LIFT. The face flux is first computed into registers, then written over
-// the state's LDS region (the two are never live together), which keeps the
-// block's LDS at max(state, flux) instead of their sum. Own node values stay
-// in registers for the epilogue.
-template <int NP, int NFP, int NB, bool CIN, int EPI>
-__global__ void __launch_bounds__(kFusedWG) k_fused2(StageArgs io) {
-  const StageArgs &d = io;
-  constexpr int TPE = (NP + NB - 1) / NB;  // threads per element
-  constexpr int EPB = kFusedWG / TPE;      // elements per block
-  constexpr int NF4 = 4 * NFP;
-  constexpr int FPT = (NF4 + TPE - 1) / TPE;  // face nodes per thread
-  constexpr int SU = EPB * NP * 6, SF = EPB * NF4 * 6;
-  __shared__ double smem[SU > SF ? SU : SF];
-  const int tid = threadIdx.x;
-  const int64_t e0 = (int64_t)blockIdx.x * EPB;
-  const int64_t rem = d.n_elem - e0;
-  const int nel = rem < EPB ? (int)rem : EPB;
-  {
-    const int64_t g0 = e0 * NP * 6;
-    const int cnt = nel * NP * 6;
-    for (int q = tid; q < cnt; q += kFusedWG) smem[q] = load_state<CIN>(io, g0 + q);
-  }
-  __syncthreads();
-  const int le = tid / TPE, t = tid - (tid / TPE) * TPE;
-  const bool active = tid < nel * TPE;
-  const int64_t e = e0 + le;
-  int node[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) node[b] = t + b * TPE;
-  double acc[NB][6], own[NB][6], flx[FPT][6];
-  double ji[9];
-  if (active) {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) ji[q] = d.jinv[e * 9 + q];
-    const double *ub = smem + le * NP * 6;
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int c = 0; c < 6; ++c) own[b][c] = node[b] < NP ? ub[node[b] * 6 + c] : 0.0;
-    // Volume: r[b][k][c] = sum_j D_k[node_b][j] u[j][c].
-    double r[NB][3][6];
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int c = 0; c < 6; ++c) r[b][k][c] = 0.0;
-#pragma unroll 2
-    for (int j = 0; j < NP; ++j) {
-      double u[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) u[c] = ub[j * 6 + c];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int ib = node[b] < NP ? node[b] : 0;
-        const double d0 = d.dT[(0 * NP + j) * NP + ib];
-        const double d1 = d.dT[(1 * NP + j) * NP + ib];
-        const double d2 = d.dT[(2 * NP + j) * NP + ib];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          r[b][0][c] = fma(d0, u[c], r[b][0][c]);
-          r[b][1][c] = fma(d1, u[c], r[b][1][c]);
-          r[b][2][c] = fma(d2, u[c], r[b][2][c]);
-        }
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      auto pd = [&](int p, int c) {
-        return ji[0 * 3 + p] * r[b][0][c] + ji[1 * 3 + p] * r[b][1][c] + ji[2 * 3 + p] * r[b][2][c];
-      };
-      acc[b][0] = pd(1, 5) - pd(2, 4);
-      acc[b][1] = pd(2, 3) - pd(0, 5);
-      acc[b][2] = pd(0, 4) - pd(1, 3);
-      acc[b][3] = -(pd(1, 2) - pd(2, 1));
-      acc[b][4] = -(pd(2, 0) - pd(0, 2));
-      acc[b][5] = -(pd(0, 1) - pd(1, 0));
-    }
-    // Face flux into registers, scaled by 0.5*fscale (folded into g).
-#pragma unroll
-    for (int s2 = 0; s2 < FPT; ++s2) {
-      const int fm = t + s2 * TPE;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) flx[s2][c] = 0.0;
-      if (fm >= NF4) continue;
-      const int f = fm / NFP, m = fm - f * NFP;
-      double g[3];
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        const double r0 = ji[0 * 3 + x], r1 = ji[1 * 3 + x], r2 = ji[2 * 3 + x];
-        g[x] = f == 0 ? -r2 : f == 1 ? -r0 : f == 2 ? -r1 : (r0 + r1 + r2);
-      }
-      const double inv_g = 1.0 / sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-      const int code = d.fcode[e * 4 + f];
-      const int vi = d.face_nodes[f * NFP + m];
-      double em[3], hm[3], je[3], jh[3];
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        em[x] = ub[vi * 6 + x];
-        hm[x] = ub[vi * 6 + 3 + x];
-      }
-      double a = d.alpha;
-      if (code >= 0) {
-        const int64_t nb = code >> 5;
-        const int nlf = (code >> 3) & 3, o = code & 7;
-        const int vj = d.nbr_node[((f * 4 + nlf) * 6 + o) * NFP + m];
-        const int64_t nbb = (nb * NP + vj) * 6;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          je[x] = em[x] - load_state<CIN>(io, nbb + x);
-          jh[x] = hm[x] - load_state<CIN>(io, nbb + 3 + x);
-        }
-      } else if (code == kCodePec) {
-        const double ig2 = inv_g * inv_g;
-        const double edg = (g[0] * em[0] + g[1] * em[1] + g[2] * em[2]) * ig2;
-        const double hdg = (g[0] * hm[0] + g[1] * hm[1] + g[2] * hm[2]) * ig2;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          je[x] = 2.0 * (em[x] - edg * g[x]);
-          jh[x] = 2.0 * hdg * g[x];
-        }
-      } else {
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          je[x] = em[x];
-          jh[x] = hm[x];
-        }
-        a = 1.0;
-      }
-      double gxh[3], gxe[3], gge[3], ggh[3];
-      cross3(g, jh, gxh);
-      cross3(g, je, gxe);
-      cross3(g, gxe, gge);
-      cross3(g, gxh, ggh);
-      const double ha = 0.5 * a * inv_g;
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        flx[s2][x] = -0.5 * gxh[x] + ha * gge[x];
-        flx[s2][3 + x] = 0.5 * gxe[x] + ha * ggh[x];
-      }
-    }
-  }
-  __syncthreads();  // every thread is done reading the state in LDS
-  if (active) {
-#pragma unroll
-    for (int s2 = 0; s2 < FPT; ++s2) {
-      const int fm = t + s2 * TPE;
-      if (fm >= NF4) continue;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) smem[(le * NF4 + fm) * 6 + c] = flx[s2][c];
-    }
-  }
-  __syncthreads();
-  if (!active) return;
-  {
-    const double *fb = smem + le * NF4 * 6;
-#pragma unroll 2
-    for (int fm = 0; fm < NF4; ++fm) {
-      double F[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) F[c] = fb[fm * 6 + c];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int ib = node[b] < NP ? node[b] : 0;
-        const double w = d.lift[fm * NP + ib];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) acc[b][c] = fma(w, F[c], acc[b][c]);
-      }
-    }
-  }
-  const double *mt = d.mat_tab + 12 * d.mat[e];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    if (node[b] >= NP) continue;
-    double kv[6];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      kv[c] = mt[c] * acc[b][c] - mt[6 + c] * own[b][c];
-      kv[3 + c] = mt[3 + c] * acc[b][3 + c] - mt[9 + c] * own[b][3 + c];
-    }
-    const int64_t base = (e * NP + node[b]) * 6;
-    if (io.src_kind == RFHIP_DRIVE_POINT) {
-      const int64_t off = io.src_dof - base;
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-        if (off == c) kv[c] += io.g;
-    } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
-#pragma unroll
-      for (int c = 0; c < 6; ++c) kv[c] += io.src_vec[base + c] * io.g;
-    }
-    if constexpr (EPI == 0) {
-#pragma unroll
-      for (int c = 0; c < 6; ++c) io.dy[base + c] = kv[c];
-    } else if constexpr (EPI == 1) {
-      double *pp = static_cast<double *>(io.p);
-      double *yo = static_cast<double *>(io.yout);
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        const double pn = io.sa * pp[base + c] + io.dt * kv[c];
-        pp[base + c] = pn;
-        yo[base + c] = own[b][c] + io.sb * pn;
-      }
-    } else {
-      float *pp = static_cast<float *>(io.p);
-      float *yh = static_cast<float *>(io.yout);
-      uint16_t *yl = static_cast<uint16_t *>(io.yout_lo);
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        const float pn = (float)(io.sa * (double)pp[base + c] + io.dt * kv[c]);
-        pp[base + c] = pn;
-        const double yn = own[b][c] + io.sb * (double)pn;
-        const float h = (float)yn;
-        yh[base + c] = h;
-        yl[base + c] = bf16_rne((float)(yn - (double)h));
-      }
-    }
-  }
-}
-
 // Unit normal and 0.5*fscale of local face f from J^-1 (rows of ji):
 // g_f = {-row2, -row0, -row1, row0+row1+row2}, n = g/|g|, fscale = |g|.
 __device__ __forceinline__ void face_geom(const double *ji, int f, double *n, double &cf) {
@@ -590,7 +192,7 @@ __device__ __forceinline__ GroupRange group_range(bool xcd, int64_t ngroups) {
   return {(int64_t)blockIdx.x, ngroups, (int64_t)gridDim.x};
 }
 
-// Latency-oriented variant of k_fused (same thread mapping):
+// One thread per (element, node), 256/Np elements per block:
 //  * the face codes and neighbour-node indices of a thread's face nodes are
 //    resolved before anything else (they are a chain of dependent loads),
 //    and with PREV the neighbour traces themselves are requested too, so
@@ -903,32 +505,17 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
   }
 }
 
-// ---- FP64 MFMA volume --------------------------------------------------
+// ---- shared pieces of the flux and the epilogue --------------------------
 //
-// k_fused4 moves the volume contraction onto the matrix cores: for one
-// field (H, then E) of a block of EPB elements,
-//   R[s][col] = sum_j [Dr; Ds; Dt][s][j] * U[j][col],  s = k*Np + i,
-//   col = 3*element + component,
-// is a (3Np x Np) x (Np x 3 EPB) product done with
-// v_mfma_f64_16x16x4f64 (A fragments precomputed on the host, B read from
-// the state in LDS). The products land in LDS as R[e][i][k][c]; each
-// (element, node) thread then applies J^-1 and forms the curl on the VALU,
-// which meanwhile also does the flux, LIFT and the stage update. Wave w of
-// the block takes output tiles w, w + waves, ...; every MFMA is executed by
-// whole waves (the tile loop is wave-uniform). Operand layout of
-// v_mfma_f64_16x16x4f64 (checked on gfx950): lane l supplies A[l%16][l/16]
-// and B[l/16][l%16] and receives C[4r + l/16][l%16] in element r.
+// Operand layout of v_mfma_f64_16x16x4f64 (checked on gfx950): lane l
+// supplies A[l%16][l/16] and B[l/16][l%16] and receives C[4r + l/16][l%16]
+// in element r.
 typedef double mf_d4 __attribute__((ext_vector_type(4)));
-
-template <int NP>
-struct MfmaCfg {
-  static constexpr int EPB = NP == 35 ? 8 : 16;             // elements per block
-  static constexpr int NT = ((EPB * NP + 63) / 64) * 64;     // threads per block
-  static constexpr int MT = (3 * NP + 15) / 16;              // 16-row tiles of [Dr; Ds; Dt]
-  static constexpr int KS = (NP + 3) / 4;                    // k-steps of 4
-  static constexpr int CH = EPB * 3;                         // columns of one field
-  static constexpr int NTL = (CH + 15) / 16;                 // 16-column tiles
-};
+// Operands of v_mfma_f32_16x16x32_bf16: lane l supplies A[l%16][8(l/16) + t]
+// and B[8(l/16) + t][l%16] (t = 0..7) and receives C[4(l/16) + r][l%16] in
+// element r.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Upwind flux of one face node (rhs.rs:1372-1440) with unit normal n and
 // cf = 0.5*fscale, own trace u and neighbour trace v (interior faces):
@@ -1017,500 +604,9 @@ __device__ __forceinline__ void node_epilogue(const StageArgs &io, int64_t base,
   }
 }
 
-template <int NP, int NFP, bool CIN, int EPI, bool LMF>
-__global__ void __launch_bounds__(MfmaCfg<NP>::NT) k_fused4(StageArgs io) {
-  using Cfg = MfmaCfg<NP>;
-  constexpr int EPB = Cfg::EPB, NT = Cfg::NT, MT = Cfg::MT, KS = Cfg::KS, NTL = Cfg::NTL;
-  constexpr int NWAVE = NT / 64;
-  constexpr int NF4 = 4 * NFP;
-  constexpr int FPT = (NF4 + NP - 1) / NP;  // face nodes per thread
-  constexpr int SU = EPB * NP * 6, SF = EPB * NF4 * 6;
-  __shared__ __align__(16) double sx[SU > SF ? SU : SF];  // the state, then the face flux
-  __shared__ __align__(16) double sr[EPB * NP * 9];        // R[e][i][k][c] of one field
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int le = tid / NP, i = tid - (tid / NP) * NP;
-  const int64_t e0 = (int64_t)blockIdx.x * EPB;
-  const int64_t rem = io.n_elem - e0;
-  const int nel = rem < EPB ? (int)rem : EPB;
-  const bool active = tid < nel * NP;
-  const int64_t e = e0 + le;
-
-  // Neighbour traces of this thread's face nodes: requested first.
-  double nv[FPT][6];
-  int fcodes[FPT];
-  if (active) {
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) {
-      const int fm = i + s * NP;
-      fcodes[s] = kCodePec;
-      if (fm >= NF4) continue;
-      const int f = fm / NFP, m = fm - f * NFP;
-      const int code = io.fcode[e * 4 + f];
-      fcodes[s] = code;
-      if (code < 0) continue;
-      const int64_t nb = code >> 5;
-      const int nlf = (code >> 3) & 3, o = code & 7;
-      const int vj = io.nbr_node[((f * 4 + nlf) * 6 + o) * NFP + m];
-      load_trace<CIN>(io, (nb * NP + vj) * 6, nv[s]);
-    }
-  }
-  {
-    const int64_t g0 = e0 * NP * 6;
-    const int cnt2 = nel * NP * 3;
-    for (int q = tid; q < cnt2; q += NT) {
-      double2 v;
-      if constexpr (CIN) {
-        const float2 h = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + g0)[q];
-        const uint32_t l = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + g0)[q];
-        v.x = (double)h.x + (double)__uint_as_float(l << 16);
-        v.y = (double)h.y + (double)__uint_as_float(l & 0xFFFF0000u);
-      } else {
-        v = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + g0)[q];
-      }
-      reinterpret_cast<double2 *>(sx)[q] = v;
-    }
-  }
-  double ji[9];
-  if (active) {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) ji[q] = io.jinv[e * 9 + q];
-  }
-  __syncthreads();
-
-  double own[6], acc[6], fl[FPT][6];
-  if (active) {
-    const double *ub = sx + le * NP * 6;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) own[c] = ub[i * 6 + c];
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) {
-      const int fm = i + s * NP;
-      if (fm >= NF4) continue;
-      const int f = fm / NFP, m = fm - f * NFP;
-      double n[3], cf;
-      face_geom(ji, f, n, cf);
-      const int vi = io.face_nodes[f * NFP + m];
-      face_flux(fcodes[s], io.alpha, n, cf, ub + vi * 6, nv[s], fl[s]);
-    }
-  }
-
-  // Volume on the matrix cores: field H (half 0, gives dE), then E (dH).
-#pragma unroll 1
-  for (int h = 0; h < 2; ++h) {
-    const int cbase = h == 0 ? 3 : 0;
-    for (int t = wave; t < MT * NTL; t += NWAVE) {
-      const int mt = t / NTL, nt = t - (t / NTL) * NTL;
-      const int col = nt * 16 + (lane & 15);
-      const int ce = col / 3, cc = col - (col / 3) * 3;
-      const bool cv = col < nel * 3;
-      const double *ap = io.afrag + (size_t)mt * KS * 64 + lane;
-      mf_d4 cacc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const int j = ks * 4 + (lane >> 4);
-        const double a = ap[ks * 64];
-        const double b = (cv && j < NP) ? sx[(ce * NP + j) * 6 + cbase + cc] : 0.0;
-        cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, cacc, 0, 0, 0);
-      }
-      if (cv) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int srow = mt * 16 + r * 4 + (lane >> 4);  // C[4r + lane/16][lane%16]
-          if (srow < 3 * NP) {
-            const int k = srow / NP, ii = srow - (srow / NP) * NP;
-            sr[((ce * NP + ii) * 3 + k) * 3 + cc] = cacc[r];
-          }
-        }
-      }
-    }
-    __syncthreads();
-    if (active) {
-      const double *rb = sr + (le * NP + i) * 9;
-      double rv[9];
-#pragma unroll
-      for (int q = 0; q < 9; ++q) rv[q] = rb[q];
-      // d(p, c) = d(field_c)/dx_p = sum_k J^-1[k][p] R_k[c]
-      auto dd = [&](int p, int c) { return ji[p] * rv[c] + ji[3 + p] * rv[3 + c] + ji[6 + p] * rv[6 + c]; };
-      if (h == 0) {
-        acc[0] = dd(1, 2) - dd(2, 1);
-        acc[1] = dd(2, 0) - dd(0, 2);
-        acc[2] = dd(0, 1) - dd(1, 0);
-      } else {
-        acc[3] = -(dd(1, 2) - dd(2, 1));
-        acc[4] = -(dd(2, 0) - dd(0, 2));
-        acc[5] = -(dd(0, 1) - dd(1, 0));
-      }
-    }
-    __syncthreads();  // sr is rewritten by the next field; after E the state is no longer read
-  }
-
-  if (active) {
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) {
-      const int fm = i + s * NP;
-      if (fm >= NF4) continue;
-      double2 *out = reinterpret_cast<double2 *>(sx + (le * NF4 + fm) * 6);
-      out[0] = make_double2(fl[s][0], fl[s][1]);
-      out[1] = make_double2(fl[s][2], fl[s][3]);
-      out[2] = make_double2(fl[s][4], fl[s][5]);
-    }
-  }
-  __syncthreads();
-  if constexpr (LMF) {
-    // LIFT on the matrix cores: L[i][col] = sum_fm LIFT[i][fm] F[fm][col],
-    // col = 6*element + component, F[e][fm][c] in LDS; L lands in sr as
-    // L[e][i][c].
-    constexpr int MTL = (NP + 15) / 16, KSL = (NF4 + 3) / 4, NTLL = (EPB * 6 + 15) / 16;
-    for (int t = wave; t < MTL * NTLL; t += NWAVE) {
-      const int mt = t / NTLL, nt = t - (t / NTLL) * NTLL;
-      const int col = nt * 16 + (lane & 15);
-      const int ce = col / 6, cc = col - (col / 6) * 6;
-      const bool cv = col < nel * 6;
-      const double *ap = io.lfrag + (size_t)mt * KSL * 64 + lane;
-      mf_d4 cacc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int ks = 0; ks < KSL; ++ks) {
-        const int fm = ks * 4 + (lane >> 4);
-        const double a = ap[ks * 64];
-        const double b = (cv && fm < NF4) ? sx[(ce * NF4 + fm) * 6 + cc] : 0.0;
-        cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, cacc, 0, 0, 0);
-      }
-      if (cv) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = mt * 16 + r * 4 + (lane >> 4);
-          if (row < NP) sr[(ce * NP + row) * 6 + cc] = cacc[r];
-        }
-      }
-    }
-    __syncthreads();
-    if (!active) return;
-    const double *lb = sr + (le * NP + i) * 6;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) acc[c] += lb[c];
-  } else {
-    if (!active) return;
-    const double *fb = sx + le * NF4 * 6;
-#pragma unroll 4
-    for (int fm = 0; fm < NF4; ++fm) {
-      const double w = io.lift[fm * NP + i];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) acc[c] = fma(w, fb[fm * 6 + c], acc[c]);
-    }
-  }
-  node_epilogue<EPI>(io, (e * NP + i) * 6, io.mat_tab + 12 * io.mat[e], acc, own);
-}
-
-// ---- MFMA volume + MFMA LIFT with linear operand addressing -------------
-//
-// k_fused5 runs both dense contractions of a stage on the matrix cores
-// (v_mfma_f64_16x16x4f64, FP64) and keeps every LDS array component-major
-// with the element index innermost, so the MFMA operand and result
-// addresses are linear in the lane index:
-//   state      sS[(c*Np + j)*E2 + e]         (c = 0..5)
-//   products   sR[(c'*3Np + s)*E2 + e]        (one field, s = k*Np + i)
-//   flux       sF[(c'*4Nfp + fm)*E2 + e]      (one field)
-//   lifted     sL[(c'*Np + i)*E2 + e]         (one field, over sR)
-// with column = c'*EPB + e of each GEMM and E2 = EPB + 1 (bank padding).
-// Per element group: volume(H) -> curl(H) -> volume(E) -> curl(E) ->
-// LIFT(E flux) -> LIFT(H flux), each MFMA phase split over the block's
-// waves by output tile, the VALU phases one thread per (element, node).
-template <int NP>
-struct Mfma5Cfg {
-  static constexpr int EPB = NP == 35 ? 8 : 16;
-  static constexpr int E2 = EPB + 1;
-  static constexpr int NT = ((EPB * NP + 63) / 64) * 64;
-  static constexpr int MT = (3 * NP + 15) / 16;  // volume row tiles
-  static constexpr int KS = (NP + 3) / 4;        // volume k-steps
-  static constexpr int NTC = (3 * EPB + 15) / 16;  // column tiles of one field
-};
-
-template <int NP, int NFP, bool CIN, int EPI, bool XCD = false>
-__global__ void __launch_bounds__(Mfma5Cfg<NP>::NT) k_fused5(StageArgs io) {
-  using Cfg = Mfma5Cfg<NP>;
-  constexpr int EPB = Cfg::EPB, E2 = Cfg::E2, NT = Cfg::NT, MT = Cfg::MT, KS = Cfg::KS, NTC = Cfg::NTC;
-  constexpr int NWAVE = NT / 64;
-  constexpr int NF4 = 4 * NFP;
-  constexpr int FPT = (NF4 + NP - 1) / NP;
-  constexpr int MTL = (NP + 15) / 16, KSL = NF4 / 4;  // LIFT tiles (4Nfp is a multiple of 4)
-  constexpr int SS = 6 * NP * E2, SF = 3 * NF4 * E2, SR = 3 * 3 * NP * E2;
-  constexpr bool COLPAD = (3 * EPB) % 16 != 0;  // last column tile partly empty
-  // Fixed row tile per wave when there are enough waves: its A fragments are
-  // loaded once and every MFMA chain of the wave shares them.
-  constexpr bool VFIX = MT <= NWAVE, LFIX = MTL <= NWAVE;
-  __shared__ __align__(16) double sA[SS > SF ? SS : SF];  // state, then one field's flux
-  __shared__ __align__(16) double sB[SR];                 // products, then lifted flux
-  __shared__ __align__(16) double sg[EPB * 16];            // per face: n, 0.5*fscale
-  __shared__ __align__(16) double sm[EPB * 12];            // material row per element
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int le = tid / NP, i = tid - (tid / NP) * NP;
-  const int64_t grp = group_range(XCD, (io.n_elem + EPB - 1) / EPB).first;
-  if (grp >= (io.n_elem + EPB - 1) / EPB) return;  // padding blocks of the XCD order
-  const int64_t e0 = grp * EPB;
-  const int64_t rem = io.n_elem - e0;
-  const int nel = rem < EPB ? (int)rem : EPB;
-  const bool active = tid < nel * NP;
-  const int64_t e = e0 + le;
-  const int64_t base = (e * NP + i) * 6;
-  const int lcol = lane & 15, lrow = lane >> 4;
-
-  // P0: every global read of the group is issued here (neighbour traces,
-  // state, J^-1, old p, source, material row, A fragments) so the block
-  // waits for one memory latency instead of a chain of them.
-  double nv[FPT][6];
-  int fcodes[FPT];
-  double ji[9];
-  // old p of this node: f32 for the compressed form (kept narrow until use)
-  using PT = typename std::conditional<EPI == 2, float, double>::type;
-  PT pold[6];
-  if (active) {
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) {
-      const int fm = i + s * NP;
-      fcodes[s] = kCodePec;
-      if (fm >= NF4) continue;
-      const int nbv = io.nbase[e * NF4 + fm];
-      fcodes[s] = nbv;
-      if (nbv >= 0) load_trace<CIN>(io, nbv, nv[s]);
-    }
-#pragma unroll
-    for (int q = 0; q < 9; ++q) ji[q] = io.jinv[e * 9 + q];
-    if constexpr (EPI == 1) {
-      const double2 *pp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.p) + base);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const double2 v = pp[q];
-        pold[2 * q] = v.x;
-        pold[2 * q + 1] = v.y;
-      }
-    } else if constexpr (EPI == 2) {
-      const float2 *pp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.p) + base);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const float2 v = pp[q];
-        pold[2 * q] = v.x;
-        pold[2 * q + 1] = v.y;
-      }
-    }
-    for (int q = i; q < 12; q += NP) sm[le * 12 + q] = io.mat_tab[12 * io.mat[e] + q];
-  }
-  double av[VFIX ? KS : 1];
-  if constexpr (VFIX) {
-    const double *ap = io.afrag + (size_t)(wave < MT ? wave : 0) * KS * 64 + lane;
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) av[ks] = ap[ks * 64];
-  }
-  {
-    const int64_t g0 = e0 * NP * 6;
-    const int cnt2 = nel * NP * 3;
-    for (int q = tid; q < cnt2; q += NT) {
-      double2 v;
-      if constexpr (CIN) {
-        const float2 h = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + g0)[q];
-        const uint32_t l = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + g0)[q];
-        v.x = (double)h.x + (double)__uint_as_float(l << 16);
-        v.y = (double)h.y + (double)__uint_as_float(l & 0xFFFF0000u);
-      } else {
-        v = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + g0)[q];
-      }
-      const int ee = q / (3 * NP), r2 = q - ee * (3 * NP);
-      const int j = r2 / 3, c = 2 * (r2 - j * 3);
-      sA[(c * NP + j) * E2 + ee] = v.x;
-      sA[((c + 1) * NP + j) * E2 + ee] = v.y;
-    }
-  }
-  if (active && i < 4) {
-    double n[3], cf;
-    face_geom(ji, i, n, cf);
-    double *o = sg + le * 16 + i * 4;
-    o[0] = n[0];
-    o[1] = n[1];
-    o[2] = n[2];
-    o[3] = cf;
-  }
-  __syncthreads();
-
-  double own[6], acc[6], fl[FPT][6];
-  if (active) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) own[c] = sA[(c * NP + i) * E2 + le];
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) {
-      const int fm = i + s * NP;
-      if (fm >= NF4) continue;
-      const int f = fm / NFP, m = fm - f * NFP;
-      const double *gq = sg + le * 16 + f * 4;
-      const double n[3] = {gq[0], gq[1], gq[2]};
-      const int vi = io.face_nodes[f * NFP + m];
-      double u[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) u[c] = sA[(c * NP + vi) * E2 + le];
-      face_flux(fcodes[s] >= 0 ? 0 : fcodes[s], io.alpha, n, gq[3], u, nv[s], fl[s]);
-    }
-  }
-
-  // Volume of one field on the matrix cores, R[s][c'*EPB + e] into sB: the
-  // wave keeps one row tile and runs the NTC column tiles as independent
-  // accumulation chains.
-  auto volume = [&](int cbase) {
-    for (int mt = wave; mt < MT; mt += NWAVE) {
-      double a[KS];
-      if constexpr (VFIX) {
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) a[ks] = av[ks];
-      } else {
-        const double *ap = io.afrag + (size_t)mt * KS * 64 + lane;
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) a[ks] = ap[ks * 64];
-      }
-      mf_d4 cacc[NTC];
-#pragma unroll
-      for (int nt = 0; nt < NTC; ++nt) cacc[nt] = mf_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const bool jv = NP % 4 == 0 || ks < KS - 1 || ks * 4 + lrow < NP;
-#pragma unroll
-        for (int nt = 0; nt < NTC; ++nt) {
-          const int col = nt * 16 + lcol;
-          const int cp = col / EPB, ce = col - (col / EPB) * EPB;
-          const bool cv = !COLPAD || cp < 3;
-          const double b = (cv && jv) ? sA[((cbase + (cv ? cp : 0)) * NP + ks * 4 + lrow) * E2 + ce] : 0.0;
-          cacc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b, cacc[nt], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int nt = 0; nt < NTC; ++nt) {
-        const int col = nt * 16 + lcol;
-        const int cp = col / EPB, ce = col - (col / EPB) * EPB;
-        if (COLPAD && cp >= 3) continue;
-        double *rp = sB + (cp * 3 * NP + mt * 16 + lrow) * E2 + ce;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (mt * 16 + r * 4 + lrow < 3 * NP) rp[r * 4 * E2] = cacc[nt][r];
-      }
-    }
-  };
-  // d(field_c')/dx_p at this thread's node from sB, then the curl.
-  auto curl = [&](double *out, double sign) {
-    double rv[3][3];  // rv[k][c']
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) rv[k][c] = sB[(c * 3 * NP + k * NP + i) * E2 + le];
-    auto dd = [&](int p, int c) { return ji[p] * rv[0][c] + ji[3 + p] * rv[1][c] + ji[6 + p] * rv[2][c]; };
-    out[0] = sign * (dd(1, 2) - dd(2, 1));
-    out[1] = sign * (dd(2, 0) - dd(0, 2));
-    out[2] = sign * (dd(0, 1) - dd(1, 0));
-  };
-  // LIFT of one field's flux (sA) into sB as L[c'][i][e]: wave w keeps row
-  // tile w % MTL (A fragments in lf) and takes every (NWAVE/MTL)-th column
-  // tile.
-  double lf[LFIX ? KSL : 1];
-  auto load_lift_frags = [&]() {
-    if constexpr (LFIX) {
-      const double *ap = io.lfrag + (size_t)(wave % MTL) * KSL * 64 + lane;
-#pragma unroll
-      for (int ks = 0; ks < KSL; ++ks) lf[ks] = ap[ks * 64];
-    }
-  };
-  auto lift = [&]() {
-    if constexpr (LFIX) {
-      const int mt = wave % MTL;
-      constexpr int WPM = NWAVE / MTL;  // waves per row tile (rounded down)
-      const int widx = wave / MTL;
-      if (widx >= WPM + (mt < NWAVE % MTL ? 1 : 0)) return;
-      const int nwm = WPM + (mt < NWAVE % MTL ? 1 : 0);
-      for (int nt = widx; nt < NTC; nt += nwm) {
-        const int col = nt * 16 + lcol;
-        const int cp = col / EPB, ce = col - (col / EPB) * EPB;
-        const bool cv = !COLPAD || cp < 3;
-        const double *bp = sA + ((cv ? cp : 0) * NF4 + lrow) * E2 + ce;
-        mf_d4 cacc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int ks = 0; ks < KSL; ++ks) {
-          const double b = cv ? bp[ks * 4 * E2] : 0.0;
-          cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(lf[ks], b, cacc, 0, 0, 0);
-        }
-        if (cv) {
-          double *lp = sB + (cp * NP + mt * 16 + lrow) * E2 + ce;
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (mt * 16 + r * 4 + lrow < NP) lp[r * 4 * E2] = cacc[r];
-        }
-      }
-    } else {
-      for (int t = wave; t < MTL * NTC; t += NWAVE) {
-        const int mt = t / NTC, nt = t - (t / NTC) * NTC;
-        const int col = nt * 16 + lcol;
-        const int cp = col / EPB, ce = col - (col / EPB) * EPB;
-        const bool cv = !COLPAD || cp < 3;
-        const double *ap = io.lfrag + (size_t)mt * KSL * 64 + lane;
-        const double *bp = sA + ((cv ? cp : 0) * NF4 + lrow) * E2 + ce;
-        mf_d4 cacc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int ks = 0; ks < KSL; ++ks) {
-          const double b = cv ? bp[ks * 4 * E2] : 0.0;
-          cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[ks * 64], b, cacc, 0, 0, 0);
-        }
-        if (cv) {
-          double *lp = sB + (cp * NP + mt * 16 + lrow) * E2 + ce;
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (mt * 16 + r * 4 + lrow < NP) lp[r * 4 * E2] = cacc[r];
-        }
-      }
-    }
-  };
-  auto write_flux = [&](int cbase) {
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) {
-      const int fm = i + s * NP;
-      if (fm >= NF4) continue;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) sA[(c * NF4 + fm) * E2 + le] = fl[s][cbase + c];
-    }
-  };
-
-  volume(3);  // H -> dE
-  __syncthreads();
-  if (active) curl(acc, 1.0);
-  __syncthreads();
-  volume(0);  // E -> dH
-  __syncthreads();
-  load_lift_frags();
-  if (active) {
-    curl(acc + 3, -1.0);
-    write_flux(0);  // the state is no longer read
-  }
-  __syncthreads();
-  lift();  // E flux
-  __syncthreads();
-  if (active) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) acc[c] += sB[(c * NP + i) * E2 + le];
-    write_flux(3);
-  }
-  __syncthreads();
-  lift();  // H flux
-  __syncthreads();
-  if (!active) return;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) acc[3 + c] += sB[(c * NP + i) * E2 + le];
-  double po[6], srcv[6];
-#pragma unroll
-  for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
-  if (io.src_kind == RFHIP_DRIVE_VECTOR) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
-  }
-  node_update<EPI>(io, base, sm + le * 12, acc, own, po, srcv);
-}
-
 // ---- MFMA volume with the curl in registers -------------------------------
 //
-// k_fused7 maps wave w, lane l of a 16-element block to node 4w + l/16 of
+// k_fused8 maps wave w, lane l of a 16-element block to node 4w + l/16 of
 // element l%16. Its volume A operand interleaves the stacked [Dr; Ds; Dt]
 // by node (row 4k + q <-> derivative k of node 4w + q, a zero row for
 // k = 3), so v_mfma_f64_16x16x4f64 hands each lane, in element r of its
@@ -1526,206 +622,6 @@ struct Mfma7Cfg {
   static constexpr int NT = NW * 64;
   static constexpr int KS = (NP + 3) / 4;
 };
-
-template <int NP, int NFP, bool CIN, int EPI, bool XCD>
-__global__ void __launch_bounds__(Mfma7Cfg<NP>::NT) k_fused7(StageArgs io) {
-  using Cfg = Mfma7Cfg<NP>;
-  constexpr int EPB = 16, E2 = 17, NW = Cfg::NW, NT = Cfg::NT, KS = Cfg::KS;
-  constexpr int NF4 = 4 * NFP;
-  constexpr int FPT = (NF4 + NP - 1) / NP;
-  constexpr int MTL = (NP + 15) / 16, KSL = NF4 / 4;
-  constexpr int SS = 6 * NP * E2, SF = 6 * NF4 * E2;
-  constexpr bool LFIX = MTL <= NW;
-  __shared__ __align__(16) double sA[SS];  // state, then the lifted flux L[c][i][e]
-  __shared__ __align__(16) double sF[SF];  // face flux F[c][fm][e]
-  __shared__ __align__(16) double sg[EPB * 16];           // per face: n, 0.5*fscale
-  __shared__ __align__(16) double sm[EPB * 12];           // material row per element
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int el = lane & 15, q = lane >> 4;
-  const int i = 4 * wave + q;
-  const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;
-  const int64_t grp = group_range(XCD, ngroups).first;
-  if (grp >= ngroups) return;  // padding blocks of the XCD order
-  const int64_t e0 = grp * EPB;
-  const int64_t rem = io.n_elem - e0;
-  const int nel = rem < EPB ? (int)rem : EPB;
-  const bool active = i < NP && el < nel;
-  const int64_t e = e0 + el;
-  const int64_t base = (e * NP + i) * 6;
-
-  // P0: all global reads of the group.
-  double nv[FPT][6];
-  int fcodes[FPT];
-  double ji[9];
-  using PT = typename std::conditional<EPI == 2, float, double>::type;
-  PT pold[6];
-  if (active) {
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) {
-      const int fm = i + s * NP;
-      fcodes[s] = kCodePec;
-      if (fm >= NF4) continue;
-      const int nbv = io.nbase[e * NF4 + fm];
-      fcodes[s] = nbv >= 0 ? 0 : nbv;
-      if (nbv >= 0) load_trace<CIN>(io, nbv, nv[s]);
-    }
-  }
-  if (el < nel) {
-#pragma unroll
-    for (int c = 0; c < 9; ++c) ji[c] = io.jinv[e * 9 + c];
-  }
-  if (active) {
-    if constexpr (EPI == 1) {
-      const double2 *pp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.p) + base);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double2 v = pp[c];
-        pold[2 * c] = v.x;
-        pold[2 * c + 1] = v.y;
-      }
-    } else if constexpr (EPI == 2) {
-      const float2 *pp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.p) + base);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float2 v = pp[c];
-        pold[2 * c] = v.x;
-        pold[2 * c + 1] = v.y;
-      }
-    }
-  }
-  for (int t = tid; t < nel * 12; t += NT) sm[t] = io.mat_tab[12 * io.mat[e0 + t / 12] + t % 12];
-  double av[KS];
-  {
-    const double *ap = io.afrag7 + (size_t)wave * KS * 64 + lane;
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) av[ks] = ap[ks * 64];
-  }
-  {
-    const int64_t g0 = e0 * NP * 6;
-    const int cnt2 = nel * NP * 3;
-    for (int t = tid; t < cnt2; t += NT) {
-      double2 v;
-      if constexpr (CIN) {
-        const float2 h = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + g0)[t];
-        const uint32_t l = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + g0)[t];
-        v.x = (double)h.x + (double)__uint_as_float(l << 16);
-        v.y = (double)h.y + (double)__uint_as_float(l & 0xFFFF0000u);
-      } else {
-        v = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + g0)[t];
-      }
-      const int ee = t / (3 * NP), r2 = t - ee * (3 * NP);
-      const int j = r2 / 3, c = 2 * (r2 - j * 3);
-      sA[(c * NP + j) * E2 + ee] = v.x;
-      sA[((c + 1) * NP + j) * E2 + ee] = v.y;
-    }
-  }
-  if (wave == 0 && q < 4 && el < nel) {  // face q of element el
-    double n[3], cf;
-    face_geom(ji, q, n, cf);
-    double *o = sg + el * 16 + q * 4;
-    o[0] = n[0];
-    o[1] = n[1];
-    o[2] = n[2];
-    o[3] = cf;
-  }
-  __syncthreads();
-
-  // P1: own values, face flux into LDS, and both volume products with their
-  // curls.
-  double own[6], acc[6];
-  if (active) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) own[c] = sA[(c * NP + i) * E2 + el];
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) {
-      const int fm = i + s * NP;
-      if (fm >= NF4) continue;
-      const int f = fm / NFP, m = fm - f * NFP;
-      const double *gq = sg + el * 16 + f * 4;
-      const double n[3] = {gq[0], gq[1], gq[2]};
-      const int vi = io.face_nodes[f * NFP + m];
-      double u[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) u[c] = sA[(c * NP + vi) * E2 + el];
-      double fo[6];
-      face_flux(fcodes[s], io.alpha, n, gq[3], u, nv[s], fo);
-#pragma unroll
-      for (int c = 0; c < 6; ++c) sF[(c * NF4 + fm) * E2 + el] = fo[c];
-    }
-  }
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int cbase = h == 0 ? 3 : 0;
-    mf_d4 cacc[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) cacc[c] = mf_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const bool jv = NP % 4 == 0 || ks < KS - 1 || ks * 4 + q < NP;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double b = jv ? sA[((cbase + c) * NP + ks * 4 + q) * E2 + el] : 0.0;
-        cacc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], b, cacc[c], 0, 0, 0);
-      }
-    }
-    // cacc[c'][k] = d(field_c')/dr_k at node i of element el.
-    auto dd = [&](int p, int c) { return ji[p] * cacc[c][0] + ji[3 + p] * cacc[c][1] + ji[6 + p] * cacc[c][2]; };
-    const double sgn = h == 0 ? 1.0 : -1.0;
-    acc[3 * h + 0] = sgn * (dd(1, 2) - dd(2, 1));
-    acc[3 * h + 1] = sgn * (dd(2, 0) - dd(0, 2));
-    acc[3 * h + 2] = sgn * (dd(0, 1) - dd(1, 0));
-  }
-  __syncthreads();  // the state in LDS is no longer read; the flux is complete
-
-  // P2: LIFT of all six flux components (columns c*16 + e) into sA as L[c][i][e].
-  {
-    const int lcol = lane & 15, lrow = lane >> 4;
-    if constexpr (LFIX) {
-      const int mt = wave % MTL;
-      const int nwm = NW / MTL + (mt < NW % MTL ? 1 : 0);
-      double lf[KSL];
-      const double *ap = io.lfrag + (size_t)mt * KSL * 64 + lane;
-#pragma unroll
-      for (int ks = 0; ks < KSL; ++ks) lf[ks] = ap[ks * 64];
-      for (int nt = wave / MTL; nt < 6; nt += nwm) {
-        const double *bp = sF + (nt * NF4 + lrow) * E2 + lcol;
-        mf_d4 c4 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int ks = 0; ks < KSL; ++ks) c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(lf[ks], bp[ks * 4 * E2], c4, 0, 0, 0);
-        double *lp = sA + (nt * NP + mt * 16 + lrow) * E2 + lcol;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (mt * 16 + r * 4 + lrow < NP) lp[r * 4 * E2] = c4[r];
-      }
-    } else {
-      for (int t = wave; t < MTL * 6; t += NW) {
-        const int mt = t / 6, nt = t - (t / 6) * 6;
-        const double *ap = io.lfrag + (size_t)mt * KSL * 64 + lane;
-        const double *bp = sF + (nt * NF4 + lrow) * E2 + lcol;
-        mf_d4 c4 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int ks = 0; ks < KSL; ++ks) c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[ks * 64], bp[ks * 4 * E2], c4, 0, 0, 0);
-        double *lp = sA + (nt * NP + mt * 16 + lrow) * E2 + lcol;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (mt * 16 + r * 4 + lrow < NP) lp[r * 4 * E2] = c4[r];
-      }
-    }
-  }
-  __syncthreads();
-  if (!active) return;
-#pragma unroll
-  for (int c = 0; c < 6; ++c) acc[c] += sA[(c * NP + i) * E2 + el];
-  double po[6], srcv[6];
-#pragma unroll
-  for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
-  if (io.src_kind == RFHIP_DRIVE_VECTOR) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
-  }
-  node_update<EPI>(io, base, sm + el * 12, acc, own, po, srcv);
-}
 
 // Raw field values of one node as stored (f32 pairs + bf16 pairs, or f64
 // pairs), converted to f64 only where they are used, so a load needs no
@@ -1792,7 +688,7 @@ __device__ __forceinline__ void touch_trace(const TraceRaw<CIN> &r) {
 #define RF_K8_WPE 3  // waves per SIMD the persistent kernel is compiled for
 #endif
 
-// Persistent k_fused7: each block walks over element groups and reads the
+// k_fused8 is persistent: each block walks over element groups and reads the
 // next group's state, neighbour-table entries and material indices into
 // registers while the current group computes, so a group starts without
 // waiting on HBM. The group's own J^-1, old p, material rows and neighbour
@@ -1969,7 +865,7 @@ k_fused8(StageArgs io) {
       for (int s = 0; s < FPT; ++s) {
         const int fm = i + s * NP;
         if (fm >= NF4) continue;
-        const int f = fm / NFP, m = fm - f * NFP;
+        const int f = fm / NFP;
         double n[3], cf;
         face_geom(ji, f, n, cf);
         const int vi = sfn[fm];
@@ -2080,7 +976,7 @@ k_fused8(StageArgs io) {
 //     where a load is issued and the compiler counts outstanding loads
 //     exactly;
 //   * face geometry is formed in the flux and the face-node table is in LDS.
-template <int NP, int NFP, bool CIN, int EPI, int UNR, bool MLIFT = false>
+template <int NP, int NFP, bool CIN, int EPI, int UNR>
 __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3))) k_fused9(StageArgs io) {
   constexpr int EPB = kFusedWG / NP;
   constexpr int NF4 = 4 * NFP;
@@ -2313,509 +1209,6 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
 #pragma unroll
     for (int k = 0; k < MR; ++k) asm volatile("" ::"v"(mrow[k]));
     double own[6];
-    if constexpr (MLIFT) {
-      // LIFT on the matrix cores: L[i][(e,c)] = sum_fm LIFT[i][fm] F[fm][(e,c)]
-      // as 16x16x4 tiles over rows i (MTL tiles) and columns 6e + c (NTL
-      // tiles), the A operand from the LDS copy of LIFT and B from the flux.
-      // L goes back to LDS over the flux (after all tiles have read it) and
-      // each thread takes its node's six values.
-      constexpr int MTL = (NP + 15) / 16, NCOL = EPB * 6, NTL = (NCOL + 15) / 16, KSL = NF4 / 4;
-      constexpr int TPW = (MTL * NTL + 3) / 4;  // tiles per wave
-      const int lane = tid & 63, wave = tid >> 6;
-      const int lc = lane & 15, lr = lane >> 4;
-      mf_d4 ct[TPW];
-#pragma unroll
-      for (int k = 0; k < TPW; ++k) {
-        const int t = wave + 4 * k;
-        ct[k] = mf_d4{0.0, 0.0, 0.0, 0.0};
-        if (t < MTL * NTL) {
-          const int mt = t / NTL, nt = t - (t / NTL) * NTL;
-          const int row = 16 * mt + lc, n = 16 * nt + lc;
-          const double *ap = lift + (row < NP ? row : 0) * LS + lr;
-          const int ne = n < NCOL ? n / 6 : 0, nc = n - (n / 6) * 6;
-          const double *bp = sf + ne * SFS + lr * 6 + nc;
-          const bool av = row < NP, bv = n < NCOL;
-          mf_d4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-          for (int ks = 0; ks < KSL; ++ks) {
-            const double x = av ? ap[4 * ks] : 0.0;
-            const double y = bv ? bp[24 * ks] : 0.0;
-            if (ks & 1)
-              c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c1, 0, 0, 0);
-            else
-              c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c0, 0, 0, 0);
-          }
-#pragma unroll
-          for (int r = 0; r < 4; ++r) ct[k][r] = c0[r] + c1[r];
-        }
-      }
-      __syncthreads();  // every tile has read the flux; L takes its place
-#pragma unroll
-      for (int k = 0; k < TPW; ++k) {
-        const int t = wave + 4 * k;
-        if (t < MTL * NTL) {
-          const int mt = t / NTL, nt = t - (t / NTL) * NTL;
-          const int n = 16 * nt + lc;
-          if (n < NCOL) {
-            const int ne = n / 6, nc = n - ne * 6;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int row = 16 * mt + 4 * r + lr;
-              if (row < NP) sf[(ne * NP + row) * 6 + nc] = ct[k][r];
-            }
-          }
-        }
-      }
-      __syncthreads();  // L complete
-      if (active) {
-        const double *lp = sf + (le * NP + i) * 6;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) acc[c] += lp[c];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) own[c] = ub[i * 6 + c];
-      }
-    } else {
-    if (active) {
-      const double *fb = sf + le * SFS;
-      const double *lr = lift + i * LS;
-#pragma unroll UNR
-      for (int fm = 0; fm < NF4; fm += 2) {  // NF4 = 4 Nfp is even
-        const double2 w = *reinterpret_cast<const double2 *>(lr + fm);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) acc[c] = fma(w.x, fb[fm * 6 + c], acc[c]);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) acc[c] = fma(w.y, fb[fm * 6 + 6 + c], acc[c]);
-      }
-#pragma unroll
-      for (int c = 0; c < 6; ++c) own[c] = ub[i * 6 + c];
-    }
-    }
-    __syncthreads();  // su and sf are read; su takes the next group's state
-    if (grp + gridDim.x < ngroups) {
-      const int64_t rn = io.n_elem - (grp + gridDim.x) * EPB;
-      write_state(rn < EPB ? (int)rn : EPB);
-    }
-    // The remaining read-ahead registers are taken here, before this
-    // group's stores: a wait placed after the (lane-predicated) stores would
-    // be counted as if they had not been issued and would wait on them.
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) asm volatile("" : "+v"(nbn[s]));
-    asm volatile("" : "+v"(mnext));
-    if (active) {
-      double po[6], srcv[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
-      if (io.src_kind == RFHIP_DRIVE_VECTOR) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
-      }
-      node_update<EPI>(io, base, sm + le * 12, acc, own, po, srcv);
-    }
-  }
-}
-
-// ---- sparse volume contraction (k_fused10) ------------------------------
-//
-// The equispaced nodal D_k are sparse. With barycentric multi-indices b (row
-// node) and a (column node), dL_a/dlambda_m at node b can be nonzero only if
-// a_m >= 1 and b_q >= a_q for every q != m (L_a is a product of 1-D factors
-// that vanish at the nodes below a); D_r = d/dlambda_2 - d/dlambda_1 and
-// likewise s (lambda_3) and t (lambda_4). k_fused10 gives each of a block's 4
-// waves a fixed group of nodes (one lane per element and node slot), so a
-// wave visits only the columns some node of its group needs, with the column
-// set known at compile time. The groups balance the waves' column counts;
-// they are the output of gen_node_groups.py (beside this file), which also
-// states the counts. The operator build checks that every D_k entry outside
-// the pattern is zero to round-off before this kernel may run.
-template <int NP>
-struct NodeGroups;
-template <>
-struct NodeGroups<4> {
-  static constexpr int RPW = 1;
-  static constexpr int g[4][1] = {{1}, {0}, {2}, {3}};
-};
-template <>
-struct NodeGroups<10> {
-  static constexpr int RPW = 3;
-  static constexpr int g[4][3] = {{2, 6, 7}, {1, 3, 4}, {5, 8, 9}, {0, -1, -1}};
-};
-template <>
-struct NodeGroups<20> {
-  static constexpr int RPW = 5;
-  static constexpr int g[4][5] = {{0, 3, 11, 12, 19}, {4, 7, 10, 15, 18}, {1, 13, 14, 16, 17}, {2, 5, 6, 8, 9}};
-};
-template <>
-struct NodeGroups<35> {
-  static constexpr int RPW = 9;
-  static constexpr int g[4][9] = {{0, 2, 5, 16, 17, 18, 24, 25, 30},
-                                  {3, 4, 7, 8, 9, 11, 13, 19, 23},
-                                  {1, 6, 10, 12, 14, 22, 31, 32, 33},
-                                  {15, 20, 21, 26, 27, 28, 29, 34, -1}};
-};
-
-// Barycentric multi-index (p - i - j - k, i, j, k) of every node, in node order
-// (i, then j, then k ascending).
-template <int NP>
-struct NodeBary {
-  int b[NP][4];
-  constexpr NodeBary() : b{} {
-    const int p = NP == 4 ? 1 : NP == 10 ? 2 : NP == 20 ? 3 : 4;
-    int n = 0;
-    for (int i = 0; i <= p; ++i)
-      for (int j = 0; j <= p - i; ++j)
-        for (int k = 0; k <= p - i - j; ++k) {
-          b[n][0] = p - i - j - k;
-          b[n][1] = i;
-          b[n][2] = j;
-          b[n][3] = k;
-          ++n;
-        }
-  }
-};
-
-__host__ __device__ constexpr bool d_may_be_nonzero(const int *b, const int *a, int m) {
-  if (a[m] < 1) return false;
-  for (int q = 0; q < 4; ++q)
-    if (q != m && b[q] < a[q]) return false;
-  return true;
-}
-
-// Bit j set: column j of D_k (k = 0 r, 1 s, 2 t) may be nonzero in row i.
-template <int NP>
-__host__ __device__ constexpr uint64_t d_row_mask(int i, int k) {
-  constexpr NodeBary<NP> nb{};
-  uint64_t m = 0;
-  for (int j = 0; j < NP; ++j)
-    if (d_may_be_nonzero(nb.b[i], nb.b[j], k + 1) || d_may_be_nonzero(nb.b[i], nb.b[j], 0)) m |= 1ull << j;
-  return m;
-}
-
-template <int NP>
-__host__ __device__ constexpr uint64_t d_wave_mask(int w, int k) {
-  uint64_t m = 0;
-  for (int s = 0; s < NodeGroups<NP>::RPW; ++s)
-    if (NodeGroups<NP>::g[w][s] >= 0) m |= d_row_mask<NP>(NodeGroups<NP>::g[w][s], k);
-  return m;
-}
-
-// r[k][c] = sum_j D_k[i][j] u[j][c] over the columns of wave W's pattern
-// (dT in LDS as [k][j][i]); the first product of each k initialises it.
-#ifndef SVB
-#define SVB 2
-#endif
-template <int NP, int W>
-__device__ __forceinline__ void sparse_volume(const double *ub, const double *dT, int i, double (&r)[3][6]) {
-  constexpr uint64_t m0 = d_wave_mask<NP>(W, 0), m1 = d_wave_mask<NP>(W, 1), m2 = d_wave_mask<NP>(W, 2);
-  constexpr uint64_t mu = m0 | m1 | m2;
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
-#pragma unroll
-  for (int j = 0; j < NP; ++j) {
-    if (!((mu >> j) & 1)) continue;
-    const double2 *up = reinterpret_cast<const double2 *>(ub + j * 6);
-    const double2 u01 = up[0], u23 = up[1], u45 = up[2];
-    const double u[6] = {u01.x, u01.y, u23.x, u23.y, u45.x, u45.y};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const uint64_t mk = k == 0 ? m0 : k == 1 ? m1 : m2;
-      if (!((mk >> j) & 1)) continue;
-      const double d = dT[(k * NP + j) * NP + i];
-      const bool first = (mk & ((1ull << j) - 1)) == 0;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) r[k][c] = first ? d * u[c] : fma(d, u[c], r[k][c]);
-    }
-    // Bound the scheduler's look-ahead to two visited columns: unbounded,
-    // it hoists every column's LDS reads and spills.
-    if (__builtin_popcountll(mu & ((2ull << j) - 1)) % SVB == 0) __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
-// The same contraction with the wave's column masks as wave-uniform runtime
-// values (scalar registers): one code path for all waves, the skipped
-// columns being scalar branches.
-template <int NP>
-__device__ __forceinline__ void sparse_volume_rt(const double *ub, const double *dT, int i, double (&r)[3][6],
-                                                 uint64_t m0, uint64_t m1, uint64_t m2) {
-  const uint64_t mu = m0 | m1 | m2;
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
-  // Not unrolled: unrolled, the scheduler hoists the reads of many columns
-  // past the scalar branches and the kernel spills.
-#pragma unroll 1
-  for (int j = 0; j < NP; ++j) {
-    if ((mu >> j) & 1) {
-      const double2 *up = reinterpret_cast<const double2 *>(ub + j * 6);
-      const double2 u01 = up[0], u23 = up[1], u45 = up[2];
-      const double u[6] = {u01.x, u01.y, u23.x, u23.y, u45.x, u45.y};
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const uint64_t mk = k == 0 ? m0 : k == 1 ? m1 : m2;
-        if ((mk >> j) & 1) {
-          const double d = dT[(k * NP + j) * NP + i];
-#pragma unroll
-          for (int c = 0; c < 6; ++c) r[k][c] = fma(d, u[c], r[k][c]);
-        }
-      }
-    }
-  }
-}
-
-template <int NP>
-__device__ __forceinline__ uint64_t wave_mask_rt(int wave, int k) {
-  return wave == 0 ? d_wave_mask<NP>(0, k)
-         : wave == 1 ? d_wave_mask<NP>(1, k)
-         : wave == 2 ? d_wave_mask<NP>(2, k)
-                     : d_wave_mask<NP>(3, k);
-}
-
-// The volume term of node i: the sparse contraction for wave W's columns,
-// then J^-1 and the curl, so that only the six curl values leave the
-// wave-specific code (its 18 reference derivatives stay inside).
-template <int NP, int W>
-__device__ __forceinline__ void sparse_curl(const double *ub, const double *dT, int i, const double *ji,
-                                            double *acc) {
-  double r[3][6];
-  sparse_volume<NP, W>(ub, dT, i, r);
-  auto pd = [&](int p, int c) { return ji[p] * r[0][c] + ji[3 + p] * r[1][c] + ji[6 + p] * r[2][c]; };
-  acc[0] = pd(1, 5) - pd(2, 4);
-  acc[1] = pd(2, 3) - pd(0, 5);
-  acc[2] = pd(0, 4) - pd(1, 3);
-  acc[3] = -(pd(1, 2) - pd(2, 1));
-  acc[4] = -(pd(2, 0) - pd(0, 2));
-  acc[5] = -(pd(0, 1) - pd(1, 0));
-}
-
-// k_fused10: k_fused9 with node-major lanes (wave w holds node group w of
-// every element of the block) and the sparse volume contraction above.
-template <int NP, int NFP, bool CIN, int EPI, int UNR>
-__global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3))) k_fused10(StageArgs io) {
-  constexpr int RPW = NodeGroups<NP>::RPW;  // node slots per wave
-  constexpr int EPB = 64 / RPW;              // elements per block (one lane per element and slot)
-  constexpr int NF4 = 4 * NFP;
-  constexpr int FPT = (NF4 + NP - 1) / NP;
-  // D_k as [k][j][i] (a wave reads one column entry for each of its nodes)
-  // and LIFT as [i][fm] (row stride LS, two fm per 16-byte read). The state
-  // of an element is padded to SUS doubles so a wave's elements spread over
-  // the LDS banks.
-  constexpr int LS = NF4 + 2, SUS = NP * 6 + 2;
-  constexpr int ND = 3 * NP * NP;
-  constexpr int SD = ND + NP * LS;
-  constexpr int SFS = NF4 * 6 + 2;
-  constexpr int PPT = (EPB * NP * 3 + kFusedWG - 1) / kFusedWG;
-  __shared__ __align__(16) double su[EPB * SUS];
-  __shared__ __align__(16) double sf[EPB * SFS];
-  __shared__ __align__(16) double sd[SD];
-  __shared__ __align__(16) double sm[EPB * 12];
-  __shared__ int sfn[NF4];
-  __shared__ int sng[4 * RPW];
-  for (int q = threadIdx.x; q < SD; q += kFusedWG) {
-    double v = 0.0;
-    if (q < ND) {
-      v = io.dT[q];
-    } else {
-      const int i = (q - ND) / LS, fm = (q - ND) - i * LS;
-      if (fm < NF4) v = io.lift[fm * NP + i];
-    }
-    sd[q] = v;
-  }
-  for (int q = threadIdx.x; q < NF4; q += kFusedWG) sfn[q] = io.face_nodes[q];
-  for (int q = threadIdx.x; q < 4 * RPW; q += kFusedWG) sng[q] = NodeGroups<NP>::g[q / RPW][q % RPW];
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  __syncthreads();  // lane_ids reads the node-group table from here on
-  const double *dT = sd;
-  const double *lift = sd + ND;
-  const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;
-  // Lane ids come from an opaque copy of threadIdx.x wherever the group loop
-  // uses them, so per-lane addresses are not hoisted out of it and held in
-  // registers.
-  // Lane l of wave w: slot l / EPB of the wave's node group (node i, or -1
-  // for an empty slot or a spare lane) and element l % EPB.
-  auto lane_ids = [&](int &tid, int &le, int &i) {
-    int tv = threadIdx.x;
-    asm volatile("" : "+v"(tv));
-    tid = tv;
-    const int l = tv & 63, slot = l / EPB;
-    le = l - slot * EPB;
-    const int g = slot < RPW ? sng[wave * RPW + slot] : -1;
-    i = (g >= 0 && g < NP) ? g : -1;  // only a valid node index ever becomes an address
-  };
-
-  using HT = typename std::conditional<CIN, float2, double2>::type;
-  HT sh[PPT];
-  uint32_t sl[PPT];
-  int nbn[FPT];
-  int mnext;
-  auto read_ahead = [&](int64_t g) {
-    int tid, le, i;
-    lane_ids(tid, le, i);
-    const int64_t rg = io.n_elem - g * EPB;
-    const int ng = rg < EPB ? (int)rg : EPB;
-    const int64_t gb = g * EPB * NP * 6;
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-      const int q0 = tid + k * kFusedWG;
-      const int q = q0 < ng * NP * 3 ? q0 : ng * NP * 3 - 1;
-      if constexpr (CIN) {
-        sh[k] = reinterpret_cast<const float2 *>(static_cast<const float *>(io.yin) + gb)[q];
-        sl[k] = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(io.yin_lo) + gb)[q];
-      } else {
-        sh[k] = reinterpret_cast<const double2 *>(static_cast<const double *>(io.yin) + gb)[q];
-      }
-    }
-    const int64_t eg = g * EPB + (le < ng ? le : ng - 1);
-    const int ic = i >= 0 ? i : 0;
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) {
-      const int fm = ic + s * NP;
-      const int v = io.nbase[eg * NF4 + (fm < NF4 ? fm : NF4 - 1)];
-      nbn[s] = (i >= 0 && fm < NF4) ? v : kCodePec;
-    }
-    mnext = io.mat[eg];
-  };
-  auto write_state = [&](int ng) {
-    int tid, le, i;
-    lane_ids(tid, le, i);
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-      const int q = tid + k * kFusedWG;
-      if (q < ng * NP * 3) {
-        double2 v;
-        if constexpr (CIN) {
-          v.x = (double)sh[k].x + (double)__uint_as_float(sl[k] << 16);
-          v.y = (double)sh[k].y + (double)__uint_as_float(sl[k] & 0xFFFF0000u);
-        } else {
-          v = sh[k];
-        }
-        const int ee = q / (3 * NP), r2 = q - ee * (3 * NP);
-        reinterpret_cast<double2 *>(su + ee * SUS)[r2] = v;
-      }
-    }
-  };
-  // The launch holds at most ngroups blocks; the last group of a block reads
-  // itself ahead again, which keeps every read-ahead load unconditional.
-  read_ahead(blockIdx.x);
-  {
-    const int64_t r0 = io.n_elem - (int64_t)blockIdx.x * EPB;
-    write_state(r0 < EPB ? (int)r0 : EPB);
-  }
-
-#pragma unroll 1
-  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    int tid, le, i;
-    lane_ids(tid, le, i);
-    const int64_t e0 = grp * EPB;
-    const int64_t rem = io.n_elem - e0;
-    const int nel = rem < EPB ? (int)rem : EPB;
-    const bool active = i >= 0 && le < nel;
-    const int ic = i >= 0 ? i : 0;  // clamped: loads stay in range
-    const int64_t e = e0 + (le < nel ? le : nel - 1);
-    const int64_t base = (e * NP + ic) * 6;
-
-    double ji[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) ji[q] = io.jinv[e * 9 + q];
-    TraceRaw<CIN> nr[FPT];
-    int fcodes[FPT];
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) {
-      fcodes[s] = nbn[s] >= 0 ? 0 : nbn[s];
-      load_trace_raw<CIN>(io, nbn[s] >= 0 ? nbn[s] : 0, nr[s]);
-    }
-    // material row entries i, i + Np, ... (< 12) of element le
-    constexpr int MR = (12 + NP - 1) / NP;
-    double mrow[MR];
-#pragma unroll
-    for (int k = 0; k < MR; ++k) {
-      const int q = ic + k * NP;
-      mrow[k] = io.mat_tab[12 * mnext + (q < 12 ? q : 0)];
-    }
-    read_ahead(grp + gridDim.x < ngroups ? grp + gridDim.x : grp);
-    __syncthreads();  // the group's state is in LDS
-    // Needed only after the flux; issued here, behind the traces, to keep
-    // them out of the registers of the volume contraction.
-    using PT = typename std::conditional<EPI == 2, float, double>::type;
-    PT pold[6];
-    if constexpr (EPI == 1) {
-      const double2 *pp = reinterpret_cast<const double2 *>(static_cast<const double *>(io.p) + base);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const double2 v = pp[q];
-        pold[2 * q] = v.x;
-        pold[2 * q + 1] = v.y;
-      }
-    } else if constexpr (EPI == 2) {
-      const float2 *pp = reinterpret_cast<const float2 *>(static_cast<const float *>(io.p) + base);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const float2 v = pp[q];
-        pold[2 * q] = v.x;
-        pold[2 * q + 1] = v.y;
-      }
-    }
-
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    const double *ub = su + le * SUS;
-    if (active) {
-      {
-        double r[3][6];
-        // wave-uniform column masks: each wave visits its own column set
-        sparse_volume_rt<NP>(ub, dT, i, r, wave_mask_rt<NP>(wave, 0), wave_mask_rt<NP>(wave, 1),
-                             wave_mask_rt<NP>(wave, 2));
-        auto pd = [&](int p, int c) { return ji[p] * r[0][c] + ji[3 + p] * r[1][c] + ji[6 + p] * r[2][c]; };
-        acc[0] = pd(1, 5) - pd(2, 4);
-        acc[1] = pd(2, 3) - pd(0, 5);
-        acc[2] = pd(0, 4) - pd(1, 3);
-        acc[3] = -(pd(1, 2) - pd(2, 1));
-        acc[4] = -(pd(2, 0) - pd(0, 2));
-        acc[5] = -(pd(0, 1) - pd(1, 0));
-      }
-      // The curl is formed here, before the flux: left to itself the
-      // compiler sinks it past the flux and keeps the 18 reference
-      // derivatives live there.
-      asm volatile("" ::"v"(acc[0]), "v"(acc[1]), "v"(acc[2]), "v"(acc[3]), "v"(acc[4]), "v"(acc[5]) : "memory");
-
-#pragma unroll
-      for (int s = 0; s < FPT; ++s) {
-        const int fm = i + s * NP;
-        if (fm >= NF4) continue;
-        const int f = fm / NFP;
-        double n[3], cf;
-        face_geom(ji, f, n, cf);
-        double u[6], nv[6], fo[6];
-        const double *uv = ub + sfn[fm] * 6;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) u[c] = uv[c];
-        trace_f64<CIN>(nr[s], nv);
-        face_flux(fcodes[s], io.alpha, n, cf, u, nv, fo);
-        double2 *out = reinterpret_cast<double2 *>(sf + le * SFS + fm * 6);
-        out[0] = make_double2(fo[0], fo[1]);
-        out[1] = make_double2(fo[2], fo[3]);
-        out[2] = make_double2(fo[4], fo[5]);
-      }
-#pragma unroll
-      for (int k = 0; k < MR; ++k)
-        if (i + k * NP < 12) sm[le * 12 + i + k * NP] = mrow[k];
-    }
-    __syncthreads();  // flux and materials complete
-    // Every register loaded for this group is taken here on all lanes: left
-    // to lanes that are active only, a lane-predicated branch would leave the
-    // loads pending on its skip path, and the next group start would wait for
-    // all outstanding loads before reusing those registers.
-#pragma unroll
-    for (int q = 0; q < 9; ++q) asm volatile("" ::"v"(ji[q]));
-#pragma unroll
-    for (int s = 0; s < FPT; ++s) touch_trace<CIN>(nr[s]);
-    if constexpr (EPI != 0) {
-#pragma unroll
-      for (int c = 0; c < 6; ++c) asm volatile("" ::"v"(pold[c]));
-    }
-#pragma unroll
-    for (int k = 0; k < MR; ++k) asm volatile("" ::"v"(mrow[k]));
-    double own[6];
     if (active) {
       const double *fb = sf + le * SFS;
       const double *lr = lift + i * LS;
@@ -2860,8 +1253,8 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
 // the LIFT reads) stays inside one wave, so after the initial table load
 // there are no block-wide barriers: a wave stalled on memory or LDS does not
 // hold the block's other waves.
-template <int NP, int NFP, bool CIN, int EPI, int UNR>
-__global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3))) k_fused11(StageArgs io) {
+template <int NP, int NFP, bool CIN, int EPI, int UNR, int VMF = 0>
+__global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(VMF ? 2 : 3))) k_fused11(StageArgs io) {
   constexpr int EPB = 64 / NP;  // elements per WAVE: a wave owns whole elements
   constexpr int NWB = kFusedWG / 64;
   constexpr int NF4 = 4 * NFP;
@@ -2870,12 +1263,32 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
   // a 16-byte LDS read then returns two consecutive j (or fm) entries of a
   // thread's row; the padded strides spread the rows over the banks.
   constexpr int DS = NP + (NP & 1) + 2, LS = NF4 + 2;
-  constexpr int ND = 3 * NP * DS;
+  constexpr int ND = VMF ? 0 : 3 * NP * DS;  // with the volume on the matrix cores D is not in LDS
   constexpr int SD = ND + NP * LS;
   constexpr int SFS = NF4 * 6 + 2;
   constexpr int PPT = (EPB * NP * 3 + 63) / 64;
+  // VMF > 0: the volume contraction R = [Dr; Ds; Dt] U of the wave's group
+  // (3Np x Np times Np x 6 EPB, columns element-major field components) on
+  // v_mfma_f32_16x16x32_bf16, both operands split into VMF bf16 pieces
+  // (D_k from the host, the state when it is written to LDS), the products
+  // of pieces p, q with p + q <= VMF + 1 summed smallest first in FP32.
+  // The pieces of the state, B[piece][col][node] with a node stride SBS,
+  // share the wave's flux buffer (free between the LIFT of one group and
+  // the flux of the next); the products go back through the same buffer as
+  // R[e][i][k][c] in f32. J^-1, the curl, the flux, LIFT and the update stay
+  // FP64.
+  constexpr int KSV = (NP + 31) / 32;        // k-steps of 32 nodes
+  constexpr int MTV = (3 * NP + 15) / 16;    // 16-row tiles of [Dr; Ds; Dt]
+  constexpr int NCOL = 6 * EPB;              // columns of U
+  constexpr int NTV = (NCOL + 15) / 16;      // 16-column tiles
+  constexpr int SBS = (NP + 3) & ~3;         // node stride of a column (8-byte aligned reads)
+  constexpr int SBE = NTV * 16 * SBS + 32 * KSV;  // entries per piece, with the read slack of the last tile
+  constexpr int SFW_V = VMF ? ((VMF * SBE * 2 + 7) / 8 > (EPB * NP * 18 * 4 + 7) / 8 ? (VMF * SBE * 2 + 7) / 8
+                                                                                       : (EPB * NP * 18 * 4 + 7) / 8)
+                            : 0;
+  constexpr int SFW = EPB * SFS > SFW_V ? EPB * SFS : SFW_V;
   __shared__ __align__(16) double su_b[NWB][EPB * NP * 6];
-  __shared__ __align__(16) double sf_b[NWB][EPB * SFS];
+  __shared__ __align__(16) double sf_b[NWB][SFW];
   __shared__ __align__(16) double sd[SD];
   __shared__ __align__(16) double sm_b[NWB][EPB * 12];
   __shared__ int sfn[NF4];
@@ -2967,6 +1380,61 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
         reinterpret_cast<double2 *>(su)[q] = v;
       }
     }
+    if constexpr (VMF > 0) {
+      // bf16 pieces of the state, transposed to B[piece][col][node]; zeros for
+      // the node padding, the elements past ng and the read slack after the
+      // last column, so every entry a valid column's fragment reads is finite.
+      uint16_t *sb = reinterpret_cast<uint16_t *>(sf);
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        const int q = tid + k * 64;
+        if (q < ng * NP * 3) {
+          const int e = q / (NP * 3), r = q - e * (NP * 3), j = r / 3, c = 2 * (r - j * 3);
+          uint16_t pc[2][VMF];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            if constexpr (CIN) {
+              // f32 hi + bf16 lo: the hi part splits exactly, lo joins the last piece
+              float t = h ? sh[k].y : sh[k].x;
+              const float lo = __uint_as_float(h ? (sl[k] & 0xFFFF0000u) : (sl[k] << 16));
+#pragma unroll
+              for (int pz = 0; pz < VMF; ++pz) {
+                const float x = pz == VMF - 1 ? t + lo : t;
+                pc[h][pz] = bf16_rne(x);
+                t -= bf16_to_f(pc[h][pz]);
+              }
+            } else {
+              double t = h ? sh[k].y : sh[k].x;
+#pragma unroll
+              for (int pz = 0; pz < VMF; ++pz) {
+                pc[h][pz] = bf16_rne((float)t);
+                t -= (double)bf16_to_f(pc[h][pz]);
+              }
+            }
+          }
+          const int col = 6 * e + c;
+#pragma unroll
+          for (int pz = 0; pz < VMF; ++pz) {
+            sb[pz * SBE + col * SBS + j] = pc[0][pz];
+            sb[pz * SBE + (col + 1) * SBS + j] = pc[1][pz];
+          }
+        }
+      }
+      constexpr int NPAD = SBS - NP;  // node padding per column
+      const int nvalid = ng * 6;
+      const int zcount = nvalid * NPAD + (NCOL - nvalid) * SBS + 32 * KSV;
+      for (int q = tid; q < VMF * zcount; q += 64) {
+        const int pz = q / zcount, z = q - pz * zcount;
+        int at;
+        if (z < nvalid * NPAD) {
+          const int col = NPAD ? z / (NPAD ? NPAD : 1) : 0;
+          at = col * SBS + NP + (z - col * NPAD);
+        } else {
+          at = nvalid * SBS + (z - nvalid * NPAD);
+        }
+        sb[pz * SBE + at] = 0;
+      }
+    }
   };
   // Waves without a group at all (the last block) just leave; the last
   // group of a wave reads itself ahead again, which keeps every read-ahead
@@ -3033,33 +1501,96 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
 
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const double *ub = su + le * NP * 6;
-    if (active) {
-      double r[3][6];
+    double r[3][6];
+    if constexpr (VMF > 0) {
+      // every lane takes part in the MFMAs, active or not
+      const uint16_t *sb = reinterpret_cast<const uint16_t *>(sf);
+      const uint4 *afr = static_cast<const uint4 *>(io.afragb);
+      const int lc = tid & 15, lq = tid >> 4;
+      // all B pieces of the group into registers first, so each product tile
+      // can go back to the (shared) buffer as soon as it is complete
+      bf16x8 bq[KSV][NTV][VMF];
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
+      for (int ks = 0; ks < KSV; ++ks)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
-      const double *dr = dT + i * DS;
-      auto vol_step = [&](int j, double d0, double d1, double d2) {
-        double u[6];
+        for (int nt = 0; nt < NTV; ++nt)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) u[c] = ub[j * 6 + c];
+          for (int pz = 0; pz < VMF; ++pz) {
+            const uint2 *src = reinterpret_cast<const uint2 *>(sb + pz * SBE + (16 * nt + lc) * SBS + 32 * ks + 8 * lq);
+            const uint2 w0 = src[0], w1 = src[1];
+            bq[ks][nt][pz] = __builtin_bit_cast(bf16x8, make_uint4(w0.x, w0.y, w1.x, w1.y));
+          }
+      wave_sync();  // the pieces are read; the products take their place
+      float *sR = reinterpret_cast<float *>(sf);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          r[0][c] = fma(d0, u[c], r[0][c]);
-          r[1][c] = fma(d1, u[c], r[1][c]);
-          r[2][c] = fma(d2, u[c], r[2][c]);
+      for (int mt = 0; mt < MTV; ++mt) {
+        f32x4 cacc[NTV];
+#pragma unroll
+        for (int nt = 0; nt < NTV; ++nt) cacc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < KSV; ++ks) {
+          bf16x8 aq[VMF];
+#pragma unroll
+          for (int pz = 0; pz < VMF; ++pz)
+            aq[pz] = __builtin_bit_cast(bf16x8, afr[((size_t)(pz * MTV + mt) * KSV + ks) * 64 + tid]);
+#pragma unroll
+          for (int nt = 0; nt < NTV; ++nt)
+            // pieces pa, pb (0-based) with pa + pb <= VMF - 1, smallest products first
+#pragma unroll
+            for (int sum = VMF - 1; sum >= 0; --sum)
+#pragma unroll
+              for (int pa = 0; pa <= sum; ++pa)
+                cacc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[pa], bq[ks][nt][sum - pa], cacc[nt], 0, 0, 0);
         }
-      };
-#pragma unroll UNR
-      for (int j = 0; j < NP - 1; j += 2) {
-        const double2 a0 = *reinterpret_cast<const double2 *>(dr + j);
-        const double2 a1 = *reinterpret_cast<const double2 *>(dr + NP * DS + j);
-        const double2 a2 = *reinterpret_cast<const double2 *>(dr + 2 * NP * DS + j);
-        vol_step(j, a0.x, a1.x, a2.x);
-        vol_step(j + 1, a0.y, a1.y, a2.y);
+#pragma unroll
+        for (int nt = 0; nt < NTV; ++nt)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = 16 * mt + 4 * lq + q, col = 16 * nt + lc;
+            if (row < 3 * NP && col < NCOL) {
+              const int k = row / NP, ii = row - k * NP, ee = col / 6, c = col - ee * 6;
+              sR[((ee * NP + ii) * 3 + k) * 6 + c] = cacc[nt][q];
+            }
+          }
       }
-      if constexpr (NP & 1) vol_step(NP - 1, dr[NP - 1], dr[NP * DS + NP - 1], dr[2 * NP * DS + NP - 1]);
+      wave_sync();
+      if (active) {
+        const float *rr = sR + (le * NP + i) * 18;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) r[k][c] = (double)rr[k * 6 + c];
+      }
+      wave_sync();  // the products are read before the flux overwrites them
+    }
+    if (active) {
+      if constexpr (VMF == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) r[k][c] = 0.0;
+        const double *dr = dT + i * DS;
+        auto vol_step = [&](int j, double d0, double d1, double d2) {
+          double u[6];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) u[c] = ub[j * 6 + c];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) {
+            r[0][c] = fma(d0, u[c], r[0][c]);
+            r[1][c] = fma(d1, u[c], r[1][c]);
+            r[2][c] = fma(d2, u[c], r[2][c]);
+          }
+        };
+#pragma unroll UNR
+        for (int j = 0; j < NP - 1; j += 2) {
+          const double2 a0 = *reinterpret_cast<const double2 *>(dr + j);
+          const double2 a1 = *reinterpret_cast<const double2 *>(dr + NP * DS + j);
+          const double2 a2 = *reinterpret_cast<const double2 *>(dr + 2 * NP * DS + j);
+          vol_step(j, a0.x, a1.x, a2.x);
+          vol_step(j + 1, a0.y, a1.y, a2.y);
+        }
+        if constexpr (NP & 1) vol_step(NP - 1, dr[NP - 1], dr[NP * DS + NP - 1], dr[2 * NP * DS + NP - 1]);
+      }
       auto pd = [&](int p, int c) { return ji[p] * r[0][c] + ji[3 + p] * r[1][c] + ji[6 + p] * r[2][c]; };
       acc[0] = pd(1, 5) - pd(2, 4);
       acc[1] = pd(2, 3) - pd(0, 5);
@@ -3180,147 +1711,101 @@ inline int64_t resident_blocks_nt(const void *kernel, int nt) {
 // launches use this many).
 inline int64_t resident_blocks(const void *kernel) { return resident_blocks_nt(kernel, kFusedWG); }
 
-// Kernel variant per launch: 0 = k_fused (one node per thread), 1/2 =
-// k_fused2 with 2/1 nodes per thread, 3 = k_fused3 (early neighbour
-// indices), 4 = k_fused3 with early neighbour traces, 5 = k_fused3 with D
-// and LIFT in LDS over a grid-stride loop, 6 = k_fused4 (FP64 MFMA volume),
-// 7 = k_fused4 with LIFT on MFMA too, 8 = k_fused5 (MFMA volume and LIFT,
-// linear operand addressing), 9 = variant 5 and 10 = variant 8 with the
-// XCD-aware group order, 11 = k_fused7 (curl in registers; Np <= 20, else
-// variant 5), 12/13 = variant 5 with the next group's state read ahead
-// (loops unrolled by 4 / 2), 14 = variant 13 with the read-ahead issued
-// after the group's own reads and the face geometry formed in the flux,
-// 15 = k_fused8 (persistent k_fused7 with read-ahead; Np <= 20),
-// 16 / 17 = k_fused9 (k_fused3 layout with the memory traffic pipelined;
-// loops unrolled by 2 / 4 nodes), 18 / 19 = k_fused9 with the LIFT on the
-// matrix cores, 20 / 21 = k_fused10 (sparse volume, LIFT unrolled by 2 / 4;
-// only when the operator's D_k fit the equispaced pattern), 22 / 23 =
-// k_fused11 (whole elements per wave, no block barriers in the group loop). Chosen by the
-// operator (autotuned or set).
+// Kernel variant per launch, each measured fastest for some order and
+// register form on MI355X:
+//   0     k_fused3, D and LIFT in LDS, grid-stride over element groups;
+//   1, 2  k_fused3 with the next group's state read ahead into registers,
+//         loops unrolled by 4 (1) or by 2 with the read-ahead issued after
+//         the group's own reads (2);
+//   3     k_fused8: FP64 MFMA volume with the curl in registers, persistent,
+//         read-ahead (Np <= 20);
+//   4, 5  k_fused9: the k_fused3 layout with the group's memory traffic
+//         pipelined, loops unrolled by 2 / 4 nodes;
+//   6, 7  k_fused11: k_fused9 with whole elements per wave and no
+//         block-wide barriers in the group loop, unrolled by 2 / 4.
+// Variants 3 to 7 read the neighbour-trace table (built for conforming
+// meshes); variant 3 needs Np <= 20 (variant_available).
 
 template <int NP, bool CIN, int EPI>
 void fused_np(int variant, const StageArgs &a, hipStream_t s) {
   constexpr int F = Nfp<NP>::v;
   constexpr int EPB1 = kFusedWG / NP;
   const int64_t groups1 = (a.n_elem + EPB1 - 1) / EPB1;
-  if (variant == 0) {
-    hipLaunchKernelGGL((k_fused<NP, F, CIN, EPI>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
-  } else if (variant == 1) {
-    constexpr int EPB = kFusedWG / ((NP + 1) / 2);
-    const int64_t groups = (a.n_elem + EPB - 1) / EPB;
-    hipLaunchKernelGGL((k_fused2<NP, F, 2, CIN, EPI>), dim3((unsigned)groups), dim3(kFusedWG), 0, s, a);
-  } else if (variant == 2) {
-    hipLaunchKernelGGL((k_fused2<NP, F, 1, CIN, EPI>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
-  } else if (variant == 3) {
-    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, false, false>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
-  } else if (variant == 4) {
-    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, false, true>), dim3((unsigned)groups1), dim3(kFusedWG), 0, s, a);
-  } else if (variant == 6 || variant == 7) {
-    using Cfg = MfmaCfg<NP>;
-    const int64_t groups = (a.n_elem + Cfg::EPB - 1) / Cfg::EPB;
-    if (variant == 6)
-      hipLaunchKernelGGL((k_fused4<NP, F, CIN, EPI, false>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_fused4<NP, F, CIN, EPI, true>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
-  } else if ((variant == 8 || variant == 10) && a.nbase) {
-    using Cfg = Mfma5Cfg<NP>;
-    const int64_t groups = (a.n_elem + Cfg::EPB - 1) / Cfg::EPB;
-    if (variant == 8)
-      hipLaunchKernelGGL((k_fused5<NP, F, CIN, EPI, false>), dim3((unsigned)groups), dim3(Cfg::NT), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_fused5<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))), dim3(Cfg::NT),
-                         0, s, a);
-  } else if ((variant == 16 || variant == 17) && a.nbase) {
-    const void *kf = variant == 16 ? reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 1>)
-                                   : reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 2>);
+  auto persistent = [&](const void *kf, int64_t groups) {
     const int64_t res = resident_blocks(kf);
-    const int64_t grid = groups1 < res ? groups1 : res;
-    if (grid > 0) {
-      if (variant == 16)
-        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
-    }
-  } else if ((variant == 22 || variant == 23) && a.nbase) {
-    const void *kf = variant == 22 ? reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 1>)
-                                   : reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 2>);
-    const int64_t res = resident_blocks(kf);
-    const int64_t epw = 64 / NP;
-    const int64_t wgroups = (a.n_elem + epw - 1) / epw;
-    const int64_t groups = (wgroups + 3) / 4;  // blocks of 4 waves
-    const int64_t grid = groups < res ? groups : res;
-    if (grid > 0) {
-      if (variant == 22)
-        hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
-    }
-  } else if ((variant == 20 || variant == 21) && a.nbase && a.dsparse) {
-    const void *kf = variant == 20 ? reinterpret_cast<const void *>(&k_fused10<NP, F, CIN, EPI, 1>)
-                                   : reinterpret_cast<const void *>(&k_fused10<NP, F, CIN, EPI, 2>);
-    const int64_t res = resident_blocks(kf);
-    const int64_t epb = 64 / NodeGroups<NP>::RPW;
-    const int64_t groups = (a.n_elem + epb - 1) / epb;
-    const int64_t grid = groups < res ? groups : res;
-    if (grid > 0) {
-      if (variant == 20)
-        hipLaunchKernelGGL((k_fused10<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_fused10<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
-    }
-  } else if ((variant == 18 || variant == 19) && a.nbase) {
-    const void *kf = variant == 18 ? reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 1, true>)
-                                   : reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 2, true>);
-    const int64_t res = resident_blocks(kf);
-    const int64_t grid = groups1 < res ? groups1 : res;
-    if (grid > 0) {
-      if (variant == 18)
-        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 1, true>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2, true>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
-    }
-  } else if (variant == 15 && a.nbase && NP <= 20) {
+    return groups < res ? groups : res;
+  };
+  if (variant == 3 && a.nbase && NP <= 20) {
     const int64_t res = resident_blocks_nt(reinterpret_cast<const void *>(&k_fused8<NP, F, CIN, EPI>), Mfma7Cfg<NP>::NT);
     const int64_t groups = (a.n_elem + 15) / 16;
     const int64_t grid = groups < res ? groups : res;
     if (grid > 0)
       hipLaunchKernelGGL((k_fused8<NP, F, CIN, EPI>), dim3((unsigned)grid), dim3(Mfma7Cfg<NP>::NT), 0, s, a);
-  } else if (variant == 11 && a.nbase && NP <= 20) {
-    const int64_t groups = (a.n_elem + 15) / 16;
-    hipLaunchKernelGGL((k_fused7<NP, F, CIN, EPI, true>), dim3((unsigned)(8 * ((groups + 7) / 8))),
-                       dim3(Mfma7Cfg<NP>::NT), 0, s, a);
-  } else if (variant == 14) {
-    const int64_t res = resident_blocks(
-          reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 2, true>));
-    const int64_t grid = groups1 < res ? groups1 : res;
-    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 2, true>), dim3((unsigned)grid),
-                       dim3(kFusedWG), 0, s, a);
-  } else if (variant == 12 || variant == 13) {
-    // variant 5 with the next group's state read ahead (12), and with loops
-    // unrolled by 2 instead of 4 (13)
-    const void *kf = variant == 12
-                         ? reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 4>)
-                         : reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 2>);
-    const int64_t res = resident_blocks(kf);
-    const int64_t grid = groups1 < res ? groups1 : res;
-    if (variant == 12)
-      hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 4>), dim3((unsigned)grid),
-                         dim3(kFusedWG), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 2>), dim3((unsigned)grid),
-                         dim3(kFusedWG), 0, s, a);
-  } else if (variant == 9) {
-    const int64_t resident = resident_blocks(reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, true>));
-    const int64_t cap = 8 * ((groups1 + 7) / 8);
-    int64_t grid = (resident / 8) * 8;
-    if (grid < 8) grid = 8;
-    if (grid > cap) grid = cap;
-    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, true>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+  } else if ((variant == 4 || variant == 5) && a.nbase) {
+    const void *kf = variant == 4 ? reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 1>)
+                                  : reinterpret_cast<const void *>(&k_fused9<NP, F, CIN, EPI, 2>);
+    const int64_t grid = persistent(kf, groups1);
+    if (grid > 0) {
+      if (variant == 4)
+        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+    }
+  } else if ((variant == 6 || variant == 7) && a.nbase) {
+    const void *kf = variant == 6 ? reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 1>)
+                                  : reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 2>);
+    const int64_t epw = 64 / NP;
+    const int64_t wgroups = (a.n_elem + epw - 1) / epw;
+    const int64_t grid = persistent(kf, (wgroups + 3) / 4);  // blocks of 4 waves
+    if (grid > 0) {
+      if (variant == 6)
+        hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+    }
+  } else if (variant == 2) {
+    const void *kf = reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 2, true>);
+    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 2, true>),
+                       dim3((unsigned)persistent(kf, groups1)), dim3(kFusedWG), 0, s, a);
+  } else if (variant == 1) {
+    const void *kf = reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 4>);
+    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 4>),
+                       dim3((unsigned)persistent(kf, groups1)), dim3(kFusedWG), 0, s, a);
   } else {
-    const int64_t resident = resident_blocks(reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false>));
-    const int64_t grid = groups1 < resident ? groups1 : resident;
-    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+    const void *kf = reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false>);
+    hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false>), dim3((unsigned)persistent(kf, groups1)),
+                       dim3(kFusedWG), 0, s, a);
   }
+}
+
+// The precision forms: k_fused11 with the volume on the matrix cores in
+// VMF bf16 pieces, variant 0 / 1 = loops unrolled by 1 / 2.
+template <int NP, bool CIN, int EPI, int VMF>
+void vmf_np(int variant, const StageArgs &a, hipStream_t s) {
+  constexpr int F = Nfp<NP>::v;
+  const void *kf = variant == 0 ? reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 1, VMF>)
+                                : reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 2, VMF>);
+  const int64_t res = resident_blocks(kf);
+  const int64_t epw = 64 / NP;
+  const int64_t wgroups = (a.n_elem + epw - 1) / epw;
+  const int64_t groups = (wgroups + 3) / 4;  // blocks of 4 waves
+  const int64_t grid = groups < res ? groups : res;
+  if (grid <= 0) return;
+  if (variant == 0)
+    hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 1, VMF>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 2, VMF>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+}
+
+template <bool CIN, int EPI>
+void vmf_dispatch(int np, int vmf, int variant, const StageArgs &a, hipStream_t s) {
+  switch (np) {
+    case 10: vmf == 3 ? vmf_np<10, CIN, EPI, 3>(variant, a, s) : vmf_np<10, CIN, EPI, 2>(variant, a, s); break;
+    case 20: vmf == 3 ? vmf_np<20, CIN, EPI, 3>(variant, a, s) : vmf_np<20, CIN, EPI, 2>(variant, a, s); break;
+    case 35: vmf == 3 ? vmf_np<35, CIN, EPI, 3>(variant, a, s) : vmf_np<35, CIN, EPI, 2>(variant, a, s); break;
+    default: throw Error{RFHIP_EUNSUPPORTED, "the precision forms run orders 2 to 4"};
+  }
+  RF_HIP(hipGetLastError());
 }
 
 template <bool CIN, int EPI>
@@ -3416,35 +1901,13 @@ __global__ void __launch_bounds__(256) k_port_source(StageArgs sa, DevOp dv, int
   }
 }
 
-template <int NP>
-static bool d_fits_pattern_np(const double *dT) {
-  for (int k = 0; k < 3; ++k) {
-    double mx = 0.0;
-    for (int q = 0; q < NP * NP; ++q) mx = std::fmax(mx, std::fabs(dT[k * NP * NP + q]));
-    for (int j = 0; j < NP; ++j)
-      for (int i = 0; i < NP; ++i)
-        if (!((d_row_mask<NP>(i, k) >> j) & 1) && std::fabs(dT[(k * NP + j) * NP + i]) > 1e-13 * mx) return false;
-  }
-  return true;
-}
-
-bool d_fits_sparse_pattern(int order, const double *dT) {
-  switch (order) {
-    case 1: return d_fits_pattern_np<4>(dT);
-    case 2: return d_fits_pattern_np<10>(dT);
-    case 3: return d_fits_pattern_np<20>(dT);
-    case 4: return d_fits_pattern_np<35>(dT);
-    default: return false;
-  }
-}
-
 void launch_port_source(const rfhip_op *op, int64_t n_elems, const int64_t *elem, const int64_t *off,
                         const int32_t *lf, const double *prof, double *b, hipStream_t s) {
   if (n_elems == 0) return;
   int64_t blocks = (n_elems * op->np + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   const unsigned g = (unsigned)blocks;
-  if (op->form == RFHIP_FORM_FUSED_F64 || op->form == RFHIP_FORM_FUSED_F64C)
+  if (form_fused(op->form))
     hipLaunchKernelGGL((k_port_source<true, double>), dim3(g), dim3(256), 0, s, op->sa, op->dev, n_elems, elem, off,
                        lf, prof, b);
   else if (op->form == RFHIP_FORM_PLAIN_F64)
@@ -3456,26 +1919,105 @@ void launch_port_source(const rfhip_op *op, int64_t n_elems, const int64_t *elem
   RF_HIP(hipGetLastError());
 }
 
-bool variant_available(int order, int variant, const StageArgs &a) {
-  const int np = (order + 1) * (order + 2) * (order + 3) / 6;
-  switch (variant) {
-    case 8: case 10: case 16: case 17: case 18: case 19: case 22: case 23:
-      return a.nbase != nullptr;
-    case 20: case 21:
-      return a.nbase != nullptr && a.dsparse != 0;
-    case 11: case 15:
-      return a.nbase != nullptr && np <= 20;
-    default:
-      return variant >= 0 && variant < 24;
+// The held port drive of one LSERK4 stage with its waveform sample on the
+// device, g = gseq[*gidx], over the DOFs of the elements with port faces
+// (the only nonzero entries of the port source b, rhs.rs:1157-1199):
+//   mode 0 (plain forms, before the stage kernel):  k += b g;
+//   mode 1 (fused FP64, after the stage kernel):    p += dt b g, y += sb dt b g;
+//   mode 2 (fused compressed registers): the same on p (f32) and on y
+//          (f32 hi + bf16 lo, recompressed as the stage kernel does).
+// The stage kernel has run without the drive, so together they give
+// p = a p + dt (A y + b g), y = y + sb p.
+template <int MODE, class T>
+__global__ void __launch_bounds__(256) k_port_stage(int64_t n_elems, const int64_t *__restrict__ elem, int ndof_e,
+                                                    const double *__restrict__ b, const double *__restrict__ gseq,
+                                                    const int64_t *__restrict__ gidx, T *__restrict__ kp,
+                                                    void *__restrict__ yh, uint16_t *__restrict__ yl, double dt,
+                                                    double sb) {
+  const double g = gseq[gidx[0]];
+  const int64_t total = n_elems * ndof_e;
+  for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < total; q += (int64_t)gridDim.x * 256) {
+    const int64_t u = q / ndof_e;
+    const int64_t d = elem[u] * ndof_e + (q - u * ndof_e);
+    const double bg = b[d] * g;
+    if constexpr (MODE == 0) {
+      kp[d] = (T)((double)kp[d] + bg);
+    } else if constexpr (MODE == 1) {
+      double *y = static_cast<double *>(yh);
+      const double pn = (double)kp[d] + dt * bg;
+      kp[d] = (T)pn;
+      y[d] += sb * dt * bg;
+    } else {
+      float *y = static_cast<float *>(yh);
+      kp[d] = (T)((double)kp[d] + dt * bg);
+      const double yn = (double)y[d] + (double)bf16_to_f(yl[d]) + sb * dt * bg;
+      const float h = (float)yn;
+      y[d] = h;
+      yl[d] = bf16_rne((float)(yn - (double)h));
+    }
   }
+}
+
+__global__ void k_step_advance(int64_t *gidx) {
+  if (threadIdx.x == 0) gidx[0] += 1;
+}
+
+void launch_port_stage(const rfhip_op *op, bool before, const StageArgs &a, void *k, double dt, double sb,
+                       hipStream_t s) {
+  const int64_t ne = op->ps_count;
+  if (ne == 0) return;
+  const int ndof_e = 6 * op->np;
+  int64_t blocks = (ne * ndof_e + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  const dim3 g((unsigned)blocks), b(256);
+  const int64_t *el = op->ps_elem.as<int64_t>();
+  const double *src = op->src.as<double>(), *gs = op->gseq.as<double>();
+  const int64_t *gi = op->gctr.as<int64_t>();
+  if (before) {  // plain forms: k += b g before the stage kernel
+    if (op->form == RFHIP_FORM_PLAIN_F32)
+      hipLaunchKernelGGL((k_port_stage<0, float>), g, b, 0, s, ne, el, ndof_e, src, gs, gi, static_cast<float *>(k),
+                         nullptr, nullptr, dt, sb);
+    else
+      hipLaunchKernelGGL((k_port_stage<0, double>), g, b, 0, s, ne, el, ndof_e, src, gs, gi,
+                         static_cast<double *>(k), nullptr, nullptr, dt, sb);
+  } else if (!form_compressed(op->form)) {
+    hipLaunchKernelGGL((k_port_stage<1, double>), g, b, 0, s, ne, el, ndof_e, src, gs, gi, static_cast<double *>(a.p),
+                       a.yout, nullptr, dt, sb);
+  } else {
+    hipLaunchKernelGGL((k_port_stage<2, float>), g, b, 0, s, ne, el, ndof_e, src, gs, gi, static_cast<float *>(a.p),
+                       a.yout, static_cast<uint16_t *>(a.yout_lo), dt, sb);
+  }
+  RF_HIP(hipGetLastError());
+}
+
+void launch_step_advance(const rfhip_op *op, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, op->gctr.as<int64_t>());
+  RF_HIP(hipGetLastError());
+}
+
+bool variant_available(int order, int form, int variant, const StageArgs &a) {
+  const int np = (order + 1) * (order + 2) * (order + 3) / 6;
+  if (form_vmf(form)) return variant >= 0 && variant < kStageVariantsBf16 && a.nbase != nullptr && order >= 2 && order <= 4;
+  if (variant < 0 || variant >= kStageVariants) return false;
+  if (variant == 3) return a.nbase != nullptr && np <= 20;
+  if (variant >= 4) return a.nbase != nullptr;
+  return true;
 }
 
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s) {
   const int np = (order + 1) * (order + 2) * (order + 3) / 6;
   const int variant = a.variant;
+  if (const int vmf = form_vmf(form)) {
+    if (!a.nbase || !a.afragb) throw Error{RFHIP_EUNSUPPORTED, "the precision forms need a conforming mesh"};
+    if (epi == 0)
+      vmf_dispatch<false, 0>(np, vmf, variant, a, s);
+    else
+      vmf_dispatch<true, 2>(np, vmf, variant, a, s);
+    return;
+  }
   if (epi == 0)
     fused_dispatch<false, 0>(np, variant, a, s);
-  else if (form == RFHIP_FORM_FUSED_F64C)
+  else if (form_compressed(form))
     fused_dispatch<true, 2>(np, variant, a, s);
   else
     fused_dispatch<false, 1>(np, variant, a, s);

@@ -109,6 +109,13 @@ void upload(DevBuf &b, const T *host, size_t count, hipStream_t s) {
 
 // ---- operator ----------------------------------------------------------
 
+// Forms with the fused stage kernel; with compressed state registers (y f32
+// hi + bf16 lo, p f32); and the BF16 pieces of the precision forms' volume
+// contraction (0: FP64).
+inline bool form_fused(int f) { return f >= RFHIP_FORM_FUSED_F64 && f <= RFHIP_FORM_FUSED_BF16X2; }
+inline bool form_compressed(int f) { return f >= RFHIP_FORM_FUSED_F64C && f <= RFHIP_FORM_FUSED_BF16X2; }
+inline int form_vmf(int f) { return f == RFHIP_FORM_FUSED_BF16X3 ? 3 : f == RFHIP_FORM_FUSED_BF16X2 ? 2 : 0; }
+
 // Per-face code of the fused kernels, one int32 per (element, face):
 //   interior: (neighbour << 5) | (neighbour_local_face << 3) | orientation
 //   PEC wall: -1;  port p: -2 - p.
@@ -135,9 +142,9 @@ struct StageArgs {
   int variant;            // kernel variant (rfhip_fused.hip)
   const double *dT;       // 3*Np*Np: dT[(k*Np + j)*Np + i] = D_k[i][j]
   const double *lift;     // LIFT column-major: lift[fm*Np + i]
-  const double *afrag;    // FP64 MFMA A fragments of [Dr; Ds; Dt] (rfhip_fused.hip k_fused4)
-  const double *lfrag;    // FP64 MFMA A fragments of LIFT (k_fused4 with LIFT on the matrix cores)
-  const double *afrag7;   // node-interleaved A fragments of [Dr; Ds; Dt] (k_fused7)
+  const void *afragb;     // split-BF16 A fragments of [Dr; Ds; Dt] (precision forms, k_fused11 VMF)
+  const double *lfrag;    // FP64 MFMA A fragments of LIFT (k_fused8)
+  const double *afrag7;   // node-interleaved A fragments of [Dr; Ds; Dt] (k_fused8)
   const double *jinv;     // 9K
   const int *fcode;       // 4K
   const int *mat;         // K
@@ -145,7 +152,6 @@ struct StageArgs {
   const int *face_nodes;  // 4*Nfp
   const int *nbr_node;    // [(f*4 + nlf)*6 + orientation][Nfp] -> neighbour volume node
   const int *nbase;       // [e*4Nfp + fm]: AoS index of the neighbour trace, or the (negative) face code
-  int dsparse;            // D_k zero outside the equispaced sparsity pattern (k_fused10 may run)
   const void *yin;        // double, or float hi (compressed)
   const void *yin_lo;     // bf16 lo (compressed) or null
   void *yout, *yout_lo;
@@ -178,7 +184,7 @@ struct rfhip_op {
   // device operator data
   rfhip::DevBuf dr, ds, dt, lift, face_nodes, jinv, inv_eps, inv_mu, sig_eps, sig_mu, normal, fscale, nbr,
       nbr_lf, port, perm;
-  rfhip::DevBuf dT, liftc, afrag, lfrag, afrag7, jinv64, fcode, mat, mat_tab, nbr_node, nbase;
+  rfhip::DevBuf dT, liftc, lfrag, afrag7, afragb, jinv64, fcode, mat, mat_tab, nbr_node, nbase;
   int64_t n_vec = 0;  // length of device vectors (n_dof)
   // Debye dispersive block: slot -> element and (a, g, 1/eps_inf); the P
   // block follows the 6 Np K values of [E,H] in every state vector
@@ -203,6 +209,9 @@ struct rfhip_op {
   std::vector<PortEntries> ports;
   rfhip::DevBuf port_dof, port_prof, port_w, port_off, port_norm, port_rows;
   rfhip::DevBuf ps_elem, ps_off, ps_lf, ps_prof;  // port-source assembly input
+  int64_t ps_count = 0;  // elements with port faces of the assembled source (ps_elem)
+  // waveform samples of a batched port drive and the step index into them
+  rfhip::DevBuf gseq, gctr;
   bool src_resident = false;  // src holds a device-assembled port source
   // wall time of the device part of the last Krylov or KCL call (input
   // resident to output ready, host transfers excluded), milliseconds
@@ -244,18 +253,29 @@ void launch_add_point(const rfhip_op *op, void *k, int64_t dof, double val, hipS
 void launch_add_vec(const rfhip_op *op, void *k, const void *src, double g, hipStream_t s);
 
 // Fused forms: epi 0 = apply (dy), 1 = LSERK stage (FP64 or compressed
-// registers by form).
+// registers by form). Stage-kernel variants 0 .. kStageVariants-1 (the
+// precision forms: 0 .. kStageVariantsBf16-1).
+constexpr int kStageVariants = 8;
+constexpr int kStageVariantsBf16 = 2;
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s);
 // Whether stage-kernel variant `variant` runs as itself on this operator's
-// data (some need the neighbour-trace table, the sparse D_k pattern or
-// Np <= 20; launch_stage runs the base variant for them otherwise).
-bool variant_available(int order, int variant, const StageArgs &a);
+// data (some need the neighbour-trace table or Np <= 20; launch_stage runs
+// the base variant for them otherwise).
+bool variant_available(int order, int form, int variant, const StageArgs &a);
 // Debye block after an applied derivative (dy = A y; f32: plain FP32 form),
 // with the drive of the P rows, and after a fused LSERK4 stage
 // (rfhip_debye.hip). No-ops without Debye elements.
 void launch_debye_apply(const rfhip_op *op, const void *y, void *dy, bool f32, int kind, int64_t dof,
                         const double *vec, double g, hipStream_t s);
 void launch_debye_stage(const rfhip_op *op, const StageArgs &a, hipStream_t s);
+
+// The held port drive of one LSERK4 stage read from op->gseq[*op->gctr]
+// over the port elements of the assembled source (rfhip_fused.hip): before
+// (plain forms) k += b g; after a fused stage p += dt b g, y_out += sb dt b g.
+void launch_port_stage(const rfhip_op *op, bool before, const StageArgs &a, void *k, double dt, double sb,
+                       hipStream_t s);
+// *op->gctr += 1: the next step's waveform sample.
+void launch_step_advance(const rfhip_op *op, hipStream_t s);
 
 // Port source b (rhs.rs:1157-1199) into the zeroed f64 vector b for the port
 // faces grouped by element: element elem[u] has faces [off[u], off[u+1]) with
@@ -270,9 +290,6 @@ void launch_port_source(const rfhip_op *op, int64_t n_elems, const int64_t *elem
 void assemble_sparse(rfhip_csr *csr, const rfhip_op_desc &d, int64_t n_disp, const int64_t *disp_elem,
                      const double *disp_coef);
 
-// True when every entry of dT (3*Np*Np, [k][j][i]) outside the equispaced
-// nodal sparsity pattern is zero to round-off (|x| <= 1e-13 * max |D_k|).
-bool d_fits_sparse_pattern(int order, const double *dT);
 
 // Conversions between boundary AoS and the form's storage.
 void launch_f64_to_c(const double *src, float *hi, uint16_t *lo, int64_t n, hipStream_t s);

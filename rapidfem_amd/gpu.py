@@ -95,7 +95,9 @@ class GpuContext:
 class GpuOperator:
     """gpu/operator.rs:78-153 GpuOperator; form selects the kernel family
     ('plain_f32' = the reference's structure and precision, 'plain_f64',
-    'fused_f64', 'fused_f64c')."""
+    'fused_f64', 'fused_f64c', and the precision forms 'fused_bf16x3' /
+    'fused_bf16x2': fused_f64c with the volume contraction on the matrix
+    cores in 3- or 2-piece split BF16 with FP32 accumulation)."""
 
     def __init__(self, ctx: GpuContext, op: MaxwellOperator | None, form: str = "fused_f64", *, desc=None,
                  mass=None, det=None, dispersive=None):
@@ -266,6 +268,31 @@ class GpuOperator:
         self.last_dim = dim.value
         return out
 
+    def expmv_adaptive(self, ctx, v, t_end, ctl=None, proposer=None, log_cap: int = 100_000, hess_dim: int = 0):
+        """exp(t_end A) v with (h, m) chosen per step (rfhip_op_expmv_adaptive;
+        no reference counterpart: the reference's exponential route steps at a
+        fixed h and m, session.rs:700-726). ctl: an ExpController; proposer:
+        a trained Proposer, or None for the plain controller. Returns
+        (out, step records (numpy EXP_STEP_DTYPE), Hessenberg blocks
+        (n_steps, hess_dim+1, hess_dim) or None)."""
+        from .proposer import ExpController
+
+        ctl = ctl or ExpController()
+        v = self._vec(v, np.float64)
+        out = np.empty_like(v)
+        steps = np.zeros(max(int(log_cap), 1), L.EXP_STEP_DTYPE)
+        hess = np.zeros((max(int(log_cap), 1), hess_dim + 1, hess_dim)) if hess_dim > 0 else None
+        n = C.c_int64()
+        cc = ctl.c()
+        L.check(L.lib.rfhip_op_expmv_adaptive(self._h, v, float(t_end), C.byref(cc),
+                                              None if proposer is None else proposer._h,
+                                              steps.ctypes.data_as(C.c_void_p), int(log_cap),
+                                              None if hess is None else hess.ctypes.data_as(C.c_void_p),
+                                              int(hess_dim), C.byref(n), out))
+        k = min(n.value, int(log_cap))
+        self.last_steps = n.value
+        return out, steps[:k].copy(), None if hess is None else hess[:k].copy()
+
     # gpu/operator.rs:1267-1298 (CPU algorithm propagator.rs:338-373)
     def etd_step(self, ctx, y, b, h, m, tol=0.0):
         y = self._vec(y, np.float64)
@@ -292,6 +319,9 @@ class GpuOperator:
     @property
     def variant(self) -> int:
         return int(L.lib.rfhip_op_variant(self._h))
+
+    def variant_available(self, v: int) -> bool:
+        return bool(L.lib.rfhip_op_variant_available(self._h, int(v)))
 
     def poison_lds(self) -> None:
         """Fill every CU's LDS with NaN (a test aid: a kernel that reads LDS
@@ -341,6 +371,30 @@ def batch_lserk(ops: list[GpuOperator], dts, nsteps: int) -> None:
     handles = (C.c_void_p * n)(*[g._h.value for g in ops])
     d = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, np.float64), (n,)))
     L.check(L.lib.rfhip_batch_lserk(handles, n, d, int(nsteps)))
+
+
+def batch_lserk_ports(ops: list[GpuOperator], dts, nsteps: int, waves) -> None:
+    """As batch_lserk, each operator with a waveform (waves[i], nsteps
+    samples held per step; None = free) driven by its device-assembled port
+    source (GpuOperator.port_source) (the batching form with port
+    excitations; no reference counterpart)."""
+    n = len(ops)
+    if len(waves) != n:
+        raise ValueError("one waveform (or None) per operator")
+    handles = (C.c_void_p * n)(*[g._h.value for g in ops])
+    d = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, np.float64), (n,)))
+    keep = []
+    ptrs = (C.c_void_p * n)()
+    for i, w in enumerate(waves):
+        if w is None:
+            ptrs[i] = None
+            continue
+        w = np.ascontiguousarray(w, np.float64)
+        if w.shape != (int(nsteps),):
+            raise ValueError(f"waveform {i} needs {nsteps} samples, got {w.shape}")
+        keep.append(w)
+        ptrs[i] = w.ctypes.data_as(C.c_void_p).value
+    L.check(L.lib.rfhip_batch_lserk_ports(handles, n, d, int(nsteps), ptrs))
 
 
 @dataclass

@@ -9,6 +9,10 @@ benchmark and the parity tests run on.
                 s=0.05, c=(0.4,0.5,0.5), PEC, upwind.
   modes_box     n^3 jittered unit box (55^3*6 = 998,250 tets at full size),
                 N=4, sum of 16 PEC cavity modes with N(0,1) amplitudes.
+  port_box      jittered box with a dielectric block and four port planes
+                (the x and y walls) carrying TE10-shaped incident profiles:
+                one mesh of the batching workload (port_batch: 8 seeded
+                boxes of 20k-300k tets at orders 2/3/4, 4 excitations each).
   slab_box      unit box graded along x toward a thin slab at a seeded
                 position (cell widths 20:1 between far field and slab),
                 nx x ny x nz cells x 6 tets (40x30x28 = 201,600 tets at full
@@ -22,7 +26,8 @@ import math
 
 import numpy as np
 
-from .maxwell import Mesh, MaxwellOperator, structured_box, structured_box_jittered
+from .maxwell import (Mesh, MaxwellOperator, boundary_faces_on_plane, port_face_data, structured_box,
+                      structured_box_jittered)
 
 
 def te101_field(op: MaxwellOperator) -> np.ndarray:
@@ -51,8 +56,8 @@ def pulse_box(nx=60, ny=30, nz=30, order=3, eps_r=4.0, flux_alpha=1.0) -> tuple[
     return op, y.reshape(-1)
 
 
-def slab_box(nx=40, ny=30, nz=28, order=3, seed=5, ratio=20.0, width=0.02, flux_alpha=1.0
-             ) -> tuple[MaxwellOperator, np.ndarray]:
+def slab_box(nx=40, ny=30, nz=28, order=3, seed=5, ratio=20.0, width=0.02, flux_alpha=1.0, eps_r=10.0,
+             sigma=50.0) -> tuple[MaxwellOperator, np.ndarray]:
     rng = np.random.default_rng(seed)
     x0 = float(rng.uniform(0.35, 0.65))
     # cell widths along x: 1 far from the slab, 1/ratio at it (Gaussian
@@ -69,15 +74,88 @@ def slab_box(nx=40, ny=30, nz=28, order=3, seed=5, ratio=20.0, width=0.02, flux_
     cx = nodes[mesh.tets].mean(axis=1)[:, 0]
     slab = np.abs(cx - x0) <= 0.5 * width + 0.5 * np.diff(xs).min()
     eps = np.ones((mesh.n_tets, 3))
-    eps[slab] = 10.0
-    sigma = np.where(slab, 50.0, 0.0)
-    op = MaxwellOperator(mesh, order, flux_alpha, materials={"eps": eps, "sigma": sigma})
+    eps[slab] = eps_r
+    sig = np.where(slab, sigma, 0.0)
+    op = MaxwellOperator(mesh, order, flux_alpha, materials={"eps": eps, "sigma": sig})
     x = op.node_coords()
     xc0 = x0 - 0.2 if x0 > 0.5 else x0 + 0.2
     r2 = (x[..., 0] - xc0) ** 2 + (x[..., 1] - 0.5) ** 2 + (x[..., 2] - 0.5) ** 2
     y = np.zeros((op.n_elem, op.re.n_nodes, 6))
     y[..., 0] = np.exp(-r2 / (2 * 0.05 ** 2)) * np.sin(2 * math.pi * 5 * x[..., 0])
     return op, y.reshape(-1)
+
+
+# Port walls of port_box: (axis, side) for x = 0, x = Lx, y = 0, y = Ly.
+PORT_WALLS = ((0, 0), (0, 1), (1, 0), (1, 1))
+
+
+def te10_profile(axis: int, side: int, lengths):
+    """Incident TE10-shaped profile on the wall x[axis] = 0 (side 0) or
+    = lengths[axis] (side 1): e_t along the wall's second transverse axis v,
+    sin(pi x_u / L_u) across the first one u, and h_t = k x e_t with k the
+    unit normal into the box (a unit-impedance wave entering the domain)."""
+    u, v = (axis + 1) % 3, (axis + 2) % 3
+    k = np.zeros(3)
+    k[axis] = 1.0 if side == 0 else -1.0
+
+    def prof(x):
+        e = np.zeros_like(x)
+        e[:, v] = np.sin(math.pi * x[:, u] / lengths[u])
+        return e, np.cross(k, e)
+
+    return prof
+
+
+def port_box(nx, ny, nz, order, seed, lengths=(1.0, 1.0, 1.0), eps_r=None, flux_alpha=1.0):
+    """Jittered box (+-10% h, walls kept planar) with a dielectric block
+    (eps_r seeded in [2, 6] unless given) in a seeded sub-box, PEC walls
+    except the four port walls of PORT_WALLS, each tagged as its own port
+    (ids 0..3, rhs.rs:862-880). Returns (operator, ports) with
+    ports[r] = (elem, local_face, profiles, weights) for GpuOperator.set_port."""
+    rng = np.random.default_rng(seed)
+    mesh = structured_box_jittered(nx, ny, nz, *lengths, 0.1, seed)
+    cen = mesh.nodes[mesh.tets].mean(axis=1)
+    lo = rng.uniform(0.2, 0.5, 3) * np.asarray(lengths)
+    hi = lo + rng.uniform(0.2, 0.4, 3) * np.asarray(lengths)
+    inside = np.all((cen >= lo) & (cen <= hi), axis=1)
+    eps = np.ones((mesh.n_tets, 3))
+    eps[inside] = float(rng.uniform(2.0, 6.0)) if eps_r is None else eps_r
+    probe = MaxwellOperator(mesh, order, flux_alpha, materials={"eps": eps})
+    walls = [boundary_faces_on_plane(probe, axis, 0.0 if side == 0 else lengths[axis]) for axis, side in PORT_WALLS]
+    face_port = np.full((mesh.n_tets, 4), -1, np.int32)
+    for r, faces in enumerate(walls):
+        for e, f in faces:
+            face_port[e, f] = r
+    op = MaxwellOperator(mesh, order, flux_alpha, materials={"eps": eps}, face_port=face_port)
+    ports = [port_face_data(op, faces, te10_profile(axis, side, lengths))
+             for faces, (axis, side) in zip(walls, PORT_WALLS)]
+    return op, ports
+
+
+def port_batch_specs(seed=2024, n_meshes=8, tets=(20_000, 300_000)):
+    """Seeded sizes of the batching workload: (nx, ny, nz, order) per mesh,
+    tet counts log-uniform in `tets`, orders 2, 3, 4 in turn."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(n_meshes):
+        target = float(np.exp(rng.uniform(np.log(tets[0]), np.log(tets[1]))))
+        c = (target / 6.0) ** (1.0 / 3.0)
+        asp = rng.uniform(0.8, 1.25)
+        nx = max(2, int(round(c * asp)))
+        ny = max(2, int(round(c / asp)))
+        nz = max(2, int(round(target / (6.0 * nx * ny))))
+        out.append((nx, ny, nz, (2, 3, 4)[b % 3]))
+    return out
+
+
+def port_waveforms(seed, n, t0_scale=4.0):
+    """n Gaussian-modulated pulses (session.rs:242-261) with centre
+    frequencies seeded in [3, 8]: (t0, tau, f0) with tau = 1.5 / f0 and
+    t0 = t0_scale * tau."""
+    rng = np.random.default_rng(seed)
+    f0 = rng.uniform(3.0, 8.0, n)
+    tau = 1.5 / f0
+    return [(float(t0_scale * t), float(t), float(f)) for t, f in zip(tau, f0)]
 
 
 def cavity_modes_field(op: MaxwellOperator, n_modes=16, seed=0, lengths=(1.0, 1.0, 1.0)) -> np.ndarray:
@@ -137,7 +215,7 @@ def flops_per_dof_stage(order: int) -> float:
 
 def bytes_per_dof_stage(order: int, form: str) -> float:
     np_ = (order + 1) * (order + 2) * (order + 3) // 6
-    if form == "fused_f64c":
+    if form in ("fused_f64c", "fused_bf16x3", "fused_bf16x2"):
         state = 6 + 6 + 4 + 4  # y (f32+bf16) read+write, p (f32) read+write
         geom = 9 * 8 + 4 * 4 + 4  # J^-1 f64, 4 face codes, material index
     elif form == "fused_f64":

@@ -1,0 +1,113 @@
+"""The precision forms: the volume contraction D_k u on the matrix cores in
+BF16 split into 3 pieces ('fused_bf16x3') or hi + lo ('fused_bf16x2') with
+FP32 accumulation, everything else in FP64, registers as fused_f64c.
+
+  oracle size  apply and 6 LSERK4 steps against the oracle at orders 2-4
+               (ports, PEC walls, materials, conductivity), every variant,
+               LDS poisoned first; the error sits at the split's precision:
+               below 1e-5 (3 pieces, ~24 bits) and 1e-3 (2 pieces, ~16 bits)
+               relative L2 for the apply, and it is a real loss against the
+               FP64 forms (above 1e-9), so the MFMA path is the one running;
+  C2, C3       5000 (324,000 tets, N=3) and 2000 (998,250 tets, N=4) LSERK4
+               steps at the CFL step against the FP64 fused form; the error
+               is measured and reported (the accuracy bar of the stepper is
+               1e-6), and bounded by 1e-3.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ["fused_bf16x3", "fused_bf16x2"]
+APPLY_TOL = {"fused_bf16x3": 1e-5, "fused_bf16x2": 1e-3}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from rapidfem_amd import GpuContext
+
+    return GpuContext()
+
+
+@pytest.mark.parametrize("order", [2, 3, 4])
+@pytest.mark.parametrize("form", FORMS)
+def test_precision_form_matches_oracle_at_its_precision(ctx, oracle, order, form):
+    import rapidfem_amd as R
+    from conftest import oracle_desc
+
+    xyz, tets = oracle.box_mesh(3, 3, 3, 1.0, 1.0, 1.0, 0.1, 7, True)
+    K = tets.shape[0]
+    eps = np.ones((K, 3))
+    eps[::3] = 4.0
+    sig = np.zeros(K)
+    sig[::5] = 0.3
+    port = np.full(4 * K, -1, np.int32)
+    port[0:40:4] = 0
+    oop = oracle.Operator(xyz, tets, order, 1.0, eps=eps, sigma=sig, face_port=port)
+    n = oop.n_dof
+    y = np.sin(0.3 + 0.017 * np.arange(n))
+    want_apply = oop.apply(y)
+    dt = 0.01
+    want = oop.lserk(y, dt, 6)
+    d, arrs, _, ex = oracle_desc(oop)
+    g = R.GpuOperator.from_desc(ctx, d, form, mass=None, det=None)
+    assert g.n_variants() == 2
+    for v in range(g.n_variants()):
+        assert g.variant_available(v)
+        g.set_variant(v)
+        g.poison_lds()
+        e_apply = rel_l2(g.apply_f64(y), want_apply)
+        g.poison_lds()
+        g.state_load(y)
+        g.state_lserk(dt, 6)
+        e_step = rel_l2(g.state_store(), want)
+        print(f"precision form={form} order={order} v={v}: apply {e_apply:.3e}, 6 LSERK4 steps {e_step:.3e}",
+              flush=True)
+        assert 1e-9 < e_apply < APPLY_TOL[form]
+        assert e_step < APPLY_TOL[form]
+
+
+def _long_run(ctx, op, y0, steps, forms):
+    import rapidfem_amd as R
+
+    s = R.TdSession(op, 1.0, "fused_f64", ctx)
+    dt = s.cfl_dt()
+    ref = s.gop
+    ref.state_load(y0)
+    ref.state_lserk(dt, steps)
+    want = ref.state_store()
+    del s, ref
+    errs = {}
+    for form in forms:
+        g = R.GpuOperator(ctx, op, form)
+        g.state_load(y0)
+        g.state_lserk(dt, steps)
+        errs[form] = rel_l2(g.state_store(), want)
+        del g
+    return dt, errs
+
+
+@pytest.mark.timeout(900)
+def test_c2_precision_forms_5000_steps_against_fp64(ctx):
+    from rapidfem_amd import workloads as W
+
+    op, y0 = W.pulse_box(60, 30, 30, 3)
+    dt, errs = _long_run(ctx, op, y0, 5000, FORMS + ["fused_f64c"])
+    print(f"C2 5000 steps at dt {dt:.4e}, rel L2 vs FP64: " + ", ".join(f"{k} {v:.3e}" for k, v in errs.items()),
+          flush=True)
+    for v in errs.values():
+        assert np.isfinite(v) and v < 1e-3
+    assert errs["fused_bf16x3"] < errs["fused_bf16x2"]
+
+
+@pytest.mark.timeout(900)
+def test_c3_precision_form_2000_steps_against_fp64(ctx):
+    from rapidfem_amd import workloads as W
+
+    op, y0 = W.modes_box(55, 4)
+    dt, errs = _long_run(ctx, op, y0, 2000, ["fused_bf16x3"])
+    print(f"C3 2000 steps at dt {dt:.4e}, rel L2 vs FP64: " + ", ".join(f"{k} {v:.3e}" for k, v in errs.items()),
+          flush=True)
+    assert np.isfinite(errs["fused_bf16x3"]) and errs["fused_bf16x3"] < 1e-3

@@ -1,0 +1,165 @@
+"""The adaptive Krylov exponential and its learned (h, m) proposer on the GPU.
+
+  network    the HIP forward pass and the HIP loss gradient equal the numpy
+             restatement (oracle/mlp.py) to FP32 round-off; Adam training
+             on the GPU fits a smooth synthetic map;
+  controller the plain controller's result on an oracle-size mesh equals the
+             oracle's fixed-step expmv replayed over the logged (h, m)
+             sequence, and lies within the run's tolerance of a fine-step
+             FP64 propagation; every logged step satisfies its own bound;
+             a network (any parameters) proposing the steps still meets the
+             same tolerance, its rejected proposals falling back.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from rapidfem_amd import GpuContext
+
+    return GpuContext()
+
+
+def _rand_net(prop, seed):
+    from oracle import mlp as M
+
+    rng = np.random.default_rng(seed)
+    p = rng.normal(scale=0.25, size=M.N_PARAMS).astype(np.float32)
+    norm = np.concatenate([rng.normal(size=12), rng.uniform(0.5, 2.0, 12), rng.normal(size=2),
+                           rng.uniform(0.5, 3.0, 2)]).astype(np.float32)
+    prop.set_params(p, norm)
+    return p.astype(np.float64), norm.astype(np.float64)
+
+
+def test_forward_matches_numpy_restatement(ctx):
+    from oracle import mlp as M
+    from rapidfem_amd.proposer import Proposer
+
+    prop = Proposer(ctx, seed=11)
+    p, norm = _rand_net(prop, 3)
+    x = np.random.default_rng(4).normal(size=(1000, 12)).astype(np.float32)
+    got = prop.forward(x)
+    want = M.forward(p, norm, x.astype(np.float64))
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"forward: max rel err vs numpy {err:.2e}", flush=True)
+    assert err < 2e-5
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_loss_gradient_matches_numpy_restatement(ctx, weighted):
+    from oracle import mlp as M
+    from rapidfem_amd.proposer import Proposer
+
+    prop = Proposer(ctx, seed=12)
+    p, norm = _rand_net(prop, 5)
+    rng = np.random.default_rng(6)
+    n = 777  # not a multiple of the gradient kernel's 128 blocks
+    x = rng.normal(size=(n, 12)).astype(np.float32)
+    t = rng.normal(size=(n, 2)).astype(np.float32)
+    w = rng.uniform(0.1, 3.0, n).astype(np.float32) if weighted else None
+    loss, g = prop.loss_grad(x, t, w)
+    want_loss, want_g = M.loss_grad(p, norm, x.astype(np.float64), t.astype(np.float64),
+                                    None if w is None else w.astype(np.float64))
+    gerr = np.linalg.norm(g - want_g) / np.linalg.norm(want_g)
+    print(f"loss {loss:.6e} (numpy {want_loss:.6e}), gradient rel L2 err {gerr:.2e}", flush=True)
+    assert abs(loss - want_loss) <= 1e-5 * want_loss
+    assert gerr < 1e-5
+    # each parameter block is right, not only the total
+    for lo, hi in ((0, 768), (768, 832), (832, 4928), (4928, 4992), (4992, 5120), (5120, 5122)):
+        assert np.linalg.norm(g[lo:hi] - want_g[lo:hi]) <= 1e-4 * np.linalg.norm(want_g[lo:hi]) + 1e-9
+
+
+def test_training_fits_a_smooth_map_and_matches_numpy_after_training(ctx):
+    from oracle import mlp as M
+    from rapidfem_amd.proposer import Proposer
+
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, size=(8192, 12)).astype(np.float32)
+    t = np.stack([np.sin(2 * x[:, 0]) + 0.5 * x[:, 1] * x[:, 2], 10 + 20 * np.tanh(x[:, 3] - x[:, 4])], axis=1)
+    t = t.astype(np.float32)
+    prop = Proposer(ctx, seed=1)
+    prop.fit_normalisation(x, t)
+    l0, _ = prop.loss_grad(x, t)
+    hist = prop.train(x, t, epochs=40, batch=256, lr=3e-3, seed=2)
+    l1, _ = prop.loss_grad(x, t)
+    print(f"training loss {l0:.4f} -> {l1:.5f} (epochs {hist[0]:.4f} ... {hist[-1]:.5f})", flush=True)
+    assert l1 < 0.02 * l0
+    assert np.all(np.isfinite(hist)) and hist[-1] < hist[0]
+    p, norm = prop.params()
+    got = prop.forward(x[:256])
+    want = M.forward(p.astype(np.float64), norm.astype(np.float64), x[:256].astype(np.float64))
+    assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+
+
+def _slab_small():
+    from rapidfem_amd import workloads as W
+
+    return W.slab_box(6, 4, 4, order=2, seed=5)
+
+
+def test_plain_controller_matches_oracle_replay_and_fine_propagation(ctx, oracle):
+    import rapidfem_amd as R
+    from rapidfem_amd.proposer import ExpController
+
+    op, y0 = _slab_small()
+    oop = oracle.Operator(op.mesh.nodes, op.mesh.tets, 2, 1.0, eps=1.0 / op.inv_eps,
+                          sigma=op.sigma_eps[:, 0] / op.inv_eps[:, 0])
+    g = R.GpuOperator(ctx, op, "fused_f64")
+    cfl = R.TdSession(op, 1.0, "fused_f64", ctx).cfl_dt()
+    t_end = 60 * cfl
+    tol = 1e-7
+    ctl = ExpController(tol=tol, m=20)
+    out, steps, _ = g.expmv_adaptive(ctx, y0, t_end, ctl)
+    n = len(steps)
+    print(f"plain controller: {n} steps over {t_end / cfl:.0f} CFL steps, h/cfl {np.round(steps['h'] / cfl, 2)}, "
+          f"judged {steps['n_judged'].sum()}", flush=True)
+    assert n >= 2 and abs(steps["h"].sum() - t_end) <= 1e-12 * t_end
+    assert np.all(steps["est"] <= steps["bound"] * (1 + 1e-12))
+    assert np.all(steps["m"] == 20)
+    # replay: the oracle's fixed-dimension expmv over the same steps
+    y = y0.copy()
+    for s in steps:
+        y, _ = oop.expmv(y, float(s["h"]), int(s["m"]), 0.0)
+    rep = rel_l2(out, y)
+    # fine propagation: 4x the steps at m = 40 without a tolerance
+    yf = y0.copy()
+    nf = 4 * n
+    for _ in range(nf):
+        yf = g.expmv(ctx, yf, t_end / nf, 40, 0.0)
+    err = rel_l2(out, yf)
+    print(f"replay rel L2 {rep:.2e}; vs fine propagation {err:.2e} (tol {tol:g})", flush=True)
+    assert rep < 1e-11
+    assert err < 2 * tol
+
+
+def test_network_proposals_are_judged_and_meet_the_tolerance(ctx):
+    import rapidfem_amd as R
+    from rapidfem_amd.proposer import ExpController, Proposer
+
+    op, y0 = _slab_small()
+    g = R.GpuOperator(ctx, op, "fused_f64")
+    cfl = R.TdSession(op, 1.0, "fused_f64", ctx).cfl_dt()
+    t_end = 60 * cfl
+    tol = 1e-7
+    prop = Proposer(ctx, seed=9)
+    # an untrained network centred on an aggressive proposal: log(h s) ~ 2, m ~ 12
+    p, _ = prop.params()
+    norm = np.concatenate([np.zeros(12), np.ones(12), [2.0, 12.0], [1.0, 4.0]]).astype(np.float32)
+    prop.set_params(p, norm)
+    ctl = ExpController(tol=tol, m=20, m_min=10, m_max=30)
+    out, steps, _ = g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=prop)
+    out_p, steps_p, _ = g.expmv_adaptive(ctx, y0, t_end, ctl)
+    print(f"network: {len(steps)} steps, m {steps['m'].tolist()}, first proposals accepted "
+          f"{int((steps['n_judged'] == 1).sum())}; plain: {len(steps_p)} steps", flush=True)
+    assert np.all(steps["proposer"] == 1)
+    assert np.all((steps["m"] >= 10) & (steps["m"] <= 30))
+    assert np.all(steps["est"] <= steps["bound"] * (1 + 1e-12))
+    assert abs(steps["h"].sum() - t_end) <= 1e-12 * t_end
+    assert rel_l2(out, out_p) < 2 * tol

@@ -109,8 +109,13 @@ def test_adaptive_route_holds_the_waveform_per_frame(ctx, oracle):
     print(f"adaptive route: {st.accepted}/{st.rejected} vs oracle {acc}/{rej}, rel L2 {err:.3e}", flush=True)
     assert (st.accepted, st.rejected) == (acc, rej)
     assert err < 1e-11
-    assert abs(st.h_min - hmin / C_WAVE) <= 1e-9 * st.h_min
-    assert abs(st.h_max - hmax / C_WAVE) <= 1e-9 * st.h_max
+    # The next step is h * safety * e^-0.175 ... with e the embedded error
+    # norm, a sum of (bhat - b) f terms that cancel to O(h^4) of their size:
+    # round-off of 1e-16 in the applies moves e by ~1e-7 relative and the
+    # proposed steps by ~2e-8 (measured 2.3e-8 on h_min), so h_min and h_max
+    # are compared at 1e-6.
+    assert abs(st.h_min - hmin / C_WAVE) <= 1e-6 * st.h_min
+    assert abs(st.h_max - hmax / C_WAVE) <= 1e-6 * st.h_max
 
 
 @pytest.mark.parametrize("drive", ["free", "point", "vector"])
