@@ -42,6 +42,7 @@ DTYPE_LABEL = {
     "fused_f64c": "f64 arithmetic; y stored f32+bf16, p f32",
     "fused_bf16x3": "f64 arithmetic, volume contraction bf16x3 MFMA with f32 accumulation; y f32+bf16, p f32",
     "fused_bf16x2": "f64 arithmetic, volume contraction bf16 hi+lo MFMA with f32 accumulation; y f32+bf16, p f32",
+    "fused_bf16x4": "f64 arithmetic, volume contraction bf16x4 MFMA with f32 accumulation; y f32+bf16, p f32",
     "plain_f64": "f64",
     "plain_f32": "f32",
 }
@@ -87,6 +88,8 @@ def parse():
     ap.add_argument("--proposer-cfl", type=float, default=200.0, help="horizon in CFL steps")
     ap.add_argument("--proposer-tol", type=float, default=1e-6)
     ap.add_argument("--proposer-weights", default="")
+    ap.add_argument("--m-sweep", action="store_true",
+                    help="with --proposer: also time the plain controller at fixed dimensions 10..60")
     ap.add_argument("--kcl", action="store_true",
                     help="time the KCL adaptive stepper (RK4(3)5 + PI controller) on the stepper workload: one "
                          "JSON line in DOF-stage/s over accepted and rejected substeps")
@@ -205,20 +208,23 @@ def slab_c5(args):
 
 def krylov_cost_model(g, ctx, y0, cfl, dims=(10, 20, 30, 40, 50, 60)):
     """Per-step wall time of the adaptive Krylov step at fixed dimensions on
-    this operator, fitted as c0 + c1 m + c2 m (m + 1) / 2 (ms): per step,
-    per operator apply, per basis column swept."""
+    this operator, fitted as c0 + c1 m + c2 m (m + 1) / 2 (ms; per step, per
+    operator apply, per basis column swept) by non-negative least squares
+    over every step of a 40-CFL-step run at each dimension (first steps
+    left out)."""
     import numpy as np
+    from scipy.optimize import nnls
 
     from rapidfem_amd.proposer import ExpController
 
     rows, ms = [], []
     for m in dims:
-        ctl = ExpController(tol=1e-6, m=m, h0=cfl)
-        _, steps, _ = g.expmv_adaptive(ctx, y0, 3 * cfl, ctl, log_cap=64)
-        for s in steps[1:] if len(steps) > 1 else steps:
+        ctl = ExpController(tol=1e-6, m=m, h0=cfl, growth_limit=1.5, h_scale=cfl)
+        _, steps, _ = g.expmv_adaptive(ctx, y0, 40 * cfl, ctl, log_cap=256)
+        for s in steps[1:]:
             rows.append([1.0, s["m"], 0.5 * s["m"] * (s["m"] + 1)])
             ms.append(s["ms"])
-    coef, *_ = np.linalg.lstsq(np.array(rows), np.array(ms), rcond=None)
+    coef, _ = nnls(np.array(rows), np.array(ms))
     return [float(c) for c in coef], len(ms)
 
 
@@ -249,14 +255,23 @@ def proposer_bench(args) -> int:
         print(json.dumps({"metric": "adaptive Krylov step cost model", "cost_model_ms": cost, "steps_fitted": nfit,
                           "config": {"workload": f"slab box {op.n_elem} tets N={args.order}"}}))
         return 0
-    fine_ctl = ExpController(tol=tol * 1e-3, m=40)
+    fine_ctl = ExpController(tol=tol * 1e-3, m=40, h_scale=cfl)
     yf, sf, _ = g.expmv_adaptive(ctx, y0, t_end, fine_ctl)
     log(f"  fine run: {len(sf)} steps (m 40, tol {tol * 1e-3:g}), {sf['ms'].sum():.0f} ms")
+    sweep = {}
+    if args.m_sweep:
+        # the plain controller at each fixed dimension: what choosing m can gain on this workload
+        for m in (10, 16, 20, 24, 28, 32, 40, 48, 60):
+            ctl = ExpController(tol=tol, m=m, h_scale=cfl)
+            y, st, _ = g.expmv_adaptive(ctx, y0, t_end, ctl, log_cap=4096)
+            sweep[m] = {"device_ms": round(g.last_call_ms, 1), "steps": int(len(st)),
+                        "rel_l2_vs_fine": float(np.linalg.norm(y - yf) / np.linalg.norm(yf))}
+            log(f"  plain m={m}: {len(st)} steps, device {g.last_call_ms:.0f} ms")
     res = {}
     prop = Proposer(ctx)
     prop.load(args.proposer_weights or WEIGHTS)
     for name, p in (("plain", None), ("learned", prop)):
-        ctl = ExpController(tol=tol, m=40, m_min=10, m_max=60)
+        ctl = ExpController(tol=tol, m=40, m_min=10, m_max=60, h_scale=cfl)
         g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=p, log_cap=4096)  # warm-up
         t0 = time.perf_counter()
         y, st, _ = g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=p, log_cap=4096)
@@ -264,7 +279,7 @@ def proposer_bench(args) -> int:
         err = float(np.linalg.norm(y - yf) / np.linalg.norm(yf))
         res[name] = {"device_ms": g.last_call_ms, "call_ms": wall, "steps": int(len(st)),
                      "applies": int(st["m"].sum()), "first_proposals_rejected": int((st["n_judged"] > 1).sum()),
-                     "m_mean": float(st["m"].mean()), "rel_l2_vs_fine": err,
+                     "m_mean": float(st["m"].mean()), "m_first": [int(m) for m in st["m"][:40]], "rel_l2_vs_fine": err,
                      "h_over_cfl": [round(float(h / cfl), 2) for h in st["h"][:40]]}
         log(f"  {name}: {len(st)} steps, {int(st['m'].sum())} applies, device {g.last_call_ms:.0f} ms, "
             f"rel L2 vs fine {err:.2e}")
@@ -273,7 +288,12 @@ def proposer_bench(args) -> int:
            "value": round(gain, 3), "unit": "x (plain ms / learned ms)", "n_gpus": 1, "higher_is_better": True,
            "dtype": "f64 (Arnoldi, CGS2, Pade-13), f32 network", "data": "synthetic", "tol": tol,
            "plain": res["plain"], "learned": res["learned"], "fine": {"steps": int(len(sf)), "tol": tol * 1e-3},
-           "cost_model_ms": cost,
+           "cost_model_ms": cost, "plain_fixed_m": sweep,
+           # the plain controller at the dimension that was best in hindsight on this workload
+           "vs_best_fixed_m": (None if not sweep else
+                               {"m": min(sweep, key=lambda m: sweep[m]["device_ms"]),
+                                "gain": round(min(v["device_ms"] for v in sweep.values())
+                                              / res["learned"]["device_ms"], 3)}),
            "config": {"workload": f"slab box {args.krylov_cells[0]}x{args.krylov_cells[1]}x{args.krylov_cells[2]} x 6 "
                                   f"tets graded 20:1 to an eps_r 10, sigma 50 slab, N={args.order}, pulse field, "
                                   f"t_end {args.proposer_cfl} CFL steps",
@@ -303,17 +323,36 @@ def kcl_bench(args) -> int:
     variant, tune_ms = g.autotune()
     log(f"kcl: {op.n_elem} tets, N={args.order}, n={n}, cfl dt {dt:.4e}, frame {frame:.4e}, "
         f"stage kernel variant {variant}")
-    runs = []
-    for r in range(3):
-        t0 = time.perf_counter()
-        _, st = g.transient_kcl(ctx, y0, frame, 2, precision="f64")
-        wall = (time.perf_counter() - t0) * 1e3
-        ms = g.last_call_ms
-        sub = st.accepted + st.rejected
-        log(f"  rep {r}: {st.accepted} accepted + {st.rejected} rejected substeps, device part {ms:.2f} ms, "
-            f"call {wall:.1f} ms")
-        if r > 0:
-            runs.append((ms, sub, st.accepted, st.rejected, wall))
+    def reps(gop, label):
+        runs, last = [], None
+        for r in range(3):
+            t0 = time.perf_counter()
+            traj, st = gop.transient_kcl(ctx, y0, frame, 2, precision="f64")
+            wall = (time.perf_counter() - t0) * 1e3
+            ms = gop.last_call_ms
+            sub = st.accepted + st.rejected
+            log(f"  {label} rep {r}: {st.accepted} accepted + {st.rejected} rejected substeps, device part "
+                f"{ms:.2f} ms, call {wall:.1f} ms")
+            if r > 0:
+                runs.append((ms, sub, st.accepted, st.rejected, wall))
+            last = traj[-1]
+        return runs, last
+
+    runs, y_f64 = reps(g, "f64 registers")
+    # the compressed-register form with compressed KCL registers
+    gc = R.GpuOperator(ctx, op, "fused_f64c")
+    vc, _ = gc.autotune()
+    gc.set_kcl_registers("compressed")
+    runs_c, y_c = reps(gc, "compressed registers")
+    ms_c = float(np.median([x[0] for x in runs_c]))
+    sub_c = runs_c[-1][1]
+    compressed = {"form": "fused_f64c", "variant": vc, "ms_per_call": ms_c, "substeps": sub_c,
+                  "accepted": runs_c[-1][2], "rejected": runs_c[-1][3], "ms_per_substep": ms_c / max(sub_c, 1),
+                  "value": n * 5 * sub_c / (ms_c * 1e-3),
+                  "rel_l2_vs_f64_registers": float(np.linalg.norm(y_c - y_f64) / np.linalg.norm(y_f64))}
+    log(f"  compressed registers: {compressed['ms_per_substep']:.3f} ms per substep, "
+        f"rel L2 vs FP64 registers {compressed['rel_l2_vs_f64_registers']:.2e}")
+    del gc
     ms = float(np.median([x[0] for x in runs]))
     wall = float(np.median([x[4] for x in runs]))
     sub, acc, rej = runs[-1][1], runs[-1][2], runs[-1][3]
@@ -321,7 +360,7 @@ def kcl_bench(args) -> int:
     out = {"metric": "KCL adaptive stepper DOF-stage/s (RK4(3)5[2R+]C + PI controller)", "value": value,
            "unit": "DOF-stage/s", "n_gpus": 1, "reps": len(runs), "higher_is_better": True, "dtype": "f64",
            "data": "synthetic", "ms_per_call": ms, "substeps": sub, "accepted": acc, "rejected": rej,
-           "ms_per_substep": ms / max(sub, 1),
+           "ms_per_substep": ms / max(sub, 1), "compressed_registers": compressed,
            "host_inclusive": {"ms_per_call": wall, "value": n * 5 * sub / (wall * 1e-3),
                               "note": "the whole call: state upload, layout conversions and the 3 frames "
                                       "downloaded as f64 (311 MB each)"},

@@ -47,17 +47,18 @@ extern "C" {
  *             + materials + stage update), FP64 state and arithmetic.
  *  FUSED_F64C the fused kernel with FP64 arithmetic and compressed
  *             registers: y stored as f32 hi + bf16 lo, p as f32.
- *  FUSED_BF16X3, FUSED_BF16X2  the precision forms: FUSED_F64C's registers,
- *             with the volume contraction D_k u on the matrix cores in BF16
- *             split into 3 (or 2: hi + lo) pieces per operand, products
- *             accumulated in FP32; J^-1, curl, flux, LIFT and the update stay
- *             FP64. Orders 2..4. */
+ *  FUSED_BF16X3, FUSED_BF16X2, FUSED_BF16X4  the precision forms:
+ *             FUSED_F64C's registers, with the volume contraction D_k u on the
+ *             matrix cores in BF16 split into 3 (2: hi + lo; 4) pieces per
+ *             operand, products accumulated in FP32; J^-1, curl, flux, LIFT
+ *             and the update stay FP64. Orders 2..4. */
 #define RFHIP_FORM_PLAIN_F32 0
 #define RFHIP_FORM_PLAIN_F64 1
 #define RFHIP_FORM_FUSED_F64 2
 #define RFHIP_FORM_FUSED_F64C 3
 #define RFHIP_FORM_FUSED_BF16X3 4
 #define RFHIP_FORM_FUSED_BF16X2 5
+#define RFHIP_FORM_FUSED_BF16X4 6
 
 /* Drive of dy/dt = A*y + b*g(t) (session.rs:107-114 Drive). */
 #define RFHIP_DRIVE_FREE 0
@@ -233,6 +234,17 @@ int rfhip_op_transient_kcl_f64(rfhip_op *op, const double *y0, double dt,
                                const rfhip_controller *ctl, double *out,
                                rfhip_kcl_stats *stats);
 
+/* KCL registers (no counterpart in the reference, whose KCL keeps f32
+ * registers, gpu/operator.rs:823-893). RFHIP_KCL_REG_F64 (default): y, err,
+ * the stage input and the substep backup in FP64, 48 bytes per entry per
+ * stage update. RFHIP_KCL_REG_COMPRESSED (forms with compressed state
+ * registers only, else RFHIP_EUNSUPPORTED): y and its backup as f32 hi +
+ * bf16 lo, err as f32, the stage input FP64, 36 bytes; the controller and
+ * every apply still in FP64. Applies to later transient_kcl calls. */
+#define RFHIP_KCL_REG_F64 0
+#define RFHIP_KCL_REG_COMPRESSED 1
+int rfhip_op_set_kcl_registers(rfhip_op *op, int32_t mode);
+
 /* GpuOperator::expmv (gpu/operator.rs:1175-1197): out = exp(t*A)*v, FP64
  * Arnoldi (CGS2) with the Hessenberg exponential, following
  * propagator.rs:186-326: max_dim caps the dimension, tol > 0 enables the
@@ -269,13 +281,16 @@ typedef struct {
   double growth_limit;     /* clamps of that factor */
   double shrink_limit;
   int64_t max_steps;       /* accepted steps before RFHIP_EINVAL */
+  double h_scale;          /* characteristic step of the proposer's features
+                              and outputs (the CFL step); <= 0: 1/|H[1][0]|
+                              of the first basis */
 } rfhip_expctl;
 
 /* One accepted step of an adaptive run (the accept/reject log). */
 typedef struct {
   double t;                /* operator time at the step's start */
   double h;                /* accepted step */
-  double h_first;          /* first proposal (== h when it was accepted) */
+  double h_first;          /* first proposal (plain: == h when accepted) */
   double est_first, bound_first;  /* its estimate and bound */
   double est, bound;       /* estimate and bound of the accepted step */
   double beta;             /* norm of the propagated vector */
@@ -283,7 +298,7 @@ typedef struct {
   double ms;               /* wall time of the step: Arnoldi, exponentials */
   int32_t m;               /* Krylov dimension built */
   int32_t proposer;        /* 0 plain controller, 1 network */
-  int32_t n_judged;        /* proposals judged (1: the first was accepted) */
+  int32_t n_judged;        /* proposals judged (1: the first was admissible) */
   int32_t breakdown;       /* 1: the Krylov subspace was invariant (exact) */
   float feat[12];          /* the proposer's features for this step */
 } rfhip_exp_step;
@@ -292,8 +307,9 @@ typedef struct {
  * ctl->m every step, the next step grown from the last estimate, a rejected
  * proposal shrunk on the same basis, which costs no operator applies).
  * Otherwise, after two Arnoldi columns of each step the network proposes
- * (h, m), m clamped to [m_min, m_max]; a rejected proposal falls back to the
- * largest admissible step <= h on the built basis. steps (capacity
+ * (h, m), m clamped to [m_min, m_max]; the step taken is the largest one
+ * the estimate admits on that basis (searched from the proposed h, up or
+ * down, to 2^-10 in log h). steps (capacity
  * steps_cap, may be NULL) receives each step's record; hess (may be NULL)
  * receives each logged step's Hessenberg matrix as a zero-padded
  * (hess_dim+1) x hess_dim row-major block. n_steps (may be NULL) gets the
@@ -330,7 +346,7 @@ int rfhip_expm_pade(const double *a, int32_t n, double *out);
  * FP32, run and trained by the library's own HIP kernels. Parameters
  * (5122 floats): W1[64][12] b1[64] W2[64][64] b2[64] W3[2][64] b3[2],
  * row-major; normalisation (28 floats): in_mean[12] in_inv_scale[12]
- * out_mean[2] out_scale[2]. Outputs are (log(h * |H[1][0]|), m). */
+ * out_mean[2] out_scale[2]. Outputs are (log(h / h_scale), m). */
 int rfhip_mlp_create(rfhip_ctx *ctx, uint64_t seed, rfhip_mlp **out);
 void rfhip_mlp_destroy(rfhip_mlp *mlp);
 int rfhip_mlp_n_params(void);

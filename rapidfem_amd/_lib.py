@@ -31,7 +31,9 @@ FORM_FUSED_F64 = 2
 FORM_FUSED_F64C = 3
 FORM_FUSED_BF16X3 = 4
 FORM_FUSED_BF16X2 = 5
-FORMS = {"plain_f32": 0, "plain_f64": 1, "fused_f64": 2, "fused_f64c": 3, "fused_bf16x3": 4, "fused_bf16x2": 5}
+FORM_FUSED_BF16X4 = 6
+FORMS = {"plain_f32": 0, "plain_f64": 1, "fused_f64": 2, "fused_f64c": 3, "fused_bf16x3": 4, "fused_bf16x2": 5,
+         "fused_bf16x4": 6}
 
 DRIVE_FREE, DRIVE_POINT, DRIVE_VECTOR = 0, 1, 2
 
@@ -61,7 +63,7 @@ class ExpCtlC(C.Structure):
     """rfhip_expctl: the adaptive Krylov exponential's controller."""
 
     _fields_ = [("tol", _d), ("m", _i32), ("m_min", _i32), ("m_max", _i32), ("h0", _d), ("safety", _d),
-                ("growth_limit", _d), ("shrink_limit", _d), ("max_steps", _i64)]
+                ("growth_limit", _d), ("shrink_limit", _d), ("max_steps", _i64), ("h_scale", _d)]
 
 
 # rfhip_exp_step as a numpy record (same layout as the C struct)
@@ -101,6 +103,7 @@ SIGNATURES = {
     "rfhip_op_transient_ports_f64": (C.c_int, [_vp, _dp, _d, _i64, _i64, C.c_int, _i64, _vp, _vp, _i64, _dp, _vp]),
     "rfhip_op_transient_kcl": (C.c_int, [_vp, _fp, C.c_float, _i64, C.c_int, _i64, _vp, _vp, _i64,
                                          C.POINTER(ControllerC), _fp, C.POINTER(KclStatsC)]),
+    "rfhip_op_set_kcl_registers": (C.c_int, [_vp, C.c_int32]),
     "rfhip_op_transient_kcl_f64": (C.c_int, [_vp, _dp, _d, _i64, C.c_int, _i64, _vp, _vp, _i64,
                                              C.POINTER(ControllerC), _dp, C.POINTER(KclStatsC)]),
     "rfhip_op_expmv": (C.c_int, [_vp, _dp, _d, _i32, _d, _dp, C.POINTER(_i32)]),

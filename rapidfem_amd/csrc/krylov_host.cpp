@@ -179,26 +179,29 @@ double max_admissible_h(const double *hess, int ld, int j, double beta, double r
 }
 
 // Features of one step's proposal, from the first two Arnoldi columns of
-// the step (hess with row stride ld; cols >= 2) and the previous step.
+// the step (hess with row stride ld; cols >= 2) and the run so far, in units
+// of the run's characteristic step hs (the CFL step when the caller gives
+// it): the decay of the propagated norm since the start, log |H[1][0]| hs,
+// the first Hessenberg entries times hs, the decay rate over the previous
+// step, the fraction of the run done, log |H[1][0]| hs of the first basis,
+// the size of the system, the tolerance and the time left.
 void proposal_features(const double *hess, int ld, const ProposalHistory &hist, double beta, double tol,
-                       double t_left, float *f) {
+                       double t_left, double hs, int64_t n_dof, float *f) {
   const double h00 = hess[0], h01 = hess[1], h10 = hess[(size_t)ld], h11 = hess[(size_t)ld + 1],
                h21 = hess[(size_t)2 * ld + 1];
-  const double s = std::max(std::fabs(h10), 1e-300);
   auto lg = [](double x) { return std::log(std::max(x, 1e-300)); };
-  f[0] = (float)(hist.h_prev > 0.0 ? lg(hist.h_prev * s) : 0.0);
-  f[1] = (float)(h00 / s);
-  f[2] = (float)(h11 / s);
-  f[3] = (float)(h21 / s);
-  f[4] = (float)(h01 / s);
-  f[5] = (float)(hist.s_prev > 0.0 ? lg(s / hist.s_prev) : 0.0);
-  f[6] = (float)(hist.beta_prev > 0.0 ? lg(beta / hist.beta_prev) : 0.0);
-  f[7] = (float)(hist.m_prev / 60.0);
-  f[8] = (float)(hist.bound_prev > 0.0 ? std::clamp(std::log10(std::max(hist.est_prev, 1e-300) / hist.bound_prev), -10.0, 2.0)
-                                       : -10.0);
-  f[9] = (float)(hist.accepted_prev ? 1.0 : 0.0);
+  f[0] = (float)(hist.beta0 > 0.0 ? lg(beta / hist.beta0) : 0.0);
+  f[1] = (float)lg(std::fabs(h10) * hs);
+  f[2] = (float)(h00 * hs);
+  f[3] = (float)(h11 * hs);
+  f[4] = (float)(h21 * hs);
+  f[5] = (float)(h01 * hs);
+  f[6] = (float)(hist.h_prev > 0.0 && hist.beta_prev > 0.0 ? lg(beta / hist.beta_prev) / (hist.h_prev / hs) : 0.0);
+  f[7] = (float)(hist.t_end > 0.0 ? std::clamp((hist.t_end - t_left) / hist.t_end, 0.0, 1.0) : 0.0);
+  f[8] = (float)lg((hist.h10_0 > 0.0 ? hist.h10_0 : std::fabs(h10)) * hs);
+  f[9] = (float)(std::log10((double)std::max<int64_t>(n_dof, 1)) / 8.0);
   f[10] = (float)std::log10(tol);
-  f[11] = (float)lg(std::min(t_left * s, 1e6));
+  f[11] = (float)lg(std::min(t_left / hs, 1e6));
 }
 
 }  // namespace rfhip

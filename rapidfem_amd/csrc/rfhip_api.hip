@@ -368,6 +368,11 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
           }
     upload(op->afragb, fragb.data(), fragb.size(), s);
   }
+  // Component-lane stage kernel table (variant 8), when the reference
+  // element is the compiled equispaced one.
+  std::vector<double> clane;
+  const bool has_clane = clane_pack(np, d.face_nodes, D, d.lift, clane);
+  if (has_clane) upload(op->clane, clane.data(), clane.size(), s);
   upload(op->jinv64, d.jinv, 9 * (size_t)K, s);
   upload(op->fcode, fcode.data(), fcode.size(), s);
   upload(op->mat, mat.data(), mat.size(), s);
@@ -405,6 +410,7 @@ void build_fused(rfhip_op *op, const rfhip_op_desc &d) {
   a.face_nodes = op->face_nodes.as<int>();
   a.nbr_node = op->nbr_node.as<int>();
   a.nbase = nbase.empty() ? nullptr : op->nbase.as<int>();
+  a.clane = has_clane ? op->clane.as<double>() : nullptr;
   a.variant = default_variant(op->order, op->form);
 }
 
@@ -590,6 +596,10 @@ double factor(const rfhip_controller &c, double err_norm, double prev, bool reje
   return std::min(std::max(f, c.shrink_limit), c.growth_limit);
 }
 
+// Bytes of a compressed register of n entries: f32 hi parts, then bf16 lo
+// parts padded to an even count (a whole number of 4-byte words).
+size_t creg_bytes(int64_t n) { return 4 * (size_t)n + 2 * (size_t)((n + 1) & ~int64_t(1)); }
+
 // KCL RK4(3)5[2R+]C transient with the PI controller (gpu/operator.rs:823-893
 // structure, session.rs:781-827 control logic) on the register vector ys.
 // The controller runs on the host; each substep reads back one double.
@@ -598,12 +608,27 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
               const rfhip_controller &ctl, rfhip_kcl_stats &st, const std::function<void(int64_t)> &emit) {
   const int64_t n = nvec(op);
   const bool f64 = f64_regs(op->form);
+  // compressed registers: y and its backup as f32 hi + bf16 lo ([hi | lo]
+  // in one buffer), err as f32; every stage's apply reads st in FP64, the
+  // last stage leaving the accepted y there for the next substep
+  const bool cmp = op->kcl_reg == 1 && form_compressed(op->form);
   const size_t w = f64 ? 8 : 4;
   hipStream_t s = op->stream;
   op->kst.alloc(w * n);
-  op->kerr.alloc(w * n);
-  op->kbak.alloc(w * n);
+  op->kerr.alloc((cmp ? 4 : w) * n);
+  op->kbak.alloc(cmp ? creg_bytes(n) : w * n);
   for (DevBuf *b : {&op->kst, &op->kerr, &op->kbak}) launch_fill(b->p, (int64_t)b->bytes, s);
+  float *yh = nullptr, *bh = nullptr;
+  uint16_t *yl = nullptr, *bl = nullptr;
+  if (cmp) {
+    op->kyc.alloc(creg_bytes(n));
+    yh = op->kyc.as<float>();
+    yl = reinterpret_cast<uint16_t *>(yh + n);
+    bh = op->kbak.as<float>();
+    bl = reinterpret_cast<uint16_t *>(bh + n);
+    launch_f64_to_c(static_cast<const double *>(ys), yh, yl, n, s);
+    launch_copy(op->kst.p, ys, 8 * n, s);
+  }
   const int nblk = weighted_err_blocks(n);
   op->red.alloc(8 * (nblk + 1));
   double *part = op->red.as<double>();
@@ -626,6 +651,11 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_begin).count() - emit_ms;
   };
   timed_emit(0);
+  // the frame of the current y in ys (FP64), where the caller reads it
+  auto frame = [&](int64_t r) {
+    if (cmp) launch_c_to_f64(yh, yl, static_cast<double *>(ys), n, s);
+    timed_emit(r);
+  };
   for (int64_t fr = 0; fr < steps; ++fr) {
     double t_rel = 0.0;
     const double gv = kind == RFHIP_DRIVE_FREE ? 0.0 : (double)g[fr];
@@ -637,12 +667,14 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
       // pass also saves the starting y (kbak); the last one reduces the
       // error norm (session.rs:160-171) against it.
       for (int sg = 0; sg < 5; ++sg) {
-        rhs_into(op, sg == 0 ? ys : op->kst.p, op->k.p, kind, dof, gv);
+        rhs_into(op, sg == 0 && !cmp ? ys : op->kst.p, op->k.p, kind, dof, gv);
         KclStage a{};
-        a.y = ys;
+        a.y = cmp ? (void *)yh : ys;
+        a.y_lo = yl;
         a.err = op->kerr.p;
         a.st = op->kst.p;
         a.bak = op->kbak.p;
+        a.bak_lo = bl;
         a.k = op->k.p;
         a.dt = ht;
         a.b = kKclB[sg];
@@ -656,7 +688,10 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
         a.partials = part;
         a.nblk = nblk;
         a.err_sum = res;
-        launch_kcl_update(f64, a, s);
+        if (cmp)
+          launch_kcl_update_c(a, s);
+        else
+          launch_kcl_update(f64, a, s);
       }
       double ssum = 0.0;
       RF_HIP(hipMemcpyAsync(&ssum, res, 8, hipMemcpyDeviceToHost, s));
@@ -671,7 +706,12 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
         hmax = std::max(hmax, h);
       } else {
         ++rej;
-        launch_copy(ys, op->kbak.p, (int64_t)(w * n), s);
+        if (cmp) {
+          launch_copy(op->kyc.p, op->kbak.p, (int64_t)creg_bytes(n), s);
+          launch_c_to_f64(bh, bl, op->kst.as<double>(), n, s);
+        } else {
+          launch_copy(ys, op->kbak.p, (int64_t)(w * n), s);
+        }
         h = ht * factor(ctl, std::isfinite(e) ? e : kKclNonfinite, prev, true);
       }
       if (h < h_floor) {
@@ -682,7 +722,7 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
                                          " rejected substeps"};
       }
     }
-    timed_emit(fr + 1);
+    frame(fr + 1);
   }
   finish_timing();
   st = {acc, rej, hmin, hmax};
@@ -932,9 +972,9 @@ struct Arnoldi {
 };
 
 // exp(t_end A) v with a step size and Krylov dimension chosen per step: the
-// plain error-controlled controller, or the network's proposals judged by the
-// same a-posteriori estimate with the plain controller's largest admissible
-// step as the fallback.
+// plain error-controlled controller, or the network's proposals: its m sets
+// the basis and its h seeds a search for the largest step the same
+// a-posteriori estimate admits on that basis.
 void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip_expctl &ctl, rfhip_mlp *mlp,
                    rfhip_exp_step *log, int64_t log_cap, double *hess, int hess_dim, int64_t *n_steps, double *out) {
   const int cap = std::max(ctl.m, ctl.m_max);
@@ -946,7 +986,7 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
   const auto t_begin = std::chrono::steady_clock::now();
   KrylovScratch ws;
   ProposalHistory hist;
-  double t = 0.0, h_next = ctl.h0 > 0.0 ? ctl.h0 : 0.0, rate = 0.0;
+  double t = 0.0, h_next = ctl.h0 > 0.0 ? ctl.h0 : 0.0, rate = 0.0, hs = 1.0;
   int64_t nst = 0;
   bool zero = false;
   while (t_end - t > 1e-14 * t_end) {
@@ -966,7 +1006,13 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
     rec.beta = ar.beta;
     rec.t_left = t_left;
     const bool early = ar.broke;  // an invariant subspace of dimension 1 or 2
-    if (!early) proposal_features(ar.H.data(), cap, hist, ar.beta, ctl.tol, t_left, rec.feat);
+    if (nst == 0) {
+      hs = ctl.h_scale > 0.0 ? ctl.h_scale : 1.0 / std::max(std::fabs(ar.H[(size_t)cap]), 1e-300);
+      hist.t_end = t_end;
+      hist.beta0 = ar.beta;
+      hist.h10_0 = std::fabs(ar.H[(size_t)cap]);
+    }
+    if (!early) proposal_features(ar.H.data(), cap, hist, ar.beta, ctl.tol, t_left, hs, op->n_dof, rec.feat);
     double h_prop;
     int m_prop;
     if (early) {
@@ -976,8 +1022,7 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
       float yo[2];
       int rc = rfhip_mlp_forward(mlp, rec.feat, 1, yo);
       if (rc != RFHIP_OK) throw Error{rc, "proposer forward pass failed"};
-      const double s = std::max(std::fabs(ar.H[(size_t)cap]), 1e-300);
-      const double hp = std::exp(std::clamp((double)yo[0], -700.0, 700.0)) / s;
+      const double hp = hs * std::exp(std::clamp((double)yo[0], -700.0, 700.0));
       h_prop = std::isfinite(hp) ? std::clamp(hp, 1e-9 * t_end, t_left) : t_left;
       m_prop = std::clamp((int)std::lround(std::isfinite(yo[1]) ? yo[1] : (float)ctl.m), std::max(ctl.m_min, 2), ctl.m_max);
       rec.proposer = 1;
@@ -998,23 +1043,24 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
     rec.est_first = est;
     rec.bound_first = rate * h_prop;
     rec.n_judged = 1;
-    if (!(est <= rate * h)) {
-      if (mlp) {  // fallback: the plain controller's largest admissible step on this basis
-        h = max_admissible_h(ar.H.data(), cap, d, ar.beta, rate, h_prop, h_prop, 12, ws);
+    if (mlp && !ar.broke) {
+      // the network's step seeds a search for the largest step the estimate
+      // admits on this basis (up or down from it); a proposal the estimate
+      // rejects counts as a second judgement
+      if (!(est <= rate * h)) ++rec.n_judged;
+      h = max_admissible_h(ar.H.data(), cap, d, ar.beta, rate, h_prop, t_left, 10, ws);
+      RF_REQUIRE(h > 0.0, "adaptive exponential: no admissible step on the proposed basis");
+      est = krylov_estimate(ar.H.data(), cap, d, ar.beta, h, ws);
+    } else if (!(est <= rate * h)) {  // shrink on the same basis until the estimate accepts
+      for (;;) {
+        const double r = est / (rate * h);
+        h *= std::max(ctl.shrink_limit, std::min(ctl.safety * std::pow(r, -1.0 / d), 1.0 - 1e-3));
         est = krylov_estimate(ar.H.data(), cap, d, ar.beta, h, ws);
         ++rec.n_judged;
-      } else {  // shrink on the same basis until the estimate accepts
-        for (;;) {
-          const double r = est / (rate * h);
-          h *= std::max(ctl.shrink_limit, std::min(ctl.safety * std::pow(r, -1.0 / d), 1.0 - 1e-3));
-          est = krylov_estimate(ar.H.data(), cap, d, ar.beta, h, ws);
-          ++rec.n_judged;
-          if (est <= rate * h) break;
-          RF_REQUIRE(h > 1e-12 * t_end && rec.n_judged < 200,
-                     "adaptive exponential: step size collapsed (tolerance too tight for this dimension)");
-        }
+        if (est <= rate * h) break;
+        RF_REQUIRE(h > 1e-12 * t_end && rec.n_judged < 200,
+                   "adaptive exponential: step size collapsed (tolerance too tight for this dimension)");
       }
-      RF_REQUIRE(h > 0.0, "adaptive exponential: step size collapsed");
     }
     rec.h = h;
     rec.est = est;
@@ -1034,12 +1080,7 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
     const double grow = r > 0.0 ? ctl.safety * std::pow(r, -1.0 / std::max(d, 1)) : ctl.growth_limit;
     h_next = h * std::clamp(grow, ctl.shrink_limit, ctl.growth_limit);
     hist.h_prev = h;
-    hist.s_prev = std::fabs(ar.H[(size_t)cap]);
     hist.beta_prev = ar.beta;
-    hist.est_prev = est;
-    hist.bound_prev = rate * h;
-    hist.m_prev = d;
-    hist.accepted_prev = rec.n_judged == 1;
     t += h;
     std::swap(x, y);
   }
@@ -1278,7 +1319,7 @@ int rfhip_op_create_dispersive(rfhip_ctx *ctx, const rfhip_op_desc *d, int form,
     RF_REQUIRE(ctx && d && out, "null argument");
     RF_REQUIRE(n_disp >= 0, "negative number of dispersive elements");
     RF_REQUIRE(n_disp == 0 || (disp_elem && disp_a && disp_g && disp_inv_eps_inf), "missing dispersive array");
-    RF_REQUIRE(form >= RFHIP_FORM_PLAIN_F32 && form <= RFHIP_FORM_FUSED_BF16X2, "unknown form");
+    RF_REQUIRE(form >= RFHIP_FORM_PLAIN_F32 && form <= RFHIP_FORM_FUSED_BF16X4, "unknown form");
     RF_REQUIRE(d->n_elem >= 1, "empty operator");
     const int p = order_of_np(d->np);
     RF_REQUIRE(p >= 1 && d->nfp == (p + 1) * (p + 2) / 2, "inconsistent Np / Nfp");
@@ -1596,6 +1637,16 @@ int rfhip_op_transient_kcl(rfhip_op *op, const float *y0, float dt, int64_t step
                            const double *src, const float *g, int64_t n_g, const rfhip_controller *ctl, float *out,
                            rfhip_kcl_stats *stats) {
   return kcl_entry<float, float>(op, y0, (double)dt, steps, kind, dof, src, g, n_g, ctl, out, stats);
+}
+
+int rfhip_op_set_kcl_registers(rfhip_op *op, int32_t mode) {
+  return guarded([&] {
+    RF_REQUIRE(op, "null operator");
+    RF_REQUIRE(mode == RFHIP_KCL_REG_F64 || mode == RFHIP_KCL_REG_COMPRESSED, "unknown KCL register mode");
+    if (mode == RFHIP_KCL_REG_COMPRESSED && !form_compressed(op->form))
+      throw Error{RFHIP_EUNSUPPORTED, "compressed KCL registers need a form with compressed state registers"};
+    op->kcl_reg = mode;
+  });
 }
 
 int rfhip_op_transient_kcl_f64(rfhip_op *op, const double *y0, double dt, int64_t steps, int kind, int64_t dof,

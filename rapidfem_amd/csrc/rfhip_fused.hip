@@ -22,13 +22,16 @@
 // (element, node). The block's state is staged in LDS with coalesced loads;
 // D^T and LIFT^T are read from L1 with consecutive nodes on consecutive
 // lanes.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
 #include <type_traits>
+#include <vector>
 
+#include "rfhip_elem_tables.hpp"
 #include "rfhip_impl.hpp"
 
 namespace rfhip {
@@ -1679,6 +1682,287 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(V
   }
 }
 
+// ---- component lanes ------------------------------------------------------
+//
+// k_clane maps lane 6q + c of a wave to field component c (Ex..Hz) of
+// element q of the wave's group of kClEpw = 10 elements (lanes 60..63 idle).
+// Every lane then runs the same node sequence, so the entries of D_k and
+// LIFT are the same for all lanes of a wave: they come through the scalar
+// cache as SGPR operands of the FMAs, read in the order the kernel uses them
+// from a table the host packs once per operator (StageArgs::clane), and the
+// zeros of the equispaced D_k (ElemTab::DM: 258 of 400 per D_k at N=3) are
+// skipped when the kernel is compiled. A lane holds its component at every
+// node of its element in registers (u) and accumulates its component of the
+// right-hand side (acc); D and LIFT are never read from LDS. What one
+// component needs of the others moves between the element's six lanes with
+// ds_bpermute: two physical derivatives per node for the curl, and four
+// face jumps per face node for the flux.
+//
+// The J^-1 entries are loaded with the lane's component axis rotated to the
+// front (jr[k][a] = J^-1[k][(p + a) % 3] for field axis p), so the face
+// normal each lane forms is (n_p, n_p+1, n_p+2) and the flux of component p
+// is one formula for every lane:
+//   F_p = cf (a ((n . j_same) n_p - j_p) + s (n_p+1 j'_p+2 - n_p+2 j'_p+1)),
+// s = -1 for E, +1 for H, j' the other field's jumps (rhs.rs:1372-1440; the
+// PEC and port rules enter through the lane's own jump, see face_flux).
+constexpr int kClEpw = 10;
+constexpr int kClJb = 9;  // columns of a D_k row per batch of table entries
+
+template <int NP>
+struct ClaneWaves {
+  static constexpr int v = NP >= 20 ? 2 : 3;
+};
+
+// Entries of the packed table before (i, j, k) in (i, j, k) order (k
+// fastest): the nonzeros of D_r, D_s, D_t. clane_nd<NP>(NP, 0, 0) counts all
+// of them, so LIFT's entries start there.
+template <int NP>
+constexpr int clane_nd(int i_end, int j_end, int k_end) {
+  int s = 0;
+  for (int i = 0; i < NP; ++i)
+    for (int j = 0; j < NP; ++j)
+      for (int k = 0; k < 3; ++k) {
+        if (i > i_end || (i == i_end && (j > j_end || (j == j_end && k >= k_end)))) return s;
+        s += (int)((ElemTab<NP>::DM[k][i] >> j) & 1ull);
+      }
+  return s;
+}
+
+// f(std::integral_constant<int, B>), ..., f(... E - 1): loop bodies whose
+// index must be a compile-time constant (register arrays, table entries).
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+__device__ __forceinline__ double bperm_f64(int src_lane, double v) {
+  const int2 w = __builtin_bit_cast(int2, v);
+  const int lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, w.x);
+  const int hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, w.y);
+  return __builtin_bit_cast(double, make_int2(lo, hi));
+}
+
+template <int NP, bool CIN, int EPI>
+__global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(ClaneWaves<NP>::v)))
+k_clane(StageArgs io) {
+  using T = ElemTab<NP>;
+  constexpr int NFP = T::NFP;
+  constexpr int NWB = kFusedWG / 64;
+  constexpr int NDV = clane_nd<NP>(NP, 0, 0);
+  typedef const double __attribute__((address_space(4))) *cptr;
+  // neighbour volume node per ((f*4 + nlf)*6 + o, m); rows 96 + f: the
+  // element's own face nodes (boundary faces read their own trace)
+  __shared__ int snb[100 * NFP];
+  for (int q = threadIdx.x; q < 100 * NFP; q += kFusedWG)
+    snb[q] = q < 96 * NFP ? io.nbr_node[q] : io.face_nodes[q - 96 * NFP];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int qg = lane / 6 < kClEpw ? lane / 6 : kClEpw - 1;  // idle lanes mirror element 9
+  const int c = lane % 6;
+  const int fld = c >= 3 ? 1 : 0;
+  const int pc = c - 3 * fld;
+  const int p1 = pc == 2 ? 0 : pc + 1, p2 = pc == 0 ? 2 : pc - 1;
+  const int srcS1 = 6 * qg + 3 * fld + p1, srcS2 = 6 * qg + 3 * fld + p2;
+  const int srcO1 = 6 * qg + 3 * (1 - fld) + p1, srcO2 = 6 * qg + 3 * (1 - fld) + p2;
+  const double sgn = fld ? -1.0 : 1.0;  // E: +curl H, H: -curl E
+  const int64_t ngroups = (io.n_elem + kClEpw - 1) / kClEpw;
+  const float *yh = static_cast<const float *>(io.yin);
+  const uint16_t *yl = static_cast<const uint16_t *>(io.yin_lo);
+  const double *yd = static_cast<const double *>(io.yin);
+  using RT = typename std::conditional<CIN, float, double>::type;
+  struct Raw {
+    RT h;
+    uint32_t l;
+  };
+  auto load_raw = [&](int64_t idx, Raw &r) {
+    if constexpr (CIN) {
+      r.h = yh[idx];
+      r.l = yl[idx];
+    } else {
+      r.h = yd[idx];
+      r.l = 0;
+    }
+  };
+  auto raw_f64 = [&](const Raw &r) -> double {
+    if constexpr (CIN)
+      return (double)r.h + (double)__uint_as_float(r.l << 16);
+    else
+      return r.h;
+  };
+#pragma unroll 1
+  for (int64_t grp = (int64_t)blockIdx.x * NWB + wave; grp < ngroups; grp += (int64_t)gridDim.x * NWB) {
+    const int64_t e0 = grp * kClEpw;
+    const int64_t rem = io.n_elem - e0;
+    const int nel = rem < kClEpw ? (int)rem : kClEpw;
+    const bool active = lane < 6 * nel;
+    const int64_t e = e0 + (qg < nel ? qg : nel - 1);
+    const int64_t b0 = e * NP * 6 + c;  // index of node 0 of this lane's component
+    // The table's address is laundered per group: its entries are loop
+    // invariant, and hoisted out of the group loop they would all have to
+    // stay in registers.
+    const double *csp = io.clane;
+    asm volatile("" : "+s"(csp));
+    // Loads from the constant address space are taken never to be written,
+    // so the scheduler would issue every table load of the group at once
+    // (more SGPRs than a wave has). The entries are used in batches (part of
+    // a D_k row, a LIFT row); after each batch the table address is
+    // laundered with the results of the batch before it as inputs, so a
+    // batch's entries load while the previous batch computes and at most two
+    // batches hold SGPRs.
+    cptr cs = (cptr)csp;  // wave-uniform: scalar loads
+    double lag[3] = {0.0, 0.0, 0.0};
+    auto next_batch = [&](double a, double b, double c) {
+      asm volatile("" : "+s"(csp) : "v"(lag[0]), "v"(lag[1]), "v"(lag[2]));
+      cs = (cptr)csp;
+      lag[0] = a;
+      lag[1] = b;
+      lag[2] = c;
+    };
+    double jr[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      jr[k][0] = io.jinv[e * 9 + 3 * k + pc];
+      jr[k][1] = io.jinv[e * 9 + 3 * k + p1];
+      jr[k][2] = io.jinv[e * 9 + 3 * k + p2];
+    }
+    int code[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) code[f] = io.fcode[e * 4 + f];
+    const int mi = io.mat[e];
+    const double m_inv = io.mat_tab[12 * mi + c];
+    const double m_sig = io.mat_tab[12 * mi + 6 + c];
+    double u[NP];
+    {
+      Raw ur[NP];
+      static_for<0, NP>([&](auto j) { load_raw(b0 + 6 * j.value, ur[j.value]); });
+      static_for<0, NP>([&](auto j) { u[j.value] = raw_f64(ur[j.value]); });
+    }
+    // neighbour trace of face f, this lane's component (own node on a
+    // boundary face, so every load is valid and unconditional)
+    Raw nb[NFP];
+    auto issue_face = [&](auto fc) {
+      constexpr int f = decltype(fc)::value;
+      const int cd = code[f];
+      const int64_t ne = cd >= 0 ? (int64_t)(cd >> 5) : e;
+      const int tb = (cd >= 0 ? (f * 4 + ((cd >> 3) & 3)) * 6 + (cd & 7) : 96 + f) * NFP;
+      static_for<0, NFP>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        load_raw((ne * NP + snb[tb + m]) * 6 + c, nb[m]);
+      });
+    };
+    issue_face(std::integral_constant<int, 0>{});
+
+    // volume: reference derivatives of u at node i (sparse D_k rows), the
+    // two physical derivatives this component feeds into the curl, and the
+    // curl component of the lane's own output from its element's lanes
+    double acc[NP];
+    static_for<0, NP>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      double d[3] = {0.0, 0.0, 0.0};
+      static_for<0, NP>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        constexpr bool any = ((T::DM[0][i] | T::DM[1][i] | T::DM[2][i]) >> j) & 1ull;
+        if constexpr (any) {
+          const double uj = u[j];
+          static_for<0, 3>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if constexpr ((T::DM[k][i] >> j) & 1ull) {
+              constexpr int at = clane_nd<NP>(i, j, k);
+              d[k] = fma(cs[at], uj, d[k]);
+            }
+          });
+        }
+        if constexpr ((j + 1) % kClJb == 0 || j + 1 == NP) next_batch(d[0], d[1], d[2]);
+      });
+      const double d1 = jr[0][1] * d[0] + jr[1][1] * d[1] + jr[2][1] * d[2];  // along axis p+1
+      const double d2 = jr[0][2] * d[0] + jr[1][2] * d[1] + jr[2][2] * d[2];  // along axis p+2
+      // (curl H)_p = d/dx_p+1 H_p+2 - d/dx_p+2 H_p+1: lane H_p+2's d2 and lane H_p+1's d1
+      acc[i] = sgn * (bperm_f64(srcO2, d2) - bperm_f64(srcO1, d1));
+      // formed here: left alone the compiler gathers the exchanges of many
+      // nodes and keeps their derivatives live meanwhile
+      asm volatile("" : "+v"(acc[i]));
+    });
+
+    // faces: jumps, flux, LIFT (entries after the volume's in the table)
+    static_for<0, 4>([&](auto fc) {
+      constexpr int f = decltype(fc)::value;
+      const int cd = code[f];
+      // g_f = {-row2, -row0, -row1, row0 + row1 + row2} of J^-1, rotated
+      double g[3];
+#pragma unroll
+      for (int x = 0; x < 3; ++x)
+        g[x] = f == 0 ? -jr[2][x] : f == 1 ? -jr[0][x] : f == 2 ? -jr[1][x] : jr[0][x] + jr[1][x] + jr[2][x];
+      const double s2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+      const double hs = 0.5 * s2;
+      double rs = __builtin_amdgcn_rsq(s2);
+      rs = rs * fma(-hs * rs, rs, 1.5);
+      rs = rs * fma(-hs * rs, rs, 1.5);
+      const double n0 = g[0] * rs, n1 = g[1] * rs, n2 = g[2] * rs;
+      const double cf = hs * rs;
+      // own jump = wo * u - wn * neighbour: interior (1, 1), PEC E (2, 0),
+      // PEC H (0, 0), port (1, 0); a = 1 on ports
+      const double wo = cd >= 0 ? 1.0 : cd == kCodePec ? (fld ? 0.0 : 2.0) : 1.0;
+      const double wn = cd >= 0 ? 1.0 : 0.0;
+      const double a = cd >= 0 || cd == kCodePec ? io.alpha : 1.0;
+      double fl[NFP];
+      static_for<0, NFP>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const double jo = fma(-wn, raw_f64(nb[m]), wo * u[T::FN[f][m]]);
+        const double js1 = bperm_f64(srcS1, jo), js2 = bperm_f64(srcS2, jo);
+        const double jo1 = bperm_f64(srcO1, jo), jo2 = bperm_f64(srcO2, jo);
+        const double sd = n0 * jo + n1 * js1 + n2 * js2;
+        fl[m] = cf * (a * (sd * n0 - jo) - sgn * (n1 * jo2 - n2 * jo1));
+        asm volatile("" : "+v"(fl[m]));
+      });
+      // the next face's traces load while this face is lifted
+      if constexpr (f < 3) issue_face(std::integral_constant<int, f + 1>{});
+      // LIFT rows of this face: node i takes its Nfp entries in one go
+      static_for<0, NP>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        static_for<0, NFP>([&](auto mc) {
+          constexpr int m = decltype(mc)::value;
+          acc[i] = fma(cs[NDV + (f * NP + i) * NFP + m], fl[m], acc[i]);
+        });
+        next_batch(acc[i], acc[i], acc[i]);
+      });
+    });
+
+    // materials, held source, stage update
+    using PT = typename std::conditional<EPI == 2, float, double>::type;
+    if (active) {
+      static_for<0, NP>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const int64_t idx = b0 + 6 * i;
+        const double ui = u[i];
+        double kv = m_inv * acc[i] - m_sig * ui;
+        if (io.src_kind == RFHIP_DRIVE_POINT) {
+          if (io.src_dof == idx) kv += io.g;
+        } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+          kv += io.src_vec[idx] * io.g;
+        }
+        if constexpr (EPI == 0) {
+          io.dy[idx] = kv;
+        } else if constexpr (EPI == 1) {
+          const double pn = io.sa * static_cast<const double *>(io.p)[idx] + io.dt * kv;
+          static_cast<double *>(io.p)[idx] = pn;
+          static_cast<double *>(io.yout)[idx] = ui + io.sb * pn;
+        } else {
+          const float pn = (float)(io.sa * (double)static_cast<const float *>(io.p)[idx] + io.dt * kv);
+          static_cast<float *>(io.p)[idx] = pn;
+          const double yn = ui + io.sb * (double)pn;
+          const float h = (float)yn;
+          static_cast<float *>(io.yout)[idx] = h;
+          static_cast<uint16_t *>(io.yout_lo)[idx] = bf16_rne((float)(yn - (double)h));
+        }
+      });
+    }
+  }
+}
+
 template <int NP>
 struct Nfp {
   static constexpr int v = NP == 4 ? 3 : NP == 10 ? 6 : NP == 20 ? 10 : 15;
@@ -1722,7 +2006,10 @@ inline int64_t resident_blocks(const void *kernel) { return resident_blocks_nt(k
 //   4, 5  k_fused9: the k_fused3 layout with the group's memory traffic
 //         pipelined, loops unrolled by 2 / 4 nodes;
 //   6, 7  k_fused11: k_fused9 with whole elements per wave and no
-//         block-wide barriers in the group loop, unrolled by 2 / 4.
+//         block-wide barriers in the group loop, unrolled by 2 / 4;
+//   8     k_clane: one lane per (element, field component), D_k and LIFT
+//         as wave-uniform scalar operands, the zeros of D_k skipped
+//         (operators whose reference element matches ElemTab).
 // Variants 3 to 7 read the neighbour-trace table (built for conforming
 // meshes); variant 3 needs Np <= 20 (variant_available).
 
@@ -1763,6 +2050,11 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
       else
         hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
     }
+  } else if (variant == 8 && a.clane) {
+    const void *kf = reinterpret_cast<const void *>(&k_clane<NP, CIN, EPI>);
+    const int64_t wgroups = (a.n_elem + kClEpw - 1) / kClEpw;
+    const int64_t grid = persistent(kf, (wgroups + 3) / 4);  // blocks of 4 waves
+    if (grid > 0) hipLaunchKernelGGL((k_clane<NP, CIN, EPI>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
   } else if (variant == 2) {
     const void *kf = reinterpret_cast<const void *>(&k_fused3<NP, F, CIN, EPI, true, false, false, true, 2, true>);
     hipLaunchKernelGGL((k_fused3<NP, F, CIN, EPI, true, false, false, true, 2, true>),
@@ -1797,12 +2089,22 @@ void vmf_np(int variant, const StageArgs &a, hipStream_t s) {
     hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 2, VMF>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
 }
 
+template <int NP, bool CIN, int EPI>
+void vmf_pieces(int vmf, int variant, const StageArgs &a, hipStream_t s) {
+  if (vmf == 2)
+    vmf_np<NP, CIN, EPI, 2>(variant, a, s);
+  else if (vmf == 3)
+    vmf_np<NP, CIN, EPI, 3>(variant, a, s);
+  else
+    vmf_np<NP, CIN, EPI, 4>(variant, a, s);
+}
+
 template <bool CIN, int EPI>
 void vmf_dispatch(int np, int vmf, int variant, const StageArgs &a, hipStream_t s) {
   switch (np) {
-    case 10: vmf == 3 ? vmf_np<10, CIN, EPI, 3>(variant, a, s) : vmf_np<10, CIN, EPI, 2>(variant, a, s); break;
-    case 20: vmf == 3 ? vmf_np<20, CIN, EPI, 3>(variant, a, s) : vmf_np<20, CIN, EPI, 2>(variant, a, s); break;
-    case 35: vmf == 3 ? vmf_np<35, CIN, EPI, 3>(variant, a, s) : vmf_np<35, CIN, EPI, 2>(variant, a, s); break;
+    case 10: vmf_pieces<10, CIN, EPI>(vmf, variant, a, s); break;
+    case 20: vmf_pieces<20, CIN, EPI>(vmf, variant, a, s); break;
+    case 35: vmf_pieces<35, CIN, EPI>(vmf, variant, a, s); break;
     default: throw Error{RFHIP_EUNSUPPORTED, "the precision forms run orders 2 to 4"};
   }
   RF_HIP(hipGetLastError());
@@ -1999,9 +2301,47 @@ bool variant_available(int order, int form, int variant, const StageArgs &a) {
   const int np = (order + 1) * (order + 2) * (order + 3) / 6;
   if (form_vmf(form)) return variant >= 0 && variant < kStageVariantsBf16 && a.nbase != nullptr && order >= 2 && order <= 4;
   if (variant < 0 || variant >= kStageVariants) return false;
+  if (variant == 8) return a.clane != nullptr;
   if (variant == 3) return a.nbase != nullptr && np <= 20;
   if (variant >= 4) return a.nbase != nullptr;
   return true;
+}
+
+namespace {
+template <int NP>
+bool clane_pack_np(const int *face_nodes, const double *const D[3], const double *lift, std::vector<double> &out) {
+  using T = ElemTab<NP>;
+  constexpr int NFP = T::NFP;
+  for (int f = 0; f < 4; ++f)
+    for (int m = 0; m < NFP; ++m)
+      if (face_nodes[f * NFP + m] != T::FN[f][m]) return false;
+  for (int k = 0; k < 3; ++k) {
+    double mx = 0.0;
+    for (int q = 0; q < NP * NP; ++q) mx = std::max(mx, std::fabs(D[k][q]));
+    for (int i = 0; i < NP; ++i)
+      for (int j = 0; j < NP; ++j)
+        if ((std::fabs(D[k][i * NP + j]) > 1e-9 * mx) != (((T::DM[k][i] >> j) & 1ull) != 0)) return false;
+  }
+  out.clear();
+  for (int i = 0; i < NP; ++i)
+    for (int j = 0; j < NP; ++j)
+      for (int k = 0; k < 3; ++k)
+        if ((T::DM[k][i] >> j) & 1ull) out.push_back(D[k][i * NP + j]);
+  for (int f = 0; f < 4; ++f)
+    for (int i = 0; i < NP; ++i)
+      for (int m = 0; m < NFP; ++m) out.push_back(lift[(size_t)i * 4 * NFP + f * NFP + m]);
+  return true;
+}
+}  // namespace
+
+bool clane_pack(int np, const int *face_nodes, const double *const D[3], const double *lift, std::vector<double> &out) {
+  switch (np) {
+    case 4: return clane_pack_np<4>(face_nodes, D, lift, out);
+    case 10: return clane_pack_np<10>(face_nodes, D, lift, out);
+    case 20: return clane_pack_np<20>(face_nodes, D, lift, out);
+    case 35: return clane_pack_np<35>(face_nodes, D, lift, out);
+    default: return false;
+  }
 }
 
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s) {

@@ -112,9 +112,11 @@ void upload(DevBuf &b, const T *host, size_t count, hipStream_t s) {
 // Forms with the fused stage kernel; with compressed state registers (y f32
 // hi + bf16 lo, p f32); and the BF16 pieces of the precision forms' volume
 // contraction (0: FP64).
-inline bool form_fused(int f) { return f >= RFHIP_FORM_FUSED_F64 && f <= RFHIP_FORM_FUSED_BF16X2; }
-inline bool form_compressed(int f) { return f >= RFHIP_FORM_FUSED_F64C && f <= RFHIP_FORM_FUSED_BF16X2; }
-inline int form_vmf(int f) { return f == RFHIP_FORM_FUSED_BF16X3 ? 3 : f == RFHIP_FORM_FUSED_BF16X2 ? 2 : 0; }
+inline bool form_fused(int f) { return f >= RFHIP_FORM_FUSED_F64 && f <= RFHIP_FORM_FUSED_BF16X4; }
+inline bool form_compressed(int f) { return f >= RFHIP_FORM_FUSED_F64C && f <= RFHIP_FORM_FUSED_BF16X4; }
+inline int form_vmf(int f) {
+  return f == RFHIP_FORM_FUSED_BF16X3 ? 3 : f == RFHIP_FORM_FUSED_BF16X2 ? 2 : f == RFHIP_FORM_FUSED_BF16X4 ? 4 : 0;
+}
 
 // Per-face code of the fused kernels, one int32 per (element, face):
 //   interior: (neighbour << 5) | (neighbour_local_face << 3) | orientation
@@ -152,6 +154,7 @@ struct StageArgs {
   const int *face_nodes;  // 4*Nfp
   const int *nbr_node;    // [(f*4 + nlf)*6 + orientation][Nfp] -> neighbour volume node
   const int *nbase;       // [e*4Nfp + fm]: AoS index of the neighbour trace, or the (negative) face code
+  const double *clane;    // nonzeros of D_k then LIFT, in k_clane's order (clane_pack), or null
   const void *yin;        // double, or float hi (compressed)
   const void *yin_lo;     // bf16 lo (compressed) or null
   void *yout, *yout_lo;
@@ -184,18 +187,19 @@ struct rfhip_op {
   // device operator data
   rfhip::DevBuf dr, ds, dt, lift, face_nodes, jinv, inv_eps, inv_mu, sig_eps, sig_mu, normal, fscale, nbr,
       nbr_lf, port, perm;
-  rfhip::DevBuf dT, liftc, lfrag, afrag7, afragb, jinv64, fcode, mat, mat_tab, nbr_node, nbase;
+  rfhip::DevBuf dT, liftc, lfrag, afrag7, afragb, clane, jinv64, fcode, mat, mat_tab, nbr_node, nbase;
   int64_t n_vec = 0;  // length of device vectors (n_dof)
   // Debye dispersive block: slot -> element and (a, g, 1/eps_inf); the P
   // block follows the 6 Np K values of [E,H] in every state vector
   int64_t n_disp = 0;
+  int kcl_reg = 0;  // KCL registers: 0 FP64, 1 compressed (rfhip_op_set_kcl_registers)
   rfhip::DevBuf disp_elem, disp_coef;
   int order = 0;
   // state registers (precision depends on form)
   rfhip::DevBuf y[2], ylo[2], p, k, src;
   int cur = 0;  // index of the current state buffer for fused forms
   // KCL / Krylov / scratch buffers
-  rfhip::DevBuf kst, kerr, kbak, red, aos;
+  rfhip::DevBuf kst, kerr, kbak, kyc, red, aos;
   rfhip::DevBuf kb_basis, kb_w, kb_proj, kb_part, kb_out, kb_x;
   rfhip::HostPinned kb_host;  // Krylov projections and norm, copied without a sync per pass
   rfhip::DevBuf probe_idx, probe_rows;  // device-side probe recording
@@ -255,13 +259,18 @@ void launch_add_vec(const rfhip_op *op, void *k, const void *src, double g, hipS
 // Fused forms: epi 0 = apply (dy), 1 = LSERK stage (FP64 or compressed
 // registers by form). Stage-kernel variants 0 .. kStageVariants-1 (the
 // precision forms: 0 .. kStageVariantsBf16-1).
-constexpr int kStageVariants = 8;
+constexpr int kStageVariants = 9;
 constexpr int kStageVariantsBf16 = 2;
 void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t s);
 // Whether stage-kernel variant `variant` runs as itself on this operator's
 // data (some need the neighbour-trace table or Np <= 20; launch_stage runs
 // the base variant for them otherwise).
 bool variant_available(int order, int form, int variant, const StageArgs &a);
+// The table of the component-lane stage kernel (variant 8): the nonzeros of
+// D_r, D_s, D_t (row-major D_k[i][j], the order of (i, j, k)) and then LIFT
+// ((face, node, face node) order). False when the operator's face nodes or
+// the zeros of its D_k differ from the compiled equispaced tables.
+bool clane_pack(int np, const int *face_nodes, const double *const D[3], const double *lift, std::vector<double> &out);
 // Debye block after an applied derivative (dy = A y; f32: plain FP32 form),
 // with the drive of the P rows, and after a fused LSERK4 stage
 // (rfhip_debye.hip). No-ops without Debye elements.
@@ -313,6 +322,7 @@ void launch_gather(const void *y, const uint16_t *lo, int kind, const int64_t *i
 // double, +inf for a non-finite y) through nblk partials.
 struct KclStage {
   void *y, *err, *st, *bak;
+  uint16_t *y_lo, *bak_lo;  // compressed registers: bf16 lo parts (launch_kcl_update_c)
   const void *k;
   double dt, b, ew, amb;
   bool first, next;
@@ -323,6 +333,9 @@ struct KclStage {
   double *err_sum;
 };
 void launch_kcl_update(bool f64, const KclStage &a, hipStream_t s);
+// Compressed registers (y, bak f32 + bf16, err f32, st f64); the first
+// stage is never the last.
+void launch_kcl_update_c(const KclStage &a, hipStream_t s);
 void launch_copy(void *dst, const void *src, int64_t nbytes, hipStream_t s);
 int weighted_err_blocks(int64_t n);
 

@@ -16,14 +16,16 @@ struct KrylovScratch {
 };
 
 // What the proposer's features read of the previous step.
+// What the features take from the run so far. None of it depends on the
+// controller's own choices of step or dimension (the previous step enters
+// only as the interval of a decay rate), so a network trained on one
+// controller's logs sees the same features under another.
 struct ProposalHistory {
-  double h_prev = 0.0;      // step taken
-  double s_prev = 0.0;      // |H[1][0]| of its basis
-  double beta_prev = 0.0;   // norm of the vector it propagated
-  double est_prev = 0.0;    // estimate of the accepted proposal
-  double bound_prev = 0.0;  // and its bound
-  int m_prev = 0;           // Krylov dimension built
-  bool accepted_prev = false;  // whether its first proposal was accepted
+  double t_end = 0.0;      // the run's horizon
+  double beta0 = 0.0;      // norm of the initial vector
+  double h10_0 = 0.0;      // |H[1][0]| of the first step's basis
+  double h_prev = 0.0;     // previous step, 0 before the first
+  double beta_prev = 0.0;  // norm of the vector it propagated
 };
 
 constexpr int kProposalFeatures = 12;
@@ -37,8 +39,9 @@ double krylov_estimate(const double *hess, int ld, int j, double beta, double h,
 // Largest h <= h_cap with krylov_estimate(h) <= rate * h.
 double max_admissible_h(const double *hess, int ld, int j, double beta, double rate, double h_ref, double h_cap,
                         int iters, KrylovScratch &ws);
-// The 12 proposal features (needs the first two Arnoldi columns).
+// The 12 proposal features (needs the first two Arnoldi columns), in units
+// of the characteristic step hs.
 void proposal_features(const double *hess, int ld, const ProposalHistory &hist, double beta, double tol,
-                       double t_left, float *f);
+                       double t_left, double hs, int64_t n_dof, float *f);
 
 }  // namespace rfhip

@@ -162,6 +162,43 @@ __global__ void __launch_bounds__(kBS) k_kcl_update(T *__restrict__ y, T *__rest
   }
 }
 
+// The same pass with compressed registers: y and the substep's starting y
+// (bak) as f32 hi + bf16 lo, err as f32, the next stage's input st in FP64
+// (what the apply reads). 6 + 6 + 4 + 4 + 8 (+ 8 for k) bytes per entry
+// against 48 with FP64 registers. LAST: no next stage; st = y_new, the next
+// substep's first input, and the block's partial sum of the error norm.
+template <bool FIRST, bool LAST>
+__global__ void __launch_bounds__(kBS) k_kcl_update_c(float *__restrict__ yh, uint16_t *__restrict__ yl,
+                                                      float *__restrict__ err, double *__restrict__ st,
+                                                      const double *__restrict__ k, float *__restrict__ bh,
+                                                      uint16_t *__restrict__ bl, double dt, double b, double ew,
+                                                      double amb, double atol, double rtol, int64_t n,
+                                                      double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    const float h0 = yh[i];
+    const uint16_t l0 = yl[i];
+    const double f = dt * k[i];
+    const double yn = ((double)h0 + (double)bf16_to_f(l0)) + b * f;
+    const float hn = (float)yn;
+    yh[i] = hn;
+    yl[i] = bf16_rne((float)(yn - (double)hn));
+    if (FIRST) {
+      bh[i] = h0;
+      bl[i] = l0;
+    }
+    const double en = FIRST ? ew * f : (double)err[i] + ew * f;
+    err[i] = (float)en;
+    st[i] = LAST ? yn : yn + amb * f;
+    if (LAST) acc += werr_term((double)bh[i] + (double)bf16_to_f(bl[i]), yn, en, atol, rtol);
+  }
+  if (LAST) {
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+
 __device__ __forceinline__ double block_sum(double v, double *sh) {
   // wave64 shuffle reduction, then across the block's waves.
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -553,6 +590,25 @@ void kcl_update_t(const KclStage &a, hipStream_t s) {
       hipLaunchKernelGGL((k_kcl_update<T, false, false>), g, bs, 0, s, yy, ee, ss, kk, bb, dt, b, ew, amb, a.atol,
                          a.rtol, a.n, a.partials);
   }
+  if (!a.next) hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, a.partials, a.nblk, a.err_sum);
+  RF_HIP(hipGetLastError());
+}
+
+void launch_kcl_update_c(const KclStage &a, hipStream_t s) {
+  float *yh = static_cast<float *>(a.y), *err = static_cast<float *>(a.err), *bh = static_cast<float *>(a.bak);
+  uint16_t *yl = a.y_lo, *bl = a.bak_lo;
+  double *st = static_cast<double *>(a.st);
+  const double *k = static_cast<const double *>(a.k);
+  const dim3 g = a.next ? dim3(grid_for(a.n)) : dim3((unsigned)a.nblk), bs(kBS);
+  if (a.first && a.next)
+    hipLaunchKernelGGL((k_kcl_update_c<true, false>), g, bs, 0, s, yh, yl, err, st, k, bh, bl, a.dt, a.b, a.ew,
+                       a.amb, a.atol, a.rtol, a.n, a.partials);
+  else if (a.next)
+    hipLaunchKernelGGL((k_kcl_update_c<false, false>), g, bs, 0, s, yh, yl, err, st, k, bh, bl, a.dt, a.b, a.ew,
+                       a.amb, a.atol, a.rtol, a.n, a.partials);
+  else
+    hipLaunchKernelGGL((k_kcl_update_c<false, true>), g, bs, 0, s, yh, yl, err, st, k, bh, bl, a.dt, a.b, a.ew,
+                       a.amb, a.atol, a.rtol, a.n, a.partials);
   if (!a.next) hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, a.partials, a.nblk, a.err_sum);
   RF_HIP(hipGetLastError());
 }

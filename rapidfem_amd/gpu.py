@@ -95,9 +95,10 @@ class GpuContext:
 class GpuOperator:
     """gpu/operator.rs:78-153 GpuOperator; form selects the kernel family
     ('plain_f32' = the reference's structure and precision, 'plain_f64',
-    'fused_f64', 'fused_f64c', and the precision forms 'fused_bf16x3' /
-    'fused_bf16x2': fused_f64c with the volume contraction on the matrix
-    cores in 3- or 2-piece split BF16 with FP32 accumulation)."""
+    'fused_f64', 'fused_f64c', and the precision forms 'fused_bf16x4' /
+    'fused_bf16x3' / 'fused_bf16x2': fused_f64c with the volume contraction
+    on the matrix cores in 4-, 3- or 2-piece split BF16 with FP32
+    accumulation)."""
 
     def __init__(self, ctx: GpuContext, op: MaxwellOperator | None, form: str = "fused_f64", *, desc=None,
                  mass=None, det=None, dispersive=None):
@@ -258,6 +259,15 @@ class GpuOperator:
         L.check(fn(self._h, y0, dt, int(steps), drive.kind, drive.dof, L.ptr(drive.vec), L.ptr(gg), n_g,
                    C.byref(cc), out, C.byref(st)))
         return out.reshape(steps + 1, self._n), KclStats(st.accepted, st.rejected, st.h_min, st.h_max)
+
+    def set_kcl_registers(self, mode: str) -> None:
+        """KCL registers for later transient_kcl calls (rfhip_op_set_kcl_registers;
+        no reference counterpart): "f64" (default) or "compressed" (y and its
+        backup f32 + bf16, err f32; forms with compressed state registers)."""
+        modes = {"f64": 0, "compressed": 1}
+        if mode not in modes:
+            raise ValueError(f"unknown KCL register mode {mode!r}")
+        L.check(L.lib.rfhip_op_set_kcl_registers(self._h, modes[mode]))
 
     # gpu/operator.rs:1175-1197 (CPU algorithm propagator.rs:186-326)
     def expmv(self, ctx, v, t, m, tol=0.0):

@@ -45,10 +45,12 @@ class ExpController:
     growth_limit: float = 5.0
     shrink_limit: float = 0.2
     max_steps: int = 1_000_000
+    h_scale: float = 0.0  # the proposer's unit of time (the CFL step); 0: 1/|H[1][0]| of the first basis
 
     def c(self) -> L.ExpCtlC:
         return L.ExpCtlC(float(self.tol), int(self.m), int(self.m_min), int(self.m_max), float(self.h0),
-                         float(self.safety), float(self.growth_limit), float(self.shrink_limit), int(self.max_steps))
+                         float(self.safety), float(self.growth_limit), float(self.shrink_limit), int(self.max_steps),
+                         float(self.h_scale))
 
 
 def expm_pade(a) -> np.ndarray:
@@ -60,13 +62,13 @@ def expm_pade(a) -> np.ndarray:
     return out
 
 
-def features_and_targets(steps: np.ndarray, h_best: np.ndarray, m_best: np.ndarray, hess: np.ndarray):
-    """Rows (features, (log(h* |H[1][0]|), m*)) of the logged steps that have
-    a proposal (no breakdown within two columns) and a positive label."""
-    s = np.abs(hess[:, 1, 0])
-    keep = (steps["m"] >= 2) & (s > 0) & (h_best > 0) & ~((steps["breakdown"] == 1) & (steps["m"] <= 2))
+def features_and_targets(steps: np.ndarray, h_best: np.ndarray, m_best: np.ndarray, h_scale: float):
+    """Rows (features, (log(h* / h_scale), m*)) of the logged steps that have
+    a proposal (no breakdown within two columns) and a positive label; h_scale
+    is the run's ExpController.h_scale."""
+    keep = (steps["m"] >= 2) & (h_best > 0) & ~((steps["breakdown"] == 1) & (steps["m"] <= 2))
     x = steps["feat"][keep].astype(np.float32)
-    t = np.stack([np.log(h_best[keep] * s[keep]), m_best[keep].astype(np.float64)], axis=1).astype(np.float32)
+    t = np.stack([np.log(h_best[keep] / h_scale), m_best[keep].astype(np.float64)], axis=1).astype(np.float32)
     return x, t
 
 

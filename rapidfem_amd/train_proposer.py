@@ -5,18 +5,21 @@ route takes a fixed step and dimension, crates/rapidfem-td/src/
 session.rs:700-726).
 
 Data: the plain controller (GpuOperator.expmv_adaptive without a proposer)
-runs at Krylov dimension m_log = 60 on seeded stiff workloads (slab boxes
-graded 10:1 to 30:1 toward a lossy slab, eps_r 4..16, sigma 10..100, N=3,
-several initial pulses, tolerances 1e-9..1e-4, horizons 20..400 CFL steps),
-logging every step's features and Hessenberg matrix. Labels: for each step,
-the step h_j each grid dimension j admits under the run's tolerance and the
-j with the most time advanced per unit of cost (rfhip_krylov_labels), cost
-per step c0 + c1 j + c2 j (j+1) / 2 in milliseconds, from the workload the
-proposer will run on (measure it with `bench.py --krylov-cost`).
+runs at Krylov dimension m_log = 60 on seeded stiff workloads: slab boxes
+graded 10:1 to 30:1 toward a lossy slab (eps_r 4..16, sigma 10..100, N=3)
+in three size classes (about 1k-5k, 10k-40k and 130k-250k tets, the last
+the size of the target workload), several initial pulses each, tolerances
+1e-9..1e-4, horizons 20..400 CFL steps, the CFL step as the features' unit of
+time. Every step's features and Hessenberg matrix are logged. Labels: for
+each step, the step h_j each grid dimension j admits under the run's
+tolerance and the j with the most time advanced per unit of cost
+(rfhip_krylov_labels), cost per step c0 + c1 j + c2 j (j+1) / 2 in
+milliseconds, from the workload the proposer will run on (measure it with
+`bench.py --proposer --cost-only`).
 
 Usage:
   python -m rapidfem_amd.train_proposer --rows 50000 --cost C0 C1 C2 \
-      [--out rapidfem_amd/proposer_weights.npz] [--epochs 60]
+      [--out rapidfem_amd/proposer_weights.npz] [--epochs 60] [--max-seconds S]
 Prints one JSON line with the data and training summary.
 """
 from __future__ import annotations
@@ -43,11 +46,22 @@ def log(msg: str) -> None:
     print(f"[train_proposer] {msg}", file=sys.stderr, flush=True)
 
 
-def instance(rng: np.random.Generator):
-    """One seeded stiff workload: a small graded slab box, N=3."""
-    nx = int(rng.integers(8, 15))
-    ny = int(rng.integers(4, 8))
-    nz = int(rng.integers(4, 8))
+# size classes of the training workloads: (nx, ny, nz) ranges, initial fields
+# per mesh, share of the collection time
+SIZES = {
+    "small": (((8, 15), (4, 8), (4, 8)), 4, 0.35),
+    "medium": (((16, 27), (10, 17), (10, 17)), 3, 0.35),
+    "large": (((36, 45), (26, 33), (24, 31)), 2, 0.30),
+}
+
+
+def instance(rng: np.random.Generator, size: str = "small"):
+    """One seeded stiff workload: a graded slab box of the size class, N=3."""
+    (xr, yr, zr), _, _ = SIZES[size]
+    while True:  # never the 40x30x28 box the gain is measured on
+        nx, ny, nz = (int(rng.integers(*r)) for r in (xr, yr, zr))
+        if (nx, ny, nz) != (40, 30, 28):
+            break
     op, y0 = W.slab_box(nx, ny, nz, 3, seed=int(rng.integers(1 << 30)), ratio=float(rng.uniform(10, 30)),
                         eps_r=float(rng.uniform(4, 16)), sigma=float(rng.uniform(10, 100)))
     return op, y0
@@ -69,11 +83,14 @@ def initial_fields(op, y0, rng, count):
     return out
 
 
-def collect(ctx, rows: int, cost, seed: int = 0, nthreads: int = 15):
-    """(features, targets, summary) of at least `rows` labelled steps."""
+def collect(ctx, rows: int, cost, seed: int = 0, nthreads: int = 15, max_seconds: float = 1e9, shares=None,
+            fields=None):
+    """(features, targets, summary) of at least `rows` labelled steps, or of
+    what max_seconds of collection gave."""
     rng = np.random.default_rng(seed)
     xs, ts = [], []
     n_rows = n_steps = n_runs = n_rejected = 0
+    per_size = {}
     t0 = time.time()
     pool = cf.ThreadPoolExecutor(max_workers=1)
     pending = []  # (future, logged steps)
@@ -91,39 +108,60 @@ def collect(ctx, rows: int, cost, seed: int = 0, nthreads: int = 15):
                 keep.append((f, k))
         pending[:] = keep
 
-    def label_job(steps, hess, rate):
+    def label_job(steps, hess, rate, h_scale):
         hb, mb, _ = labels(steps, hess, rate, cost, m_grid=LABEL_GRID, nthreads=nthreads)
-        return features_and_targets(steps, hb, mb, hess)
+        return features_and_targets(steps, hb, mb, h_scale)
 
+    # the size classes take turns within their shares of the time budget
+    shares = dict(zip(SIZES, shares)) if shares else {k: v[2] for k, v in SIZES.items()}
+    fields = dict(zip(SIZES, fields)) if fields else {k: v[1] for k, v in SIZES.items()}
+    budget = {k: max(shares[k], 1e-9) * min(max_seconds, 1e7) for k in SIZES}
+    used = {k: 0.0 for k in SIZES}
     while True:
         drain(False)
-        if n_rows + sum(k for _, k in pending) >= rows:
+        if n_rows + sum(k for _, k in pending) >= rows or time.time() - t0 > max_seconds:
             drain(True)
-            if n_rows >= rows:
+            if n_rows >= rows or time.time() - t0 > max_seconds:
                 break
-        op, y0 = instance(rng)
+        size = min(SIZES, key=lambda k: used[k] / budget[k])
+        t_inst = time.time()
+        op, y0 = instance(rng, size)
         g = GpuOperator(ctx, op, "fused_f64")
         cfl = TdSession(op, 1.0, "fused_f64", ctx).cfl_dt()
-        for y in initial_fields(op, y0, rng, 4):
+        for y in initial_fields(op, y0, rng, fields[size]):
             t_end = cfl * float(np.exp(rng.uniform(np.log(20.0), np.log(400.0))))
             tol = float(10.0 ** rng.uniform(-9, -4))
-            ctl = ExpController(tol=tol, m=M_LOG, m_min=10, m_max=M_LOG)
+            ctl = ExpController(tol=tol, m=M_LOG, m_min=10, m_max=M_LOG, h_scale=cfl)
             _, steps, hess = g.expmv_adaptive(ctx, y, t_end, ctl, log_cap=4096, hess_dim=M_LOG)
             rate = tol * float(np.linalg.norm(y)) / t_end
-            pending.append((pool.submit(label_job, steps, hess, rate), len(steps)))
+            pending.append((pool.submit(label_job, steps, hess, rate, cfl), len(steps)))
             n_steps += len(steps)
             n_rejected += int((steps["n_judged"] > 1).sum())
             n_runs += 1
-        if n_runs % 20 == 0:
-            log(f"{n_runs} runs, {n_steps} logged steps, {n_rows} labelled rows, {time.time() - t0:.0f} s")
+            per_size[size] = per_size.get(size, 0) + len(steps)
+        used[size] += time.time() - t_inst
+        del g
+        log(f"{n_runs} runs ({size} {op.n_elem} tets), {n_steps} logged steps, {n_rows} labelled rows, "
+            f"{time.time() - t0:.0f} s")
     pool.shutdown()
-    x = np.concatenate(xs)
-    t = np.concatenate(ts)
-    return x, t, {"runs": n_runs, "logged_steps": n_steps, "rejected_first_proposals": n_rejected,
-                  "rows": int(len(x)), "collect_s": round(time.time() - t0, 1)}
+    x = np.concatenate(xs) if xs else np.zeros((0, 12), np.float32)
+    t = np.concatenate(ts) if ts else np.zeros((0, 2), np.float32)
+    return x, t, {"runs": n_runs, "logged_steps": n_steps, "steps_per_size": per_size,
+                  "rejected_first_proposals": n_rejected, "rows": int(len(x)), "collect_s": round(time.time() - t0, 1)}
 
 
-def train(ctx, x, t, epochs: int = 60, seed: int = 0):
+def size_weights(x: np.ndarray) -> np.ndarray:
+    """Per-row weights that give each system-size class (from the n_dof
+    feature: below 1e6, below 6.3e6, above) the same total weight, mean 1."""
+    lg = x[:, 9].astype(np.float64) * 8.0  # log10 n_dof
+    cls = np.digitize(lg, [6.0, 6.8])
+    counts = np.bincount(cls, minlength=3).astype(np.float64)
+    present = counts > 0
+    w = (len(x) / present.sum()) / counts[cls]
+    return w.astype(np.float32)
+
+
+def train(ctx, x, t, epochs: int = 60, seed: int = 0, balance: bool = True):
     prop = Proposer(ctx, seed=seed)
     prop.fit_normalisation(x, t)
     rng = np.random.default_rng(seed)
@@ -132,7 +170,8 @@ def train(ctx, x, t, epochs: int = 60, seed: int = 0):
     val, tr = idx[:n_val], idx[n_val:]
     l0, _ = prop.loss_grad(x[val], t[val])
     t0 = time.time()
-    hist = prop.train(x[tr], t[tr], epochs=epochs, batch=256, lr=2e-3, seed=seed + 1)
+    w = size_weights(x[tr]) if balance else None
+    hist = prop.train(x[tr], t[tr], w, epochs=epochs, batch=256, lr=2e-3, seed=seed + 1)
     l1, _ = prop.loss_grad(x[val], t[val])
     pred = prop.forward(x[val])
     m_err = float(np.mean(np.abs(np.rint(pred[:, 1]) - t[val, 1])))
@@ -150,13 +189,32 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--data-out", default="", help="also save the labelled rows (npz)")
+    ap.add_argument("--data-in", nargs="*", default=[], help="labelled rows saved by earlier runs, trained on too")
+    ap.add_argument("--max-seconds", type=float, default=1e9, help="stop collecting after this long")
+    ap.add_argument("--shares", type=float, nargs=3, default=None, metavar=("SMALL", "MEDIUM", "LARGE"),
+                    help="shares of the collection time per size class")
+    ap.add_argument("--fields", type=int, nargs=3, default=None, metavar=("SMALL", "MEDIUM", "LARGE"),
+                    help="initial fields per mesh of each size class")
+    ap.add_argument("--no-balance", action="store_true", help="train without the size-class weights")
     a = ap.parse_args(argv)
     ctx = GpuContext()
-    x, t, summ = collect(ctx, a.rows, a.cost, a.seed)
+    xin, tin = [], []
+    for path in a.data_in:
+        with np.load(path, allow_pickle=False) as z:
+            xin.append(z["x"])
+            tin.append(z["t"])
+    n_in = int(sum(len(v) for v in xin))
+    x, t, summ = collect(ctx, max(a.rows - n_in, 0), a.cost, a.seed, max_seconds=a.max_seconds, shares=a.shares,
+                         fields=a.fields)
     log(f"collected {summ}")
     if a.data_out:
         np.savez_compressed(a.data_out, x=x, t=t)
-    prop, tsum = train(ctx, x, t, a.epochs, a.seed)
+    if xin:
+        x = np.concatenate(xin + [x])
+        t = np.concatenate(tin + [t])
+        summ["rows_from_earlier_runs"] = n_in
+        summ["rows"] = int(len(x))
+    prop, tsum = train(ctx, x, t, a.epochs, a.seed, balance=not a.no_balance)
     prop.save(a.out)
     summ.update(tsum)
     summ["cost_model_ms"] = list(a.cost)

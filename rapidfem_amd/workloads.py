@@ -215,7 +215,7 @@ def flops_per_dof_stage(order: int) -> float:
 
 def bytes_per_dof_stage(order: int, form: str) -> float:
     np_ = (order + 1) * (order + 2) * (order + 3) // 6
-    if form in ("fused_f64c", "fused_bf16x3", "fused_bf16x2"):
+    if form in ("fused_f64c", "fused_bf16x4", "fused_bf16x3", "fused_bf16x2"):
         state = 6 + 6 + 4 + 4  # y (f32+bf16) read+write, p (f32) read+write
         geom = 9 * 8 + 4 * 4 + 4  # J^-1 f64, 4 face codes, material index
     elif form == "fused_f64":

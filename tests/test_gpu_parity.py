@@ -127,6 +127,50 @@ def test_kcl_adaptive_matches_oracle(ctx, oracle, form):
     assert abs(st.h_min - hmin) <= 1e-9 * hmin and abs(st.h_max - hmax) <= 1e-9 * hmax
 
 
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("drive", ["free", "vector"])
+def test_kcl_compressed_registers_match_oracle(ctx, oracle, order, drive):
+    """KCL with compressed registers (y and its backup f32 + bf16, err f32,
+    stage input FP64) on the compressed-register form: the oracle's
+    accept/reject counts, the state at the form's precision, and a
+    rejection restores y from its compressed backup."""
+    from rapidfem_amd import Controller, Drive, RfhipError
+
+    oop, y0 = _te101(oracle, 3, order)
+    n = oop.n_dof
+    g = _gop(ctx, oop, "fused_f64c")
+    g.set_kcl_registers("compressed")
+    kw, okw = {}, {}
+    if drive == "vector":
+        b = np.cos(0.37 * np.arange(n))
+        wave = np.array([0.4, -0.3, 0.2])
+        kw = dict(drive=Drive.vector(b), g=wave)
+        okw = dict(drive=("vector", b), g=wave)
+    # a tight controller and a long first step force rejections
+    ctl = Controller(atol=1e-9, rtol=1e-7)
+    dt, steps = 0.05, 3
+    traj, st = g.transient_kcl(ctx, y0, dt, steps, ctl=ctl, precision="f64", **kw)
+    octl = oracle.Controller.default()
+    octl.atol, octl.rtol = 1e-9, 1e-7
+    y, acc, rej, _, _ = oop.adaptive(y0, dt, steps, octl, **okw)
+    errs = []
+    for k in range(1, steps + 1):
+        yk, _, _, _, _ = oop.adaptive(y0, dt, k, octl, **okw)
+        errs.append(rel_l2(traj[k], yk))
+    print(f"kcl compressed order={order} drive={drive}: gpu {st.accepted}/{st.rejected}, oracle {acc}/{rej}, "
+          f"rel_l2 per frame {[f'{e:.2e}' for e in errs]}", flush=True)
+    assert (st.accepted, st.rejected) == (acc, rej)
+    assert rej > 0
+    assert max(errs) < 1e-7
+    g.set_kcl_registers("f64")
+    traj64, st64 = g.transient_kcl(ctx, y0, dt, steps, ctl=ctl, precision="f64", **kw)
+    assert (st64.accepted, st64.rejected) == (acc, rej)
+    assert rel_l2(traj64[-1], y) < 1e-11
+    g64 = _gop(ctx, oop, "fused_f64")
+    with pytest.raises(RfhipError):
+        g64.set_kcl_registers("compressed")
+
+
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
 @pytest.mark.parametrize("variant", [5, 7])
 @pytest.mark.parametrize("drive", ["free", "vector"])

@@ -8,7 +8,8 @@
              sequence, and lies within the run's tolerance of a fine-step
              FP64 propagation; every logged step satisfies its own bound;
              a network (any parameters) proposing the steps still meets the
-             same tolerance, its rejected proposals falling back.
+             same tolerance, every step the largest the estimate admits on
+             the network's basis.
 """
 import math
 
@@ -140,6 +141,7 @@ def test_plain_controller_matches_oracle_replay_and_fine_propagation(ctx, oracle
 
 
 def test_network_proposals_are_judged_and_meet_the_tolerance(ctx):
+    from oracle import mlp as M
     import rapidfem_amd as R
     from rapidfem_amd.proposer import ExpController, Proposer
 
@@ -154,7 +156,7 @@ def test_network_proposals_are_judged_and_meet_the_tolerance(ctx):
     norm = np.concatenate([np.zeros(12), np.ones(12), [2.0, 12.0], [1.0, 4.0]]).astype(np.float32)
     prop.set_params(p, norm)
     ctl = ExpController(tol=tol, m=20, m_min=10, m_max=30)
-    out, steps, _ = g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=prop)
+    out, steps, hess = g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=prop, hess_dim=30)
     out_p, steps_p, _ = g.expmv_adaptive(ctx, y0, t_end, ctl)
     print(f"network: {len(steps)} steps, m {steps['m'].tolist()}, first proposals accepted "
           f"{int((steps['n_judged'] == 1).sum())}; plain: {len(steps_p)} steps", flush=True)
@@ -163,3 +165,11 @@ def test_network_proposals_are_judged_and_meet_the_tolerance(ctx):
     assert np.all(steps["est"] <= steps["bound"] * (1 + 1e-12))
     assert abs(steps["h"].sum() - t_end) <= 1e-12 * t_end
     assert rel_l2(out, out_p) < 2 * tol
+    # every step but the last is the largest the estimate admits on its basis
+    # (scipy's exponential as the judge): 0.2 % more is rejected
+    rate = tol * float(np.linalg.norm(y0)) / t_end
+    for s, hb in zip(steps[:-1], hess[:-1]):
+        j = int(s["m"])
+        assert M.krylov_estimate(hb, j, float(s["beta"]), float(s["h"])) <= rate * float(s["h"]) * (1 + 1e-9)
+        h_up = 1.002 * float(s["h"])
+        assert M.krylov_estimate(hb, j, float(s["beta"]), h_up) > rate * h_up
