@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--autotune", type=int, default=1, help="pick the fastest kernel variant on this mesh")
     ap.add_argument("--variant", type=int, default=-1,
                     help="pin this stage-kernel variant instead of autotuning (fused forms)")
+    ap.add_argument("--probe-c3", action="store_true",
+                    help="with --traffic-probe: the 998,250-tet N=4 cavity instead of the pulse box")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="counter-profiling run: a 1 GiB streaming copy (torch.neg, 8 B read + 8 B written "
                          "per element) as the calibration kernel, then 10 LSERK4 steps of the chosen form at a "
@@ -578,8 +580,13 @@ def traffic_probe(args) -> int:
         torch.neg(a, out=b)
     torch.cuda.synchronize()
     del a, b
-    nx, ny, nz = args.cells
-    op, y0 = W.pulse_box(nx, ny, nz, args.order)
+    if args.probe_c3:
+        op, y0 = W.modes_box(55, 4)  # the 998,250-tet N=4 cavity of the C3 line
+        order = 4
+    else:
+        nx, ny, nz = args.cells
+        op, y0 = W.pulse_box(nx, ny, nz, args.order)
+        order = args.order
     ctx = R.GpuContext()
     g = R.GpuOperator(ctx, op, args.form)
     if args.variant >= 0:
@@ -590,7 +597,8 @@ def traffic_probe(args) -> int:
     g.state_lserk(0.002, 10)
     print(json.dumps({"probe": "traffic", "copy_bytes_read": 8 * n_copy, "copy_bytes_written": 8 * n_copy,
                       "form": args.form, "variant": g.variant, "n_dof": op.n_dof, "stage_launches": 50,
-                      "algorithmic_bytes_per_launch": W.bytes_per_dof_stage(args.order, args.form) * op.n_dof}))
+                      "workload": f"{op.n_elem} tets N={order}",
+                      "algorithmic_bytes_per_launch": W.bytes_per_dof_stage(order, args.form) * op.n_dof}))
     return 0
 
 

@@ -1706,7 +1706,6 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(V
 // s = -1 for E, +1 for H, j' the other field's jumps (rhs.rs:1372-1440; the
 // PEC and port rules enter through the lane's own jump, see face_flux).
 constexpr int kClEpw = 10;
-constexpr int kClJb = 9;  // columns of a D_k row per batch of table entries
 
 template <int NP>
 struct ClaneWaves {
@@ -1808,19 +1807,15 @@ k_clane(StageArgs io) {
     asm volatile("" : "+s"(csp));
     // Loads from the constant address space are taken never to be written,
     // so the scheduler would issue every table load of the group at once
-    // (more SGPRs than a wave has). The entries are used in batches (part of
-    // a D_k row, a LIFT row); after each batch the table address is
-    // laundered with the results of the batch before it as inputs, so a
-    // batch's entries load while the previous batch computes and at most two
-    // batches hold SGPRs.
+    // (more SGPRs than a wave has). The entries are used in batches (the D_k
+    // rows of a node, a LIFT row); after each batch the table address is
+    // laundered with the batch's result as an input, so the next batch's
+    // entries load once it is done. (Letting them load one batch earlier
+    // raised the register count and measured slower.)
     cptr cs = (cptr)csp;  // wave-uniform: scalar loads
-    double lag[3] = {0.0, 0.0, 0.0};
-    auto next_batch = [&](double a, double b, double c) {
-      asm volatile("" : "+s"(csp) : "v"(lag[0]), "v"(lag[1]), "v"(lag[2]));
+    auto next_batch = [&](double dep) {
+      asm volatile("" : "+s"(csp) : "v"(dep));
       cs = (cptr)csp;
-      lag[0] = a;
-      lag[1] = b;
-      lag[2] = c;
     };
     double jr[3][3];
 #pragma unroll
@@ -1876,15 +1871,12 @@ k_clane(StageArgs io) {
             }
           });
         }
-        if constexpr ((j + 1) % kClJb == 0 || j + 1 == NP) next_batch(d[0], d[1], d[2]);
       });
       const double d1 = jr[0][1] * d[0] + jr[1][1] * d[1] + jr[2][1] * d[2];  // along axis p+1
       const double d2 = jr[0][2] * d[0] + jr[1][2] * d[1] + jr[2][2] * d[2];  // along axis p+2
       // (curl H)_p = d/dx_p+1 H_p+2 - d/dx_p+2 H_p+1: lane H_p+2's d2 and lane H_p+1's d1
       acc[i] = sgn * (bperm_f64(srcO2, d2) - bperm_f64(srcO1, d1));
-      // formed here: left alone the compiler gathers the exchanges of many
-      // nodes and keeps their derivatives live meanwhile
-      asm volatile("" : "+v"(acc[i]));
+      next_batch(acc[i]);
     });
 
     // faces: jumps, flux, LIFT (entries after the volume's in the table)
@@ -1916,7 +1908,6 @@ k_clane(StageArgs io) {
         const double jo1 = bperm_f64(srcO1, jo), jo2 = bperm_f64(srcO2, jo);
         const double sd = n0 * jo + n1 * js1 + n2 * js2;
         fl[m] = cf * (a * (sd * n0 - jo) - sgn * (n1 * jo2 - n2 * jo1));
-        asm volatile("" : "+v"(fl[m]));
       });
       // the next face's traces load while this face is lifted
       if constexpr (f < 3) issue_face(std::integral_constant<int, f + 1>{});
@@ -1927,7 +1918,7 @@ k_clane(StageArgs io) {
           constexpr int m = decltype(mc)::value;
           acc[i] = fma(cs[NDV + (f * NP + i) * NFP + m], fl[m], acc[i]);
         });
-        next_batch(acc[i], acc[i], acc[i]);
+        next_batch(acc[i]);
       });
     });
 

@@ -173,3 +173,38 @@ def test_network_proposals_are_judged_and_meet_the_tolerance(ctx):
         assert M.krylov_estimate(hb, j, float(s["beta"]), float(s["h"])) <= rate * float(s["h"]) * (1 + 1e-9)
         h_up = 1.002 * float(s["h"])
         assert M.krylov_estimate(hb, j, float(s["beta"]), h_up) > rate * h_up
+
+
+@pytest.mark.timeout(600)
+def test_learned_proposer_gain_on_the_graded_slab(ctx):
+    """The committed network against the plain controller (m = 40, the
+    session's krylov_dim) on the 201,600-tet N=3 slab box graded 20:1 toward
+    an eps_r 10, sigma 50 slab: exp(t_end A) v over 200 CFL steps at tol
+    1e-6, wall time of the device part and final error of both against a
+    fine run (the plain controller at tol 1e-9). Both must meet the
+    tolerance; the gain (target 1.5x) is reported and held above 1.4x."""
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+    from rapidfem_amd.proposer import WEIGHTS, ExpController, Proposer
+
+    op, y0 = W.slab_box(40, 30, 28, 3)
+    g = R.GpuOperator(ctx, op, "fused_f64")
+    g.autotune()
+    cfl = R.TdSession(op, 1.0, "fused_f64", ctx).cfl_dt()
+    t_end, tol = 200 * cfl, 1e-6
+    yf, sf, _ = g.expmv_adaptive(ctx, y0, t_end, ExpController(tol=tol * 1e-3, m=40, h_scale=cfl))
+    prop = Proposer(ctx)
+    prop.load(WEIGHTS)
+    res = {}
+    for name, p in (("plain", None), ("learned", prop)):
+        ctl = ExpController(tol=tol, m=40, m_min=10, m_max=60, h_scale=cfl)
+        g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=p)  # warm-up
+        y, st, _ = g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=p)
+        res[name] = (g.last_call_ms, rel_l2(y, yf), len(st), float(st["m"].mean()))
+    gain = res["plain"][0] / res["learned"][0]
+    print(f"graded slab, 200 CFL steps, tol {tol:g}: plain {res['plain'][0]:.0f} ms ({res['plain'][2]} steps, "
+          f"m {res['plain'][3]:.0f}, error {res['plain'][1]:.2e}); learned {res['learned'][0]:.0f} ms "
+          f"({res['learned'][2]} steps, mean m {res['learned'][3]:.1f}, error {res['learned'][1]:.2e}); "
+          f"gain {gain:.2f}x (target 1.5x)", flush=True)
+    assert res["plain"][1] < tol and res["learned"][1] < tol
+    assert gain > 1.4
