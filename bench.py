@@ -34,6 +34,19 @@ METRIC = "DGTD DOF-stage updates/s + %HBM/MFMA peak, chosen vs plain at 1e-6 rel
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_stage_pulse_box_traffic.json")
+TRAFFIC_FILE_C3 = os.path.join(ROOT, "profiles", "fused_stage_modes_box_n4_traffic.json")
+
+
+def traffic_entry(path, form, n, variant):
+    """HBM bytes per launch of a stage-kernel variant from a committed
+    counter summary (FETCH_SIZE, WRITE_SIZE), or None."""
+    try:
+        tr = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if tr.get("form") != form or tr.get("n_dof") != n:
+        return None
+    return tr.get("variants", {}).get(str(variant), {}).get("bytes_per_launch")
 
 
 # The arithmetic each form computes in, and how its state registers are stored.
@@ -719,7 +732,7 @@ def c3_line(ctx, args):
             "plain": {"form": "plain_f64", "ms_per_step": pms, "value": n * 5 / (pms * 1e-3),
                       "apply_kernel_ms": papply},
             "speedup_vs_plain": pms / (tot / steps), "rel_l2_vs_f64": err, "rel_l2_steps": steps + 2,
-            "roofline": roofline_record(4, args.form, n, stage)}
+            "roofline": roofline_record(4, args.form, n, stage, traffic_entry(TRAFFIC_FILE_C3, args.form, n, variant))}
 
 
 def main() -> int:
@@ -805,14 +818,7 @@ def main() -> int:
     log(f"fused_f64: {fvalue:.4e} DOF-stage/s, stage kernel {fstage:.4f} ms")
 
     # Roofline of the dominant kernel (the fused stage kernel).
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        try:
-            tr = json.load(open(TRAFFIC_FILE))
-            if tr.get("form") == args.form and tr.get("n_dof") == n:
-                traffic = tr.get("variants", {}).get(str(variant), {}).get("bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
+    traffic = traffic_entry(TRAFFIC_FILE, args.form, n, variant)
     roofline = roofline_record(args.order, args.form, n, stage_ms, traffic)
     cpu = None
     vs_oracle = None

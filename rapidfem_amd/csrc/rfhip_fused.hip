@@ -1256,7 +1256,7 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
 // the LIFT reads) stays inside one wave, so after the initial table load
 // there are no block-wide barriers: a wave stalled on memory or LDS does not
 // hold the block's other waves.
-template <int NP, int NFP, bool CIN, int EPI, int UNR, int VMF = 0, int LUNR = UNR>
+template <int NP, int NFP, bool CIN, int EPI, int UNR, int VMF = 0>
 __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(VMF ? 2 : 3))) k_fused11(StageArgs io) {
   constexpr int EPB = 64 / NP;  // elements per WAVE: a wave owns whole elements
   constexpr int NWB = kFusedWG / 64;
@@ -1647,7 +1647,7 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(V
     if (active) {
       const double *fb = sf + le * SFS;
       const double *lr = lift + i * LS;
-#pragma unroll LUNR
+#pragma unroll UNR
       for (int fm = 0; fm < NF4; fm += 2) {  // NF4 = 4 Nfp is even
         const double2 w = *reinterpret_cast<const double2 *>(lr + fm);
 #pragma unroll
@@ -2030,15 +2030,14 @@ void fused_np(int variant, const StageArgs &a, hipStream_t s) {
         hipLaunchKernelGGL((k_fused9<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
     }
   } else if ((variant == 6 || variant == 7) && a.nbase) {
-    const void *kf = variant == 6 ? reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 2, 0, 64>)
+    const void *kf = variant == 6 ? reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 1>)
                                   : reinterpret_cast<const void *>(&k_fused11<NP, F, CIN, EPI, 2>);
     const int64_t epw = 64 / NP;
     const int64_t wgroups = (a.n_elem + epw - 1) / epw;
     const int64_t grid = persistent(kf, (wgroups + 3) / 4);  // blocks of 4 waves
     if (grid > 0) {
       if (variant == 6)
-        hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 2, 0, 64>), dim3((unsigned)grid), dim3(kFusedWG), 0, s,
-                           a);
+        hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
       else
         hipLaunchKernelGGL((k_fused11<NP, F, CIN, EPI, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
     }
