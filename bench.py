@@ -83,6 +83,9 @@ def parse():
                     help="pin this stage-kernel variant instead of autotuning (fused forms)")
     ap.add_argument("--probe-c3", action="store_true",
                     help="with --traffic-probe: the 998,250-tet N=4 cavity instead of the pulse box")
+    ap.add_argument("--drift", action="store_true",
+                    help="long-run error of every register / precision form against the FP64 fused form")
+    ap.add_argument("--drift-steps", type=int, default=5000)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="counter-profiling run: a 1 GiB streaming copy (torch.neg, 8 B read + 8 B written "
                          "per element) as the calibration kernel, then 10 LSERK4 steps of the chosen form at a "
@@ -313,6 +316,50 @@ def proposer_bench(args) -> int:
                                   f"tets graded 20:1 to an eps_r 10, sigma 50 slab, N={args.order}, pulse field, "
                                   f"t_end {args.proposer_cfl} CFL steps",
                       "n_elem": op.n_elem, "n": op.n_dof, "variant": variant}}
+    print(json.dumps(out))
+    return 0
+
+
+def drift_bench(args) -> int:
+    """Long-run precision of the register and arithmetic forms: --drift-steps
+    LSERK4 steps at the CFL step of the stepper workload (C2, 5000 steps by
+    default) from the same pulse, each form's final state against the FP64
+    fused form's, with each form's ms per step. The reference's f32 GPU
+    structure is plain_f32."""
+    import numpy as np
+
+    import rapidfem_amd as R
+    from rapidfem_amd import workloads as W
+
+    nx, ny, nz = args.cells
+    op, y0 = W.pulse_box(nx, ny, nz, args.order)
+    ctx = R.GpuContext()
+    sess = R.TdSession(op, 1.0, "fused_f64", ctx)
+    dt = sess.cfl_dt()
+    steps = args.drift_steps
+    ref = sess.gop
+    ref.autotune()
+    ref.state_load(y0)
+    t_ref, _ = ref.state_lserk_timed(dt, steps)
+    want = ref.state_store()
+    log(f"drift: {op.n_elem} tets N={args.order}, {steps} steps at dt {dt:.4e}; fused_f64 {t_ref / steps:.3f} ms/step")
+    forms = {}
+    for form in ("fused_f64c", "plain_f64", "plain_f32", "fused_bf16x4", "fused_bf16x3", "fused_bf16x2"):
+        g = R.GpuOperator(ctx, op, form)
+        if form.startswith("fused"):
+            g.autotune()
+        g.state_load(y0)
+        tot, _ = g.state_lserk_timed(dt, steps)
+        y = g.state_store()
+        err = float(np.linalg.norm(y - want) / np.linalg.norm(want))
+        forms[form] = {"rel_l2_vs_fp64": err, "ms_per_step": tot / steps}
+        log(f"  {form}: rel L2 vs FP64 after {steps} steps {err:.3e}, {tot / steps:.3f} ms/step")
+        del g
+    out = {"metric": "relative L2 error against the FP64 fused form after a long LSERK4 run, per form",
+           "value": forms["fused_f64c"]["rel_l2_vs_fp64"], "unit": "rel L2 (chosen form)", "n_gpus": 1,
+           "higher_is_better": False, "steps": steps, "dt": dt, "forms": forms, "fused_f64_ms_per_step": t_ref / steps,
+           "data": "synthetic", "config": {"workload": f"pulse_box {nx}x{ny}x{nz} cells x 6 tets, N={args.order}",
+                                           "n_elem": op.n_elem, "n_dof": op.n_dof}}
     print(json.dumps(out))
     return 0
 
@@ -755,6 +802,8 @@ def main() -> int:
         return batch_bench(args)
     if args.kcl:
         return kcl_bench(args)
+    if args.drift:
+        return drift_bench(args)
     import numpy as np
 
     import rapidfem_amd as R
