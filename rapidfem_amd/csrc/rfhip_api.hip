@@ -870,16 +870,30 @@ int krylov_impl(rfhip_op *op, const double *v_host, double t, int max_dim, doubl
 
 // ---- adaptive Krylov exponential ------------------------------------------
 //
-// Arnoldi with CGS2 (the device sweeps of krylov_impl's plain path), built
-// one column at a time so a step's dimension can be chosen after its first
-// columns. The Hessenberg matrix is (cap+1) x cap, row stride cap; a column
-// j+1 of the basis is scaled from w when column j+1 is extended, so a step
-// that stops at dimension d never writes basis column d.
+// The adaptive exponential's Arnoldi basis, built one column at a time by
+// classical Gram-Schmidt with delayed reorthogonalisation (DCGS2,
+// Swirydowicz, Langou, Ananthan, Yang and Thomas, "Low synchronization
+// Gram-Schmidt and generalized minimal residual algorithms", 2021): column
+// j's second projection is taken in the same sweep as column j+1's first.
+// Column j of the basis holds the provisional vector q_j (projected once,
+// not normalised) until the next column finishes it. Per column:
+//  * z = A q_j, written into column j+1;
+//  * one Gram sweep: s = V_j^T q_j, |q_j|^2, p = V_j^T z, q_j . z;
+//  * on the host: r = |q_j - V_j s| (Pythagoras), column j-1 of H corrected
+//    by s and closed with r, and column j's first projections of
+//    A v_j = (z - A V_j s) / r = (z - V_{j+1} H s) / r;
+//  * one update sweep: v_j = (q_j - V_j s) / r in place, and the next
+//    provisional vector q_{j+1} = (z - V_j a - b q_j) / r.
+// Two passes over the basis per column where CGS2 takes three, and no
+// normalisation pass (q_j is applied unnormalised; the scale folds into
+// the coefficients). finalize() reorthogonalises the last q before H is
+// used. The step's state vector is column 0 (q_0), and the step's result
+// is written back there.
 struct Arnoldi {
   rfhip_op *op = nullptr;
   int64_t n = 0;
   int cap = 0, cols = 0, nblk = 0;
-  bool broke = false;
+  bool broke = false, pending = false;
   double beta = 0.0;
   std::vector<double> H;
 
@@ -888,13 +902,10 @@ struct Arnoldi {
       op->kb_basis.alloc(8 * (size_t)(cap + 1) * n);
       op->kb_cap = cap;
     }
-    op->kb_w.alloc(8 * n);
-    op->kb_x.alloc(8 * n);
-    op->kb_out.alloc(8 * n);
-    op->kb_proj.alloc(8 * (kMaxKrylov + 8));
-    op->kb_host.alloc(8 * (2 * kMaxKrylov + 8));
+    op->kb_proj.alloc(8 * (size_t)(6 * kMaxKrylov + 16));
+    op->kb_host.alloc(8 * (size_t)(6 * kMaxKrylov + 16));
     nblk = krylov_dot_blocks(n);
-    op->kb_part.alloc(8 * (size_t)nblk * (kMaxKrylov + 1));
+    op->kb_part.alloc(8 * (size_t)nblk * (2 * kMaxKrylov + 4));
     if (op->form == RFHIP_FORM_PLAIN_F32) {
       op->kst.alloc(4 * n);
       op->k.alloc(4 * n);
@@ -902,72 +913,134 @@ struct Arnoldi {
     H.assign((size_t)(cap + 1) * cap, 0.0);
   }
   double *basis() const { return op->kb_basis.as<double>(); }
-  // v_0 = x / |x| for the f64 layout vector x on the device.
-  void begin(const double *x) {
-    hipStream_t s = op->stream;
+  double *col(int c) const { return basis() + (int64_t)c * n; }
+  double &h(int r, int c) { return H[(size_t)r * cap + c]; }
+  // device / pinned-host regions: Gram results, update coefficients, |q|^2
+  static constexpr int kGram = 0, kCoef = 2 * kMaxKrylov + 4, kNorm = 4 * kMaxKrylov + 8;
+  // A new basis from the state vector in column 0.
+  void begin() {
     std::fill(H.begin(), H.end(), 0.0);
     cols = 0;
+    pending = true;
     broke = false;
-    double *proj = op->kb_proj.as<double>();
-    launch_multi_dot(x, n, 1, x, op->kb_part.as<double>(), nblk, proj, s);
-    double nrm2 = 0.0;
-    RF_HIP(hipMemcpyAsync(&nrm2, proj, 8, hipMemcpyDeviceToHost, s));
-    sync(op);
-    beta = std::sqrt(nrm2);
-    if (beta == 0.0) {
-      broke = true;
-      return;
-    }
-    launch_scale(x, basis(), n, 1.0 / beta, s);
+    beta = 0.0;
   }
-  // Column `cols` of H: w = A v_j, two classical Gram-Schmidt passes, the
-  // norm; sets broke (and leaves H[j+1][j] = 0) below the breakdown
-  // threshold of propagator.rs.
-  void extend() {
+  void apply(const double *x, double *w) {
     hipStream_t s = op->stream;
-    const int j = cols;
-    double *w = op->kb_w.as<double>(), *proj = op->kb_proj.as<double>(), *part = op->kb_part.as<double>();
-    if (j > 0) launch_scale(w, basis() + (int64_t)j * n, n, 1.0 / H[(size_t)j * cap + (j - 1)], s);
-    const double *bj = basis() + (int64_t)j * n;
     if (op->form == RFHIP_FORM_PLAIN_F32) {
-      launch_cast_d2f(bj, op->kst.as<float>(), n, s);
+      launch_cast_d2f(x, op->kst.as<float>(), n, s);
       plain_apply_full(op, op->kst.p, op->k.p, s);
       launch_cast_f2d(op->k.as<float>(), w, n, s);
     } else {
-      rhs_into(op, bj, w, RFHIP_DRIVE_FREE, 0, 0.0);
+      rhs_into(op, x, w, RFHIP_DRIVE_FREE, 0, 0.0);
     }
-    const int c = j + 1;
-    double *hp = op->kb_host.as<double>();
-    launch_multi_dot(basis(), n, c, w, part, nblk, proj, s);
-    RF_HIP(hipMemcpyAsync(hp, proj, 8 * c, hipMemcpyDeviceToHost, s));
-    launch_axpy_dot(w, basis(), n, c, proj, part, nblk, proj, s);
-    RF_HIP(hipMemcpyAsync(hp + kMaxKrylov, proj, 8 * c, hipMemcpyDeviceToHost, s));
-    launch_axpy_basis_norm(w, basis(), n, c, proj, part, nblk, proj + kMaxKrylov, s);
-    RF_HIP(hipMemcpyAsync(hp + 2 * kMaxKrylov, proj + kMaxKrylov, 8, hipMemcpyDeviceToHost, s));
+  }
+  // Finishes v_j (j = cols) and starts q_{j+1}; at j = 0 sets beta = |q_0|.
+  // A breakdown (|q_j - V_j s| below the threshold of propagator.rs) leaves
+  // cols = j, H[j][j-1] = 0 and broke set.
+  void extend() {
+    RF_REQUIRE(pending && !broke && cols < cap, "adaptive Arnoldi: no provisional column to extend");
+    hipStream_t s = op->stream;
+    const int j = cols;
+    double *dev = op->kb_proj.as<double>(), *hp = op->kb_host.as<double>(), *part = op->kb_part.as<double>();
+    apply(col(j), col(j + 1));
+    launch_gram2(basis(), n, j + 2, part, nblk, dev + kGram, s);
+    RF_HIP(hipMemcpyAsync(hp + kGram, dev + kGram, 8 * (size_t)(2 * (j + 2)), hipMemcpyDeviceToHost, s));
     sync(op);
-    for (int pass = 0; pass < 2; ++pass)
-      for (int i = 0; i < c; ++i) H[(size_t)i * cap + j] += hp[pass * kMaxKrylov + i];
-    const double hnext = std::sqrt(hp[2 * kMaxKrylov]);
-    cols = c;
-    if (!(hnext >= kArnoldiBreakdown)) {
-      broke = true;
+    const double *g = hp + kGram;  // g[2c] = b_c . q_j, g[2c + 1] = b_c . z
+    const double alpha = g[2 * j], gamma = g[2 * j + 1];
+    RF_REQUIRE(std::isfinite(alpha) && std::isfinite(gamma), "adaptive Arnoldi: non-finite basis vector");
+    double r2 = alpha, sp = 0.0;
+    for (int c = 0; c < j; ++c) {
+      r2 -= g[2 * c] * g[2 * c];
+      sp += g[2 * c] * g[2 * c + 1];
+    }
+    const double r = r2 > 0.0 ? std::sqrt(r2) : 0.0;
+    if (j == 0) {
+      beta = r;
+      if (beta == 0.0) {
+        broke = true;
+        pending = false;
+        return;
+      }
+    } else {
+      for (int c = 0; c < j; ++c) h(c, j - 1) += g[2 * c];
+      if (!(r >= kArnoldiBreakdown)) {
+        h(j, j - 1) = 0.0;
+        broke = true;
+        pending = false;
+        return;
+      }
+      h(j, j - 1) = r;
+    }
+    // t = H[0..j][0..j-1] s, the first projections of A v_j onto V_{j+1}
+    // and the next provisional vector's coefficients
+    std::vector<double> t((size_t)j + 1, 0.0), cv((size_t)j + 1);
+    for (int k = 0; k <= j; ++k)
+      for (int i = 0; i < j; ++i) t[k] += h(k, i) * g[2 * i];
+    for (int c = 0; c < j; ++c) h(c, j) = (g[2 * c + 1] - t[c]) / r;
+    h(j, j) = ((gamma - sp) / r - t[j]) / r;
+    for (int k = 0; k <= j; ++k) cv[k] = t[k] + r * h(k, j);
+    double *cf = hp + kCoef;
+    for (int c = 0; c < j; ++c) {
+      cf[c] = g[2 * c];
+      cf[j + c] = cv[c] - g[2 * c] * cv[j] / r;
+    }
+    cf[2 * j] = cv[j] / r;
+    cf[2 * j + 1] = 1.0 / r;
+    RF_HIP(hipMemcpyAsync(dev + kCoef, cf, 8 * (size_t)(2 * j + 2), hipMemcpyHostToDevice, s));
+    launch_dcgs_update(basis(), n, j, dev + kCoef, part, nblk, dev + kNorm, s);
+    RF_HIP(hipMemcpyAsync(hp + kNorm, dev + kNorm, 8, hipMemcpyDeviceToHost, s));
+    cols = j + 1;
+  }
+  // H[cols][cols-1] from |q_cols| (the first-pass residual norm), for the
+  // proposer's features before the column is finished.
+  void provisional() {
+    if (!pending || broke || cols == 0) return;
+    sync(op);
+    h(cols, cols - 1) = std::sqrt(std::max(op->kb_host.as<double>()[kNorm], 0.0));
+  }
+  // Reorthogonalises the last provisional vector: closes column cols-1 of H
+  // (its correction and H[cols][cols-1]); a breakdown sets broke.
+  void finalize() {
+    if (!pending || broke || cols == 0) {
+      pending = false;
       return;
     }
-    H[(size_t)c * cap + j] = hnext;
+    hipStream_t s = op->stream;
+    const int j = cols;
+    double *dev = op->kb_proj.as<double>(), *hp = op->kb_host.as<double>();
+    launch_multi_dot(basis(), n, j + 1, col(j), op->kb_part.as<double>(), nblk, dev + kGram, s);
+    RF_HIP(hipMemcpyAsync(hp + kGram, dev + kGram, 8 * (size_t)(j + 1), hipMemcpyDeviceToHost, s));
+    sync(op);
+    double r2 = hp[kGram + j];
+    for (int c = 0; c < j; ++c) {
+      h(c, j - 1) += hp[kGram + c];
+      r2 -= hp[kGram + c] * hp[kGram + c];
+    }
+    const double r = r2 > 0.0 ? std::sqrt(r2) : 0.0;
+    pending = false;
+    if (!(r >= kArnoldiBreakdown)) {
+      h(j, j - 1) = 0.0;
+      broke = true;
+    } else {
+      h(j, j - 1) = r;
+    }
   }
-  // out = beta V_d exp(h H_d) e_1 (f64 layout vector on the device).
-  void combine(int d, double h, double *out, KrylovScratch &ws) {
+  // out = beta V_d exp(h H_d) e_1 (f64 layout vector on the device; may be
+  // column 0).
+  void combine(int d, double hstep, double *out, KrylovScratch &ws) {
     ws.m.assign((size_t)d * d, 0.0);
     ws.e.assign((size_t)d * d, 0.0);
     for (int a = 0; a < d; ++a)
-      for (int b = 0; b < d; ++b) ws.m[(size_t)a * d + b] = h * H[(size_t)a * cap + b];
+      for (int b = 0; b < d; ++b) ws.m[(size_t)a * d + b] = hstep * H[(size_t)a * cap + b];
     expm_pade13(ws.m.data(), d, ws.e.data(), ws.work);
-    std::vector<double> coef(d);
-    for (int i = 0; i < d; ++i) coef[i] = beta * ws.e[(size_t)i * d];
-    double *proj = op->kb_proj.as<double>();
-    RF_HIP(hipMemcpyAsync(proj, coef.data(), 8 * d, hipMemcpyHostToDevice, op->stream));
-    launch_lincomb(basis(), n, d, proj, out, op->stream);
-    sync(op);  // coef lives on this stack frame
+    double *cf = op->kb_host.as<double>() + kCoef;
+    for (int i = 0; i < d; ++i) cf[i] = beta * ws.e[(size_t)i * d];
+    double *dev = op->kb_proj.as<double>() + kCoef;
+    RF_HIP(hipMemcpyAsync(dev, cf, 8 * (size_t)d, hipMemcpyHostToDevice, op->stream));
+    launch_lincomb_alias(basis(), n, d, dev, out, op->stream);
+    sync(op);
   }
 };
 
@@ -979,8 +1052,7 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
                    rfhip_exp_step *log, int64_t log_cap, double *hess, int hess_dim, int64_t *n_steps, double *out) {
   const int cap = std::max(ctl.m, ctl.m_max);
   Arnoldi ar(op, cap);
-  const int64_t n = ar.n;
-  double *x = op->kb_x.as<double>(), *y = op->kb_out.as<double>();
+  double *x = ar.col(0);  // the state: each step's basis starts from it and its result goes back there
   h2d_layout_f64(op, v_host, x);
   sync(op);
   const auto t_begin = std::chrono::steady_clock::now();
@@ -988,19 +1060,16 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
   ProposalHistory hist;
   double t = 0.0, h_next = ctl.h0 > 0.0 ? ctl.h0 : 0.0, rate = 0.0, hs = 1.0;
   int64_t nst = 0;
-  bool zero = false;
   while (t_end - t > 1e-14 * t_end) {
     RF_REQUIRE(nst < ctl.max_steps, "adaptive exponential: more than max_steps steps");
     const auto t_step = std::chrono::steady_clock::now();
-    ar.begin(x);
-    if (ar.beta == 0.0) {
-      zero = true;
-      break;
-    }
+    ar.begin();
+    ar.extend();
+    if (ar.beta == 0.0) break;  // a zero state stays zero
     if (nst == 0) rate = ctl.tol * ar.beta / t_end;
     const double t_left = t_end - t;
-    ar.extend();
     if (!ar.broke) ar.extend();
+    ar.provisional();
     rfhip_exp_step rec{};
     rec.t = t;
     rec.beta = ar.beta;
@@ -1031,6 +1100,7 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
       m_prop = ctl.m;
     }
     while (!ar.broke && ar.cols < m_prop) ar.extend();
+    ar.finalize();
     const int d = ar.cols;
     rec.m = d;
     rec.breakdown = ar.broke ? 1 : 0;
@@ -1065,7 +1135,7 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
     rec.h = h;
     rec.est = est;
     rec.bound = rate * h;
-    ar.combine(d, h, y, ws);
+    ar.combine(d, h, x, ws);
     if (hess && hess_dim > 0 && nst < log_cap) {
       double *blk = hess + (size_t)nst * (hess_dim + 1) * hess_dim;
       std::fill(blk, blk + (size_t)(hess_dim + 1) * hess_dim, 0.0);
@@ -1082,9 +1152,7 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
     hist.h_prev = h;
     hist.beta_prev = ar.beta;
     t += h;
-    std::swap(x, y);
   }
-  if (zero) launch_fill(x, 8 * n, op->stream);
   sync(op);
   op->last_call_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_begin).count();
   if (n_steps) *n_steps = nst;

@@ -345,6 +345,14 @@ void launch_multi_dot(const double *basis, int64_t n, int cols, const double *w,
                       double *proj, hipStream_t s);
 void launch_axpy_basis_norm(double *w, const double *basis, int64_t n, int cols, const double *proj,
                             double *partials, int nblk, double *norm2, hipStream_t s);
+// CGS2 with delayed reorthogonalisation (rfhip_vec.hip): out[2c + r] =
+// b_c . b_{cols-2+r} for c < cols; the update sweep of column j (coef =
+// [s (j), a (j), bq, 1/r]) and |b_{j+1}|^2; out = sum coef_c b_c with out
+// allowed to be a column.
+void launch_gram2(const double *basis, int64_t n, int cols, double *partials, int nblk, double *out, hipStream_t s);
+void launch_dcgs_update(double *basis, int64_t n, int j, const double *coef, double *partials, int nblk,
+                        double *norm2, hipStream_t s);
+void launch_lincomb_alias(const double *basis, int64_t n, int dim, const double *coef, double *out, hipStream_t s);
 // w -= basis[:, :cols] proj, then proj_out = basis[:, :cols]^T w (the updated
 // w): a CGS2 first-pass update fused with the second pass's projections.
 // proj_out may be proj: it is written after every block has read proj.

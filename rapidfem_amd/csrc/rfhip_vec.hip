@@ -390,6 +390,151 @@ __global__ void __launch_bounds__(kBS) k_axpy_dot(double *__restrict__ w, const 
     }
   }
 }
+// ---- CGS2 with delayed reorthogonalisation (the adaptive Arnoldi) --------
+//
+// Per-block partials of the Gram products of basis columns 0..cols-1 with
+// the last two, b_{cols-2} and b_{cols-1} (the provisional vector q and the
+// new product z = A q): part[b*2cols + 2c + r] = b_c . b_{cols-2+r} over the
+// block's contiguous slice, kDotTileM columns per pass. One sweep gives the
+// reorthogonalisation coefficients of q, the projections of z, |q|^2 and
+// q . z.
+template <bool VEC>
+__global__ void __launch_bounds__(kBS) k_gram2(const double *__restrict__ basis, int64_t n, int cols,
+                                               double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  const int64_t m = VEC ? n / 2 : n;
+  const int64_t chunk = (m + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk;
+  const int64_t i1 = i0 + chunk < m ? i0 + chunk : m;
+  const double *q = basis + (int64_t)(cols - 2) * n, *z = basis + (int64_t)(cols - 1) * n;
+  for (int c0 = 0; c0 < cols; c0 += kDotTileM) {
+    double aq[kDotTileM], az[kDotTileM];
+#pragma unroll
+    for (int t = 0; t < kDotTileM; ++t) aq[t] = az[t] = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kBS) {
+      if constexpr (VEC) {
+        const double2 qi = reinterpret_cast<const double2 *>(q)[i], zi = reinterpret_cast<const double2 *>(z)[i];
+#pragma unroll
+        for (int t = 0; t < kDotTileM; ++t)
+          if (c0 + t < cols) {
+            const double2 b = reinterpret_cast<const double2 *>(basis + (int64_t)(c0 + t) * n)[i];
+            aq[t] += b.x * qi.x + b.y * qi.y;
+            az[t] += b.x * zi.x + b.y * zi.y;
+          }
+      } else {
+        const double qi = q[i], zi = z[i];
+#pragma unroll
+        for (int t = 0; t < kDotTileM; ++t)
+          if (c0 + t < cols) {
+            const double b = basis[(int64_t)(c0 + t) * n + i];
+            aq[t] += b * qi;
+            az[t] += b * zi;
+          }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kDotTileM; ++t) {
+      if (c0 + t >= cols) break;
+      const double sq = block_sum(aq[t], sh);
+      const double sz = block_sum(az[t], sh);
+      if (threadIdx.x == 0) {
+        part[(int64_t)blockIdx.x * 2 * cols + 2 * (c0 + t)] = sq;
+        part[(int64_t)blockIdx.x * 2 * cols + 2 * (c0 + t) + 1] = sz;
+      }
+    }
+  }
+}
+// The update sweep: with q = b_j and z = b_{j+1},
+//   b_j     <- (q - sum_{c<j} s_c b_c) / r          (the finished v_j)
+//   b_{j+1} <- (z - sum_{c<j} a_c b_c - bq q) / r    (the next provisional vector)
+// coef = [s_0..s_{j-1}, a_0..a_{j-1}, bq, 1/r]; the block's partial
+// |b_{j+1}|^2 goes to part[blockIdx.x].
+template <bool VEC>
+__global__ void __launch_bounds__(kBS) k_dcgs_update(double *__restrict__ basis, int64_t n, int j,
+                                                     const double *__restrict__ coef, double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  __shared__ double sc[2 * kMaxCols + 2];
+  for (int c = threadIdx.x; c < 2 * j + 2; c += kBS) sc[c] = coef[c];
+  __syncthreads();
+  const double bq = sc[2 * j], ir = sc[2 * j + 1];
+  double *q = basis + (int64_t)j * n, *z = basis + (int64_t)(j + 1) * n;
+  double acc = 0.0;
+  const int64_t m = VEC ? n / 2 : n;
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < m; i += (int64_t)gridDim.x * kBS) {
+    if constexpr (VEC) {
+      const double2 qi = reinterpret_cast<const double2 *>(q)[i];
+      double2 zi = reinterpret_cast<const double2 *>(z)[i];
+      zi.x -= bq * qi.x;
+      zi.y -= bq * qi.y;
+      double2 qn = qi;
+      for (int c0 = 0; c0 < j; c0 += kDotTile) {
+        double2 b[kDotTile];
+#pragma unroll
+        for (int t = 0; t < kDotTile; ++t)
+          if (c0 + t < j) b[t] = reinterpret_cast<const double2 *>(basis + (int64_t)(c0 + t) * n)[i];
+#pragma unroll
+        for (int t = 0; t < kDotTile; ++t)
+          if (c0 + t < j) {
+            qn.x -= sc[c0 + t] * b[t].x;
+            qn.y -= sc[c0 + t] * b[t].y;
+            zi.x -= sc[j + c0 + t] * b[t].x;
+            zi.y -= sc[j + c0 + t] * b[t].y;
+          }
+      }
+      qn.x *= ir;
+      qn.y *= ir;
+      zi.x *= ir;
+      zi.y *= ir;
+      reinterpret_cast<double2 *>(q)[i] = qn;
+      reinterpret_cast<double2 *>(z)[i] = zi;
+      acc += zi.x * zi.x + zi.y * zi.y;
+    } else {
+      const double qi = q[i];
+      double zi = z[i] - bq * qi, qn = qi;
+      for (int c0 = 0; c0 < j; c0 += kDotTile) {
+        double b[kDotTile];
+#pragma unroll
+        for (int t = 0; t < kDotTile; ++t)
+          if (c0 + t < j) b[t] = basis[(int64_t)(c0 + t) * n + i];
+#pragma unroll
+        for (int t = 0; t < kDotTile; ++t)
+          if (c0 + t < j) {
+            qn -= sc[c0 + t] * b[t];
+            zi -= sc[j + c0 + t] * b[t];
+          }
+      }
+      const double zn = zi * ir;
+      q[i] = qn * ir;
+      z[i] = zn;
+      acc += zn * zn;
+    }
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+// out = sum_c coef_c basis_c where out may be one of the columns (each
+// entry is read by the thread that writes it, so in place is safe; no
+// restrict on these pointers).
+__global__ void __launch_bounds__(kBS) k_lincomb_alias(const double *basis, int64_t n, int dim,
+                                                       const double *__restrict__ coef, double *out) {
+  __shared__ double sc[kMaxCols];
+  for (int c = threadIdx.x; c < dim; c += kBS) sc[c] = coef[c];
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+    double acc = 0.0;
+    for (int c0 = 0; c0 < dim; c0 += kDotTile) {
+      double b[kDotTile];
+#pragma unroll
+      for (int t = 0; t < kDotTile; ++t)
+        if (c0 + t < dim) b[t] = basis[(int64_t)(c0 + t) * n + i];
+#pragma unroll
+      for (int t = 0; t < kDotTile; ++t)
+        if (c0 + t < dim) acc += sc[c0 + t] * b[t];
+    }
+    out[i] = acc;
+  }
+}
+
 __global__ void __launch_bounds__(kBS) k_scale(const double *__restrict__ w, double *__restrict__ d, int64_t n,
                                                double inv) {
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) d[i] = w[i] * inv;
@@ -667,6 +812,31 @@ void launch_axpy_dot(double *w, const double *basis, int64_t n, int cols, const 
 #undef RF_AXPY_DOT
   }
   hipLaunchKernelGGL(k_dot_finish, dim3(cols), dim3(kBS), 0, s, partials, nblk, cols, proj_out);
+  RF_HIP(hipGetLastError());
+}
+void launch_gram2(const double *basis, int64_t n, int cols, double *partials, int nblk, double *out,
+                  hipStream_t s) {
+  RF_REQUIRE(cols >= 2 && cols <= kMaxCols, "Krylov dimension above 64");
+  if (n % 2 == 0)
+    hipLaunchKernelGGL(k_gram2<true>, dim3(nblk), dim3(kBS), 0, s, basis, n, cols, partials);
+  else
+    hipLaunchKernelGGL(k_gram2<false>, dim3(nblk), dim3(kBS), 0, s, basis, n, cols, partials);
+  hipLaunchKernelGGL(k_dot_finish, dim3(2 * cols), dim3(kBS), 0, s, partials, nblk, 2 * cols, out);
+  RF_HIP(hipGetLastError());
+}
+void launch_dcgs_update(double *basis, int64_t n, int j, const double *coef, double *partials, int nblk,
+                        double *norm2, hipStream_t s) {
+  RF_REQUIRE(j >= 0 && j + 1 < kMaxCols, "Krylov dimension above 64");
+  if (n % 2 == 0)
+    hipLaunchKernelGGL(k_dcgs_update<true>, dim3(nblk), dim3(kBS), 0, s, basis, n, j, coef, partials);
+  else
+    hipLaunchKernelGGL(k_dcgs_update<false>, dim3(nblk), dim3(kBS), 0, s, basis, n, j, coef, partials);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, partials, nblk, norm2);
+  RF_HIP(hipGetLastError());
+}
+void launch_lincomb_alias(const double *basis, int64_t n, int dim, const double *coef, double *out, hipStream_t s) {
+  RF_REQUIRE(dim <= kMaxCols, "Krylov dimension above 64");
+  hipLaunchKernelGGL(k_lincomb_alias, dim3(grid_for(n)), dim3(kBS), 0, s, basis, n, dim, coef, out);
   RF_HIP(hipGetLastError());
 }
 void launch_scale(const double *w, double *dst, int64_t n, double inv, hipStream_t s) {
