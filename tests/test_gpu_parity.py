@@ -671,6 +671,30 @@ def test_debye_kcl_matches_oracle(ctx, oracle, form):
     assert err < (1e-7 if form == "fused_f64c" else 1e-11)
 
 
+def test_debye_kcl_compressed_registers_match_oracle(ctx, oracle):
+    """Compressed KCL registers on the augmented system at N=4 with 81 Debye
+    elements: n = 42,525 is odd, so the bf16 halves of y and its backup end
+    on a padded word (the rejection restore copies whole words)."""
+    from rapidfem_amd import Controller
+
+    oop, g, elems = _debye_pair(ctx, oracle, "fused_f64c", 4)
+    n = oop.n_dof
+    assert n % 2 == 1 and len(elems) == 81
+    g.set_kcl_registers("compressed")
+    y0 = np.cos(0.2 + 0.013 * np.arange(n))
+    ctl = Controller(atol=1e-9, rtol=1e-7)
+    traj, st = g.transient_kcl(ctx, y0, 0.05, 2, ctl=ctl, precision="f64")
+    octl = oracle.Controller.default()
+    octl.atol, octl.rtol = 1e-9, 1e-7
+    y, acc, rej, _, _ = oop.adaptive(y0, 0.05, 2, octl)
+    err = rel_l2(traj[-1], y)
+    err_p = rel_l2(traj[-1][6 * oop.K * 35:], y[6 * oop.K * 35:])
+    print(f"debye kcl compressed N=4: gpu {st.accepted}/{st.rejected} oracle {acc}/{rej} rel_l2 {err:.3e} "
+          f"P {err_p:.3e}", flush=True)
+    assert (st.accepted, st.rejected) == (acc, rej)
+    assert err < 1e-7 and err_p < 1e-7
+
+
 @pytest.mark.parametrize("form", ["plain_f64", "fused_f64"])
 def test_debye_expmv_matches_oracle(ctx, oracle, form):
     """Krylov exp(tA)v of the augmented operator (the P block is ordinary

@@ -208,3 +208,21 @@ def test_learned_proposer_gain_on_the_graded_slab(ctx):
           f"gain {gain:.2f}x (target 1.5x)", flush=True)
     assert res["plain"][1] < tol and res["learned"][1] < tol
     assert gain > 1.4
+
+
+def test_zero_state_and_a_horizon_below_one_step(ctx):
+    """A zero state propagates to zero in no steps; a horizon far below
+    what the estimate admits is covered by one step of exactly that length,
+    whose result is the state to first order."""
+    import rapidfem_amd as R
+    from rapidfem_amd.proposer import ExpController
+
+    op, y0 = _slab_small()
+    g = R.GpuOperator(ctx, op, "fused_f64")
+    cfl = R.TdSession(op, 1.0, "fused_f64", ctx).cfl_dt()
+    out, steps, _ = g.expmv_adaptive(ctx, np.zeros_like(y0), 10 * cfl, ExpController(tol=1e-8, m=12))
+    assert len(steps) == 0 and not np.any(out)
+    # a horizon far below one step: one step of the full length, exp(hA)v ~ v
+    out, steps, _ = g.expmv_adaptive(ctx, y0, 1e-6 * cfl, ExpController(tol=1e-8, m=12))
+    assert len(steps) == 1 and abs(steps["h"][0] - 1e-6 * cfl) <= 1e-12 * cfl
+    assert rel_l2(out, y0) < 1e-5
