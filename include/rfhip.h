@@ -268,7 +268,8 @@ int rfhip_op_etd_step(rfhip_op *op, const double *y, const double *b,
  *   est(h, m) = beta * H[m][m-1] * |[exp(h H_m)]_{m-1,0}|,
  * accepted when est <= tol * |v| * h / t_end (so the estimates of a run add
  * up to at most tol * |v|). Hessenberg exponentials: degree-13 Padé with
- * scaling and squaring, FP64; Arnoldi: CGS2 on the device, FP64. */
+ * scaling and squaring, FP64; Arnoldi: classical Gram-Schmidt with delayed
+ * reorthogonalisation (DCGS2) on the device, FP64. */
 typedef struct rfhip_mlp rfhip_mlp;
 
 typedef struct {
