@@ -240,9 +240,16 @@ int rfhip_op_transient_kcl_f64(rfhip_op *op, const double *y0, double dt,
  * stage update. RFHIP_KCL_REG_COMPRESSED (forms with compressed state
  * registers only, else RFHIP_EUNSUPPORTED): y and its backup as f32 hi +
  * bf16 lo, err as f32, the stage input FP64, 36 bytes; the controller and
- * every apply still in FP64. Applies to later transient_kcl calls. */
+ * every apply still in FP64. On an operator without Debye elements whose
+ * mesh has the neighbour-trace table (conforming) and whose form keeps the
+ * volume in FP64, the update runs in the stage kernel's epilogue, so k is
+ * never stored (one launch per stage instead of two).
+ * RFHIP_KCL_REG_COMPRESSED_SEPARATE: the same registers with the update
+ * always in a pass of its own after each apply. Applies to later
+ * transient_kcl calls. */
 #define RFHIP_KCL_REG_F64 0
 #define RFHIP_KCL_REG_COMPRESSED 1
+#define RFHIP_KCL_REG_COMPRESSED_SEPARATE 2
 int rfhip_op_set_kcl_registers(rfhip_op *op, int32_t mode);
 
 /* GpuOperator::expmv (gpu/operator.rs:1175-1197): out = exp(t*A)*v, FP64

@@ -611,7 +611,14 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
   // compressed registers: y and its backup as f32 hi + bf16 lo ([hi | lo]
   // in one buffer), err as f32; every stage's apply reads st in FP64, the
   // last stage leaving the accepted y there for the next substep
-  const bool cmp = op->kcl_reg == 1 && form_compressed(op->form);
+  const bool cmp = op->kcl_reg != RFHIP_KCL_REG_F64 && form_compressed(op->form);
+  // with them, the stage kernel does the register update in its epilogue
+  // where it can (no Debye P block, which the stage kernel does not
+  // update; conforming mesh): k is never written, and the stage input
+  // alternates between st and the k buffer (neighbours still read a
+  // stage's input while its next one is written)
+  const bool fuse = cmp && op->kcl_reg == RFHIP_KCL_REG_COMPRESSED && op->n_disp == 0 &&
+                    n == 6 * (int64_t)op->np * op->n_elem && kcl_stage_fusable(op->form, op->sa);
   const size_t w = f64 ? 8 : 4;
   hipStream_t s = op->stream;
   op->kst.alloc(w * n);
@@ -630,9 +637,17 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
     launch_copy(op->kst.p, ys, 8 * n, s);
   }
   const int nblk = weighted_err_blocks(n);
-  op->red.alloc(8 * (nblk + 1));
+  const int64_t nslot = fuse ? kcl_stage_slots(op->order, op->sa) : 0;
+  const int64_t npart = std::max<int64_t>(nblk, nslot);
+  op->red.alloc(8 * (npart + 1));
   double *part = op->red.as<double>();
-  double *res = part + nblk;
+  double *res = part + npart;
+  double *stb[2] = {op->kst.as<double>(), nullptr};
+  int sc = 0;  // stb[sc]: the next stage's input
+  if (fuse) {
+    op->k.alloc(8 * n);
+    stb[1] = op->k.as<double>();
+  }
   double h = dt, prev = 0.0, hmin = INFINITY, hmax = 0.0;
   int64_t acc = 0, rej = 0;
   const double h_floor = ctl.min_step_factor * dt;
@@ -666,7 +681,30 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
       // last stage, builds the next stage's input y + (a - b) f. The first
       // pass also saves the starting y (kbak); the last one reduces the
       // error norm (session.rs:160-171) against it.
-      for (int sg = 0; sg < 5; ++sg) {
+      for (int sg = 0; fuse && sg < 5; ++sg) {
+        StageArgs a = op->sa;
+        a.yin = stb[sc];
+        a.yout = stb[1 - sc];
+        fill_source(op, a, kind, dof, gv);
+        a.ky = yh;
+        a.ky_lo = yl;
+        a.kerr = op->kerr.as<float>();
+        a.kbak = bh;
+        a.kbak_lo = bl;
+        a.dt = ht;
+        a.kb = kKclB[sg];
+        a.kew = kKclBhat[sg] - kKclB[sg];
+        a.kamb = sg < 4 ? kKclA[sg] - kKclB[sg] : 0.0;
+        a.katol = ctl.atol;
+        a.krtol = ctl.rtol;
+        a.kfirst = sg == 0;
+        a.klast = sg == 4;
+        a.kpart = part;
+        launch_kcl_stage(op->order, a, s);
+        sc = 1 - sc;
+        if (sg == 4) launch_sum_partials(part, (int)nslot, res, s);
+      }
+      for (int sg = 0; !fuse && sg < 5; ++sg) {
         rhs_into(op, sg == 0 && !cmp ? ys : op->kst.p, op->k.p, kind, dof, gv);
         KclStage a{};
         a.y = cmp ? (void *)yh : ys;
@@ -708,7 +746,7 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
         ++rej;
         if (cmp) {
           launch_copy(op->kyc.p, op->kbak.p, (int64_t)creg_bytes(n), s);
-          launch_c_to_f64(bh, bl, op->kst.as<double>(), n, s);
+          launch_c_to_f64(bh, bl, stb[sc], n, s);
         } else {
           launch_copy(ys, op->kbak.p, (int64_t)(w * n), s);
         }
@@ -1710,8 +1748,10 @@ int rfhip_op_transient_kcl(rfhip_op *op, const float *y0, float dt, int64_t step
 int rfhip_op_set_kcl_registers(rfhip_op *op, int32_t mode) {
   return guarded([&] {
     RF_REQUIRE(op, "null operator");
-    RF_REQUIRE(mode == RFHIP_KCL_REG_F64 || mode == RFHIP_KCL_REG_COMPRESSED, "unknown KCL register mode");
-    if (mode == RFHIP_KCL_REG_COMPRESSED && !form_compressed(op->form))
+    RF_REQUIRE(mode == RFHIP_KCL_REG_F64 || mode == RFHIP_KCL_REG_COMPRESSED ||
+                   mode == RFHIP_KCL_REG_COMPRESSED_SEPARATE,
+               "unknown KCL register mode");
+    if (mode != RFHIP_KCL_REG_F64 && !form_compressed(op->form))
       throw Error{RFHIP_EUNSUPPORTED, "compressed KCL registers need a form with compressed state registers"};
     op->kcl_reg = mode;
   });

@@ -16,18 +16,13 @@
 //    sb dt k_d to y_E', reading E and P from the stage's input buffer (the
 //    stage writes the other one), and steps P itself.
 #include "rfhip_impl.hpp"
+#include "rfhip_regs.hpp"
 
 namespace rfhip {
 namespace {
 
 constexpr int kDB = 256;
 
-__device__ __forceinline__ uint16_t bf16_rne(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // Register formats: 0 = f64, 1 = f32, 2 = f32 hi + bf16 lo.
 template <int FMT>

@@ -33,18 +33,13 @@
 
 #include "rfhip_elem_tables.hpp"
 #include "rfhip_impl.hpp"
+#include "rfhip_regs.hpp"
 
 namespace rfhip {
 namespace {
 
 constexpr int kFusedWG = 256;
 
-__device__ __forceinline__ uint16_t bf16_rne(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 // Two floats to bf16 (round to nearest even, as bf16_rne for finite values),
 // a in the low half: one v_cvt_pk_bf16_f32 on gfx950.
 __device__ __forceinline__ uint32_t bf16_pair(float a, float b) {
@@ -93,8 +88,91 @@ __device__ __forceinline__ void load_trace(const StageArgs &io, int64_t nb, doub
 }
 
 // EPI: 0 = apply (dy = A y, FP64), 1 = LSERK stage with FP64 registers,
-// 2 = LSERK stage with compressed registers (y f32+bf16, p f32).
+// 2 = LSERK stage with compressed registers (y f32+bf16, p f32), 3 = KCL
+// stage with compressed registers (k_fused11 only, node_kcl).
 //
+// KCL stage update of one node with compressed registers
+// (explicit_adaptive.rs:107-149), the same arithmetic as the separate pass
+// k_kcl_update_c: f = h k, y += b f, err = e f on the first stage and
+// err += e f after it, the next stage's input y + (a - b) f (y itself after
+// the last stage) into yout. The first stage saves the starting y (already
+// in registers) to the backup; the last reads the backup and adds the
+// node's error-norm terms (session.rs:160-171) to wsum. own: the stage
+// input, yh / yl / ev: the node's y (f32 hi, bf16 lo pairs) and err.
+__device__ __forceinline__ void node_kcl(const StageArgs &io, int64_t base, const double *mt, const double *acc,
+                                         const double *own, const double *srcv, const float *yh,
+                                         const uint32_t *yl, const float *ev, double &wsum) {
+  // the node's right-hand side: materials and the held source
+  // (rhs.rs:1443-1454), written out here as in node_update (a helper shared
+  // by the two, with an output array, raised the N=3 stage kernels' spilled
+  // VGPRs from 10 to 44-62)
+  double kv[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    kv[c] = mt[c] * acc[c] - mt[6 + c] * own[c];
+    kv[3 + c] = mt[3 + c] * acc[3 + c] - mt[9 + c] * own[3 + c];
+  }
+  if (io.src_kind == RFHIP_DRIVE_POINT) {
+    const int64_t off = io.src_dof - base;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (off == c) kv[c] += io.g;
+  } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) kv[c] += srcv[c] * io.g;
+  }
+  float hn[6], rn[6], en[6];
+  double sn[6];
+  float bh[6];
+  uint32_t bl[3];
+  if (io.klast) {
+    const float2 *bp = reinterpret_cast<const float2 *>(io.kbak + base);
+    const uint32_t *lp = reinterpret_cast<const uint32_t *>(io.kbak_lo + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float2 v = bp[q];
+      bh[2 * q] = v.x;
+      bh[2 * q + 1] = v.y;
+      bl[q] = lp[q];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const uint32_t lw = c & 1 ? yl[c >> 1] & 0xFFFF0000u : yl[c >> 1] << 16;
+    const double f = io.dt * kv[c];
+    const double yn = ((double)yh[c] + (double)__uint_as_float(lw)) + io.kb * f;
+    hn[c] = (float)yn;
+    rn[c] = (float)(yn - (double)hn[c]);
+    const double e = io.kfirst ? io.kew * f : (double)ev[c] + io.kew * f;
+    en[c] = (float)e;
+    sn[c] = io.klast ? yn : yn + io.kamb * f;
+    if (io.klast) {
+      const uint32_t bw = c & 1 ? bl[c >> 1] & 0xFFFF0000u : bl[c >> 1] << 16;
+      wsum += werr_term((double)bh[c] + (double)__uint_as_float(bw), yn, e, io.katol, io.krtol);
+    }
+  }
+  float2 *yo = reinterpret_cast<float2 *>(io.ky + base);
+  uint32_t *lo = reinterpret_cast<uint32_t *>(io.ky_lo + base);
+  float2 *eo = reinterpret_cast<float2 *>(io.kerr + base);
+  double2 *so = reinterpret_cast<double2 *>(static_cast<double *>(io.yout) + base);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    yo[q] = make_float2(hn[2 * q], hn[2 * q + 1]);
+    lo[q] = bf16_pair(rn[2 * q], rn[2 * q + 1]);
+    eo[q] = make_float2(en[2 * q], en[2 * q + 1]);
+    so[q] = make_double2(sn[2 * q], sn[2 * q + 1]);
+  }
+  if (io.kfirst) {
+    float2 *bo = reinterpret_cast<float2 *>(io.kbak + base);
+    uint32_t *blo = reinterpret_cast<uint32_t *>(io.kbak_lo + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      bo[q] = make_float2(yh[2 * q], yh[2 * q + 1]);
+      blo[q] = yl[q];
+    }
+  }
+}
+
 // Stage update with the old p already in registers (prefetched by the
 // kernels that use it): materials, held source, p and y.
 template <int EPI>
@@ -1439,10 +1517,18 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(V
       }
     }
   };
-  // Waves without a group at all (the last block) just leave; the last
-  // group of a wave reads itself ahead again, which keeps every read-ahead
-  // load unconditional.
-  if (gw0 >= ngroups) return;
+  // KCL stage (EPI 3): the wave's sum of the error-norm terms over its
+  // nodes, left in io.kpart[wave of the grid] by the last stage.
+  const int64_t wslot = (int64_t)blockIdx.x * NWB + wave;
+  double wsum = 0.0;
+  // Waves without a group at all (the last block) just leave (with a zero
+  // partial); the last group of a wave reads itself ahead again, which
+  // keeps every read-ahead load unconditional.
+  if (gw0 >= ngroups) {
+    if constexpr (EPI == 3)
+      if (io.klast && (threadIdx.x & 63) == 0) io.kpart[wslot] = 0.0;
+    return;
+  }
   read_ahead(gw0);
   {
     const int64_t r0 = io.n_elem - gw0 * EPB;
@@ -1637,12 +1723,30 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(V
     for (int q = 0; q < 9; ++q) asm volatile("" ::"v"(ji[q]));
 #pragma unroll
     for (int s = 0; s < FPT; ++s) touch_trace<CIN>(nr[s]);
-    if constexpr (EPI != 0) {
+    if constexpr (EPI == 1 || EPI == 2) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) asm volatile("" ::"v"(pold[c]));
     }
 #pragma unroll
     for (int k = 0; k < MR; ++k) asm volatile("" ::"v"(mrow[k]));
+    // KCL registers of the lane's node (every lane, clamped like the other
+    // group loads), requested here so the LIFT covers their latency
+    float kyh[6], kev[6];
+    uint32_t kyl[3];
+    if constexpr (EPI == 3) {
+      const float2 *yp = reinterpret_cast<const float2 *>(io.ky + base);
+      const uint32_t *lp = reinterpret_cast<const uint32_t *>(io.ky_lo + base);
+      const float2 *ep = reinterpret_cast<const float2 *>(io.kerr + base);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float2 a = yp[q], b = ep[q];
+        kyh[2 * q] = a.x;
+        kyh[2 * q + 1] = a.y;
+        kev[2 * q] = b.x;
+        kev[2 * q + 1] = b.y;
+        kyl[q] = lp[q];
+      }
+    }
     double own[6];
     if (active) {
       const double *fb = sf + le * SFS;
@@ -1669,15 +1773,34 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(V
 #pragma unroll
     for (int s = 0; s < FPT; ++s) asm volatile("" : "+v"(nbn[s]));
     asm volatile("" : "+v"(mnext));
-    if (active) {
-      double po[6], srcv[6];
+    if constexpr (EPI == 3) {
 #pragma unroll
-      for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
+      for (int c = 0; c < 6; ++c) asm volatile("" : "+v"(kyh[c]), "+v"(kev[c]));
+#pragma unroll
+      for (int q = 0; q < 3; ++q) asm volatile("" : "+v"(kyl[q]));
+    }
+    if (active) {
+      double srcv[6];
       if (io.src_kind == RFHIP_DRIVE_VECTOR) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
       }
-      node_update<EPI>(io, base, sm + le * 12, acc, own, po, srcv);
+      if constexpr (EPI == 3) {
+        node_kcl(io, base, sm + le * 12, acc, own, srcv, kyh, kyl, kev, wsum);
+      } else {
+        double po[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) po[c] = (double)pold[c];
+        node_update<EPI>(io, base, sm + le * 12, acc, own, po, srcv);
+      }
+    }
+  }
+  if constexpr (EPI == 3) {
+    if (io.klast) {
+      // fixed butterfly order: the same sum for the same grid
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o, 64);
+      if ((threadIdx.x & 63) == 0) io.kpart[wslot] = wsum;
     }
   }
 }
@@ -2300,6 +2423,35 @@ bool variant_available(int order, int form, int variant, const StageArgs &a) {
 
 namespace {
 template <int NP>
+const void *kcl_kernel(int variant) {
+  constexpr int F = Nfp<NP>::v;
+  return variant == 6 ? reinterpret_cast<const void *>(&k_fused11<NP, F, false, 3, 1>)
+                      : reinterpret_cast<const void *>(&k_fused11<NP, F, false, 3, 2>);
+}
+
+// Blocks of the KCL stage launch: one wave per group of 64/Np elements,
+// four waves per block, at most the resident blocks (persistent grid).
+template <int NP>
+int64_t kcl_grid(const StageArgs &a) {
+  const int64_t epw = 64 / NP;
+  const int64_t wgroups = (a.n_elem + epw - 1) / epw;
+  const int64_t groups = (wgroups + 3) / 4;
+  const int64_t res = resident_blocks(kcl_kernel<NP>(a.variant));
+  return groups < res ? groups : res;
+}
+
+template <int NP>
+void kcl_np(const StageArgs &a, hipStream_t s) {
+  constexpr int F = Nfp<NP>::v;
+  const int64_t grid = kcl_grid<NP>(a);
+  if (grid <= 0) return;
+  if (a.variant == 6)
+    hipLaunchKernelGGL((k_fused11<NP, F, false, 3, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_fused11<NP, F, false, 3, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+}
+
+template <int NP>
 bool clane_pack_np(const int *face_nodes, const double *const D[3], const double *lift, std::vector<double> &out) {
   using T = ElemTab<NP>;
   constexpr int NFP = T::NFP;
@@ -2324,6 +2476,31 @@ bool clane_pack_np(const int *face_nodes, const double *const D[3], const double
   return true;
 }
 }  // namespace
+
+bool kcl_stage_fusable(int form, const StageArgs &a) { return !form_vmf(form) && a.nbase != nullptr; }
+
+int64_t kcl_stage_slots(int order, const StageArgs &a) {
+  switch ((order + 1) * (order + 2) * (order + 3) / 6) {
+    case 4: return kcl_grid<4>(a) * (kFusedWG / 64);
+    case 10: return kcl_grid<10>(a) * (kFusedWG / 64);
+    case 20: return kcl_grid<20>(a) * (kFusedWG / 64);
+    case 35: return kcl_grid<35>(a) * (kFusedWG / 64);
+    default: throw Error{RFHIP_EUNSUPPORTED, "the fused stage kernels run orders 1 to 4"};
+  }
+}
+
+void launch_kcl_stage(int order, const StageArgs &a, hipStream_t s) {
+  RF_REQUIRE(a.nbase != nullptr, "the fused KCL stage needs the neighbour-trace table");
+  RF_REQUIRE(!a.klast || a.kpart != nullptr, "the last KCL stage needs the error partials");
+  switch ((order + 1) * (order + 2) * (order + 3) / 6) {
+    case 4: kcl_np<4>(a, s); break;
+    case 10: kcl_np<10>(a, s); break;
+    case 20: kcl_np<20>(a, s); break;
+    case 35: kcl_np<35>(a, s); break;
+    default: throw Error{RFHIP_EUNSUPPORTED, "the fused stage kernels run orders 1 to 4"};
+  }
+  RF_HIP(hipGetLastError());
+}
 
 bool clane_pack(int np, const int *face_nodes, const double *const D[3], const double *lift, std::vector<double> &out) {
   switch (np) {

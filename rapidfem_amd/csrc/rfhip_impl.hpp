@@ -165,6 +165,15 @@ struct StageArgs {
   int64_t src_dof;
   const double *src_vec;
   double g;
+  // KCL stage with compressed registers (launch_kcl_stage): yin is the
+  // stage input (FP64) and yout the next one; y and the substep backup as
+  // f32 hi + bf16 lo, err f32.
+  float *ky, *kerr, *kbak;
+  uint16_t *ky_lo, *kbak_lo;
+  double kb, kew, kamb;   // b_s, bhat_s - b_s, a_s - b_s (0 on the last stage)
+  double katol, krtol;
+  int kfirst, klast;
+  double *kpart;          // last stage: one partial sum of the error norm per wave of the grid
 };
 
 }  // namespace rfhip
@@ -192,7 +201,7 @@ struct rfhip_op {
   // Debye dispersive block: slot -> element and (a, g, 1/eps_inf); the P
   // block follows the 6 Np K values of [E,H] in every state vector
   int64_t n_disp = 0;
-  int kcl_reg = 0;  // KCL registers: 0 FP64, 1 compressed (rfhip_op_set_kcl_registers)
+  int kcl_reg = 0;  // KCL registers: RFHIP_KCL_REG_* (rfhip_op_set_kcl_registers)
   rfhip::DevBuf disp_elem, disp_coef;
   int order = 0;
   // state registers (precision depends on form)
@@ -266,6 +275,15 @@ void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t 
 // data (some need the neighbour-trace table or Np <= 20; launch_stage runs
 // the base variant for them otherwise).
 bool variant_available(int order, int form, int variant, const StageArgs &a);
+// One KCL stage with the register update in the stage kernel's epilogue
+// (k_fused11, CIN off, EPI 3): the apply of a.yin, then the compressed
+// register update and the next stage's input into a.yout. Needs the
+// neighbour-trace table and a form without the matrix-core volume; runs
+// variant 6 when a.variant is 6 and variant 7 otherwise. kcl_stage_slots:
+// the number of partial sums the last stage leaves in a.kpart.
+bool kcl_stage_fusable(int form, const StageArgs &a);
+int64_t kcl_stage_slots(int order, const StageArgs &a);
+void launch_kcl_stage(int order, const StageArgs &a, hipStream_t s);
 // The table of the component-lane stage kernel (variant 8): the nonzeros of
 // D_r, D_s, D_t (row-major D_k[i][j], the order of (i, j, k)) and then LIFT
 // ((face, node, face node) order). False when the operator's face nodes or
@@ -338,6 +356,8 @@ void launch_kcl_update(bool f64, const KclStage &a, hipStream_t s);
 void launch_kcl_update_c(const KclStage &a, hipStream_t s);
 void launch_copy(void *dst, const void *src, int64_t nbytes, hipStream_t s);
 int weighted_err_blocks(int64_t n);
+// out[0] = the sum of part[0 .. n) in a fixed order (one block).
+void launch_sum_partials(const double *part, int n, double *out, hipStream_t s);
 
 // Krylov (FP64).
 int krylov_dot_blocks(int64_t n);

@@ -7,6 +7,7 @@
 #include <type_traits>
 
 #include "rfhip_impl.hpp"
+#include "rfhip_regs.hpp"
 
 namespace rfhip {
 namespace {
@@ -20,12 +21,6 @@ inline unsigned grid_for(int64_t n, int64_t cap = 8192) {
   return (unsigned)g;
 }
 
-__device__ __forceinline__ uint16_t bf16_rne(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __global__ void __launch_bounds__(kBS) k_f64_to_c(const double *__restrict__ s, float *__restrict__ hi,
                                                   uint16_t *__restrict__ lo, int64_t n) {
@@ -120,14 +115,6 @@ __global__ void __launch_bounds__(kBS) k_copy(uint32_t *__restrict__ d, const ui
 // ---- KCL RK4(3)5[2R+]C (explicit_adaptive.rs:84-152) -------------------
 
 __device__ __forceinline__ double block_sum(double v, double *sh);
-
-// One entry of the controller's error norm (session.rs:160-171):
-// (err / (atol + rtol max(|y_old|, |y_new|)))^2, +inf for a non-finite y_new.
-__device__ __forceinline__ double werr_term(double yo, double yn, double e, double atol, double rtol) {
-  if (!isfinite(yn)) return INFINITY;
-  const double r = e / (atol + rtol * fmax(fabs(yo), fabs(yn)));
-  return r * r;
-}
 
 // After stage s of a KCL substep: f = h k, y += b f, err = e f on the first
 // stage and err += e f after it, and, when NEXT, the next stage's input
@@ -767,6 +754,11 @@ void launch_kcl_update(bool f64, const KclStage &a, hipStream_t s) {
 }
 
 int weighted_err_blocks(int64_t n) { return (int)grid_for(n, 2048); }
+
+void launch_sum_partials(const double *part, int n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, part, n, out);
+  RF_HIP(hipGetLastError());
+}
 
 
 // 1024 blocks: k_multi_dot streams 5.7 TB/s over 1024 blocks and 5.3 over

@@ -205,6 +205,65 @@ def test_kcl_on_stage_kernel_variants_matches_oracle(ctx, oracle, order, variant
     assert err < 1e-11
 
 
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [5, 6, 7])
+@pytest.mark.parametrize("drive", ["free", "vector"])
+def test_kcl_fused_update_matches_separate_pass(ctx, oracle, order, variant, drive):
+    """Compressed KCL registers with the update in the stage kernel's
+    epilogue (no k stored; the stage input alternates between two buffers;
+    the error norm summed per wave) against the same registers with the
+    update as a pass of its own after each apply (mode
+    "compressed_separate"), on the operator's variant (5: the fused update
+    runs on variant 7's kernel): the same accept/reject sequence and step
+    range, the same states to the registers' rounding, and the oracle's
+    counts and state within that rounding. The tight
+    controller forces rejections, whose restore rebuilds the stage input in
+    whichever buffer the next substep reads."""
+    from rapidfem_amd import Controller, Drive
+
+    oop, y0 = _te101(oracle, 3, order)
+    n = oop.n_dof
+    g = _gop(ctx, oop, "fused_f64c")
+    g.set_variant(variant)
+    kw, okw = {}, {}
+    if drive == "vector":
+        b = np.cos(0.37 * np.arange(n))
+        wave = np.array([0.4, -0.3, 0.2])
+        kw = dict(drive=Drive.vector(b), g=wave)
+        okw = dict(drive=("vector", b), g=wave)
+    ctl = Controller(atol=1e-9, rtol=1e-7)
+    dt, steps = 0.05, 3
+    runs = {}
+    for mode in ["compressed_separate", "compressed"]:
+        g.set_kcl_registers(mode)
+        g.poison_lds()
+        runs[mode] = g.transient_kcl(ctx, y0, dt, steps, ctl=ctl, precision="f64", **kw)
+    (ts, ss), (tf, sf) = runs["compressed_separate"], runs["compressed"]
+    octl = oracle.Controller.default()
+    octl.atol, octl.rtol = 1e-9, 1e-7
+    y, acc, rej, _, _ = oop.adaptive(y0, dt, steps, octl, **okw)
+    d = max(rel_l2(tf[k], ts[k]) for k in range(1, steps + 1))
+    err = rel_l2(tf[-1], y)
+    print(f"kcl fused update order={order} variant={variant} drive={drive}: fused {sf.accepted}/{sf.rejected}, "
+          f"separate {ss.accepted}/{ss.rejected}, oracle {acc}/{rej}; fused vs separate {d:.2e}, vs oracle {err:.2e}",
+          flush=True)
+    assert (sf.accepted, sf.rejected) == (ss.accepted, ss.rejected)
+    # the registers' rounding can move a decision near the controller's
+    # threshold against the FP64 oracle (N=4, vector drive: 193/5 against
+    # 196/4), for both paths alike
+    assert abs(sf.accepted - acc) <= 4 and abs(sf.rejected - rej) <= 2
+    assert sf.rejected > 0
+    # the two paths sum the error norm in different orders; a difference in
+    # the last bit of h moves some compressed registers across a rounding
+    # boundary of their bf16 part (2^-32 relative), and at rtol 1e-7 the
+    # embedded error estimate sees that rounding at about 1e-3 of its size,
+    # so the step sequences drift apart (by 2.4e-3 in h_min over 127
+    # substeps at N=4), the states by the integration error that makes
+    assert abs(sf.h_min - ss.h_min) <= 1e-2 * ss.h_min and abs(sf.h_max - ss.h_max) <= 1e-2 * ss.h_max
+    assert d < 1e-8
+    assert err < 1e-7
+
+
 def test_kcl_collapse_is_an_error(ctx, oracle):
     """A step collapse is reported (session.rs:817-824), not a hang."""
     from rapidfem_amd import Controller, RfhipError
