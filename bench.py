@@ -301,12 +301,27 @@ def proposer_bench(args) -> int:
                      "h_over_cfl": [round(float(h / cfl), 2) for h in st["h"][:40]]}
         log(f"  {name}: {len(st)} steps, {int(st['m'].sum())} applies, device {g.last_call_ms:.0f} ms, "
             f"rel L2 vs fine {err:.2e}")
+    # the same two runs with the finished basis columns stored as f32 + bf16
+    g.set_krylov_basis("compressed")
+    comp = {}
+    for name, p in (("plain", None), ("learned", prop)):
+        ctl = ExpController(tol=tol, m=40, m_min=10, m_max=60, h_scale=cfl)
+        g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=p, log_cap=4096)  # warm-up
+        y, st, _ = g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=p, log_cap=4096)
+        err = float(np.linalg.norm(y - yf) / np.linalg.norm(yf))
+        comp[name] = {"device_ms": g.last_call_ms, "steps": int(len(st)), "applies": int(st["m"].sum()),
+                      "m_mean": float(st["m"].mean()), "rel_l2_vs_fine": err,
+                      "speedup_vs_f64_basis": round(res[name]["device_ms"] / g.last_call_ms, 3)}
+        log(f"  {name}, compressed basis: {len(st)} steps, {int(st['m'].sum())} applies, device "
+            f"{g.last_call_ms:.0f} ms ({comp[name]['speedup_vs_f64_basis']:.2f}x the FP64 basis), "
+            f"rel L2 vs fine {err:.2e}")
+    g.set_krylov_basis("f64")
     gain = res["plain"]["device_ms"] / res["learned"]["device_ms"]
     out = {"metric": "Krylov-exp wall time, learned (h, m) proposer vs plain controller at equal tolerance",
            "value": round(gain, 3), "unit": "x (plain ms / learned ms)", "n_gpus": 1, "higher_is_better": True,
            "dtype": "f64 (Arnoldi, CGS2, Pade-13), f32 network", "data": "synthetic", "tol": tol,
            "plain": res["plain"], "learned": res["learned"], "fine": {"steps": int(len(sf)), "tol": tol * 1e-3},
-           "cost_model_ms": cost, "plain_fixed_m": sweep,
+           "compressed_basis": comp, "cost_model_ms": cost, "plain_fixed_m": sweep,
            # the plain controller at the dimension that was best in hindsight on this workload
            "vs_best_fixed_m": (None if not sweep else
                                {"m": min(sweep, key=lambda m: sweep[m]["device_ms"]),

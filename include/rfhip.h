@@ -328,6 +328,19 @@ int rfhip_op_expmv_adaptive(rfhip_op *op, const double *v, double t_end,
                             double *hess, int32_t hess_dim, int64_t *n_steps,
                             double *out);
 
+/* Basis storage of the adaptive exponential (no counterpart in the
+ * reference; for callers that trade the basis' precision for HBM traffic).
+ * RFHIP_KRYLOV_BASIS_F64 (default): every column FP64.
+ * RFHIP_KRYLOV_BASIS_COMPRESSED: the finished columns as f32 hi + bf16 lo
+ * (6 bytes per entry, about 2^-32 relative rounding), the state, the
+ * provisional vector and the operator product FP64; the Gram and update
+ * sweeps, which read every finished column, move a quarter fewer bytes.
+ * Meant for tolerances well above 1e-9. Applies to later
+ * rfhip_op_expmv_adaptive calls on op. */
+#define RFHIP_KRYLOV_BASIS_F64 0
+#define RFHIP_KRYLOV_BASIS_COMPRESSED 1
+int rfhip_op_set_krylov_basis(rfhip_op *op, int32_t mode);
+
 /* Hindsight labels for the proposer (host computation, nthreads threads):
  * for each of count logged steps (Hessenberg blocks as written by
  * rfhip_op_expmv_adaptive with hess_dim = m_log, dim_built[q] columns

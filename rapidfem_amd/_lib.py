@@ -104,6 +104,7 @@ SIGNATURES = {
     "rfhip_op_transient_kcl": (C.c_int, [_vp, _fp, C.c_float, _i64, C.c_int, _i64, _vp, _vp, _i64,
                                          C.POINTER(ControllerC), _fp, C.POINTER(KclStatsC)]),
     "rfhip_op_set_kcl_registers": (C.c_int, [_vp, C.c_int32]),
+    "rfhip_op_set_krylov_basis": (C.c_int, [_vp, C.c_int32]),
     "rfhip_op_transient_kcl_f64": (C.c_int, [_vp, _dp, _d, _i64, C.c_int, _i64, _vp, _vp, _i64,
                                              C.POINTER(ControllerC), _dp, C.POINTER(KclStatsC)]),
     "rfhip_op_expmv": (C.c_int, [_vp, _dp, _d, _i32, _d, _dp, C.POINTER(_i32)]),

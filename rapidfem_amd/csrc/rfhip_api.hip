@@ -934,9 +934,19 @@ struct Arnoldi {
   bool broke = false, pending = false;
   double beta = 0.0;
   std::vector<double> H;
+  // compressed basis (RFHIP_KRYLOV_BASIS_COMPRESSED): the finished columns
+  // as f32 hi [cap][n] + bf16 lo [cap][nl] in kb_cbasis; the provisional
+  // vector (which starts as the state) and the product A q in kb_basis
+  bool cb = false;
+  int64_t nl = 0;
 
   Arnoldi(rfhip_op *o, int capacity) : op(o), n(nvec(o)), cap(capacity) {
-    if (op->kb_cap < cap) {
+    cb = op->kb_mode == RFHIP_KRYLOV_BASIS_COMPRESSED;
+    if (cb) {
+      nl = (n + 1) & ~int64_t(1);
+      op->kb_cbasis.alloc((size_t)cap * (4 * (size_t)n + 2 * (size_t)nl));
+      op->kb_basis.alloc(16 * (size_t)n);
+    } else if (op->kb_cap < cap) {
       op->kb_basis.alloc(8 * (size_t)(cap + 1) * n);
       op->kb_cap = cap;
     }
@@ -952,6 +962,14 @@ struct Arnoldi {
   }
   double *basis() const { return op->kb_basis.as<double>(); }
   double *col(int c) const { return basis() + (int64_t)c * n; }
+  float *chi() const { return op->kb_cbasis.as<float>(); }
+  uint16_t *clo() const { return reinterpret_cast<uint16_t *>(chi() + (int64_t)cap * n); }
+  // the FP64 state: each step's basis starts from it and its result goes
+  // back there (column 0, or the provisional vector of the compressed basis)
+  double *state() const { return col(0); }
+  // the provisional vector q_j and the product A q_j of column j
+  double *qv(int j) const { return cb ? col(0) : col(j); }
+  double *zv(int j) const { return cb ? col(1) : col(j + 1); }
   double &h(int r, int c) { return H[(size_t)r * cap + c]; }
   // device / pinned-host regions: Gram results, update coefficients, |q|^2
   static constexpr int kGram = 0, kCoef = 2 * kMaxKrylov + 4, kNorm = 4 * kMaxKrylov + 8;
@@ -981,9 +999,12 @@ struct Arnoldi {
     hipStream_t s = op->stream;
     const int j = cols;
     double *dev = op->kb_proj.as<double>(), *hp = op->kb_host.as<double>(), *part = op->kb_part.as<double>();
-    apply(col(j), col(j + 1));
-    launch_gram2(basis(), n, j + 2, part, nblk, dev + kGram, s);
-    RF_HIP(hipMemcpyAsync(hp + kGram, dev + kGram, 8 * (size_t)(2 * (j + 2)), hipMemcpyDeviceToHost, s));
+    apply(qv(j), zv(j));
+    if (cb)
+      launch_gram2c(chi(), clo(), n, nl, j, qv(j), zv(j), part, nblk, dev + kGram, s);
+    else
+      launch_gram2(basis(), n, j + 2, part, nblk, dev + kGram, s);
+    RF_HIP(hipMemcpyAsync(hp + kGram, dev + kGram, 8 * (size_t)(2 * (j + 1)), hipMemcpyDeviceToHost, s));
     sync(op);
     const double *g = hp + kGram;  // g[2c] = b_c . q_j, g[2c + 1] = b_c . z
     const double alpha = g[2 * j], gamma = g[2 * j + 1];
@@ -1027,7 +1048,10 @@ struct Arnoldi {
     cf[2 * j] = cv[j] / r;
     cf[2 * j + 1] = 1.0 / r;
     RF_HIP(hipMemcpyAsync(dev + kCoef, cf, 8 * (size_t)(2 * j + 2), hipMemcpyHostToDevice, s));
-    launch_dcgs_update(basis(), n, j, dev + kCoef, part, nblk, dev + kNorm, s);
+    if (cb)
+      launch_dcgs_updatec(chi(), clo(), n, nl, j, dev + kCoef, qv(j), zv(j), part, nblk, dev + kNorm, s);
+    else
+      launch_dcgs_update(basis(), n, j, dev + kCoef, part, nblk, dev + kNorm, s);
     RF_HIP(hipMemcpyAsync(hp + kNorm, dev + kNorm, 8, hipMemcpyDeviceToHost, s));
     cols = j + 1;
   }
@@ -1048,9 +1072,17 @@ struct Arnoldi {
     hipStream_t s = op->stream;
     const int j = cols;
     double *dev = op->kb_proj.as<double>(), *hp = op->kb_host.as<double>();
-    launch_multi_dot(basis(), n, j + 1, col(j), op->kb_part.as<double>(), nblk, dev + kGram, s);
-    RF_HIP(hipMemcpyAsync(hp + kGram, dev + kGram, 8 * (size_t)(j + 1), hipMemcpyDeviceToHost, s));
-    sync(op);
+    if (cb) {
+      // v_c . q and |q|^2 as the q-column of the Gram sweep (z = q)
+      launch_gram2c(chi(), clo(), n, nl, j, qv(j), qv(j), op->kb_part.as<double>(), nblk, dev + kGram, s);
+      RF_HIP(hipMemcpyAsync(hp + kGram, dev + kGram, 8 * (size_t)(2 * (j + 1)), hipMemcpyDeviceToHost, s));
+      sync(op);
+      for (int c = 0; c <= j; ++c) hp[kGram + c] = hp[kGram + 2 * c];
+    } else {
+      launch_multi_dot(basis(), n, j + 1, col(j), op->kb_part.as<double>(), nblk, dev + kGram, s);
+      RF_HIP(hipMemcpyAsync(hp + kGram, dev + kGram, 8 * (size_t)(j + 1), hipMemcpyDeviceToHost, s));
+      sync(op);
+    }
     double r2 = hp[kGram + j];
     for (int c = 0; c < j; ++c) {
       h(c, j - 1) += hp[kGram + c];
@@ -1077,7 +1109,10 @@ struct Arnoldi {
     for (int i = 0; i < d; ++i) cf[i] = beta * ws.e[(size_t)i * d];
     double *dev = op->kb_proj.as<double>() + kCoef;
     RF_HIP(hipMemcpyAsync(dev, cf, 8 * (size_t)d, hipMemcpyHostToDevice, op->stream));
-    launch_lincomb_alias(basis(), n, d, dev, out, op->stream);
+    if (cb)
+      launch_lincombc(chi(), clo(), n, nl, d, dev, out, op->stream);
+    else
+      launch_lincomb_alias(basis(), n, d, dev, out, op->stream);
     sync(op);
   }
 };
@@ -1090,7 +1125,7 @@ void adaptive_impl(rfhip_op *op, const double *v_host, double t_end, const rfhip
                    rfhip_exp_step *log, int64_t log_cap, double *hess, int hess_dim, int64_t *n_steps, double *out) {
   const int cap = std::max(ctl.m, ctl.m_max);
   Arnoldi ar(op, cap);
-  double *x = ar.col(0);  // the state: each step's basis starts from it and its result goes back there
+  double *x = ar.state();
   h2d_layout_f64(op, v_host, x);
   sync(op);
   const auto t_begin = std::chrono::steady_clock::now();
@@ -1754,6 +1789,14 @@ int rfhip_op_set_kcl_registers(rfhip_op *op, int32_t mode) {
     if (mode != RFHIP_KCL_REG_F64 && !form_compressed(op->form))
       throw Error{RFHIP_EUNSUPPORTED, "compressed KCL registers need a form with compressed state registers"};
     op->kcl_reg = mode;
+  });
+}
+
+int rfhip_op_set_krylov_basis(rfhip_op *op, int32_t mode) {
+  return guarded([&] {
+    RF_REQUIRE(op, "null operator");
+    RF_REQUIRE(mode == RFHIP_KRYLOV_BASIS_F64 || mode == RFHIP_KRYLOV_BASIS_COMPRESSED, "unknown Krylov basis mode");
+    op->kb_mode = mode;
   });
 }
 

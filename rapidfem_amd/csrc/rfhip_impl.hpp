@@ -202,6 +202,7 @@ struct rfhip_op {
   // block follows the 6 Np K values of [E,H] in every state vector
   int64_t n_disp = 0;
   int kcl_reg = 0;  // KCL registers: RFHIP_KCL_REG_* (rfhip_op_set_kcl_registers)
+  int kb_mode = 0;  // adaptive Arnoldi basis: RFHIP_KRYLOV_BASIS_* (rfhip_op_set_krylov_basis)
   rfhip::DevBuf disp_elem, disp_coef;
   int order = 0;
   // state registers (precision depends on form)
@@ -210,6 +211,7 @@ struct rfhip_op {
   // KCL / Krylov / scratch buffers
   rfhip::DevBuf kst, kerr, kbak, kyc, red, aos;
   rfhip::DevBuf kb_basis, kb_w, kb_proj, kb_part, kb_out, kb_x;
+  rfhip::DevBuf kb_cbasis;  // compressed finished columns of the adaptive Arnoldi
   rfhip::HostPinned kb_host;  // Krylov projections and norm, copied without a sync per pass
   rfhip::DevBuf probe_idx, probe_rows;  // device-side probe recording
   // port modal extraction: host-side per-port entries, device copies
@@ -373,6 +375,18 @@ void launch_gram2(const double *basis, int64_t n, int cols, double *partials, in
 void launch_dcgs_update(double *basis, int64_t n, int j, const double *coef, double *partials, int nblk,
                         double *norm2, hipStream_t s);
 void launch_lincomb_alias(const double *basis, int64_t n, int dim, const double *coef, double *out, hipStream_t s);
+// The same three on a compressed basis (rfhip_vec.hip): finished columns as
+// f32 hi[c n + i] + bf16 lo[c nl + i] (nl = n rounded up to even), the
+// provisional q and the product z FP64 vectors of their own. gram2c: out[2c
+// + r] = b_c . (q, z)_r for c <= j, b_j = q; dcgs_updatec: writes v_j as
+// column j and the next provisional vector into q, |q|^2 to norm2;
+// lincombc: out = sum_{c<dim} coef_c v_c.
+void launch_gram2c(const float *hi, const uint16_t *lo, int64_t n, int64_t nl, int j, const double *q,
+                   const double *z, double *partials, int nblk, double *out, hipStream_t s);
+void launch_dcgs_updatec(float *hi, uint16_t *lo, int64_t n, int64_t nl, int j, const double *coef, double *q,
+                         const double *z, double *partials, int nblk, double *norm2, hipStream_t s);
+void launch_lincombc(const float *hi, const uint16_t *lo, int64_t n, int64_t nl, int dim, const double *coef,
+                     double *out, hipStream_t s);
 // w -= basis[:, :cols] proj, then proj_out = basis[:, :cols]^T w (the updated
 // w): a CGS2 first-pass update fused with the second pass's projections.
 // proj_out may be proj: it is written after every block has read proj.

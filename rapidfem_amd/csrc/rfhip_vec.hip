@@ -522,6 +522,207 @@ __global__ void __launch_bounds__(kBS) k_lincomb_alias(const double *basis, int6
   }
 }
 
+// ---- the adaptive Arnoldi on a compressed basis ---------------------------
+//
+// Finished columns v_c held as f32 hi[c n + i] plus the bf16 of the
+// remainder lo[c nl + i] (nl = n rounded up to even, so each column's lo
+// part starts on a 4-byte word): 6 bytes per entry. The provisional vector
+// q and the product z = A q stay FP64 in buffers of their own. Each lane
+// takes V consecutive entries: V = 4 when n is a multiple of 4 (a 16-byte hi
+// and an 8-byte lo load per column, two 16-byte loads of q and z), else 2
+// (n even) or 1.
+template <int V>
+__device__ __forceinline__ void cb_load(const float *hi, const uint16_t *lo, int64_t n, int64_t nl, int c, int64_t i,
+                                        double (&b)[V]) {
+  if constexpr (V == 4) {
+    const float4 h = reinterpret_cast<const float4 *>(hi + (int64_t)c * n)[i];
+    const uint2 l = reinterpret_cast<const uint2 *>(lo + (int64_t)c * nl)[i];
+    b[0] = (double)h.x + (double)__uint_as_float(l.x << 16);
+    b[1] = (double)h.y + (double)__uint_as_float(l.x & 0xFFFF0000u);
+    b[2] = (double)h.z + (double)__uint_as_float(l.y << 16);
+    b[3] = (double)h.w + (double)__uint_as_float(l.y & 0xFFFF0000u);
+  } else if constexpr (V == 2) {
+    const float2 h = reinterpret_cast<const float2 *>(hi + (int64_t)c * n)[i];
+    const uint32_t l = reinterpret_cast<const uint32_t *>(lo + (int64_t)c * nl)[i];
+    b[0] = (double)h.x + (double)__uint_as_float(l << 16);
+    b[1] = (double)h.y + (double)__uint_as_float(l & 0xFFFF0000u);
+  } else {
+    b[0] = (double)hi[(int64_t)c * n + i] + (double)bf16_to_f(lo[(int64_t)c * nl + i]);
+  }
+}
+template <int V>
+__device__ __forceinline__ void cb_store(float *hi, uint16_t *lo, int64_t n, int64_t nl, int c, int64_t i,
+                                         const double (&v)[V]) {
+  float h[V];
+  uint32_t r[V];
+#pragma unroll
+  for (int e = 0; e < V; ++e) {
+    h[e] = (float)v[e];
+    r[e] = bf16_rne((float)(v[e] - (double)h[e]));
+  }
+  if constexpr (V == 4) {
+    reinterpret_cast<float4 *>(hi + (int64_t)c * n)[i] = make_float4(h[0], h[1], h[2], h[3]);
+    reinterpret_cast<uint2 *>(lo + (int64_t)c * nl)[i] = make_uint2(r[0] | (r[1] << 16), r[2] | (r[3] << 16));
+  } else if constexpr (V == 2) {
+    reinterpret_cast<float2 *>(hi + (int64_t)c * n)[i] = make_float2(h[0], h[1]);
+    reinterpret_cast<uint32_t *>(lo + (int64_t)c * nl)[i] = r[0] | (r[1] << 16);
+  } else {
+    hi[(int64_t)c * n + i] = h[0];
+    lo[(int64_t)c * nl + i] = (uint16_t)r[0];
+  }
+}
+template <int V>
+__device__ __forceinline__ void f64_load(const double *p, int64_t i, double (&v)[V]) {
+  if constexpr (V == 1) {
+    v[0] = p[i];
+  } else {
+#pragma unroll
+    for (int e = 0; e < V; e += 2) {
+      const double2 w = reinterpret_cast<const double2 *>(p)[i * (V / 2) + e / 2];
+      v[e] = w.x;
+      v[e + 1] = w.y;
+    }
+  }
+}
+template <int V>
+__device__ __forceinline__ void f64_store(double *p, int64_t i, const double (&v)[V]) {
+  if constexpr (V == 1) {
+    p[i] = v[0];
+  } else {
+#pragma unroll
+    for (int e = 0; e < V; e += 2) reinterpret_cast<double2 *>(p)[i * (V / 2) + e / 2] = make_double2(v[e], v[e + 1]);
+  }
+}
+
+// k_gram2 on the compressed basis: "columns" 0..j-1 are v_0..v_{j-1} and
+// column j is q; part[b*2(j+1) + 2c + r] = b_c . (q, z)_r over the block's
+// slice, so c = j gives |q|^2 and q . z.
+template <int V>
+__global__ void __launch_bounds__(kBS) k_gram2c(const float *__restrict__ hi, const uint16_t *__restrict__ lo,
+                                                int64_t n, int64_t nl, int j, const double *q, const double *z,
+                                                double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  const int cols = j + 1;
+  const int64_t m = n / V;
+  const int64_t chunk = (m + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk;
+  const int64_t i1 = i0 + chunk < m ? i0 + chunk : m;
+  for (int c0 = 0; c0 < cols; c0 += kDotTileM) {
+    double aq[kDotTileM], az[kDotTileM];
+#pragma unroll
+    for (int t = 0; t < kDotTileM; ++t) aq[t] = az[t] = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kBS) {
+      double qi[V], zi[V];
+      f64_load<V>(q, i, qi);
+      f64_load<V>(z, i, zi);
+#pragma unroll
+      for (int t = 0; t < kDotTileM; ++t)
+        if (c0 + t < cols) {
+          double b[V];
+          if (c0 + t < j) {
+            cb_load<V>(hi, lo, n, nl, c0 + t, i, b);
+          } else {
+#pragma unroll
+            for (int e = 0; e < V; ++e) b[e] = qi[e];
+          }
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            aq[t] += b[e] * qi[e];
+            az[t] += b[e] * zi[e];
+          }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kDotTileM; ++t) {
+      if (c0 + t >= cols) break;
+      const double sq = block_sum(aq[t], sh);
+      const double sz = block_sum(az[t], sh);
+      if (threadIdx.x == 0) {
+        part[(int64_t)blockIdx.x * 2 * cols + 2 * (c0 + t)] = sq;
+        part[(int64_t)blockIdx.x * 2 * cols + 2 * (c0 + t) + 1] = sz;
+      }
+    }
+  }
+}
+// k_dcgs_update on the compressed basis: with coef = [s (j), a (j), bq, 1/r],
+//   v_j <- (q - sum_{c<j} s_c v_c) / r   (stored compressed as column j)
+//   q   <- (z - sum_{c<j} a_c v_c - bq q) / r   (the next provisional vector)
+// and the block's partial |q|^2 in part[blockIdx.x]. Each entry of q is read
+// and rewritten by the same thread.
+template <int V>
+__global__ void __launch_bounds__(kBS) k_dcgs_updatec(float *__restrict__ hi, uint16_t *__restrict__ lo, int64_t n,
+                                                      int64_t nl, int j, const double *__restrict__ coef,
+                                                      double *__restrict__ q, const double *__restrict__ z,
+                                                      double *__restrict__ part) {
+  __shared__ double sh[kBS / 64];
+  __shared__ double sc[2 * kMaxCols + 2];
+  for (int c = threadIdx.x; c < 2 * j + 2; c += kBS) sc[c] = coef[c];
+  __syncthreads();
+  const double bq = sc[2 * j], ir = sc[2 * j + 1];
+  double acc = 0.0;
+  const int64_t m = n / V;
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < m; i += (int64_t)gridDim.x * kBS) {
+    double qn[V], zi[V];
+    f64_load<V>(q, i, qn);
+    f64_load<V>(z, i, zi);
+#pragma unroll
+    for (int e = 0; e < V; ++e) zi[e] -= bq * qn[e];
+    for (int c0 = 0; c0 < j; c0 += kDotTile) {
+      double b[kDotTile][V];
+#pragma unroll
+      for (int t = 0; t < kDotTile; ++t)
+        if (c0 + t < j) cb_load<V>(hi, lo, n, nl, c0 + t, i, b[t]);
+#pragma unroll
+      for (int t = 0; t < kDotTile; ++t)
+        if (c0 + t < j) {
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            qn[e] -= sc[c0 + t] * b[t][e];
+            zi[e] -= sc[j + c0 + t] * b[t][e];
+          }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      qn[e] *= ir;
+      zi[e] *= ir;
+      acc += zi[e] * zi[e];
+    }
+    cb_store<V>(hi, lo, n, nl, j, i, qn);
+    f64_store<V>(q, i, zi);
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+// out = sum_{c<dim} coef_c v_c over the compressed columns (FP64 out).
+template <int V>
+__global__ void __launch_bounds__(kBS) k_lincombc(const float *__restrict__ hi, const uint16_t *__restrict__ lo,
+                                                  int64_t n, int64_t nl, int dim, const double *__restrict__ coef,
+                                                  double *__restrict__ out) {
+  __shared__ double sc[kMaxCols];
+  for (int c = threadIdx.x; c < dim; c += kBS) sc[c] = coef[c];
+  __syncthreads();
+  const int64_t m = n / V;
+  for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < m; i += (int64_t)gridDim.x * kBS) {
+    double acc[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) acc[e] = 0.0;
+    for (int c0 = 0; c0 < dim; c0 += kDotTile) {
+      double b[kDotTile][V];
+#pragma unroll
+      for (int t = 0; t < kDotTile; ++t)
+        if (c0 + t < dim) cb_load<V>(hi, lo, n, nl, c0 + t, i, b[t]);
+#pragma unroll
+      for (int t = 0; t < kDotTile; ++t)
+        if (c0 + t < dim) {
+#pragma unroll
+          for (int e = 0; e < V; ++e) acc[e] += sc[c0 + t] * b[t][e];
+        }
+    }
+    f64_store<V>(out, i, acc);
+  }
+}
+
 __global__ void __launch_bounds__(kBS) k_scale(const double *__restrict__ w, double *__restrict__ d, int64_t n,
                                                double inv) {
   for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) d[i] = w[i] * inv;
@@ -829,6 +1030,49 @@ void launch_dcgs_update(double *basis, int64_t n, int j, const double *coef, dou
 void launch_lincomb_alias(const double *basis, int64_t n, int dim, const double *coef, double *out, hipStream_t s) {
   RF_REQUIRE(dim <= kMaxCols, "Krylov dimension above 64");
   hipLaunchKernelGGL(k_lincomb_alias, dim3(grid_for(n)), dim3(kBS), 0, s, basis, n, dim, coef, out);
+  RF_HIP(hipGetLastError());
+}
+// Entries per lane of the compressed-basis sweeps (see k_gram2c).
+inline int cb_vec(int64_t n) { return n % 4 == 0 ? 4 : n % 2 == 0 ? 2 : 1; }
+void launch_gram2c(const float *hi, const uint16_t *lo, int64_t n, int64_t nl, int j, const double *q,
+                   const double *z, double *partials, int nblk, double *out, hipStream_t s) {
+  RF_REQUIRE(j >= 0 && j + 1 <= kMaxCols, "Krylov dimension above 64");
+  const int cols = j + 1;
+  switch (cb_vec(n)) {
+    case 4: hipLaunchKernelGGL(k_gram2c<4>, dim3(nblk), dim3(kBS), 0, s, hi, lo, n, nl, j, q, z, partials); break;
+    case 2: hipLaunchKernelGGL(k_gram2c<2>, dim3(nblk), dim3(kBS), 0, s, hi, lo, n, nl, j, q, z, partials); break;
+    default: hipLaunchKernelGGL(k_gram2c<1>, dim3(nblk), dim3(kBS), 0, s, hi, lo, n, nl, j, q, z, partials); break;
+  }
+  hipLaunchKernelGGL(k_dot_finish, dim3(2 * cols), dim3(kBS), 0, s, partials, nblk, 2 * cols, out);
+  RF_HIP(hipGetLastError());
+}
+void launch_dcgs_updatec(float *hi, uint16_t *lo, int64_t n, int64_t nl, int j, const double *coef, double *q,
+                         const double *z, double *partials, int nblk, double *norm2, hipStream_t s) {
+  RF_REQUIRE(j >= 0 && j + 1 < kMaxCols, "Krylov dimension above 64");
+  switch (cb_vec(n)) {
+    case 4:
+      hipLaunchKernelGGL(k_dcgs_updatec<4>, dim3(nblk), dim3(kBS), 0, s, hi, lo, n, nl, j, coef, q, z, partials);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_dcgs_updatec<2>, dim3(nblk), dim3(kBS), 0, s, hi, lo, n, nl, j, coef, q, z, partials);
+      break;
+    default:
+      hipLaunchKernelGGL(k_dcgs_updatec<1>, dim3(nblk), dim3(kBS), 0, s, hi, lo, n, nl, j, coef, q, z, partials);
+      break;
+  }
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBS), 0, s, partials, nblk, norm2);
+  RF_HIP(hipGetLastError());
+}
+void launch_lincombc(const float *hi, const uint16_t *lo, int64_t n, int64_t nl, int dim, const double *coef,
+                     double *out, hipStream_t s) {
+  RF_REQUIRE(dim <= kMaxCols, "Krylov dimension above 64");
+  const int v = cb_vec(n);
+  const dim3 g(grid_for(n / v));
+  switch (v) {
+    case 4: hipLaunchKernelGGL(k_lincombc<4>, g, dim3(kBS), 0, s, hi, lo, n, nl, dim, coef, out); break;
+    case 2: hipLaunchKernelGGL(k_lincombc<2>, g, dim3(kBS), 0, s, hi, lo, n, nl, dim, coef, out); break;
+    default: hipLaunchKernelGGL(k_lincombc<1>, g, dim3(kBS), 0, s, hi, lo, n, nl, dim, coef, out); break;
+  }
   RF_HIP(hipGetLastError());
 }
 void launch_scale(const double *w, double *dst, int64_t n, double inv, hipStream_t s) {

@@ -272,6 +272,15 @@ class GpuOperator:
             raise ValueError(f"unknown KCL register mode {mode!r}")
         L.check(L.lib.rfhip_op_set_kcl_registers(self._h, modes[mode]))
 
+    def set_krylov_basis(self, mode: str) -> None:
+        """Basis storage of later expmv_adaptive calls (rfhip_op_set_krylov_basis;
+        no reference counterpart): "f64" (default) or "compressed" (finished
+        columns f32 + bf16, 6 bytes per entry; for tolerances well above 1e-9)."""
+        modes = {"f64": 0, "compressed": 1}
+        if mode not in modes:
+            raise ValueError(f"unknown Krylov basis mode {mode!r}")
+        L.check(L.lib.rfhip_op_set_krylov_basis(self._h, modes[mode]))
+
     # gpu/operator.rs:1175-1197 (CPU algorithm propagator.rs:186-326)
     def expmv(self, ctx, v, t, m, tol=0.0):
         v = self._vec(v, np.float64)

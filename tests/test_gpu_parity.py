@@ -754,6 +754,32 @@ def test_debye_kcl_compressed_registers_match_oracle(ctx, oracle):
     assert err < 1e-7 and err_p < 1e-7
 
 
+def test_compressed_krylov_basis_on_an_odd_length(ctx, oracle):
+    """The adaptive exponential with its finished basis columns stored as f32
+    + bf16 on the augmented Debye system at N=4 (n = 42,525 is odd: the
+    sweeps take their one-entry path and the bf16 halves of the columns are
+    padded to whole words) against the oracle's fixed-dimension expmv over
+    fine steps: within the tolerance, as with the FP64 basis."""
+    from rapidfem_amd.proposer import ExpController
+
+    oop, g, elems = _debye_pair(ctx, oracle, "fused_f64", 4)
+    n = oop.n_dof
+    assert n % 2 == 1
+    y0 = np.cos(0.2 + 0.013 * np.arange(n))
+    t_end, tol = 0.05, 1e-7
+    yf = y0.copy()
+    for _ in range(10):
+        yf, _ = oop.expmv(yf, t_end / 10, 40, 0.0)
+    ctl = ExpController(tol=tol, m=16)
+    out64, st64, _ = g.expmv_adaptive(ctx, y0, t_end, ctl)
+    g.set_krylov_basis("compressed")
+    out, st, _ = g.expmv_adaptive(ctx, y0, t_end, ctl)
+    e, e64 = rel_l2(out, yf), rel_l2(out64, yf)
+    print(f"compressed Krylov basis, Debye N=4 n={n}: {len(st)} steps (FP64 basis {len(st64)}), vs fine "
+          f"{e:.2e} (FP64 basis {e64:.2e})", flush=True)
+    assert len(st) >= 2 and e < 2 * tol and e64 < 2 * tol
+
+
 @pytest.mark.parametrize("form", ["plain_f64", "fused_f64"])
 def test_debye_expmv_matches_oracle(ctx, oracle, form):
     """Krylov exp(tA)v of the augmented operator (the P block is ordinary

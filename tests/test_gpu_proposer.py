@@ -210,6 +210,46 @@ def test_learned_proposer_gain_on_the_graded_slab(ctx):
     assert gain > 1.4
 
 
+def test_compressed_basis_meets_the_tolerance(ctx):
+    """The finished basis columns stored as f32 + bf16 (rfhip_op_set_krylov_basis):
+    the plain controller and a network-driven run still meet the tolerance
+    against a fine FP64 propagation, every step within its own bound, with
+    the step count of the FP64 basis to within one; switching back gives the
+    FP64-basis result bit for bit."""
+    import rapidfem_amd as R
+    from rapidfem_amd.proposer import ExpController, Proposer
+
+    op, y0 = _slab_small()
+    g = R.GpuOperator(ctx, op, "fused_f64")
+    cfl = R.TdSession(op, 1.0, "fused_f64", ctx).cfl_dt()
+    t_end, tol = 60 * cfl, 1e-7
+    ctl = ExpController(tol=tol, m=20, m_min=10, m_max=30)
+    prop = Proposer(ctx, seed=9)
+    p, _ = prop.params()
+    norm = np.concatenate([np.zeros(12), np.ones(12), [2.0, 12.0], [1.0, 4.0]]).astype(np.float32)
+    prop.set_params(p, norm)
+    yf = y0.copy()
+    for _ in range(200):
+        yf = g.expmv(ctx, yf, t_end / 200, 40, 0.0)
+    ref = {name: g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=pp) for name, pp in (("plain", None), ("net", prop))}
+    g.set_krylov_basis("compressed")
+    for name, pp in (("plain", None), ("net", prop)):
+        out, steps, _ = g.expmv_adaptive(ctx, y0, t_end, ctl, proposer=pp)
+        out64, steps64, _ = ref[name]
+        err, err64 = rel_l2(out, yf), rel_l2(out64, yf)
+        print(f"compressed basis, {name}: {len(steps)} steps (FP64 basis {len(steps64)}), vs fine {err:.2e} "
+              f"(FP64 basis {err64:.2e}), vs FP64 basis {rel_l2(out, out64):.2e}", flush=True)
+        assert abs(len(steps) - len(steps64)) <= 1
+        assert np.all(steps["est"] <= steps["bound"] * (1 + 1e-12))
+        assert abs(steps["h"].sum() - t_end) <= 1e-12 * t_end
+        assert err < 2 * tol
+    g.set_krylov_basis("f64")
+    again, _, _ = g.expmv_adaptive(ctx, y0, t_end, ctl)
+    assert np.array_equal(again, ref["plain"][0])
+    with pytest.raises(ValueError):
+        g.set_krylov_basis("bf16")
+
+
 def test_zero_state_and_a_horizon_below_one_step(ctx):
     """A zero state propagates to zero in no steps; a horizon far below
     what the estimate admits is covered by one step of exactly that length,
