@@ -243,7 +243,8 @@ int rfhip_op_transient_kcl_f64(rfhip_op *op, const double *y0, double dt,
  * every apply still in FP64. On an operator without Debye elements whose
  * mesh has the neighbour-trace table (conforming) and whose form keeps the
  * volume in FP64, the update runs in the stage kernel's epilogue, so k is
- * never stored (one launch per stage instead of two).
+ * never stored (one launch per stage instead of two), and the stage input
+ * is held as f32 hi + bf16 lo too.
  * RFHIP_KCL_REG_COMPRESSED_SEPARATE: the same registers with the update
  * always in a pass of its own after each apply. Applies to later
  * transient_kcl calls. */

@@ -614,14 +614,15 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
   const bool cmp = op->kcl_reg != RFHIP_KCL_REG_F64 && form_compressed(op->form);
   // with them, the stage kernel does the register update in its epilogue
   // where it can (no Debye P block, which the stage kernel does not
-  // update; conforming mesh): k is never written, and the stage input
-  // alternates between st and the k buffer (neighbours still read a
-  // stage's input while its next one is written)
+  // update; conforming mesh): k is never written, and the stage input is
+  // held like y (f32 hi + bf16 lo) in two buffers of kst that alternate
+  // (neighbours still read a stage's input while its next one is written)
   const bool fuse = cmp && op->kcl_reg == RFHIP_KCL_REG_COMPRESSED && op->n_disp == 0 &&
                     n == 6 * (int64_t)op->np * op->n_elem && kcl_stage_fusable(op->form, op->sa);
   const size_t w = f64 ? 8 : 4;
+  const size_t creg = creg_bytes(n), cstride = (creg + 255) & ~size_t(255);
   hipStream_t s = op->stream;
-  op->kst.alloc(w * n);
+  op->kst.alloc(fuse ? 2 * cstride : w * n);
   op->kerr.alloc((cmp ? 4 : w) * n);
   op->kbak.alloc(cmp ? creg_bytes(n) : w * n);
   for (DevBuf *b : {&op->kst, &op->kerr, &op->kbak}) launch_fill(b->p, (int64_t)b->bytes, s);
@@ -634,7 +635,17 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
     bh = op->kbak.as<float>();
     bl = reinterpret_cast<uint16_t *>(bh + n);
     launch_f64_to_c(static_cast<const double *>(ys), yh, yl, n, s);
-    launch_copy(op->kst.p, ys, 8 * n, s);
+    if (!fuse) launch_copy(op->kst.p, ys, 8 * n, s);
+  }
+  float *sth[2] = {nullptr, nullptr};  // fused: the stage inputs; sth[sc] is the next stage's
+  uint16_t *stl[2] = {nullptr, nullptr};
+  int sc = 0;
+  if (fuse) {
+    for (int i = 0; i < 2; ++i) {
+      sth[i] = reinterpret_cast<float *>(static_cast<char *>(op->kst.p) + i * cstride);
+      stl[i] = reinterpret_cast<uint16_t *>(sth[i] + n);
+    }
+    launch_copy(sth[0], op->kyc.p, (int64_t)creg, s);  // the first stage's input is y
   }
   const int nblk = weighted_err_blocks(n);
   const int64_t nslot = fuse ? kcl_stage_slots(op->order, op->sa) : 0;
@@ -642,12 +653,6 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
   op->red.alloc(8 * (npart + 1));
   double *part = op->red.as<double>();
   double *res = part + npart;
-  double *stb[2] = {op->kst.as<double>(), nullptr};
-  int sc = 0;  // stb[sc]: the next stage's input
-  if (fuse) {
-    op->k.alloc(8 * n);
-    stb[1] = op->k.as<double>();
-  }
   double h = dt, prev = 0.0, hmin = INFINITY, hmax = 0.0;
   int64_t acc = 0, rej = 0;
   const double h_floor = ctl.min_step_factor * dt;
@@ -683,8 +688,10 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
       // error norm (session.rs:160-171) against it.
       for (int sg = 0; fuse && sg < 5; ++sg) {
         StageArgs a = op->sa;
-        a.yin = stb[sc];
-        a.yout = stb[1 - sc];
+        a.yin = sth[sc];
+        a.yin_lo = stl[sc];
+        a.yout = sth[1 - sc];
+        a.yout_lo = stl[1 - sc];
         fill_source(op, a, kind, dof, gv);
         a.ky = yh;
         a.ky_lo = yl;
@@ -745,8 +752,11 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
       } else {
         ++rej;
         if (cmp) {
-          launch_copy(op->kyc.p, op->kbak.p, (int64_t)creg_bytes(n), s);
-          launch_c_to_f64(bh, bl, stb[sc], n, s);
+          launch_copy(op->kyc.p, op->kbak.p, (int64_t)creg, s);
+          if (fuse)
+            launch_copy(sth[sc], op->kbak.p, (int64_t)creg, s);
+          else
+            launch_c_to_f64(bh, bl, op->kst.as<double>(), n, s);
         } else {
           launch_copy(ys, op->kbak.p, (int64_t)(w * n), s);
         }

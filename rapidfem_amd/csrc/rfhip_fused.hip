@@ -98,7 +98,10 @@ __device__ __forceinline__ void load_trace(const StageArgs &io, int64_t nb, doub
 // the last stage) into yout. The first stage saves the starting y (already
 // in registers) to the backup; the last reads the backup and adds the
 // node's error-norm terms (session.rs:160-171) to wsum. own: the stage
-// input, yh / yl / ev: the node's y (f32 hi, bf16 lo pairs) and err.
+// input, yh / yl / ev: the node's y (f32 hi, bf16 lo pairs) and err. CIN:
+// the next stage's input is written as f32 hi + bf16 lo (yout, yout_lo),
+// else in FP64.
+template <bool CIN>
 __device__ __forceinline__ void node_kcl(const StageArgs &io, int64_t base, const double *mt, const double *acc,
                                          const double *own, const double *srcv, const float *yh,
                                          const uint32_t *yl, const float *ev, double &wsum) {
@@ -154,13 +157,25 @@ __device__ __forceinline__ void node_kcl(const StageArgs &io, int64_t base, cons
   float2 *yo = reinterpret_cast<float2 *>(io.ky + base);
   uint32_t *lo = reinterpret_cast<uint32_t *>(io.ky_lo + base);
   float2 *eo = reinterpret_cast<float2 *>(io.kerr + base);
-  double2 *so = reinterpret_cast<double2 *>(static_cast<double *>(io.yout) + base);
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     yo[q] = make_float2(hn[2 * q], hn[2 * q + 1]);
     lo[q] = bf16_pair(rn[2 * q], rn[2 * q + 1]);
     eo[q] = make_float2(en[2 * q], en[2 * q + 1]);
-    so[q] = make_double2(sn[2 * q], sn[2 * q + 1]);
+  }
+  if constexpr (CIN) {
+    float2 *sh = reinterpret_cast<float2 *>(static_cast<float *>(io.yout) + base);
+    uint32_t *sl = reinterpret_cast<uint32_t *>(static_cast<uint16_t *>(io.yout_lo) + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float h0 = (float)sn[2 * q], h1 = (float)sn[2 * q + 1];
+      sh[q] = make_float2(h0, h1);
+      sl[q] = bf16_pair((float)(sn[2 * q] - (double)h0), (float)(sn[2 * q + 1] - (double)h1));
+    }
+  } else {
+    double2 *so = reinterpret_cast<double2 *>(static_cast<double *>(io.yout) + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) so[q] = make_double2(sn[2 * q], sn[2 * q + 1]);
   }
   if (io.kfirst) {
     float2 *bo = reinterpret_cast<float2 *>(io.kbak + base);
@@ -1786,7 +1801,7 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(V
         for (int c = 0; c < 6; ++c) srcv[c] = io.src_vec[base + c];
       }
       if constexpr (EPI == 3) {
-        node_kcl(io, base, sm + le * 12, acc, own, srcv, kyh, kyl, kev, wsum);
+        node_kcl<CIN>(io, base, sm + le * 12, acc, own, srcv, kyh, kyl, kev, wsum);
       } else {
         double po[6];
 #pragma unroll
@@ -2425,8 +2440,8 @@ namespace {
 template <int NP>
 const void *kcl_kernel(int variant) {
   constexpr int F = Nfp<NP>::v;
-  return variant == 6 ? reinterpret_cast<const void *>(&k_fused11<NP, F, false, 3, 1>)
-                      : reinterpret_cast<const void *>(&k_fused11<NP, F, false, 3, 2>);
+  return variant == 6 ? reinterpret_cast<const void *>(&k_fused11<NP, F, true, 3, 1>)
+                      : reinterpret_cast<const void *>(&k_fused11<NP, F, true, 3, 2>);
 }
 
 // Blocks of the KCL stage launch: one wave per group of 64/Np elements,
@@ -2446,9 +2461,9 @@ void kcl_np(const StageArgs &a, hipStream_t s) {
   const int64_t grid = kcl_grid<NP>(a);
   if (grid <= 0) return;
   if (a.variant == 6)
-    hipLaunchKernelGGL((k_fused11<NP, F, false, 3, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+    hipLaunchKernelGGL((k_fused11<NP, F, true, 3, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
   else
-    hipLaunchKernelGGL((k_fused11<NP, F, false, 3, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+    hipLaunchKernelGGL((k_fused11<NP, F, true, 3, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
 }
 
 template <int NP>

@@ -165,9 +165,9 @@ struct StageArgs {
   int64_t src_dof;
   const double *src_vec;
   double g;
-  // KCL stage with compressed registers (launch_kcl_stage): yin is the
-  // stage input (FP64) and yout the next one; y and the substep backup as
-  // f32 hi + bf16 lo, err f32.
+  // KCL stage with compressed registers (launch_kcl_stage): yin / yin_lo
+  // is the stage input and yout / yout_lo the next one, f32 hi + bf16 lo
+  // like y and the substep backup; err f32.
   float *ky, *kerr, *kbak;
   uint16_t *ky_lo, *kbak_lo;
   double kb, kew, kamb;   // b_s, bhat_s - b_s, a_s - b_s (0 on the last stage)
@@ -278,8 +278,9 @@ void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t 
 // the base variant for them otherwise).
 bool variant_available(int order, int form, int variant, const StageArgs &a);
 // One KCL stage with the register update in the stage kernel's epilogue
-// (k_fused11, CIN off, EPI 3): the apply of a.yin, then the compressed
-// register update and the next stage's input into a.yout. Needs the
+// (k_fused11, compressed input, EPI 3): the apply of a.yin / a.yin_lo, then
+// the compressed register update and the next stage's input into a.yout /
+// a.yout_lo. Needs the
 // neighbour-trace table and a form without the matrix-core volume; runs
 // variant 6 when a.variant is 6 and variant 7 otherwise. kcl_stage_slots:
 // the number of partial sums the last stage leaves in a.kpart.
