@@ -270,6 +270,18 @@ __device__ __forceinline__ void face_geom(const double *ji, int f, double *n, do
   cf = hs * y;  // 0.5 |g|
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8).
+// With a grid of whole rounds the blocks on one XCD are renumbered to
+// consecutive indices, so that in each sweep of a grid-stride loop they take
+// consecutive element groups: an element's face neighbours in the cells
+// next to it along the two faster mesh directions are then mostly gathered
+// on the same XCD at about the same time and come from its L2 rather than
+// the fabric. Any renumbering visits every group once.
+__device__ __forceinline__ int64_t xcd_block() {
+  const unsigned nb = gridDim.x;
+  return (nb & 7u) == 0 ? (int64_t)(blockIdx.x & 7u) * (nb >> 3) + (blockIdx.x >> 3) : (int64_t)blockIdx.x;
+}
+
 // Order of element groups over the blocks. With xcd, blocks are taken to
 // run round-robin over the 8 XCDs (block b on XCD b % 8): XCD x walks the
 // contiguous groups [x*chunk, (x+1)*chunk), so the neighbours an element
@@ -285,7 +297,7 @@ __device__ __forceinline__ GroupRange group_range(bool xcd, int64_t ngroups) {
     const int64_t end = beg + chunk < ngroups ? beg + chunk : ngroups;
     return {beg + (int64_t)(blockIdx.x >> 3), end, (int64_t)(gridDim.x >> 3)};
   }
-  return {(int64_t)blockIdx.x, ngroups, (int64_t)gridDim.x};
+  return {xcd_block(), ngroups, (int64_t)gridDim.x};
 }
 
 // One thread per (element, node), 256/Np elements per block:
@@ -899,14 +911,15 @@ k_fused8(StageArgs io) {
   // own data again), so the compiler can count outstanding loads exactly
   // and never has to drain them all at a merge point. The launch holds at
   // most ngroups blocks.
-  read_ahead(blockIdx.x);
+  const int64_t b0 = xcd_block();
+  read_ahead(b0);
   {
-    const int64_t r0 = io.n_elem - (int64_t)blockIdx.x * EPB;
+    const int64_t r0 = io.n_elem - b0 * EPB;
     write_state(r0 < EPB ? (int)r0 : EPB);
   }
 
 #pragma unroll 1
-  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+  for (int64_t grp = b0; grp < ngroups; grp += gridDim.x) {
     lane_ids(tid, lane, el, q, i);
     const int64_t e0 = grp * EPB;
     const int64_t rem = io.n_elem - e0;
@@ -1167,14 +1180,15 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
   };
   // The launch holds at most ngroups blocks; the last group of a block reads
   // itself ahead again, which keeps every read-ahead load unconditional.
-  read_ahead(blockIdx.x);
+  const int64_t b0 = xcd_block();
+  read_ahead(b0);
   {
-    const int64_t r0 = io.n_elem - (int64_t)blockIdx.x * EPB;
+    const int64_t r0 = io.n_elem - b0 * EPB;
     write_state(r0 < EPB ? (int)r0 : EPB);
   }
 
 #pragma unroll 1
-  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+  for (int64_t grp = b0; grp < ngroups; grp += gridDim.x) {
     int tid, le, i;
     lane_ids(tid, le, i);
     const int64_t e0 = grp * EPB;
@@ -1407,7 +1421,7 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(V
   double *sf = sf_b[wave];
   double *sm = sm_b[wave];
   const int64_t ngroups = (io.n_elem + EPB - 1) / EPB;  // groups of EPB elements, one wave each
-  const int64_t gw0 = (int64_t)blockIdx.x * NWB + wave, gws = (int64_t)gridDim.x * NWB;
+  const int64_t gw0 = xcd_block() * NWB + wave, gws = (int64_t)gridDim.x * NWB;
   // The only block-wide barrier: D, LIFT and the face-node table are in LDS.
   __syncthreads();
   // Every other hand-off goes through the wave's own LDS slice; a wave's LDS
@@ -1931,7 +1945,7 @@ k_clane(StageArgs io) {
       return r.h;
   };
 #pragma unroll 1
-  for (int64_t grp = (int64_t)blockIdx.x * NWB + wave; grp < ngroups; grp += (int64_t)gridDim.x * NWB) {
+  for (int64_t grp = xcd_block() * NWB + wave; grp < ngroups; grp += (int64_t)gridDim.x * NWB) {
     const int64_t e0 = grp * kClEpw;
     const int64_t rem = io.n_elem - e0;
     const int nel = rem < kClEpw ? (int)rem : kClEpw;
