@@ -207,14 +207,16 @@ def test_kcl_on_stage_kernel_variants_matches_oracle(ctx, oracle, order, variant
 
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
 @pytest.mark.parametrize("variant", [5, 6, 7])
-@pytest.mark.parametrize("drive", ["free", "vector"])
+@pytest.mark.parametrize("drive", ["free", "vector", "point"])
 def test_kcl_fused_update_matches_separate_pass(ctx, oracle, order, variant, drive):
     """Compressed KCL registers with the update in the stage kernel's
     epilogue (no k stored; the stage input alternates between two buffers;
     the error norm summed per wave) against the same registers with the
     update as a pass of its own after each apply (mode
     "compressed_separate"), on the operator's variant (5: the fused update
-    runs on variant 7's kernel): the same accept/reject sequence and step
+    runs on variant 7's kernel), free, vector- and point-driven (the point
+    drive lands on one entry of one node's epilogue): the same
+    accept/reject sequence and step
     range, the same states to the registers' rounding, and the oracle's
     counts and state within that rounding. The tight
     controller forces rejections, whose restore rebuilds the stage input in
@@ -231,6 +233,10 @@ def test_kcl_fused_update_matches_separate_pass(ctx, oracle, order, variant, dri
         wave = np.array([0.4, -0.3, 0.2])
         kw = dict(drive=Drive.vector(b), g=wave)
         okw = dict(drive=("vector", b), g=wave)
+    elif drive == "point":
+        wave = np.array([0.4, -0.3, 0.2])
+        kw = dict(drive=Drive.point(n // 3 + 1), g=wave)
+        okw = dict(drive=("point", n // 3 + 1), g=wave)
     ctl = Controller(atol=1e-9, rtol=1e-7)
     dt, steps = 0.05, 3
     runs = {}
