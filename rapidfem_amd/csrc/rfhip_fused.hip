@@ -1363,8 +1363,19 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
 // the LIFT reads) stays inside one wave, so after the initial table load
 // there are no block-wide barriers: a wave stalled on memory or LDS does not
 // hold the block's other waves.
+// Waves per SIMD the compiler fits k_fused11's registers to: 3 for the
+// compressed-register stages (EPI 2, 3) but at Np = 10, else 2 (the FP64
+// registers and the apply, EPI 0 and 1; Np = 10; the volume on the matrix
+// cores). Fitted to 3, the FP64-register stage spilled 30-38 VGPRs and ran
+// 0.520 ms at N=3 against 0.418 at 2, and the compressed-register stage at
+// N=2 0.70 / 0.47 ms (unrolled by 2 / 1) against 0.41; the compressed stage
+// at N=3 runs the same at 2 and 3 (0.363 / 0.354 ms).
+template <int NP, int EPI, int VMF>
+struct K11Waves {
+  static constexpr int v = VMF ? 2 : (NP != 10 && (EPI == 2 || EPI == 3)) ? 3 : 2;
+};
 template <int NP, int NFP, bool CIN, int EPI, int UNR, int VMF = 0>
-__global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(VMF ? 2 : 3))) k_fused11(StageArgs io) {
+__global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(K11Waves<NP, EPI, VMF>::v))) k_fused11(StageArgs io) {
   constexpr int EPB = 64 / NP;  // elements per WAVE: a wave owns whole elements
   constexpr int NWB = kFusedWG / 64;
   constexpr int NF4 = 4 * NFP;
