@@ -169,8 +169,9 @@ void build_plain(rfhip_op *op, const rfhip_op_desc &d) {
 // register form in a sweep of every variant over LSERK4 steps on MI355X
 // (pulse boxes of 1.3M / 393k / 324k / 96k tets at N = 1..4; N = 2 and the
 // FP64 registers at N = 3 again on 768k / 324k tets once k_fused11 fitted
-// those cases to 2 waves per SIMD). autotune re-picks on the operator's own
-// mesh.
+// those cases to 2 waves per SIMD, N = 4 on the 998k-tet cavity once
+// k_fused9's LDS fitted two blocks per CU). autotune re-picks on the
+// operator's own mesh.
 int default_variant(int order, int form) {
   if (form_vmf(form)) return 1;
   const bool c = form == RFHIP_FORM_FUSED_F64C;
@@ -178,7 +179,7 @@ int default_variant(int order, int form) {
     case 1: return c ? 7 : 5;
     case 2: return c ? 7 : 6;
     case 3: return 7;
-    case 4: return c ? 1 : 2;
+    case 4: return 4;
     default: return 0;
   }
 }

@@ -1092,8 +1092,10 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(3
   constexpr int FPT = (NF4 + NP - 1) / NP;
   // D_k as [k][i][j] (row stride DS) and LIFT as [i][fm] (row stride LS):
   // a 16-byte LDS read then returns two consecutive j (or fm) entries of a
-  // thread's row; the padded strides spread the rows over the banks.
-  constexpr int DS = NP + (NP & 1) + 2, LS = NF4 + 2;
+  // thread's row; the padded strides spread the rows over the banks. At
+  // Np = 35 LIFT's rows go unpadded: the block's LDS then fits twice in a
+  // CU (81,664 of 163,840 bytes; 82,224 with the pad, one block per CU).
+  constexpr int DS = NP + (NP & 1) + 2, LS = NF4 + (NP == 35 ? 0 : 2);
   constexpr int ND = 3 * NP * DS;
   constexpr int SD = ND + NP * LS;
   constexpr int SFS = NF4 * 6 + 2;
