@@ -612,8 +612,9 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
   const int64_t n = nvec(op);
   const bool f64 = f64_regs(op->form);
   // compressed registers: y and its backup as f32 hi + bf16 lo ([hi | lo]
-  // in one buffer), err as f32; every stage's apply reads st in FP64, the
-  // last stage leaving the accepted y there for the next substep
+  // in one buffer), err as f32; with the separate update pass every stage's
+  // apply reads st in FP64, the last stage leaving the accepted y there for
+  // the next substep
   const bool cmp = op->kcl_reg != RFHIP_KCL_REG_F64 && form_compressed(op->form);
   // with them, the stage kernel does the register update in its epilogue
   // where it can (no Debye P block, which the stage kernel does not
