@@ -237,7 +237,10 @@ int rfhip_op_transient_kcl_f64(rfhip_op *op, const double *y0, double dt,
 /* KCL registers (no counterpart in the reference, whose KCL keeps f32
  * registers, gpu/operator.rs:823-893). RFHIP_KCL_REG_F64 (default): y, err,
  * the stage input and the substep backup in FP64, 48 bytes per entry per
- * stage update. RFHIP_KCL_REG_COMPRESSED (forms with compressed state
+ * stage update; on a fused form without Debye elements whose mesh has the
+ * neighbour-trace table the update runs in the stage kernel's epilogue (one
+ * launch per stage, k never stored); RFHIP_KCL_REG_F64_SEPARATE keeps it as
+ * a pass of its own after each apply. RFHIP_KCL_REG_COMPRESSED (forms with compressed state
  * registers only, else RFHIP_EUNSUPPORTED): y and its backup as f32 hi +
  * bf16 lo, err as f32, the stage input FP64, 36 bytes; the controller and
  * every apply still in FP64. On an operator without Debye elements whose
@@ -251,6 +254,7 @@ int rfhip_op_transient_kcl_f64(rfhip_op *op, const double *y0, double dt,
 #define RFHIP_KCL_REG_F64 0
 #define RFHIP_KCL_REG_COMPRESSED 1
 #define RFHIP_KCL_REG_COMPRESSED_SEPARATE 2
+#define RFHIP_KCL_REG_F64_SEPARATE 3
 int rfhip_op_set_kcl_registers(rfhip_op *op, int32_t mode);
 
 /* GpuOperator::expmv (gpu/operator.rs:1175-1197): out = exp(t*A)*v, FP64

@@ -615,18 +615,24 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
   // in one buffer), err as f32; with the separate update pass every stage's
   // apply reads st in FP64, the last stage leaving the accepted y there for
   // the next substep
-  const bool cmp = op->kcl_reg != RFHIP_KCL_REG_F64 && form_compressed(op->form);
+  const bool cmp = (op->kcl_reg == RFHIP_KCL_REG_COMPRESSED || op->kcl_reg == RFHIP_KCL_REG_COMPRESSED_SEPARATE) &&
+                   form_compressed(op->form);
   // with them, the stage kernel does the register update in its epilogue
   // where it can (no Debye P block, which the stage kernel does not
   // update; conforming mesh): k is never written, and the stage input is
   // held like y (f32 hi + bf16 lo) in two buffers of kst that alternate
   // (neighbours still read a stage's input while its next one is written)
-  const bool fuse = cmp && op->kcl_reg == RFHIP_KCL_REG_COMPRESSED && op->n_disp == 0 &&
-                    n == 6 * (int64_t)op->np * op->n_elem && kcl_stage_fusable(op->form, op->sa);
+  const bool fusable = form_fused(op->form) && op->n_disp == 0 && n == 6 * (int64_t)op->np * op->n_elem &&
+                       kcl_stage_fusable(op->form, op->sa);
+  const bool fuse = cmp && op->kcl_reg == RFHIP_KCL_REG_COMPRESSED && fusable;
+  // FP64 registers on a fused form: the same fusion, y / err / the backup
+  // and the two alternating stage inputs in FP64
+  const bool fuse64 = !cmp && f64 && op->kcl_reg == RFHIP_KCL_REG_F64 && fusable;
   const size_t w = f64 ? 8 : 4;
   const size_t creg = creg_bytes(n), cstride = (creg + 255) & ~size_t(255);
+  const size_t dstride = (8 * (size_t)n + 255) & ~size_t(255);
   hipStream_t s = op->stream;
-  op->kst.alloc(fuse ? 2 * cstride : w * n);
+  op->kst.alloc(fuse ? 2 * cstride : fuse64 ? 2 * dstride : w * n);
   op->kerr.alloc((cmp ? 4 : w) * n);
   op->kbak.alloc(cmp ? creg_bytes(n) : w * n);
   for (DevBuf *b : {&op->kst, &op->kerr, &op->kbak}) launch_fill(b->p, (int64_t)b->bytes, s);
@@ -651,8 +657,13 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
     }
     launch_copy(sth[0], op->kyc.p, (int64_t)creg, s);  // the first stage's input is y
   }
+  double *std64[2] = {nullptr, nullptr};  // fused FP64: the stage inputs; std64[sc] is the next stage's
+  if (fuse64) {
+    for (int i = 0; i < 2; ++i) std64[i] = reinterpret_cast<double *>(static_cast<char *>(op->kst.p) + i * dstride);
+    launch_copy(std64[0], ys, 8 * n, s);
+  }
   const int nblk = weighted_err_blocks(n);
-  const int64_t nslot = fuse ? kcl_stage_slots(op->order, op->sa) : 0;
+  const int64_t nslot = fuse || fuse64 ? kcl_stage_slots(op->order, op->sa, fuse64) : 0;
   const int64_t npart = std::max<int64_t>(nblk, nslot);
   op->red.alloc(8 * (npart + 1));
   double *part = op->red.as<double>();
@@ -690,18 +701,26 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
       // last stage, builds the next stage's input y + (a - b) f. The first
       // pass also saves the starting y (kbak); the last one reduces the
       // error norm (session.rs:160-171) against it.
-      for (int sg = 0; fuse && sg < 5; ++sg) {
+      for (int sg = 0; (fuse || fuse64) && sg < 5; ++sg) {
         StageArgs a = op->sa;
-        a.yin = sth[sc];
-        a.yin_lo = stl[sc];
-        a.yout = sth[1 - sc];
-        a.yout_lo = stl[1 - sc];
+        if (fuse64) {
+          a.yin = std64[sc];
+          a.yout = std64[1 - sc];
+          a.kyd = static_cast<double *>(ys);
+          a.kerrd = op->kerr.as<double>();
+          a.kbakd = op->kbak.as<double>();
+        } else {
+          a.yin = sth[sc];
+          a.yin_lo = stl[sc];
+          a.yout = sth[1 - sc];
+          a.yout_lo = stl[1 - sc];
+          a.ky = yh;
+          a.ky_lo = yl;
+          a.kerr = op->kerr.as<float>();
+          a.kbak = bh;
+          a.kbak_lo = bl;
+        }
         fill_source(op, a, kind, dof, gv);
-        a.ky = yh;
-        a.ky_lo = yl;
-        a.kerr = op->kerr.as<float>();
-        a.kbak = bh;
-        a.kbak_lo = bl;
         a.dt = ht;
         a.kb = kKclB[sg];
         a.kew = kKclBhat[sg] - kKclB[sg];
@@ -711,11 +730,11 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
         a.kfirst = sg == 0;
         a.klast = sg == 4;
         a.kpart = part;
-        launch_kcl_stage(op->order, a, s);
+        launch_kcl_stage(op->order, a, fuse64, s);
         sc = 1 - sc;
         if (sg == 4) launch_sum_partials(part, (int)nslot, res, s);
       }
-      for (int sg = 0; !fuse && sg < 5; ++sg) {
+      for (int sg = 0; !fuse && !fuse64 && sg < 5; ++sg) {
         rhs_into(op, sg == 0 && !cmp ? ys : op->kst.p, op->k.p, kind, dof, gv);
         KclStage a{};
         a.y = cmp ? (void *)yh : ys;
@@ -763,6 +782,7 @@ void kcl_impl(rfhip_op *op, void *ys, double dt, int64_t steps, int kind, int64_
             launch_c_to_f64(bh, bl, op->kst.as<double>(), n, s);
         } else {
           launch_copy(ys, op->kbak.p, (int64_t)(w * n), s);
+          if (fuse64) launch_copy(std64[sc], op->kbak.p, 8 * n, s);
         }
         h = ht * factor(ctl, std::isfinite(e) ? e : kKclNonfinite, prev, true);
       }
@@ -1798,9 +1818,9 @@ int rfhip_op_set_kcl_registers(rfhip_op *op, int32_t mode) {
   return guarded([&] {
     RF_REQUIRE(op, "null operator");
     RF_REQUIRE(mode == RFHIP_KCL_REG_F64 || mode == RFHIP_KCL_REG_COMPRESSED ||
-                   mode == RFHIP_KCL_REG_COMPRESSED_SEPARATE,
+                   mode == RFHIP_KCL_REG_COMPRESSED_SEPARATE || mode == RFHIP_KCL_REG_F64_SEPARATE,
                "unknown KCL register mode");
-    if (mode != RFHIP_KCL_REG_F64 && !form_compressed(op->form))
+    if ((mode == RFHIP_KCL_REG_COMPRESSED || mode == RFHIP_KCL_REG_COMPRESSED_SEPARATE) && !form_compressed(op->form))
       throw Error{RFHIP_EUNSUPPORTED, "compressed KCL registers need a form with compressed state registers"};
     op->kcl_reg = mode;
   });

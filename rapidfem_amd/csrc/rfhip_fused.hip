@@ -89,7 +89,8 @@ __device__ __forceinline__ void load_trace(const StageArgs &io, int64_t nb, doub
 
 // EPI: 0 = apply (dy = A y, FP64), 1 = LSERK stage with FP64 registers,
 // 2 = LSERK stage with compressed registers (y f32+bf16, p f32), 3 = KCL
-// stage with compressed registers (k_fused11 only, node_kcl).
+// stage with compressed registers (k_fused11 only, node_kcl), 4 = KCL stage
+// with FP64 registers (k_fused11 only, node_kcl64).
 //
 // KCL stage update of one node with compressed registers
 // (explicit_adaptive.rs:107-149), the same arithmetic as the separate pass
@@ -185,6 +186,63 @@ __device__ __forceinline__ void node_kcl(const StageArgs &io, int64_t base, cons
       bo[q] = make_float2(yh[2 * q], yh[2 * q + 1]);
       blo[q] = yl[q];
     }
+  }
+}
+
+// The same KCL stage update with FP64 registers, the arithmetic of the
+// separate pass k_kcl_update2: yd / ed: the node's y and err; the next
+// stage's input (FP64) into yout; the backup in kbakd.
+__device__ __forceinline__ void node_kcl64(const StageArgs &io, int64_t base, const double *mt, const double *acc,
+                                           const double *own, const double *srcv, const double *yd,
+                                           const double *ed, double &wsum) {
+  // the node's right-hand side (rhs.rs:1443-1454), written out as in
+  // node_update
+  double kv[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    kv[c] = mt[c] * acc[c] - mt[6 + c] * own[c];
+    kv[3 + c] = mt[3 + c] * acc[3 + c] - mt[9 + c] * own[3 + c];
+  }
+  if (io.src_kind == RFHIP_DRIVE_POINT) {
+    const int64_t off = io.src_dof - base;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      if (off == c) kv[c] += io.g;
+  } else if (io.src_kind == RFHIP_DRIVE_VECTOR) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) kv[c] += srcv[c] * io.g;
+  }
+  double yn[6], en[6], sn[6], bo[6];
+  if (io.klast) {
+    const double2 *bp = reinterpret_cast<const double2 *>(io.kbakd + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double2 v = bp[q];
+      bo[2 * q] = v.x;
+      bo[2 * q + 1] = v.y;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double f = io.dt * kv[c];
+    yn[c] = yd[c] + io.kb * f;
+    en[c] = io.kfirst ? io.kew * f : ed[c] + io.kew * f;
+    sn[c] = io.klast ? yn[c] : yn[c] + io.kamb * f;
+    if (io.klast) wsum += werr_term(bo[c], yn[c], en[c], io.katol, io.krtol);
+  }
+  double2 *yo = reinterpret_cast<double2 *>(io.kyd + base);
+  double2 *eo = reinterpret_cast<double2 *>(io.kerrd + base);
+  double2 *so = reinterpret_cast<double2 *>(static_cast<double *>(io.yout) + base);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    yo[q] = make_double2(yn[2 * q], yn[2 * q + 1]);
+    eo[q] = make_double2(en[2 * q], en[2 * q + 1]);
+    so[q] = make_double2(sn[2 * q], sn[2 * q + 1]);
+  }
+  if (io.kfirst) {
+    double2 *bp = reinterpret_cast<double2 *>(io.kbakd + base);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) bp[q] = make_double2(yd[2 * q], yd[2 * q + 1]);
   }
 }
 
@@ -1559,7 +1617,7 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(K
       }
     }
   };
-  // KCL stage (EPI 3): the wave's sum of the error-norm terms over its
+  // KCL stage (EPI 3, 4): the wave's sum of the error-norm terms over its
   // nodes, left in io.kpart[wave of the grid] by the last stage.
   const int64_t wslot = (int64_t)blockIdx.x * NWB + wave;
   double wsum = 0.0;
@@ -1567,7 +1625,7 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(K
   // partial); the last group of a wave reads itself ahead again, which
   // keeps every read-ahead load unconditional.
   if (gw0 >= ngroups) {
-    if constexpr (EPI == 3)
+    if constexpr (EPI == 3 || EPI == 4)
       if (io.klast && (threadIdx.x & 63) == 0) io.kpart[wslot] = 0.0;
     return;
   }
@@ -1789,6 +1847,19 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(K
         kyl[q] = lp[q];
       }
     }
+    double kyd[6], ked[6];
+    if constexpr (EPI == 4) {
+      const double2 *yp = reinterpret_cast<const double2 *>(io.kyd + base);
+      const double2 *ep = reinterpret_cast<const double2 *>(io.kerrd + base);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double2 a = yp[q], b = ep[q];
+        kyd[2 * q] = a.x;
+        kyd[2 * q + 1] = a.y;
+        ked[2 * q] = b.x;
+        ked[2 * q + 1] = b.y;
+      }
+    }
     double own[6];
     if (active) {
       const double *fb = sf + le * SFS;
@@ -1820,6 +1891,9 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(K
       for (int c = 0; c < 6; ++c) asm volatile("" : "+v"(kyh[c]), "+v"(kev[c]));
 #pragma unroll
       for (int q = 0; q < 3; ++q) asm volatile("" : "+v"(kyl[q]));
+    } else if constexpr (EPI == 4) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) asm volatile("" : "+v"(kyd[c]), "+v"(ked[c]));
     }
     if (active) {
       double srcv[6];
@@ -1829,6 +1903,8 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(K
       }
       if constexpr (EPI == 3) {
         node_kcl<CIN>(io, base, sm + le * 12, acc, own, srcv, kyh, kyl, kev, wsum);
+      } else if constexpr (EPI == 4) {
+        node_kcl64(io, base, sm + le * 12, acc, own, srcv, kyd, ked, wsum);
       } else {
         double po[6];
 #pragma unroll
@@ -1837,7 +1913,7 @@ __global__ void __launch_bounds__(kFusedWG) __attribute__((amdgpu_waves_per_eu(K
       }
     }
   }
-  if constexpr (EPI == 3) {
+  if constexpr (EPI == 3 || EPI == 4) {
     if (io.klast) {
       // fixed butterfly order: the same sum for the same grid
 #pragma unroll
@@ -2465,8 +2541,11 @@ bool variant_available(int order, int form, int variant, const StageArgs &a) {
 
 namespace {
 template <int NP>
-const void *kcl_kernel(int variant) {
+const void *kcl_kernel(int variant, bool r64) {
   constexpr int F = Nfp<NP>::v;
+  if (r64)
+    return variant == 6 ? reinterpret_cast<const void *>(&k_fused11<NP, F, false, 4, 1>)
+                        : reinterpret_cast<const void *>(&k_fused11<NP, F, false, 4, 2>);
   return variant == 6 ? reinterpret_cast<const void *>(&k_fused11<NP, F, true, 3, 1>)
                       : reinterpret_cast<const void *>(&k_fused11<NP, F, true, 3, 2>);
 }
@@ -2474,23 +2553,31 @@ const void *kcl_kernel(int variant) {
 // Blocks of the KCL stage launch: one wave per group of 64/Np elements,
 // four waves per block, at most the resident blocks (persistent grid).
 template <int NP>
-int64_t kcl_grid(const StageArgs &a) {
+int64_t kcl_grid(const StageArgs &a, bool r64) {
   const int64_t epw = 64 / NP;
   const int64_t wgroups = (a.n_elem + epw - 1) / epw;
   const int64_t groups = (wgroups + 3) / 4;
-  const int64_t res = resident_blocks(kcl_kernel<NP>(a.variant));
+  const int64_t res = resident_blocks(kcl_kernel<NP>(a.variant, r64));
   return groups < res ? groups : res;
 }
 
 template <int NP>
-void kcl_np(const StageArgs &a, hipStream_t s) {
+void kcl_np(const StageArgs &a, bool r64, hipStream_t s) {
   constexpr int F = Nfp<NP>::v;
-  const int64_t grid = kcl_grid<NP>(a);
+  const int64_t grid = kcl_grid<NP>(a, r64);
   if (grid <= 0) return;
-  if (a.variant == 6)
-    hipLaunchKernelGGL((k_fused11<NP, F, true, 3, 1>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_fused11<NP, F, true, 3, 2>), dim3((unsigned)grid), dim3(kFusedWG), 0, s, a);
+  const dim3 g((unsigned)grid), b(kFusedWG);
+  if (r64) {
+    if (a.variant == 6)
+      hipLaunchKernelGGL((k_fused11<NP, F, false, 4, 1>), g, b, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_fused11<NP, F, false, 4, 2>), g, b, 0, s, a);
+  } else {
+    if (a.variant == 6)
+      hipLaunchKernelGGL((k_fused11<NP, F, true, 3, 1>), g, b, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_fused11<NP, F, true, 3, 2>), g, b, 0, s, a);
+  }
 }
 
 template <int NP>
@@ -2521,24 +2608,24 @@ bool clane_pack_np(const int *face_nodes, const double *const D[3], const double
 
 bool kcl_stage_fusable(int form, const StageArgs &a) { return !form_vmf(form) && a.nbase != nullptr; }
 
-int64_t kcl_stage_slots(int order, const StageArgs &a) {
+int64_t kcl_stage_slots(int order, const StageArgs &a, bool r64) {
   switch ((order + 1) * (order + 2) * (order + 3) / 6) {
-    case 4: return kcl_grid<4>(a) * (kFusedWG / 64);
-    case 10: return kcl_grid<10>(a) * (kFusedWG / 64);
-    case 20: return kcl_grid<20>(a) * (kFusedWG / 64);
-    case 35: return kcl_grid<35>(a) * (kFusedWG / 64);
+    case 4: return kcl_grid<4>(a, r64) * (kFusedWG / 64);
+    case 10: return kcl_grid<10>(a, r64) * (kFusedWG / 64);
+    case 20: return kcl_grid<20>(a, r64) * (kFusedWG / 64);
+    case 35: return kcl_grid<35>(a, r64) * (kFusedWG / 64);
     default: throw Error{RFHIP_EUNSUPPORTED, "the fused stage kernels run orders 1 to 4"};
   }
 }
 
-void launch_kcl_stage(int order, const StageArgs &a, hipStream_t s) {
+void launch_kcl_stage(int order, const StageArgs &a, bool r64, hipStream_t s) {
   RF_REQUIRE(a.nbase != nullptr, "the fused KCL stage needs the neighbour-trace table");
   RF_REQUIRE(!a.klast || a.kpart != nullptr, "the last KCL stage needs the error partials");
   switch ((order + 1) * (order + 2) * (order + 3) / 6) {
-    case 4: kcl_np<4>(a, s); break;
-    case 10: kcl_np<10>(a, s); break;
-    case 20: kcl_np<20>(a, s); break;
-    case 35: kcl_np<35>(a, s); break;
+    case 4: kcl_np<4>(a, r64, s); break;
+    case 10: kcl_np<10>(a, r64, s); break;
+    case 20: kcl_np<20>(a, r64, s); break;
+    case 35: kcl_np<35>(a, r64, s); break;
     default: throw Error{RFHIP_EUNSUPPORTED, "the fused stage kernels run orders 1 to 4"};
   }
   RF_HIP(hipGetLastError());

@@ -174,6 +174,9 @@ struct StageArgs {
   double katol, krtol;
   int kfirst, klast;
   double *kpart;          // last stage: one partial sum of the error norm per wave of the grid
+  // the same with FP64 registers (launch_kcl_stage, r64): y, err and the
+  // backup FP64, the stage inputs yin / yout FP64
+  double *kyd, *kerrd, *kbakd;
 };
 
 }  // namespace rfhip
@@ -278,15 +281,16 @@ void launch_stage(int order, int form, int epi, const StageArgs &a, hipStream_t 
 // the base variant for them otherwise).
 bool variant_available(int order, int form, int variant, const StageArgs &a);
 // One KCL stage with the register update in the stage kernel's epilogue
-// (k_fused11, compressed input, EPI 3): the apply of a.yin / a.yin_lo, then
-// the compressed register update and the next stage's input into a.yout /
-// a.yout_lo. Needs the
-// neighbour-trace table and a form without the matrix-core volume; runs
-// variant 6 when a.variant is 6 and variant 7 otherwise. kcl_stage_slots:
-// the number of partial sums the last stage leaves in a.kpart.
+// (k_fused11): with compressed registers (EPI 3) the apply of a.yin /
+// a.yin_lo, the register update and the next stage's input into a.yout /
+// a.yout_lo; with FP64 registers (r64, EPI 4) the same on a.kyd, a.kerrd,
+// a.kbakd and FP64 stage inputs. Needs the neighbour-trace table and a form
+// without the matrix-core volume; runs variant 6 when a.variant is 6 and
+// variant 7 otherwise. kcl_stage_slots: the number of partial sums the last
+// stage leaves in a.kpart.
 bool kcl_stage_fusable(int form, const StageArgs &a);
-int64_t kcl_stage_slots(int order, const StageArgs &a);
-void launch_kcl_stage(int order, const StageArgs &a, hipStream_t s);
+int64_t kcl_stage_slots(int order, const StageArgs &a, bool r64);
+void launch_kcl_stage(int order, const StageArgs &a, bool r64, hipStream_t s);
 // The table of the component-lane stage kernel (variant 8): the nonzeros of
 // D_r, D_s, D_t (row-major D_k[i][j], the order of (i, j, k)) and then LIFT
 // ((face, node, face node) order). False when the operator's face nodes or

@@ -262,12 +262,12 @@ class GpuOperator:
 
     def set_kcl_registers(self, mode: str) -> None:
         """KCL registers for later transient_kcl calls (rfhip_op_set_kcl_registers;
-        no reference counterpart): "f64" (default), "compressed" (y and its
-        backup f32 + bf16, err f32; forms with compressed state registers; the
-        update in the stage kernel's epilogue where the operator allows it) or
-        "compressed_separate" (the same registers, the update always in a pass
-        of its own)."""
-        modes = {"f64": 0, "compressed": 1, "compressed_separate": 2}
+        no reference counterpart): "f64" (default; the update in the stage
+        kernel's epilogue where the operator allows it), "compressed" (y and
+        its backup f32 + bf16, err f32; forms with compressed state registers;
+        fused the same way), or "compressed_separate" / "f64_separate" (the
+        same registers, the update always in a pass of its own)."""
+        modes = {"f64": 0, "compressed": 1, "compressed_separate": 2, "f64_separate": 3}
         if mode not in modes:
             raise ValueError(f"unknown KCL register mode {mode!r}")
         L.check(L.lib.rfhip_op_set_kcl_registers(self._h, modes[mode]))

@@ -270,6 +270,53 @@ def test_kcl_fused_update_matches_separate_pass(ctx, oracle, order, variant, dri
     assert err < 1e-7
 
 
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [5, 7])
+@pytest.mark.parametrize("drive", ["free", "vector"])
+def test_kcl_fused_fp64_update_matches_separate_pass(ctx, oracle, order, variant, drive):
+    """FP64 KCL registers (the default) with the update in the stage kernel's
+    epilogue against the same registers with the update as a pass of its own
+    (mode "f64_separate"), on the FP64 fused form: equal accept/reject
+    counts, the step range and the states to FP64 rounding (the paths sum
+    the error norm in different orders), and the oracle's counts and state."""
+    from rapidfem_amd import Controller, Drive
+
+    oop, y0 = _te101(oracle, 3, order)
+    n = oop.n_dof
+    g = _gop(ctx, oop, "fused_f64")
+    g.set_variant(variant)
+    kw, okw = {}, {}
+    if drive == "vector":
+        b = np.cos(0.37 * np.arange(n))
+        wave = np.array([0.4, -0.3, 0.2])
+        kw = dict(drive=Drive.vector(b), g=wave)
+        okw = dict(drive=("vector", b), g=wave)
+    ctl = Controller(atol=1e-9, rtol=1e-7)
+    dt, steps = 0.05, 3
+    runs = {}
+    for mode in ["f64_separate", "f64"]:
+        g.set_kcl_registers(mode)
+        g.poison_lds()
+        runs[mode] = g.transient_kcl(ctx, y0, dt, steps, ctl=ctl, precision="f64", **kw)
+    (ts, ss), (tf, sf) = runs["f64_separate"], runs["f64"]
+    octl = oracle.Controller.default()
+    octl.atol, octl.rtol = 1e-9, 1e-7
+    y, acc, rej, _, _ = oop.adaptive(y0, dt, steps, octl, **okw)
+    d = max(rel_l2(tf[k], ts[k]) for k in range(1, steps + 1))
+    err = rel_l2(tf[-1], y)
+    print(f"kcl fused FP64 update order={order} variant={variant} drive={drive}: fused {sf.accepted}/{sf.rejected}, "
+          f"separate {ss.accepted}/{ss.rejected}, oracle {acc}/{rej}; fused vs separate {d:.2e}, vs oracle {err:.2e}",
+          flush=True)
+    assert (sf.accepted, sf.rejected) == (ss.accepted, ss.rejected) == (acc, rej)
+    assert sf.rejected > 0
+    # the embedded error estimate, a difference of stage terms about 1e4
+    # times its size, carries the summation-order difference into h at
+    # about 1e-9 relative over 127 substeps (N=4)
+    assert abs(sf.h_min - ss.h_min) <= 1e-8 * ss.h_min and abs(sf.h_max - ss.h_max) <= 1e-8 * ss.h_max
+    assert d < 1e-12
+    assert err < 1e-11
+
+
 def test_kcl_collapse_is_an_error(ctx, oracle):
     """A step collapse is reported (session.rs:817-824), not a hang."""
     from rapidfem_amd import Controller, RfhipError
